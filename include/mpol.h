@@ -1,0 +1,143 @@
+/* C ABI of the MI355X spectral-Ewald mutual-polarization library (libmpol.so).
+ *
+ * Plain C types only: device pointers are `const double*` / `double*` to
+ * memory on the context's GPU (for example the data_ptr() of a torch tensor),
+ * host pointers are marked "host".  Every call returns an mpol_status; on
+ * failure mpol_last_error() returns a message for the calling thread.
+ *
+ * Layouts (all C order):
+ *   positions    x[N][3]           FP64, device
+ *   permittivity eps[nsets][N]     FP64, device (nsets = 1 or nrhs)
+ *   dipoles/fields m[nrhs][N][3]   FP64, device -- per right-hand side the
+ *                                  3N interleaved vector [p1x p1y p1z p2x ...]
+ *                                  of the reference solver.
+ *
+ * A context owns one HIP stream (or borrows the caller's), the geometry
+ * precompute and all workspaces.  A context is not re-entrant: calls on one
+ * context must not overlap.
+ */
+#ifndef MPOL_H_
+#define MPOL_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mpol_ctx mpol_ctx;
+
+typedef enum {
+  MPOL_OK = 0,
+  MPOL_EINVAL = 1,   /* bad argument or shape        -> Python ValueError   */
+  MPOL_ERANGE = 2,   /* pair separation outside table -> Python ValueError  */
+  MPOL_EHIP = 3,     /* HIP runtime failure          -> Python RuntimeError */
+  MPOL_ENOMEM = 4,   /* device allocation failed     -> Python MemoryError  */
+  MPOL_ESTATE = 5    /* call out of order (no geometry/tables yet)          */
+} mpol_status;
+
+typedef enum { MPOL_FP64 = 0, MPOL_FP32 = 1 } mpol_precision;
+
+typedef struct {
+  double tol;          /* relative residual tolerance ||b - A x|| <= tol ||b||   */
+  int restart;         /* Krylov dimension per cycle                            */
+  int maxit;           /* maximum number of restart cycles                      */
+  int mixed;           /* 0: FP64 operator; 1: FP32 operator inside FP64
+                          iterative refinement (FP64 Arnoldi/Givens always)     */
+  int max_refine;      /* refinement sweeps in mixed mode                       */
+  double slab_factor;  /* coefficient of the slab correction                    */
+} mpol_solve_opts;
+
+typedef struct {
+  int64_t matvecs_fp64;   /* batched operator applications in FP64          */
+  int64_t matvecs_fp32;   /* batched operator applications in FP32          */
+  int32_t converged;      /* number of right-hand sides that met tol        */
+  int32_t refinements;    /* refinement sweeps performed (mixed mode)       */
+  double max_relres;      /* max over rhs of the final true ||b-Ax||/||b||  */
+} mpol_solve_info;
+
+typedef struct {
+  int64_t n;              /* particles                                       */
+  int64_t npairs;         /* directed real-space pairs with d < rc           */
+  int32_t ngrid[3];
+  int32_t P;
+  int32_t ncell[3];
+  int32_t tile_rows;      /* (y,z) grid rows per spread/contract tile         */
+  int32_t fft_e[3];       /* elements/thread of each axis FFT (0 = direct DFT) */
+} mpol_geometry_summary;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int mpol_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* mpol_last_error(void);
+
+/* Number of visible HIP devices. */
+int mpol_device_count(int* count);
+
+/* Create a context on `device`.  `stream` is a hipStream_t to launch on, or
+ * NULL for a context-owned stream. */
+int mpol_create(mpol_ctx** ctx, int device, void* stream);
+int mpol_destroy(mpol_ctx* ctx);
+/* Change the stream later calls launch on (NULL: the context's own). */
+int mpol_set_stream(mpol_ctx* ctx, void* stream);
+/* Wait for all work the context has queued. */
+int mpol_synchronize(mpol_ctx* ctx);
+
+/* Real-space coupling tables (host, FP64, ntab entries each).  perp is the
+ * I - rr coupling, para the rr coupling, rvals the separations (ascending,
+ * rvals[0] = 0 carrying the self term perp[0]).
+ * Replaces the table arguments Hperp/Hpara/rvals of
+ * magnetic_field ("CPU - Ewald test/ewald.py":1146-1168). */
+int mpol_set_tables(mpol_ctx* ctx, const double* perp_host, const double* para_host,
+                    const double* rvals_host, int64_t ntab);
+
+/* Geometry precompute for positions x (device, FP64 N x 3): cell list and
+ * directed pair list with d < rc (counting sorts on the GPU), stencil start
+ * nodes and separable Gaussian factors of every particle.
+ * Replaces cell_list / neighbor_list / prepare_real_space_pairs /
+ * prepare_grid_kernel / precompute_kspace ("CPU - Ewald test/ewald.py":84-199,
+ * 939-1016, 233-296, 523-582) as called from capacitance()
+ * ("CPU - Ewald test/capacitance.py":370-385).
+ * Returns MPOL_ERANGE when a pair separation falls outside the table. */
+int mpol_set_geometry(mpol_ctx* ctx, const double* x_dev, int64_t n, const double box[3],
+                      const int32_t ngrid[3], int32_t P, double xi, const double eta[3], double rc);
+
+/* Particle permittivities, nsets x N (device FP64); set s applies to rhs s
+ * (nsets == nrhs) or to every rhs (nsets == 1). */
+int mpol_set_permittivity(mpol_ctx* ctx, const double* eps_dev, int32_t nsets);
+
+/* out = A m for nrhs right-hand sides: wave-space + real-space + slab terms.
+ * Replaces magnetic_field ("CPU - Ewald test/ewald.py":1146-1236). */
+int mpol_matvec(mpol_ctx* ctx, const double* m_dev, double* out_dev, int32_t nrhs, int32_t precision,
+                double slab_factor);
+
+/* The three parts separately: hk (wave space), hr (real space incl. the
+ * dielectric and self terms) and hslab (= -slab_factor sum(m_z)/V on z, so
+ * A m = hk + hr - hslab).
+ * Replaces magnetic_field_components ("CPU - Ewald test/ewald.py":1239-1327). */
+int mpol_matvec_components(mpol_ctx* ctx, const double* m_dev, double* hk_dev, double* hr_dev,
+                           double* hslab_dev, int32_t nrhs, int32_t precision, double slab_factor);
+
+/* Restarted GMRES on the device for nrhs right-hand sides b (device), initial
+ * guesses x0 (device), result x (device).  info (host) may be NULL.
+ * Replaces magnetic_dipole / gmres_restarted
+ * ("CPU - Ewald test/capacitance.py":205-321, 60-199). */
+int mpol_solve(mpol_ctx* ctx, const double* b_dev, const double* x0_dev, double* x_dev, int32_t nrhs,
+               const mpol_solve_opts* opts, mpol_solve_info* info);
+
+/* Geometry summary (host). */
+int mpol_geometry_info(mpol_ctx* ctx, mpol_geometry_summary* info);
+
+/* The six real-space coupling tables at separations r (host, n values > 0),
+ * each with its r = 0 self term first: out (host) is 6 x (n+1) in the order
+ * pot_charge, pot_dip, field_dip_1, field_dip_2, field_dip_force_1,
+ * field_dip_force_2.  Replaces real_space_table
+ * ("CPU - Ewald test/ewald.py":604-931). */
+int mpol_real_space_table(double xi, const double* r_host, int64_t n, double* out_host);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPOL_H_ */

@@ -1,0 +1,800 @@
+// C ABI of the MI355X spectral-Ewald mutual-polarization library.
+//
+// Host side of libmpol.so: context and workspace management, FFT plans,
+// geometry precompute, the operator launch sequence and the device-resident
+// restarted GMRES driver.  See include/mpol.h for the contract of every entry
+// point.
+#include "mpol.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mpol_gmres.h"
+#include "mpol_kernels.h"
+
+namespace mpol {
+void real_space_tables(double xi, const double* r, int64_t n, double* out);
+}
+
+using namespace mpol;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) return fail(MPOL_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define HIPALLOC(buf, cnt)                                                                  \
+  do {                                                                                      \
+    hipError_t e_ = (buf).reserve(cnt);                                                     \
+    if (e_ != hipSuccess) return fail(MPOL_ENOMEM, std::string("device allocation failed: ") + #buf); \
+  } while (0)
+
+#define CHECK_STATUS(expr)        \
+  do {                            \
+    int s_ = (expr);              \
+    if (s_ != MPOL_OK) return s_; \
+  } while (0)
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  hipError_t reserve(size_t cnt) {
+    if (cnt <= n && p) return hipSuccess;
+    release();
+    if (cnt == 0) return hipSuccess;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), cnt * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      n = 0;
+      return e;
+    }
+    n = cnt;
+    return hipSuccess;
+  }
+  void release() {
+    // a buffer is only replaced between calls; wait for kernels that may still read it
+    if (p) (void)hipDeviceSynchronize();
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { release(); }
+};
+
+inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+FftPlan make_plan(int n) {
+  FftPlan p{};
+  p.n = n;
+  p.e = 0;
+  p.nstages = 0;
+  int m = n;
+  for (int q : {2, 3, 5})
+    while (m % q == 0) m /= q;
+  const bool smooth = (m == 1);
+  if (!smooth || n < 12) return p;
+  for (int e : {16, 24, 20, 12}) {
+    if (n % e != 0 || n / e > 256) continue;
+    bool ok = true;
+    for (int q : {2, 3, 5})
+      if (n % q == 0 && e % q != 0) ok = false;
+    if (!ok) continue;
+    FftPlan t{};
+    t.n = n;
+    t.e = e;
+    int rem = n, ns = 1, s = 0;
+    while (rem > 1 && s < kMaxStages) {
+      int r = 0;
+      for (int cand : {16, 8, 4, 2, 3, 5})
+        if (rem % cand == 0 && e % cand == 0) {
+          r = cand;
+          break;
+        }
+      if (r == 0) break;
+      t.radix[s] = r;
+      t.ns[s] = ns;
+      ns *= r;
+      rem /= r;
+      ++s;
+    }
+    if (rem != 1) continue;
+    t.nstages = s;
+    return t;
+  }
+  return p;
+}
+
+void make_twiddles(int n, std::vector<double>& tw) {
+  tw.resize(2 * (size_t)n);
+  for (int k = 0; k < n; ++k) {
+    const long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)n;
+    tw[2 * k] = (double)std::cos(ang);
+    tw[2 * k + 1] = -(double)std::sin(ang);
+  }
+}
+
+struct TileCfg {
+  int ty = 1, tz = 1, rows = 1, ncl = 2, lstride = 1, gs = 1;
+  size_t lds = 0;
+};
+
+TileCfg tile_cfg(int nx, int e, size_t csize, size_t budget) {
+  const int opts[][2] = {{4, 4}, {4, 2}, {2, 2}, {2, 1}, {1, 1}};
+  TileCfg c;
+  for (const auto& o : opts) {
+    c.ty = o[0];
+    c.tz = o[1];
+    c.rows = c.ty * c.tz;
+    c.ncl = (3 * c.rows + 1) / 2;
+    c.lstride = padded_len(nx);
+    c.lds = ((size_t)nx + (size_t)c.ncl * c.lstride * (e == 0 ? 2 : 1)) * csize;
+    if (c.lds <= budget) break;
+  }
+  c.gs = 1;
+  while (c.gs < c.rows) c.gs <<= 1;
+  return c;
+}
+
+struct ColCfg {
+  int tx = 1, lstride = 1;
+  size_t lds = 0;
+};
+
+ColCfg col_cfg(int n, int e, int kx, int lines_per_col, size_t csize, size_t budget, int tx_pref) {
+  ColCfg c;
+  int tx = std::min(tx_pref, kx);
+  for (;; tx = std::max(1, tx / 2)) {
+    c.tx = tx;
+    c.lstride = padded_len(n);
+    c.lds = ((size_t)n + (size_t)lines_per_col * tx * c.lstride * (e == 0 ? 2 : 1)) * csize;
+    if (c.lds <= budget || tx == 1) break;
+  }
+  return c;
+}
+
+constexpr size_t kLdsTile = 80 * 1024;
+constexpr size_t kLdsCol = 112 * 1024;
+
+}  // namespace
+
+struct mpol_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipStream_t own_stream = nullptr;
+  // tables
+  int ntab = 0;
+  double self_perp = 0.0;
+  DevBuf<double> perp, para, rvals;
+  // geometry
+  bool have_geo = false;
+  int n = 0;
+  double box[3] = {0, 0, 0}, h[3] = {0, 0, 0}, eta[3] = {0, 0, 0};
+  double xi = 0, rc = 0;
+  int ng[3] = {0, 0, 0}, kx = 0, P = 0, half = 0, nc[3] = {0, 0, 0};
+  int64_t npairs = 0;
+  DevBuf<double> x, base;
+  DevBuf<int> bin_key, bin_count, bin_start, bin_cursor, cell_key, cell_count, cell_start, cell_cursor,
+      cell_members, g0u, perm, g0s, rs_count, rs_off, rs_j, err;
+  DevBuf<double> fac64, rs_A, rs_BA, rs_rh;
+  DevBuf<float> fac32, rs_A32, rs_BA32, rs_rh32;
+  FftPlan plan[3];
+  DevBuf<double> tw64[3];
+  DevBuf<float> tw32[3];
+  // permittivity
+  int nsets = 0;
+  DevBuf<double> diag;
+  // workspaces
+  DevBuf<unsigned char> spec;
+  DevBuf<double> zsum;
+  // gmres
+  DevBuf<double> gV, gW, gR, gAx, gH, gcs, gsn, gg, gy, gh1, gh2, gpart, gbn, gbeta, grel, grn2, gx, gd;
+  DevBuf<int> gstate, gkdim;
+};
+
+namespace {
+
+template <typename K>
+void set_lds(K kernel, size_t bytes) {
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)bytes);
+}
+
+#define DISPATCH_E(e, KERNEL, ...)                     \
+  switch (e) {                                         \
+    case 0: KERNEL(0, __VA_ARGS__); break;             \
+    case 12: KERNEL(12, __VA_ARGS__); break;           \
+    case 16: KERNEL(16, __VA_ARGS__); break;           \
+    case 20: KERNEL(20, __VA_ARGS__); break;           \
+    case 24: KERNEL(24, __VA_ARGS__); break;           \
+    default: return fail(MPOL_EINVAL, "unsupported FFT plan"); \
+  }
+
+template <typename T>
+const void* tw_ptr(mpol_ctx* c, int axis) {
+  if constexpr (sizeof(T) == 8) return c->tw64[axis].p;
+  else return c->tw32[axis].p;
+}
+
+template <typename T>
+const void* fac_ptr(mpol_ctx* c) {
+  if constexpr (sizeof(T) == 8) return c->fac64.p;
+  else return c->fac32.p;
+}
+
+// out = A m (or its three parts when hr/hslab are given) in precision T.
+template <typename T>
+int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* hslab, int nrhs, double slab) {
+  const size_t cs = sizeof(cplx<T>);
+  const size_t spec_bytes = (size_t)3 * nrhs * c->ng[2] * c->ng[1] * c->kx * cs;
+  HIPALLOC(c->spec, spec_bytes);
+  HIPALLOC(c->zsum, (size_t)nrhs);
+  cplx<T>* spec = reinterpret_cast<cplx<T>*>(c->spec.p);
+  const int n = c->n;
+  hipStream_t st = c->stream;
+
+  hipLaunchKernelGGL(k_sum_z, dim3(nrhs), dim3(256), 0, st, m, n, c->zsum.p);
+  const double V = c->box[0] * c->box[1] * c->box[2];
+  const T* pA;
+  const T* pBA;
+  const T* prh;
+  if constexpr (sizeof(T) == 8) {
+    pA = c->rs_A.p; pBA = c->rs_BA.p; prh = c->rs_rh.p;
+  } else {
+    pA = c->rs_A32.p; pBA = c->rs_BA32.p; prh = c->rs_rh32.p;
+  }
+  hipLaunchKernelGGL((k_real_space<T>), dim3(cdiv(n, 256), nrhs), dim3(256), 0, st, n, nrhs, c->nsets, m, out,
+                     c->diag.p, c->rs_off.p, c->rs_j.p, pA, pBA, prh, c->zsum.p, slab / V, hr != nullptr ? 1 : 0, hr,
+                     hslab);
+
+  // tile (x axis) kernels
+  const TileCfg tc = tile_cfg(c->ng[0], c->plan[0].e, cs, kLdsTile);
+  TileArgs ta{};
+  ta.nx = c->ng[0]; ta.ny = c->ng[1]; ta.nz = c->ng[2]; ta.kx = c->kx;
+  ta.ty = tc.ty; ta.tz = tc.tz; ta.rows = tc.rows; ta.ncl = tc.ncl; ta.lstride = tc.lstride; ta.gs = tc.gs;
+  ta.P = c->P; ta.n = n; ta.nrhs = nrhs;
+  ta.perm = c->perm.p; ta.g0 = c->g0s.p; ta.fac = fac_ptr<T>(c); ta.bin_start = c->bin_start.p;
+  ta.tw = tw_ptr<T>(c, 0); ta.plan = c->plan[0];
+  ta.hvol = c->h[0] * c->h[1] * c->h[2];
+  const dim3 tgrid(cdiv(c->ng[1], tc.ty), cdiv(c->ng[2], tc.tz), nrhs);
+
+#define LAUNCH_SPREAD(E, ...)                                                                   \
+  do {                                                                                          \
+    set_lds(k_spread_xfft<T, E>, tc.lds);                                                       \
+    hipLaunchKernelGGL((k_spread_xfft<T, E>), tgrid, dim3(256), tc.lds, st, ta, m, spec);       \
+  } while (0)
+  DISPATCH_E(ta.plan.e, LAUNCH_SPREAD, 0);
+
+  // y axis forward
+  const ColCfg cy = col_cfg(c->ng[1], c->plan[1].e, c->kx, 1, cs, kLdsCol, sizeof(T) == 8 ? 8 : 16);
+  ColArgs ay{};
+  ay.nx = c->ng[0]; ay.ny = c->ng[1]; ay.nz = c->ng[2]; ay.kx = c->kx; ay.tx = cy.tx; ay.lstride = cy.lstride;
+  ay.plan = c->plan[1]; ay.tw = tw_ptr<T>(c, 1);
+  const dim3 ygrid(cdiv(c->kx, cy.tx), c->ng[2], 3 * nrhs);
+#define LAUNCH_FFTY(E, INV)                                                                     \
+  do {                                                                                          \
+    set_lds(k_fft_y<T, E>, cy.lds);                                                             \
+    hipLaunchKernelGGL((k_fft_y<T, E>), ygrid, dim3(256), cy.lds, st, ay, spec, INV);          \
+  } while (0)
+  DISPATCH_E(ay.plan.e, LAUNCH_FFTY, 0);
+
+  // z axis forward + scale + inverse
+  const ColCfg cz = col_cfg(c->ng[2], c->plan[2].e, c->kx, 3, cs, kLdsCol, sizeof(T) == 8 ? 8 : 16);
+  ColArgs az{};
+  az.nx = c->ng[0]; az.ny = c->ng[1]; az.nz = c->ng[2]; az.kx = c->kx; az.tx = cz.tx; az.lstride = cz.lstride;
+  az.plan = c->plan[2]; az.tw = tw_ptr<T>(c, 2);
+  ScaleParams sp{};
+  for (int d = 0; d < 3; ++d) {
+    sp.twopi_over_l[d] = 2.0 * kPi / c->box[d];
+    sp.one_minus_eta[d] = 1.0 - c->eta[d];
+  }
+  sp.inv_4xi2 = 1.0 / (4.0 * c->xi * c->xi);
+  sp.inv_g = 1.0 / ((double)c->ng[0] * (double)c->ng[1] * (double)c->ng[2]);
+  const dim3 zgrid(cdiv(c->kx, cz.tx), c->ng[1], nrhs);
+#define LAUNCH_FFTZ(E, ...)                                                                     \
+  do {                                                                                          \
+    set_lds(k_fft_z_scale<T, E>, cz.lds);                                                       \
+    hipLaunchKernelGGL((k_fft_z_scale<T, E>), zgrid, dim3(256), cz.lds, st, az, sp, spec);     \
+  } while (0)
+  DISPATCH_E(az.plan.e, LAUNCH_FFTZ, 0);
+
+  DISPATCH_E(ay.plan.e, LAUNCH_FFTY, 1);
+
+#define LAUNCH_CONTRACT(E, ...)                                                                 \
+  do {                                                                                          \
+    set_lds(k_xifft_contract<T, E>, tc.lds);                                                    \
+    hipLaunchKernelGGL((k_xifft_contract<T, E>), tgrid, dim3(256), tc.lds, st, ta, spec, out);  \
+  } while (0)
+  DISPATCH_E(ta.plan.e, LAUNCH_CONTRACT, 0);
+#undef LAUNCH_SPREAD
+#undef LAUNCH_FFTY
+#undef LAUNCH_FFTZ
+#undef LAUNCH_CONTRACT
+  HIPCHK(hipGetLastError());
+  return MPOL_OK;
+}
+
+int apply(mpol_ctx* c, const double* m, double* out, int nrhs, int precision, double slab) {
+  if (precision == MPOL_FP32) return run_operator<float>(c, m, out, nullptr, nullptr, nrhs, slab);
+  return run_operator<double>(c, m, out, nullptr, nullptr, nrhs, slab);
+}
+
+int check_ready(mpol_ctx* c) {
+  if (!c) return fail(MPOL_EINVAL, "null context");
+  if (!c->have_geo) return fail(MPOL_ESTATE, "geometry not set (call mpol_set_geometry first)");
+  if (c->nsets == 0) return fail(MPOL_ESTATE, "permittivity not set (call mpol_set_permittivity first)");
+  return MPOL_OK;
+}
+
+// squared norms / dot products of `cnt` vectors A_i (stride stride_i) with w, per rhs -> out[rhs*cnt+i]
+int dots(mpol_ctx* c, const double* A, size_t stride_i, int cnt, const double* w, int64_t n, int nrhs, double* out) {
+  hipLaunchKernelGGL(k_dots, dim3(kDotBlocks, nrhs), dim3(256), 0, c->stream, A, stride_i, cnt, w, n, nrhs,
+                     c->gpart.p);
+  hipLaunchKernelGGL(k_dots_finish, dim3(cdiv((int64_t)nrhs * cnt, 256)), dim3(256), 0, c->stream, c->gpart.p,
+                     kDotBlocks, nrhs * cnt, out);
+  return MPOL_OK;
+}
+
+// Restarted GMRES on x (in: initial guess, out: solution) for A x = b.
+int gmres_core(mpol_ctx* c, const double* b, double* x, int nrhs, double tol, int restart, int maxit, int precision,
+               double slab, int64_t* nmv, double* relres_host, int* converged_host) {
+  const int64_t n3 = 3LL * c->n;
+  const size_t tot = (size_t)nrhs * n3;
+  const size_t sv = tot;  // stride between basis vectors
+  HIPALLOC(c->gV, (size_t)(restart + 1) * tot);
+  HIPALLOC(c->gW, tot);
+  HIPALLOC(c->gR, tot);
+  HIPALLOC(c->gAx, tot);
+  HIPALLOC(c->gH, (size_t)nrhs * (restart + 1) * restart);
+  HIPALLOC(c->gcs, (size_t)nrhs * restart);
+  HIPALLOC(c->gsn, (size_t)nrhs * restart);
+  HIPALLOC(c->gg, (size_t)nrhs * (restart + 1));
+  HIPALLOC(c->gy, (size_t)nrhs * restart);
+  HIPALLOC(c->gh1, (size_t)nrhs * (restart + 1));
+  HIPALLOC(c->gh2, (size_t)nrhs * (restart + 1));
+  HIPALLOC(c->gpart, (size_t)nrhs * (restart + 1) * kDotBlocks);
+  HIPALLOC(c->gbn, (size_t)nrhs);
+  HIPALLOC(c->gbeta, (size_t)nrhs);
+  HIPALLOC(c->grel, (size_t)nrhs);
+  HIPALLOC(c->grn2, (size_t)nrhs);
+  HIPALLOC(c->gstate, (size_t)nrhs);
+  HIPALLOC(c->gkdim, (size_t)nrhs);
+  hipStream_t st = c->stream;
+  const dim3 vgrid(cdiv(n3, 256), nrhs);
+  const int sgrid = cdiv(nrhs, 256);
+
+  // ||b||
+  dots(c, b, 0, 1, b, n3, nrhs, c->grn2.p);
+  std::vector<double> bn2(nrhs);
+  HIPCHK(hipMemcpyAsync(bn2.data(), c->grn2.p, nrhs * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  std::vector<double> bn(nrhs);
+  std::vector<int> state(nrhs);
+  for (int r = 0; r < nrhs; ++r) {
+    bn[r] = std::sqrt(bn2[r]);
+    state[r] = bn[r] > 0.0 ? kWait : kDone;
+  }
+  HIPCHK(hipMemcpyAsync(c->gbn.p, bn.data(), nrhs * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->gstate.p, state.data(), nrhs * sizeof(int), hipMemcpyHostToDevice, st));
+
+  std::vector<double> rel(nrhs, 0.0);
+  bool all_done = false;
+  for (int outer = 0; outer <= maxit; ++outer) {
+    // true residual at the start of each cycle (and once after the last)
+    CHECK_STATUS(apply(c, x, c->gAx.p, nrhs, precision, slab));
+    ++*nmv;
+    hipLaunchKernelGGL(k_residual, dim3(cdiv((int64_t)tot, 256)), dim3(256), 0, st, b, c->gAx.p, c->gR.p,
+                       (int64_t)tot);
+    dots(c, c->gR.p, 0, 1, c->gR.p, n3, nrhs, c->grn2.p);
+    hipLaunchKernelGGL(k_cycle_start, dim3(sgrid), dim3(256), 0, st, nrhs, restart, c->grn2.p, c->gbn.p, tol,
+                       c->gstate.p, c->gg.p, c->gbeta.p, c->grel.p);
+    HIPCHK(hipMemcpyAsync(state.data(), c->gstate.p, nrhs * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(rel.data(), c->grel.p, nrhs * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    all_done = std::all_of(state.begin(), state.end(), [](int s) { return s == kDone; });
+    if (all_done || outer == maxit) break;
+    hipLaunchKernelGGL(k_scale_into, vgrid, dim3(256), 0, st, c->gR.p, c->gbeta.p, c->gV.p, n3, c->gstate.p);
+    for (int j = 0; j < restart; ++j) {
+      double* vj = c->gV.p + (size_t)j * sv;
+      CHECK_STATUS(apply(c, vj, c->gW.p, nrhs, precision, slab));
+      ++*nmv;
+      // classical Gram-Schmidt, twice
+      dots(c, c->gV.p, sv, j + 1, c->gW.p, n3, nrhs, c->gh1.p);
+      hipLaunchKernelGGL(k_sub_basis, vgrid, dim3(256), 0, st, c->gV.p, sv, j + 1, c->gh1.p, c->gW.p, n3,
+                         c->gstate.p);
+      dots(c, c->gV.p, sv, j + 1, c->gW.p, n3, nrhs, c->gh2.p);
+      hipLaunchKernelGGL(k_sub_basis, vgrid, dim3(256), 0, st, c->gV.p, sv, j + 1, c->gh2.p, c->gW.p, n3,
+                         c->gstate.p);
+      dots(c, c->gW.p, 0, 1, c->gW.p, n3, nrhs, c->grn2.p);
+      hipLaunchKernelGGL(k_arnoldi_finish, dim3(sgrid), dim3(256), 0, st, nrhs, restart, j, c->gh1.p, c->gh2.p,
+                         c->grn2.p, c->gH.p, c->gstate.p);
+      hipLaunchKernelGGL(k_normalize_next, vgrid, dim3(256), 0, st, c->gW.p, c->gV.p + (size_t)(j + 1) * sv,
+                         c->gH.p, restart, j, n3, c->gstate.p);
+      hipLaunchKernelGGL(k_givens, dim3(sgrid), dim3(256), 0, st, nrhs, restart, j, c->gH.p, c->gcs.p, c->gsn.p,
+                         c->gg.p, c->gbn.p, tol, c->gstate.p, c->gkdim.p);
+      hipLaunchKernelGGL(k_backsolve, dim3(sgrid), dim3(256), 0, st, nrhs, restart, c->gH.p, c->gg.p,
+                         c->gkdim.p, c->gstate.p, c->gy.p);
+      hipLaunchKernelGGL(k_update_x, vgrid, dim3(256), 0, st, c->gV.p, sv, c->gy.p, c->gkdim.p, c->gstate.p,
+                         restart, x, n3);
+      hipLaunchKernelGGL(k_solve_to_wait, dim3(sgrid), dim3(256), 0, st, nrhs, c->gstate.p);
+      HIPCHK(hipMemcpyAsync(state.data(), c->gstate.p, nrhs * sizeof(int), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      if (std::none_of(state.begin(), state.end(), [](int s) { return s == kIter; })) break;
+    }
+  }
+  HIPCHK(hipGetLastError());
+  int conv = 0;
+  for (int r = 0; r < nrhs; ++r) {
+    if (relres_host) relres_host[r] = rel[r];
+    if (rel[r] <= tol) ++conv;
+  }
+  if (converged_host) *converged_host = conv;
+  return MPOL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpol_version(void) { return 100; }
+
+const char* mpol_last_error(void) { return g_err.c_str(); }
+
+int mpol_device_count(int* count) {
+  if (!count) return fail(MPOL_EINVAL, "null count");
+  HIPCHK(hipGetDeviceCount(count));
+  return MPOL_OK;
+}
+
+int mpol_create(mpol_ctx** out, int device, void* stream) {
+  if (!out) return fail(MPOL_EINVAL, "null output pointer");
+  int nd = 0;
+  HIPCHK(hipGetDeviceCount(&nd));
+  if (device < 0 || device >= nd) return fail(MPOL_EINVAL, "device index out of range");
+  HIPCHK(hipSetDevice(device));
+  mpol_ctx* c = new mpol_ctx();
+  c->device = device;
+  if (stream) {
+    c->stream = reinterpret_cast<hipStream_t>(stream);
+  } else {
+    hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete c;
+      return fail(MPOL_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    c->stream = c->own_stream;
+  }
+  *out = c;
+  return MPOL_OK;
+}
+
+int mpol_destroy(mpol_ctx* c) {
+  if (!c) return MPOL_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+  delete c;
+  return MPOL_OK;
+}
+
+int mpol_set_stream(mpol_ctx* c, void* stream) {
+  if (!c) return fail(MPOL_EINVAL, "null context");
+  c->stream = stream ? reinterpret_cast<hipStream_t>(stream) : c->own_stream;
+  if (!c->stream) {
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+    c->stream = c->own_stream;
+  }
+  return MPOL_OK;
+}
+
+int mpol_synchronize(mpol_ctx* c) {
+  if (!c) return fail(MPOL_EINVAL, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MPOL_OK;
+}
+
+int mpol_set_tables(mpol_ctx* c, const double* perp, const double* para, const double* rvals, int64_t ntab) {
+  if (!c || !perp || !para || !rvals) return fail(MPOL_EINVAL, "null table pointer");
+  if (ntab < 2) return fail(MPOL_EINVAL, "tables need at least two entries");
+  for (int64_t i = 1; i < ntab; ++i)
+    if (!(rvals[i] > rvals[i - 1])) return fail(MPOL_EINVAL, "rvals must be strictly increasing");
+  HIPCHK(hipSetDevice(c->device));
+  HIPALLOC(c->perp, (size_t)ntab);
+  HIPALLOC(c->para, (size_t)ntab);
+  HIPALLOC(c->rvals, (size_t)ntab);
+  HIPCHK(hipMemcpyAsync(c->perp.p, perp, ntab * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->para.p, para, ntab * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->rvals.p, rvals, ntab * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->ntab = (int)ntab;
+  c->self_perp = perp[0];
+  c->have_geo = false;  // pair data depends on the tables
+  return MPOL_OK;
+}
+
+int mpol_set_geometry(mpol_ctx* c, const double* x, int64_t n, const double box[3], const int32_t ngrid[3],
+                      int32_t P, double xi, const double eta[3], double rc) {
+  if (!c || !x || !box || !ngrid || !eta) return fail(MPOL_EINVAL, "null argument");
+  if (c->ntab == 0) return fail(MPOL_ESTATE, "tables not set (call mpol_set_tables first)");
+  if (n < 1 || n > (1LL << 30)) return fail(MPOL_EINVAL, "particle count out of range");
+  for (int d = 0; d < 3; ++d) {
+    if (!(box[d] > 0.0)) return fail(MPOL_EINVAL, "box lengths must be positive");
+    if (ngrid[d] < 2 || ngrid[d] > 4096) return fail(MPOL_EINVAL, "grid size must be in [2, 4096]");
+    if (!(eta[d] > 0.0)) return fail(MPOL_EINVAL, "eta must be positive");
+  }
+  if (P < 1 || P > 64) return fail(MPOL_EINVAL, "P must be in [1, 64]");
+  if (!(xi > 0.0) || !(rc > 0.0)) return fail(MPOL_EINVAL, "xi and rc must be positive");
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = c->stream;
+  c->have_geo = false;
+  c->n = (int)n;
+  for (int d = 0; d < 3; ++d) {
+    c->box[d] = box[d];
+    c->ng[d] = ngrid[d];
+    c->h[d] = box[d] / (double)ngrid[d];
+    c->eta[d] = eta[d];
+    int ncell = (int)std::floor(box[d] / rc);
+    c->nc[d] = ncell < 3 ? 3 : ncell;
+  }
+  c->kx = c->ng[0] / 2 + 1;
+  c->P = P;
+  c->half = (P - 1) / 2;
+  c->xi = xi;
+  c->rc = rc;
+  const int nbins = c->ng[1] * c->ng[2];
+  const int64_t ncells = (int64_t)c->nc[0] * c->nc[1] * c->nc[2];
+  if (ncells > (1LL << 28)) return fail(MPOL_EINVAL, "too many real-space cells");
+  HIPALLOC(c->x, 3 * (size_t)n);
+  HIPALLOC(c->base, 3 * (size_t)n);
+  HIPALLOC(c->bin_key, (size_t)n);
+  HIPALLOC(c->bin_count, (size_t)nbins);
+  HIPALLOC(c->bin_start, (size_t)nbins + 1);
+  HIPALLOC(c->bin_cursor, (size_t)nbins);
+  HIPALLOC(c->cell_key, (size_t)n);
+  HIPALLOC(c->cell_count, (size_t)ncells);
+  HIPALLOC(c->cell_start, (size_t)ncells + 1);
+  HIPALLOC(c->cell_cursor, (size_t)ncells);
+  HIPALLOC(c->cell_members, (size_t)n);
+  HIPALLOC(c->g0u, 3 * (size_t)n);
+  HIPALLOC(c->perm, (size_t)n);
+  HIPALLOC(c->g0s, 3 * (size_t)n);
+  HIPALLOC(c->fac64, 3 * (size_t)n * P);
+  HIPALLOC(c->fac32, 3 * (size_t)n * P);
+  HIPALLOC(c->rs_count, (size_t)n);
+  HIPALLOC(c->rs_off, (size_t)n + 1);
+  HIPALLOC(c->err, 1);
+  HIPCHK(hipMemcpyAsync(c->x.p, x, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemsetAsync(c->bin_count.p, 0, nbins * sizeof(int), st));
+  HIPCHK(hipMemsetAsync(c->bin_cursor.p, 0, nbins * sizeof(int), st));
+  HIPCHK(hipMemsetAsync(c->cell_count.p, 0, ncells * sizeof(int), st));
+  HIPCHK(hipMemsetAsync(c->cell_cursor.p, 0, ncells * sizeof(int), st));
+  HIPCHK(hipMemsetAsync(c->err.p, 0, sizeof(int), st));
+
+  GeoParams g{};
+  g.n = (int)n;
+  g.nx = c->ng[0]; g.ny = c->ng[1]; g.nz = c->ng[2];
+  g.P = P; g.half = c->half;
+  double peta = 1.0;
+  for (int d = 0; d < 3; ++d) {
+    g.box[d] = box[d];
+    g.h[d] = c->h[d];
+    g.eta[d] = eta[d];
+    g.nc[d] = c->nc[d];
+  }
+  peta = (eta[0] * eta[1]) * eta[2];
+  g.xi = xi;
+  g.norm = std::pow(2.0 * xi * xi / 3.14159265358979323846, 1.5) * std::sqrt(1.0 / peta);
+  const int pg = cdiv(n, 256);
+  hipLaunchKernelGGL(k_geo_keys, dim3(pg), dim3(256), 0, st, g, c->x.p, c->bin_key.p, c->bin_count.p,
+                     c->cell_key.p, c->cell_count.p, c->base.p, c->g0u.p);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, c->bin_count.p, c->bin_start.p, nbins);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, c->cell_count.p, c->cell_start.p, (int)ncells);
+  hipLaunchKernelGGL(k_geo_scatter, dim3(pg), dim3(256), 0, st, g, c->bin_key.p, c->bin_start.p, c->bin_cursor.p,
+                     c->cell_key.p, c->cell_start.p, c->cell_cursor.p, c->base.p, c->g0u.p, c->perm.p, c->g0s.p,
+                     c->fac64.p, c->fac32.p, c->cell_members.p);
+  PairParams pp{};
+  pp.n = (int)n;
+  for (int d = 0; d < 3; ++d) {
+    pp.box[d] = box[d];
+    pp.nc[d] = c->nc[d];
+  }
+  pp.rc = rc;
+  pp.ntab = c->ntab;
+  hipLaunchKernelGGL((k_pairs<false>), dim3(pg), dim3(256), 0, st, pp, c->x.p, c->cell_key.p, c->cell_start.p,
+                     c->cell_members.p, c->rvals.p, c->perp.p, c->para.p, c->rs_count.p, nullptr, nullptr, nullptr,
+                     nullptr, nullptr, nullptr, nullptr, nullptr, c->err.p);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, c->rs_count.p, c->rs_off.p, (int)n);
+  int total = 0, errflag = 0;
+  HIPCHK(hipMemcpyAsync(&total, c->rs_off.p + n, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&errflag, c->err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipGetLastError());
+  if (errflag) {
+    return fail(MPOL_ERANGE,
+                "RealSpace interpolation: pair separations fall outside rvals table. Increase r_table max "
+                "(in RunCapacitance) or adjust xi/errortol.");
+  }
+  c->npairs = total;
+  const size_t np = total > 0 ? (size_t)total : 1;
+  HIPALLOC(c->rs_j, np);
+  HIPALLOC(c->rs_A, np);
+  HIPALLOC(c->rs_BA, np);
+  HIPALLOC(c->rs_rh, 3 * np);
+  HIPALLOC(c->rs_A32, np);
+  HIPALLOC(c->rs_BA32, np);
+  HIPALLOC(c->rs_rh32, 3 * np);
+  hipLaunchKernelGGL((k_pairs<true>), dim3(pg), dim3(256), 0, st, pp, c->x.p, c->cell_key.p, c->cell_start.p,
+                     c->cell_members.p, c->rvals.p, c->perp.p, c->para.p, nullptr, c->rs_off.p, c->rs_j.p,
+                     c->rs_A.p, c->rs_BA.p, c->rs_rh.p, c->rs_A32.p, c->rs_BA32.p, c->rs_rh32.p, c->err.p);
+  // FFT plans and twiddles
+  for (int d = 0; d < 3; ++d) {
+    c->plan[d] = make_plan(c->ng[d]);
+    std::vector<double> tw;
+    make_twiddles(c->ng[d], tw);
+    std::vector<float> tw32(tw.begin(), tw.end());
+    HIPALLOC(c->tw64[d], tw.size());
+    HIPALLOC(c->tw32[d], tw32.size());
+    HIPCHK(hipMemcpyAsync(c->tw64[d].p, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(c->tw32[d].p, tw32.data(), tw32.size() * sizeof(float), hipMemcpyHostToDevice, st));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipGetLastError());
+  c->have_geo = true;
+  return MPOL_OK;
+}
+
+}  // extern "C"
+
+namespace {
+__global__ void k_diag(const double* __restrict__ eps, int64_t total, double self_perp, double* __restrict__ diag) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < total) diag[k] = -3.0 / (4.0 * kPi * (1.0 - eps[k])) + self_perp;
+}
+}  // namespace
+
+extern "C" {
+
+int mpol_set_permittivity(mpol_ctx* c, const double* eps, int32_t nsets) {
+  if (!c || !eps) return fail(MPOL_EINVAL, "null argument");
+  if (!c->have_geo) return fail(MPOL_ESTATE, "geometry not set (call mpol_set_geometry first)");
+  if (nsets < 1) return fail(MPOL_EINVAL, "nsets must be >= 1");
+  HIPCHK(hipSetDevice(c->device));
+  const int64_t total = (int64_t)nsets * c->n;
+  HIPALLOC(c->diag, (size_t)total);
+  hipLaunchKernelGGL(k_diag, dim3(cdiv(total, 256)), dim3(256), 0, c->stream, eps, total, c->self_perp, c->diag.p);
+  HIPCHK(hipGetLastError());
+  c->nsets = nsets;
+  return MPOL_OK;
+}
+
+int mpol_matvec(mpol_ctx* c, const double* m, double* out, int32_t nrhs, int32_t precision, double slab) {
+  CHECK_STATUS(check_ready(c));
+  if (!m || !out || nrhs < 1) return fail(MPOL_EINVAL, "bad matvec arguments");
+  if (c->nsets != 1 && c->nsets != nrhs) return fail(MPOL_EINVAL, "permittivity sets must be 1 or nrhs");
+  if (precision != MPOL_FP64 && precision != MPOL_FP32) return fail(MPOL_EINVAL, "unknown precision");
+  HIPCHK(hipSetDevice(c->device));
+  return apply(c, m, out, nrhs, precision, slab);
+}
+
+int mpol_matvec_components(mpol_ctx* c, const double* m, double* hk, double* hr, double* hslab, int32_t nrhs,
+                           int32_t precision, double slab) {
+  CHECK_STATUS(check_ready(c));
+  if (!m || !hk || !hr || !hslab || nrhs < 1) return fail(MPOL_EINVAL, "bad matvec arguments");
+  if (c->nsets != 1 && c->nsets != nrhs) return fail(MPOL_EINVAL, "permittivity sets must be 1 or nrhs");
+  HIPCHK(hipSetDevice(c->device));
+  if (precision == MPOL_FP32) return run_operator<float>(c, m, hk, hr, hslab, nrhs, slab);
+  return run_operator<double>(c, m, hk, hr, hslab, nrhs, slab);
+}
+
+int mpol_solve(mpol_ctx* c, const double* b, const double* x0, double* x, int32_t nrhs, const mpol_solve_opts* o,
+               mpol_solve_info* info) {
+  CHECK_STATUS(check_ready(c));
+  if (!b || !x0 || !x || !o || nrhs < 1) return fail(MPOL_EINVAL, "bad solve arguments");
+  if (c->nsets != 1 && c->nsets != nrhs) return fail(MPOL_EINVAL, "permittivity sets must be 1 or nrhs");
+  if (o->restart < 1 || o->restart > 200 || o->maxit < 0) return fail(MPOL_EINVAL, "bad restart/maxit");
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = c->stream;
+  const int64_t n3 = 3LL * c->n;
+  const int64_t tot = n3 * nrhs;
+  mpol_solve_info inf{};
+  std::vector<double> rel(nrhs, 0.0);
+  int conv = 0;
+  if (x != x0) HIPCHK(hipMemcpyAsync(x, x0, tot * sizeof(double), hipMemcpyDeviceToDevice, st));
+  if (!o->mixed) {
+    int64_t nmv = 0;
+    CHECK_STATUS(gmres_core(c, b, x, nrhs, o->tol, o->restart, o->maxit, MPOL_FP64, o->slab_factor, &nmv,
+                            rel.data(), &conv));
+    inf.matvecs_fp64 = nmv;
+  } else {
+    // FP64 iterative refinement around an FP32-operator GMRES
+    HIPALLOC(c->gd, (size_t)tot);
+    HIPALLOC(c->gx, (size_t)tot);
+    HIPALLOC(c->gR, (size_t)tot);
+    HIPALLOC(c->gAx, (size_t)tot);
+    HIPALLOC(c->grn2, (size_t)nrhs);
+    HIPALLOC(c->gpart, (size_t)nrhs * (o->restart + 1) * kDotBlocks);
+    std::vector<double> bn2(nrhs), rn2(nrhs);
+    dots(c, b, 0, 1, b, n3, nrhs, c->grn2.p);
+    HIPCHK(hipMemcpyAsync(bn2.data(), c->grn2.p, nrhs * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    int64_t nmv64 = 0, nmv32 = 0;
+    int ref = 0;
+    for (;; ++ref) {
+      CHECK_STATUS(apply(c, x, c->gAx.p, nrhs, MPOL_FP64, o->slab_factor));
+      ++nmv64;
+      hipLaunchKernelGGL(k_residual, dim3(cdiv(tot, 256)), dim3(256), 0, st, b, c->gAx.p, c->gd.p, tot);
+      dots(c, c->gd.p, 0, 1, c->gd.p, n3, nrhs, c->grn2.p);
+      HIPCHK(hipMemcpyAsync(rn2.data(), c->grn2.p, nrhs * sizeof(double), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      double worst = 0.0, ratio = 1e300;
+      conv = 0;
+      for (int r = 0; r < nrhs; ++r) {
+        rel[r] = bn2[r] > 0.0 ? std::sqrt(rn2[r] / bn2[r]) : 0.0;
+        worst = std::max(worst, rel[r]);
+        if (rel[r] <= o->tol) ++conv;
+        else if (rn2[r] > 0.0) ratio = std::min(ratio, std::sqrt(bn2[r] / rn2[r]));
+      }
+      if (conv == nrhs || ref >= o->max_refine) break;
+      // inner tolerance relative to the current residual
+      double tin = 0.3 * o->tol * ratio;
+      tin = std::min(0.1, std::max(tin, 2e-6));
+      // residual -> rhs of the correction solve (copy into gx), correction into gd
+      hipLaunchKernelGGL(k_copy, dim3(cdiv(tot, 256)), dim3(256), 0, st, c->gd.p, c->gx.p, tot);
+      HIPCHK(hipMemsetAsync(c->gd.p, 0, tot * sizeof(double), st));
+      int64_t nmv = 0;
+      CHECK_STATUS(gmres_core(c, c->gx.p, c->gd.p, nrhs, tin, o->restart, o->maxit, MPOL_FP32, o->slab_factor, &nmv,
+                              nullptr, nullptr));
+      nmv32 += nmv;
+      hipLaunchKernelGGL(k_axpy_all, dim3(cdiv(tot, 256)), dim3(256), 0, st, 1.0, c->gd.p, x, tot);
+    }
+    inf.matvecs_fp64 = nmv64;
+    inf.matvecs_fp32 = nmv32;
+    inf.refinements = ref;
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  inf.converged = conv;
+  inf.max_relres = 0.0;
+  for (double v : rel) inf.max_relres = std::max(inf.max_relres, v);
+  if (info) *info = inf;
+  return MPOL_OK;
+}
+
+int mpol_geometry_info(mpol_ctx* c, mpol_geometry_summary* info) {
+  if (!c || !info) return fail(MPOL_EINVAL, "null argument");
+  if (!c->have_geo) return fail(MPOL_ESTATE, "geometry not set");
+  info->n = c->n;
+  info->npairs = c->npairs;
+  for (int d = 0; d < 3; ++d) {
+    info->ngrid[d] = c->ng[d];
+    info->ncell[d] = c->nc[d];
+    info->fft_e[d] = c->plan[d].e;
+  }
+  info->P = c->P;
+  info->tile_rows = tile_cfg(c->ng[0], c->plan[0].e, sizeof(cplx<double>), kLdsTile).rows;
+  return MPOL_OK;
+}
+
+int mpol_real_space_table(double xi, const double* r, int64_t n, double* out) {
+  if (!r || !out || n < 0) return fail(MPOL_EINVAL, "bad table arguments");
+  if (!(xi > 0.0)) return fail(MPOL_EINVAL, "xi must be positive");
+  for (int64_t i = 0; i < n; ++i)
+    if (!(r[i] > 0.0)) return fail(MPOL_EINVAL, "table separations must be positive");
+  real_space_tables(xi, r, n, out);
+  return MPOL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,653 @@
+// Device kernels of the spectral-Ewald dipole operator (gfx950).
+//
+// Grid spectrum layout in HBM (one buffer per context, reused every matvec):
+//   spec[b][z][y][kx],  b = rhs*3 + component,  kx in [0, nx/2] (half spectrum
+//   along x: the grids are real), complex<T>.
+// Particles are kept in "column-bin" order: key = g0z*ny + g0y where g0 is the
+// first grid index of the particle's P-wide stencil on each axis (wrapped).
+#pragma once
+
+#include "mpol_common.h"
+#include "mpol_fft.h"
+
+namespace mpol {
+
+constexpr int kMaxRanges = 160;  // candidate particle ranges per tile (2 per z row)
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+__host__ __device__ inline long long imod(long long a, long long n) {
+  long long r = a % n;
+  return r < 0 ? r + n : r;
+}
+
+// numpy's float modulus (result carries the divisor's sign), used by cell_list.
+__device__ inline double np_mod(double a, double b) {
+  double m = fmod(a, b);
+  if (m != 0.0) {
+    if ((b < 0.0) != (m < 0.0)) m += b;
+  } else {
+    m = copysign(0.0, b);
+  }
+  return m;
+}
+
+__device__ inline double round_half_away(double v) {
+  return v >= 0.0 ? floor(v + 0.5) : -floor(-v + 0.5);
+}
+
+// ---------------------------------------------------------------------------
+// geometry: per-particle stencil data and cell keys
+// ---------------------------------------------------------------------------
+struct GeoParams {
+  int n;
+  int nx, ny, nz;
+  int P, half;
+  double box[3];
+  double h[3];
+  double eta[3];
+  double xi;
+  double norm;   // (2 xi^2/pi)^{3/2} / sqrt(prod eta)
+  int nc[3];     // real-space cells per axis
+};
+
+// Pass 1: bin keys and histograms.
+__global__ void k_geo_keys(GeoParams g, const double* __restrict__ x, int* __restrict__ bin_key,
+                           int* __restrict__ bin_count, int* __restrict__ cell_key, int* __restrict__ cell_count,
+                           double* __restrict__ base, int* __restrict__ g0_unsorted) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.n) return;
+  const int ng[3] = {g.nx, g.ny, g.nz};
+  int g0[3];
+  int c[3];
+  for (int d = 0; d < 3; ++d) {
+    const double xv = x[3 * i + d];
+    const double node0 = round_half_away(xv / g.h[d]);
+    base[3 * i + d] = node0 * g.h[d] - xv;
+    g0[d] = (int)imod((long long)node0 - g.half - 1, ng[d]);
+    g0_unsorted[3 * i + d] = g0[d];
+    const double xw = np_mod(xv, g.box[d]);
+    int cc = (int)floor((xw / g.box[d]) * (double)g.nc[d]);
+    if (cc < 0) cc = 0;
+    if (cc >= g.nc[d]) cc = g.nc[d] - 1;
+    c[d] = cc;
+  }
+  const int key = g0[2] * g.ny + g0[1];
+  bin_key[i] = key;
+  atomicAdd(&bin_count[key], 1);
+  const int ck = c[0] + g.nc[0] * (c[1] + g.nc[1] * c[2]);
+  cell_key[i] = ck;
+  atomicAdd(&cell_count[ck], 1);
+}
+
+// Single-workgroup exclusive scan of n ints (out has n+1 entries; out[n] = total).
+__global__ void k_scan(const int* __restrict__ in, int* __restrict__ out, int n) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int chunk = (n + nt - 1) / nt;
+  const int lo = min(n, t * chunk), hi = min(n, lo + chunk);
+  int s = 0;
+  for (int k = lo; k < hi; ++k) s += in[k];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < nt; off <<= 1) {
+    int v = (t >= off) ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = part[t] - s;
+  for (int k = lo; k < hi; ++k) {
+    out[k] = run;
+    run += in[k];
+  }
+  if (t == nt - 1) out[n] = part[t];
+}
+
+// Pass 2: scatter into bin order and write the separable stencil factors
+//   fac[s][d][o] = exp(-2 xi^2 (base_d + (o - half) h_d)^2 / eta_d)   (x row also * norm)
+template <typename T>
+__device__ inline void write_fac(T* dst, double v) { *dst = (T)v; }
+
+__global__ void k_geo_scatter(GeoParams g, const int* __restrict__ bin_key, const int* __restrict__ bin_start,
+                              int* __restrict__ bin_cursor, const int* __restrict__ cell_key,
+                              const int* __restrict__ cell_start, int* __restrict__ cell_cursor,
+                              const double* __restrict__ base, const int* __restrict__ g0_unsorted,
+                              int* __restrict__ perm, int* __restrict__ g0s, double* __restrict__ fac64,
+                              float* __restrict__ fac32, int* __restrict__ cell_members) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.n) return;
+  const int key = bin_key[i];
+  const int s = bin_start[key] + atomicAdd(&bin_cursor[key], 1);
+  perm[s] = i;
+  const int P = g.P;
+  const double two_xi2 = 2.0 * g.xi * g.xi;
+  for (int d = 0; d < 3; ++d) {
+    g0s[3 * s + d] = g0_unsorted[3 * i + d];
+    const double b = base[3 * i + d];
+    for (int o = 0; o < P; ++o) {
+      const double r = b + (double)(o - g.half) * g.h[d];
+      double w = exp(-two_xi2 * ((r * r) / g.eta[d]));
+      if (d == 0) w *= g.norm;
+      fac64[((size_t)s * 3 + d) * P + o] = w;
+      fac32[((size_t)s * 3 + d) * P + o] = (float)w;
+    }
+  }
+  const int ck = cell_key[i];
+  cell_members[cell_start[ck] + atomicAdd(&cell_cursor[ck], 1)] = i;
+}
+
+// ---------------------------------------------------------------------------
+// real-space pair list (CSR by particle, owner computes)
+// ---------------------------------------------------------------------------
+struct PairParams {
+  int n;
+  double box[3];
+  double rc;
+  int nc[3];
+  int ntab;
+};
+
+__device__ inline int table_lower_bound(const double* __restrict__ rv, int ntab, double d) {
+  int lo = 0, hi = ntab;  // first index with rv[idx] >= d
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (rv[mid] < d) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// FILL=false: count pairs per particle.  FILL=true: write j, A, B-A, rhat.
+template <bool FILL>
+__global__ void k_pairs(PairParams p, const double* __restrict__ x, const int* __restrict__ cell_key,
+                        const int* __restrict__ cell_start, const int* __restrict__ cell_members,
+                        const double* __restrict__ rv, const double* __restrict__ perp,
+                        const double* __restrict__ para, int* __restrict__ count, const int* __restrict__ off,
+                        int* __restrict__ pj, double* __restrict__ pA, double* __restrict__ pBA,
+                        double* __restrict__ prh, float* __restrict__ pA32, float* __restrict__ pBA32,
+                        float* __restrict__ prh32, int* __restrict__ err) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p.n) return;
+  const double xi0 = x[3 * i], xi1 = x[3 * i + 1], xi2 = x[3 * i + 2];
+  const int ck = cell_key[i];
+  const int cx = ck % p.nc[0], cy = (ck / p.nc[0]) % p.nc[1], cz = ck / (p.nc[0] * p.nc[1]);
+  int cnt = 0;
+  int w = FILL ? off[i] : 0;
+  for (int dz = -1; dz <= 1; ++dz) {
+    const int nz = (int)imod(cz + dz, p.nc[2]);
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int ny = (int)imod(cy + dy, p.nc[1]);
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int nx = (int)imod(cx + dx, p.nc[0]);
+        const int nb = nx + p.nc[0] * (ny + p.nc[1] * nz);
+        for (int q = cell_start[nb]; q < cell_start[nb + 1]; ++q) {
+          const int j = cell_members[q];
+          if (j == i) continue;
+          double r0 = xi0 - x[3 * j], r1 = xi1 - x[3 * j + 1], r2 = xi2 - x[3 * j + 2];
+          r0 = r0 - p.box[0] * trunc(2.0 * r0 / p.box[0]);
+          r1 = r1 - p.box[1] * trunc(2.0 * r1 / p.box[1]);
+          r2 = r2 - p.box[2] * trunc(2.0 * r2 / p.box[2]);
+          const double d = sqrt((r0 * r0 + r1 * r1) + r2 * r2);
+          if (!(d < p.rc)) continue;
+          if (!FILL) {
+            ++cnt;
+            if (d <= rv[0] || d > rv[p.ntab - 1]) atomicExch(err, 1);
+          } else {
+            int idx = table_lower_bound(rv, p.ntab, d);
+            if (idx <= 0) idx = 1;
+            if (idx >= p.ntab) idx = p.ntab - 1;
+            const double lo = rv[idx - 1], hi = rv[idx];
+            const double t = (d - lo) / (hi - lo);
+            const double A = perp[idx - 1] * (1.0 - t) + perp[idx] * t;
+            const double B = para[idx - 1] * (1.0 - t) + para[idx] * t;
+            pj[w] = j;
+            pA[w] = A;
+            pBA[w] = B - A;
+            prh[3 * (size_t)w] = r0 / d;
+            prh[3 * (size_t)w + 1] = r1 / d;
+            prh[3 * (size_t)w + 2] = r2 / d;
+            pA32[w] = (float)A;
+            pBA32[w] = (float)(B - A);
+            prh32[3 * (size_t)w] = (float)(r0 / d);
+            prh32[3 * (size_t)w + 1] = (float)(r1 / d);
+            prh32[3 * (size_t)w + 2] = (float)(r2 / d);
+            ++w;
+          }
+        }
+      }
+    }
+  }
+  if (!FILL) count[i] = cnt;
+}
+
+// ---------------------------------------------------------------------------
+// per-matvec kernels
+// ---------------------------------------------------------------------------
+// Sum of the z components per right-hand side (slab term), deterministic.
+__global__ void k_sum_z(const double* __restrict__ m, int n, double* __restrict__ out) {
+  __shared__ double red[256];
+  const int rhs = blockIdx.x;
+  const double* mr = m + (size_t)rhs * n * 3;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += mr[3 * i + 2];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[rhs] = red[0];
+}
+
+// Real-space sum + dielectric/self diagonal + slab term.  Writes out (the
+// wave-space contraction is accumulated on top afterwards).
+template <typename T>
+__global__ void k_real_space(int n, int nrhs, int nsets, const double* __restrict__ m, double* __restrict__ out,
+                             const double* __restrict__ diag, const int* __restrict__ off,
+                             const int* __restrict__ pj, const T* __restrict__ pA, const T* __restrict__ pBA,
+                             const T* __restrict__ prh, const double* __restrict__ zsum, double slab_over_v,
+                             int write_diag_only_parts, double* __restrict__ hr_out, double* __restrict__ slab_out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int rhs = blockIdx.y;
+  if (i >= n || rhs >= nrhs) return;
+  const double* mr = m + (size_t)rhs * n * 3;
+  const int set = (nsets == 1) ? 0 : rhs;
+  const double dg = diag[(size_t)set * n + i];
+  T a0 = 0, a1 = 0, a2 = 0;
+  for (int q = off[i]; q < off[i + 1]; ++q) {
+    const int j = pj[q];
+    const T m0 = (T)mr[3 * j], m1 = (T)mr[3 * j + 1], m2 = (T)mr[3 * j + 2];
+    const T r0 = prh[3 * (size_t)q], r1 = prh[3 * (size_t)q + 1], r2 = prh[3 * (size_t)q + 2];
+    const T A = pA[q];
+    const T c = pBA[q] * (m0 * r0 + m1 * r1 + m2 * r2);
+    a0 += A * m0 + c * r0;
+    a1 += A * m1 + c * r1;
+    a2 += A * m2 + c * r2;
+  }
+  const double h0 = dg * mr[3 * i] + (double)a0;
+  const double h1 = dg * mr[3 * i + 1] + (double)a1;
+  const double h2 = dg * mr[3 * i + 2] + (double)a2;
+  const double sz = slab_over_v * zsum[rhs];
+  double* o = out + ((size_t)rhs * n + i) * 3;
+  if (write_diag_only_parts) {
+    // component mode: out <- 0 (wave part accumulates), hr_out <- Hr, slab_out <- H_slab
+    o[0] = 0.0; o[1] = 0.0; o[2] = 0.0;
+    double* hr = hr_out + ((size_t)rhs * n + i) * 3;
+    hr[0] = h0; hr[1] = h1; hr[2] = h2;
+    double* sl = slab_out + ((size_t)rhs * n + i) * 3;
+    sl[0] = 0.0; sl[1] = 0.0; sl[2] = -sz;
+  } else {
+    o[0] = h0;
+    o[1] = h1;
+    o[2] = h2 + sz;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// spread + forward x-FFT (fused), and inverse x-FFT + contract (fused)
+// ---------------------------------------------------------------------------
+struct TileArgs {
+  int nx, ny, nz, kx;
+  int ty, tz;        // grid rows (y, z) per tile
+  int rows;          // ty*tz
+  int ncl;           // complex LDS lines = ceil(3*rows/2)
+  int lstride;       // complex elements per LDS line (padded)
+  int gs;            // lanes per particle group (power of two >= rows, <= 64)
+  int P;
+  int n;             // particles
+  int nrhs;
+  const int* perm;
+  const int* g0;
+  const void* fac;
+  const int* bin_start;
+  const void* tw;    // nx twiddles
+  FftPlan plan;
+  double hvol;       // prod(h), trapezoid factor of the contraction
+};
+
+// Build the candidate particle ranges of a tile into shared arrays.
+__device__ inline void tile_ranges(const TileArgs& a, int ty0, int tz0, int* s_start, int* s_cum, int* s_nr) {
+  int ylo = ty0 - a.P + 1, ycount = a.ty + a.P - 1;
+  if (ycount >= a.ny) { ylo = 0; ycount = a.ny; }
+  int zlo = tz0 - a.P + 1, zcount = a.tz + a.P - 1;
+  if (zcount >= a.nz) { zlo = 0; zcount = a.nz; }
+  const int y0 = (int)imod(ylo, a.ny);
+  const int len1 = min(ycount, a.ny - y0);
+  const int len2 = ycount - len1;
+  // two slots per z row; lengths first
+  for (int zi = threadIdx.x; zi < zcount; zi += blockDim.x) {
+    const int gz = (int)imod(zlo + zi, a.nz);
+    const int k0 = gz * a.ny + y0;
+    s_start[2 * zi] = a.bin_start[k0];
+    s_cum[2 * zi] = a.bin_start[k0 + len1] - a.bin_start[k0];
+    if (len2 > 0) {
+      const int k1 = gz * a.ny;
+      s_start[2 * zi + 1] = a.bin_start[k1];
+      s_cum[2 * zi + 1] = a.bin_start[k1 + len2] - a.bin_start[k1];
+    } else {
+      s_start[2 * zi + 1] = 0;
+      s_cum[2 * zi + 1] = 0;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nr = 2 * zcount;
+    int run = 0;
+    for (int r = 0; r < nr; ++r) {
+      const int c = s_cum[r];
+      s_cum[r] = run;
+      run += c;
+    }
+    s_cum[nr] = run;
+    *s_nr = nr;
+  }
+  __syncthreads();
+}
+
+// Sum of the stencil factors of one axis that land on grid row `g`
+// (more than one term only when the stencil is wider than the grid).
+template <typename T>
+__device__ inline T axis_weight(const T* f, int g, int g0, int n, int P) {
+  int o = g - g0;
+  if (o < 0) o += n;
+  T w = 0;
+  for (; o < P; o += n) w += f[o];
+  return w;
+}
+
+template <typename T, int E>
+__global__ void __launch_bounds__(256) k_spread_xfft(TileArgs a, const double* __restrict__ m,
+                                                     cplx<T>* __restrict__ spec) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ int s_start[kMaxRanges], s_cum[kMaxRanges + 1], s_nr;
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* buf = tw + a.nx;
+  cplx<T>* scratch = buf + a.ncl * a.lstride;
+  const int ty0 = blockIdx.x * a.ty, tz0 = blockIdx.y * a.tz, rhs = blockIdx.z;
+  const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
+  for (int k = threadIdx.x; k < a.nx; k += blockDim.x) tw[k] = twg[k];
+  for (int k = threadIdx.x; k < a.ncl * a.lstride; k += blockDim.x) buf[k] = cmk<T>(0, 0);
+  tile_ranges(a, ty0, tz0, s_start, s_cum, &s_nr);
+
+  const T* fac = reinterpret_cast<const T*>(a.fac);
+  const int total = s_cum[s_nr];
+  const int grp = threadIdx.x / a.gs, lane = threadIdx.x % a.gs, ngrp = blockDim.x / a.gs;
+  const int P = a.P;
+  const double* mr = m + (size_t)rhs * 3 * (size_t)a.n;
+  int ri = 0;
+  for (int q = grp; q < total; q += ngrp) {
+    while (q >= s_cum[ri + 1]) ++ri;
+    const int s = s_start[ri] + (q - s_cum[ri]);
+    const int gx0 = a.g0[3 * s], gy0 = a.g0[3 * s + 1], gz0 = a.g0[3 * s + 2];
+    const int orig = a.perm[s];
+    const T* fs = fac + (size_t)s * 3 * P;
+    for (int row = lane; row < a.rows; row += a.gs) {
+      const int ly = row % a.ty, lz = row / a.ty;
+      const int y = ty0 + ly, z = tz0 + lz;
+      if (y >= a.ny || z >= a.nz) continue;
+      const T wy = axis_weight(fs + P, y, gy0, a.ny, P);
+      const T wz = axis_weight(fs + 2 * P, z, gz0, a.nz, P);
+      const T wyz = wy * wz;
+      if (wyz == (T)0) continue;
+      const double* mv = mr + 3 * (size_t)orig;
+      const T m0 = (T)mv[0], m1 = (T)mv[1], m2 = (T)mv[2];
+      const int l0 = row, l1 = a.rows + row, l2 = 2 * a.rows + row;
+      T* p0 = reinterpret_cast<T*>(buf + (l0 >> 1) * a.lstride) + (l0 & 1);
+      T* p1 = reinterpret_cast<T*>(buf + (l1 >> 1) * a.lstride) + (l1 & 1);
+      T* p2 = reinterpret_cast<T*>(buf + (l2 >> 1) * a.lstride) + (l2 & 1);
+      int x = gx0;
+      for (int o = 0; o < P; ++o) {
+        const T w = wyz * fs[o];
+        const int px = 2 * pidx(x);
+        atomicAdd(p0 + px, w * m0);
+        atomicAdd(p1 + px, w * m1);
+        atomicAdd(p2 + px, w * m2);
+        if (++x == a.nx) x = 0;
+      }
+    }
+  }
+  __syncthreads();
+  fft_lines<T, E>(buf, a.lstride, a.ncl, a.plan, tw, false, scratch);
+  // split packed pairs into the two half spectra and store
+  const int nreal = 3 * a.rows;
+  const int tot = nreal * a.kx;
+  for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+    const int l = idx / a.kx, k = idx - l * a.kx;
+    const int c = l / a.rows, row = l - c * a.rows;
+    const int y = ty0 + row % a.ty, z = tz0 + row / a.ty;
+    if (y >= a.ny || z >= a.nz) continue;
+    const cplx<T>* line = buf + (l >> 1) * a.lstride;
+    const cplx<T> Z = line[pidx(k)];
+    const cplx<T> Zc = cconj(line[pidx(k == 0 ? 0 : a.nx - k)]);
+    cplx<T> v;
+    if ((l & 1) == 0) v = cmk<T>((Z.x + Zc.x) * (T)0.5, (Z.y + Zc.y) * (T)0.5);
+    else v = cmk<T>((Z.y - Zc.y) * (T)0.5, (Zc.x - Z.x) * (T)0.5);
+    spec[(((size_t)(rhs * 3 + c) * a.nz + z) * a.ny + y) * a.kx + k] = v;
+  }
+}
+
+template <typename T, int E>
+__global__ void __launch_bounds__(256) k_xifft_contract(TileArgs a, const cplx<T>* __restrict__ spec,
+                                                        double* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ int s_start[kMaxRanges], s_cum[kMaxRanges + 1], s_nr;
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* buf = tw + a.nx;
+  cplx<T>* scratch = buf + a.ncl * a.lstride;
+  const int ty0 = blockIdx.x * a.ty, tz0 = blockIdx.y * a.tz, rhs = blockIdx.z;
+  const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
+  for (int k = threadIdx.x; k < a.nx; k += blockDim.x) tw[k] = twg[k];
+  // pack two half spectra per complex line: Z = A + iB (Hermitian extension)
+  const int nreal = 3 * a.rows;
+  const int tot = a.ncl * a.kx;
+  for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+    const int q = idx / a.kx, k = idx - q * a.kx;
+    cplx<T> A = cmk<T>(0, 0), B = cmk<T>(0, 0);
+    const int la = 2 * q, lb = 2 * q + 1;
+    {
+      const int c = la / a.rows, row = la - c * a.rows;
+      const int y = ty0 + row % a.ty, z = tz0 + row / a.ty;
+      if (y < a.ny && z < a.nz) A = spec[(((size_t)(rhs * 3 + c) * a.nz + z) * a.ny + y) * a.kx + k];
+    }
+    if (lb < nreal) {
+      const int c = lb / a.rows, row = lb - c * a.rows;
+      const int y = ty0 + row % a.ty, z = tz0 + row / a.ty;
+      if (y < a.ny && z < a.nz) B = spec[(((size_t)(rhs * 3 + c) * a.nz + z) * a.ny + y) * a.kx + k];
+    }
+    cplx<T>* line = buf + q * a.lstride;
+    const bool selfmirror = (k == 0) || (2 * k == a.nx);
+    if (selfmirror) {
+      line[pidx(k)] = cmk<T>(A.x, B.x);
+    } else {
+      line[pidx(k)] = cmk<T>(A.x - B.y, A.y + B.x);
+      line[pidx(a.nx - k)] = cmk<T>(A.x + B.y, B.x - A.y);
+    }
+  }
+  tile_ranges(a, ty0, tz0, s_start, s_cum, &s_nr);  // contains __syncthreads
+  fft_lines<T, E>(buf, a.lstride, a.ncl, a.plan, tw, true, scratch);
+
+  const T* fac = reinterpret_cast<const T*>(a.fac);
+  const int total = s_cum[s_nr];
+  const int grp = threadIdx.x / a.gs, lane = threadIdx.x % a.gs, ngrp = blockDim.x / a.gs;
+  const int P = a.P;
+  const int ngroups_iter = (total + ngrp - 1) / ngrp;
+  int ri = 0;
+  for (int it = 0; it < ngroups_iter; ++it) {
+    const int q = grp + it * ngrp;
+    T acc0 = 0, acc1 = 0, acc2 = 0;
+    int orig = -1;
+    if (q < total) {
+      while (q >= s_cum[ri + 1]) ++ri;
+      const int s = s_start[ri] + (q - s_cum[ri]);
+      const int gx0 = a.g0[3 * s], gy0 = a.g0[3 * s + 1], gz0 = a.g0[3 * s + 2];
+      orig = a.perm[s];
+      const T* fs = fac + (size_t)s * 3 * P;
+      for (int row = lane; row < a.rows; row += a.gs) {
+        const int ly = row % a.ty, lz = row / a.ty;
+        const int y = ty0 + ly, z = tz0 + lz;
+        if (y >= a.ny || z >= a.nz) continue;
+        const T wy = axis_weight(fs + P, y, gy0, a.ny, P);
+        const T wz = axis_weight(fs + 2 * P, z, gz0, a.nz, P);
+        const T wyz = wy * wz;
+        if (wyz == (T)0) continue;
+        const int l0 = row, l1 = a.rows + row, l2 = 2 * a.rows + row;
+        const T* p0 = reinterpret_cast<const T*>(buf + (l0 >> 1) * a.lstride) + (l0 & 1);
+        const T* p1 = reinterpret_cast<const T*>(buf + (l1 >> 1) * a.lstride) + (l1 & 1);
+        const T* p2 = reinterpret_cast<const T*>(buf + (l2 >> 1) * a.lstride) + (l2 & 1);
+        T s0 = 0, s1 = 0, s2 = 0;
+        int x = gx0;
+        for (int o = 0; o < P; ++o) {
+          const T w = fs[o];
+          const int px = 2 * pidx(x);
+          s0 += w * p0[px];
+          s1 += w * p1[px];
+          s2 += w * p2[px];
+          if (++x == a.nx) x = 0;
+        }
+        acc0 += wyz * s0;
+        acc1 += wyz * s1;
+        acc2 += wyz * s2;
+      }
+    }
+    // reduce over the lanes of the group (groups never straddle a wavefront)
+    for (int o = a.gs / 2; o > 0; o >>= 1) {
+      acc0 += __shfl_xor(acc0, o, a.gs);
+      acc1 += __shfl_xor(acc1, o, a.gs);
+      acc2 += __shfl_xor(acc2, o, a.gs);
+    }
+    if (lane == 0 && orig >= 0) {
+      double* po = out + ((size_t)rhs * a.n + (size_t)orig) * 3;
+      atomicAdd(po, (double)acc0 * a.hvol);
+      atomicAdd(po + 1, (double)acc1 * a.hvol);
+      atomicAdd(po + 2, (double)acc2 * a.hvol);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// y-axis FFT (both directions) and the fused z-axis FFT + k-space scale
+// ---------------------------------------------------------------------------
+struct ColArgs {
+  int nx, ny, nz, kx;
+  int tx;           // kx columns per workgroup
+  int lstride;
+  FftPlan plan;
+  const void* tw;   // twiddles for the transformed axis
+};
+
+template <typename T, int E>
+__global__ void __launch_bounds__(256) k_fft_y(ColArgs a, cplx<T>* __restrict__ spec, int inverse) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* buf = tw + a.ny;
+  cplx<T>* scratch = buf + a.tx * a.lstride;
+  const int kx0 = blockIdx.x * a.tx, z = blockIdx.y, b = blockIdx.z;
+  const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
+  for (int k = threadIdx.x; k < a.ny; k += blockDim.x) tw[k] = twg[k];
+  cplx<T>* base = spec + ((size_t)b * a.nz + z) * a.ny * a.kx;
+  const int tot = a.tx * a.ny;
+  for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+    const int c = idx % a.tx, y = idx / a.tx;
+    const int kx = kx0 + c;
+    buf[c * a.lstride + pidx(y)] = (kx < a.kx) ? base[(size_t)y * a.kx + kx] : cmk<T>(0, 0);
+  }
+  __syncthreads();
+  fft_lines<T, E>(buf, a.lstride, a.tx, a.plan, tw, inverse != 0, scratch);
+  for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+    const int c = idx % a.tx, y = idx / a.tx;
+    const int kx = kx0 + c;
+    if (kx < a.kx) base[(size_t)y * a.kx + kx] = buf[c * a.lstride + pidx(y)];
+  }
+}
+
+struct ScaleParams {
+  double twopi_over_l[3];
+  double one_minus_eta[3];
+  double inv_4xi2;
+  double inv_g;     // 1/(nx ny nz): inverse-FFT normalisation folded into the scale
+};
+
+// c(k) = 9 pi/(2|k|) J_{3/2}(|k|)^2 exp(-sum (1-eta_d) k_d^2 / (4 xi^2)) / |k|^2
+__device__ inline double kspace_coef(const ScaleParams& s, double k0, double k1, double k2) {
+  const double kk = (k0 * k0 + k1 * k1) + k2 * k2;
+  if (kk == 0.0) return 0.0;
+  const double kn = sqrt(kk);
+  const double j = sqrt(2.0 / (kPi * kn)) * (sin(kn) / kn - cos(kn));
+  const double ek = s.one_minus_eta[0] * k0 * k0 + s.one_minus_eta[1] * k1 * k1 + s.one_minus_eta[2] * k2 * k2;
+  return 9.0 * kPi / (2.0 * kn) * (j * j) * exp(-ek * s.inv_4xi2) / kk;
+}
+
+template <typename T, int E>
+__global__ void __launch_bounds__(256) k_fft_z_scale(ColArgs a, ScaleParams sp, cplx<T>* __restrict__ spec) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* buf = tw + a.nz;
+  cplx<T>* scratch = buf + 3 * a.tx * a.lstride;
+  const int kx0 = blockIdx.x * a.tx, y = blockIdx.y, rhs = blockIdx.z;
+  const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
+  for (int k = threadIdx.x; k < a.nz; k += blockDim.x) tw[k] = twg[k];
+  const size_t plane = (size_t)a.ny * a.kx;
+  const int tot = 3 * a.tx * a.nz;
+  for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+    const int c = idx / (a.tx * a.nz);
+    const int rem = idx - c * a.tx * a.nz;
+    const int z = rem / a.tx, cc = rem - z * a.tx;
+    const int kx = kx0 + cc;
+    cplx<T> v = cmk<T>(0, 0);
+    if (kx < a.kx) v = spec[((size_t)(rhs * 3 + c) * a.nz + z) * plane + (size_t)y * a.kx + kx];
+    buf[(c * a.tx + cc) * a.lstride + pidx(z)] = v;
+  }
+  __syncthreads();
+  fft_lines<T, E>(buf, a.lstride, 3 * a.tx, a.plan, tw, false, scratch);
+  // k-space scale: G = c(k) M F with M = khat khat^T, cross terms with an
+  // even-grid Nyquist axis removed (the Hermitian part of the reference's
+  // fftshift-ordered operator, so real grids stay real)
+  const int my = fft_freq(y, a.ny);
+  const bool nyq_y = (a.ny % 2 == 0) && (2 * my == -a.ny);
+  for (int idx = threadIdx.x; idx < a.tx * a.nz; idx += blockDim.x) {
+    const int cc = idx % a.tx, z = idx / a.tx;
+    const int kx = kx0 + cc;
+    if (kx >= a.kx) continue;
+    const int mx = fft_freq(kx, a.nx), mz = fft_freq(z, a.nz);
+    const bool nyq_x = (a.nx % 2 == 0) && (2 * mx == -a.nx);
+    const bool nyq_z = (a.nz % 2 == 0) && (2 * mz == -a.nz);
+    const double k0 = mx * sp.twopi_over_l[0], k1 = my * sp.twopi_over_l[1], k2 = mz * sp.twopi_over_l[2];
+    const double cf = kspace_coef(sp, k0, k1, k2) * sp.inv_g;
+    cplx<T>* f0 = buf + (0 * a.tx + cc) * a.lstride + pidx(z);
+    cplx<T>* f1 = buf + (1 * a.tx + cc) * a.lstride + pidx(z);
+    cplx<T>* f2 = buf + (2 * a.tx + cc) * a.lstride + pidx(z);
+    if (cf == 0.0) {
+      *f0 = cmk<T>(0, 0); *f1 = cmk<T>(0, 0); *f2 = cmk<T>(0, 0);
+      continue;
+    }
+    const double kn = sqrt((k0 * k0 + k1 * k1) + k2 * k2);
+    const T u0 = (T)(k0 / kn), u1 = (T)(k1 / kn), u2 = (T)(k2 / kn);
+    const T c = (T)cf;
+    const T x01 = (nyq_x || nyq_y) ? (T)0 : u0 * u1;
+    const T x02 = (nyq_x || nyq_z) ? (T)0 : u0 * u2;
+    const T x12 = (nyq_y || nyq_z) ? (T)0 : u1 * u2;
+    const cplx<T> F0 = *f0, F1 = *f1, F2 = *f2;
+    const T d00 = u0 * u0, d11 = u1 * u1, d22 = u2 * u2;
+    *f0 = cmk<T>(c * (d00 * F0.x + x01 * F1.x + x02 * F2.x), c * (d00 * F0.y + x01 * F1.y + x02 * F2.y));
+    *f1 = cmk<T>(c * (x01 * F0.x + d11 * F1.x + x12 * F2.x), c * (x01 * F0.y + d11 * F1.y + x12 * F2.y));
+    *f2 = cmk<T>(c * (x02 * F0.x + x12 * F1.x + d22 * F2.x), c * (x02 * F0.y + x12 * F1.y + d22 * F2.y));
+  }
+  __syncthreads();
+  fft_lines<T, E>(buf, a.lstride, 3 * a.tx, a.plan, tw, true, scratch);
+  for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+    const int c = idx / (a.tx * a.nz);
+    const int rem = idx - c * a.tx * a.nz;
+    const int z = rem / a.tx, cc = rem - z * a.tx;
+    const int kx = kx0 + cc;
+    if (kx < a.kx) spec[((size_t)(rhs * 3 + c) * a.nz + z) * plane + (size_t)y * a.kx + kx] =
+        buf[(c * a.tx + cc) * a.lstride + pidx(z)];
+  }
+}
+
+}  // namespace mpol

@@ -1,0 +1,10 @@
+"""MI355X spectral-Ewald mutual-polarization library (HIP kernels behind a C ABI).
+
+Submodules:
+  _lib     -- ctypes binding of libmpol.so (include/mpol.h)
+  context  -- Context: one geometry on one GPU (tables, precompute, matvec, GMRES)
+  build    -- hipcc build of libmpol.so for gfx950
+"""
+from .context import Context, SolveResult, default_tables, real_space_table, require_device, FP32, FP64
+
+__all__ = ["Context", "SolveResult", "default_tables", "real_space_table", "require_device", "FP32", "FP64"]

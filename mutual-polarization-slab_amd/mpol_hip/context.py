@@ -1,0 +1,188 @@
+"""Python handle over one libmpol context (device memory through torch-ROCm).
+
+torch provides device allocation and the stream; every computation of the
+operator and the solve runs in the HIP kernels of libmpol.so.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+
+FP64 = _lib.MPOL_FP64
+FP32 = _lib.MPOL_FP32
+
+
+def require_device() -> torch.device:
+    """The GPU the library runs on; raises if there is none (no CPU fallback)."""
+    if not torch.cuda.is_available():
+        raise RuntimeError("mutual-polarization operator needs a ROCm GPU (torch.cuda is unavailable); "
+                           "there is no CPU fallback")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def real_space_table(r, xi):
+    """The six real-space coupling tables (self term first) from the library's
+    FP64 host routine; same order as the reference's real_space_table."""
+    r = np.ascontiguousarray(np.asarray(r, dtype=np.float64).reshape(-1))
+    out = np.zeros((6, r.size + 1), dtype=np.float64)
+    lib = _lib.load()
+    _lib.check(lib.mpol_real_space_table(float(xi), _lib.dptr(r), r.size, _lib.dptr(out)))
+    return tuple(out[k] for k in range(6))
+
+
+_TABLES: dict = {}
+
+
+def default_tables(xi):
+    """field_dip_1, field_dip_2 and r on [0, 0.001:0.001:10]
+    ("CPU - Ewald test/capacitance.py":469-477), cached per xi."""
+    key = float(xi)
+    if key not in _TABLES:
+        r = np.arange(0.001, 10.0 + 1e-12, 0.001, dtype=np.float64)
+        tabs = real_space_table(r, xi)
+        _TABLES[key] = (tabs[2], tabs[3], np.concatenate(([0.0], r)))
+    return _TABLES[key]
+
+
+@dataclass
+class SolveResult:
+    x: torch.Tensor
+    matvecs_fp64: int
+    matvecs_fp32: int
+    converged: int
+    refinements: int
+    max_relres: float
+
+    @property
+    def flag(self) -> int:
+        return 0 if self.converged == self.x.shape[0] else 1
+
+
+class Context:
+    """One geometry (positions, grid, tables) on one GPU."""
+
+    def __init__(self, device: torch.device | None = None):
+        self.device = device if device is not None else require_device()
+        self._lib = _lib.load()
+        self._ctx = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            stream = torch.cuda.current_stream(self.device).cuda_stream
+            _lib.check(self._lib.mpol_create(ctypes.byref(self._ctx), self.device.index,
+                                             ctypes.c_void_p(stream)))
+        self.n = 0
+        self.nsets = 0
+        self._keep = []
+
+    # -- lifetime -------------------------------------------------------
+    def close(self):
+        if self._ctx:
+            self._lib.mpol_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _sync_stream(self):
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        _lib.check(self._lib.mpol_set_stream(self._ctx, ctypes.c_void_p(stream)))
+
+    def _dev(self, a, dtype=torch.float64):
+        if isinstance(a, torch.Tensor):
+            t = a.to(device=self.device, dtype=dtype)
+        else:
+            t = torch.as_tensor(np.asarray(a), dtype=dtype, device=self.device)
+        return t.contiguous()
+
+    # -- setup ----------------------------------------------------------
+    def set_tables(self, perp, para, rvals):
+        perp = np.ascontiguousarray(np.real(np.asarray(perp)), dtype=np.float64).reshape(-1)
+        para = np.ascontiguousarray(np.real(np.asarray(para)), dtype=np.float64).reshape(-1)
+        rvals = np.ascontiguousarray(np.asarray(rvals), dtype=np.float64).reshape(-1)
+        if not (perp.size == para.size == rvals.size):
+            raise ValueError("perp, para and rvals must have the same length")
+        self._sync_stream()
+        _lib.check(self._lib.mpol_set_tables(self._ctx, _lib.dptr(perp), _lib.dptr(para), _lib.dptr(rvals),
+                                             rvals.size))
+
+    def set_geometry(self, x, box, ngrid, P, xi, eta, rc):
+        xt = self._dev(x).reshape(-1, 3)
+        box = np.ascontiguousarray(np.asarray(box, dtype=np.float64).reshape(3))
+        ng = np.ascontiguousarray(np.asarray(ngrid, dtype=np.int64).reshape(3).astype(np.int32))
+        eta = np.ascontiguousarray(np.asarray(eta, dtype=np.float64).reshape(3))
+        self._sync_stream()
+        _lib.check(self._lib.mpol_set_geometry(self._ctx, ctypes.c_void_p(xt.data_ptr()), xt.shape[0],
+                                               _lib.dptr(box), _lib.i32ptr(ng), int(P), float(xi),
+                                               _lib.dptr(eta), float(rc)))
+        self.n = xt.shape[0]
+        self.nsets = 0
+
+    def set_permittivity(self, eps):
+        et = self._dev(eps)
+        if et.dim() == 1:
+            et = et.reshape(1, -1)
+        if et.shape[1] != self.n:
+            raise ValueError("permittivity must have N entries per set")
+        self._sync_stream()
+        _lib.check(self._lib.mpol_set_permittivity(self._ctx, ctypes.c_void_p(et.data_ptr()), et.shape[0]))
+        self.nsets = et.shape[0]
+        self._keep = [et]
+
+    def geometry_info(self) -> dict:
+        s = _lib.GeometrySummary()
+        _lib.check(self._lib.mpol_geometry_info(self._ctx, ctypes.byref(s)))
+        return {"n": s.n, "npairs": s.npairs, "ngrid": tuple(s.ngrid), "P": s.P, "ncell": tuple(s.ncell),
+                "tile_rows": s.tile_rows, "fft_e": tuple(s.fft_e)}
+
+    # -- operator -------------------------------------------------------
+    def _as_rhs(self, m):
+        mt = self._dev(m)
+        if mt.dim() == 2:
+            mt = mt.reshape(1, -1, 3)
+        if mt.shape[1] != self.n or mt.shape[2] != 3:
+            raise ValueError(f"expected dipoles of shape (nrhs, {self.n}, 3)")
+        return mt.contiguous()
+
+    def matvec(self, m, precision=FP64, slab_factor=1.0, out=None):
+        mt = self._as_rhs(m)
+        if out is None:
+            out = torch.empty_like(mt)
+        self._sync_stream()
+        _lib.check(self._lib.mpol_matvec(self._ctx, ctypes.c_void_p(mt.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                         mt.shape[0], int(precision), float(slab_factor)))
+        return out
+
+    def components(self, m, precision=FP64, slab_factor=1.0):
+        mt = self._as_rhs(m)
+        hk = torch.empty_like(mt)
+        hr = torch.empty_like(mt)
+        hs = torch.empty_like(mt)
+        self._sync_stream()
+        _lib.check(self._lib.mpol_matvec_components(
+            self._ctx, ctypes.c_void_p(mt.data_ptr()), ctypes.c_void_p(hk.data_ptr()),
+            ctypes.c_void_p(hr.data_ptr()), ctypes.c_void_p(hs.data_ptr()), mt.shape[0], int(precision),
+            float(slab_factor)))
+        return hk, hr, hs
+
+    def solve(self, b, x0, tol, restart=10, maxit=100, mixed=False, max_refine=8, slab_factor=1.0):
+        bt = self._as_rhs(b)
+        x0t = self._as_rhs(x0)
+        if x0t.shape != bt.shape:
+            raise ValueError("b and x0 must have the same shape")
+        x = torch.empty_like(bt)
+        opts = _lib.SolveOpts(float(tol), int(restart), int(maxit), 1 if mixed else 0, int(max_refine),
+                              float(slab_factor))
+        info = _lib.SolveInfo()
+        self._sync_stream()
+        _lib.check(self._lib.mpol_solve(self._ctx, ctypes.c_void_p(bt.data_ptr()), ctypes.c_void_p(x0t.data_ptr()),
+                                        ctypes.c_void_p(x.data_ptr()), bt.shape[0], ctypes.byref(opts),
+                                        ctypes.byref(info)))
+        return SolveResult(x, info.matvecs_fp64, info.matvecs_fp32, info.converged, info.refinements,
+                           info.max_relres)

@@ -1,0 +1,137 @@
+"""Parity of the HIP operator (libmpol.so through the C ABI) with the FP64
+oracle and the reference's golden vectors.
+
+Tolerances: FP64 operator 1e-12 relative to max |field| (the target is 1e-10);
+FP32 operator 2e-6 relative (mixed-mode target 1e-6 on the solution).
+"""
+import glob
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, golden
+from helpers import oracle_geometry, relerr, rsa, tables
+from oracle import ewald_oracle as eo
+
+pytestmark = pytest.mark.gpu
+
+OPERATOR_FILES = sorted(glob.glob(os.path.join(GOLDEN, "operator_*.npz")))
+
+
+def ctx_for(d):
+    from mpol_hip import Context
+    xi = float(d["xi"])
+    perp, para, rv = tables(xi)
+    c = Context()
+    c.set_tables(perp, para, rv)
+    h = d["box"] / d["ngrid"]
+    eta = int(d["P"]) * (h * xi) ** 2 / np.pi
+    c.set_geometry(d["x"], d["box"], d["ngrid"], int(d["P"]), xi, eta, float(d["rc"]))
+    c.set_permittivity(d["eps"])
+    return c
+
+
+@pytest.mark.parametrize("path", OPERATOR_FILES, ids=os.path.basename)
+def test_operator_fp64_matches_reference(gpu, path):
+    d = np.load(path)
+    print("case", os.path.basename(path), flush=True)
+    c = ctx_for(d)
+    info = c.geometry_info()
+    assert info["npairs"] == int(d["npairs"])
+    m = d["m"]
+    for slab in (0, 1):
+        hk, hr, hs = c.components(np.stack((m.real, m.imag)), slab_factor=float(slab))
+        hk = hk.cpu().numpy()
+        hr = hr.cpu().numpy()
+        hs = hs.cpu().numpy()
+        Hk = hk[0] + 1j * hk[1]
+        Hr = hr[0] + 1j * hr[1]
+        Hs = hs[0] + 1j * hs[1]
+        # the reference's wave part for complex input equals ours to the size of
+        # its Nyquist-plane anti-Hermitian part (< 1e-9 of |Hk|)
+        assert relerr(Hk, d[f"Hk_slab{slab}"]) < 2e-9
+        assert relerr(Hr, d[f"Hr_slab{slab}"]) < 1e-13
+        np.testing.assert_allclose(Hs, d[f"Hslab_slab{slab}"], rtol=1e-13, atol=1e-15)
+        out = c.matvec(np.stack((m.real, m.imag)), slab_factor=float(slab)).cpu().numpy()
+        full = d[f"Hk_slab{slab}"] + d[f"Hr_slab{slab}"] - d[f"Hslab_slab{slab}"]
+        assert relerr(out[0] + 1j * out[1], full) < 5e-11
+
+
+@pytest.mark.parametrize("path", OPERATOR_FILES, ids=os.path.basename)
+def test_operator_real_input_matches_oracle_hermitian_part(gpu, path):
+    """For real dipoles the GPU result equals the real part of the reference
+    operator to rounding (FP64)."""
+    d = np.load(path)
+    print("case", os.path.basename(path), flush=True)
+    c = ctx_for(d)
+    geo = oracle_geometry(d["x"], d["box"], d["ngrid"], int(d["P"]), float(d["xi"]), float(d["rc"]))
+    m = np.real(d["m"])
+    ref = np.real(eo.field(geo, m, d["eps"], 1.0))
+    out = c.matvec(m[None], slab_factor=1.0).cpu().numpy()[0]
+    assert relerr(out, ref) < 1e-12
+
+
+@pytest.mark.parametrize("path", OPERATOR_FILES, ids=os.path.basename)
+def test_operator_fp32(gpu, path):
+    from mpol_hip import FP32
+    d = np.load(path)
+    print("case", os.path.basename(path), flush=True)
+    c = ctx_for(d)
+    m = np.real(d["m"])
+    ref = c.matvec(m[None], slab_factor=1.0).cpu().numpy()[0]
+    out = c.matvec(m[None], precision=FP32, slab_factor=1.0).cpu().numpy()[0]
+    assert relerr(out, ref) < 2e-6
+
+
+def test_batched_rhs_and_permittivity_sets(gpu):
+    d = golden("operator_n512_g42.npz")
+    c = ctx_for(d)
+    rng = np.random.default_rng(7)
+    m = rng.standard_normal((5, d["x"].shape[0], 3))
+    eps = 1.5 + 20 * rng.random((5, d["x"].shape[0]))
+    c.set_permittivity(eps)
+    batched = c.matvec(m, slab_factor=1.0).cpu().numpy()
+    for k in range(5):
+        c.set_permittivity(eps[k])
+        single = c.matvec(m[k:k + 1], slab_factor=1.0).cpu().numpy()[0]
+        assert relerr(batched[k], single) < 1e-13
+
+
+def test_operator_is_symmetric_and_linear(gpu):
+    """<u, A v> = <A u, v> (every block of A is symmetric for real eps) and linearity."""
+    x, box = rsa(400, 0.3, 11)
+    xi, P, ng = 0.9, 8, np.array([40, 36, 30])
+    rc = np.sqrt(-np.log(1e-6)) / xi
+    from mpol_hip import Context
+    perp, para, rv = tables(xi)
+    c = Context()
+    c.set_tables(perp, para, rv)
+    h = box / ng
+    c.set_geometry(x, box, ng, P, xi, P * (h * xi) ** 2 / np.pi, rc)
+    c.set_permittivity(np.full(400, 5.0))
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal((400, 3))
+    v = rng.standard_normal((400, 3))
+    Au, Av, Auv = (c.matvec(np.stack((u, v, 2 * u - 3 * v)), slab_factor=1.0).cpu().numpy())
+    assert abs(np.sum(u * Av) - np.sum(Au * v)) / abs(np.sum(u * Av)) < 1e-12
+    assert relerr(Auv, 2 * Au - 3 * Av) < 1e-12
+    # anisotropic grid against the oracle
+    geo = eo.OperatorGeometry(x, box, ng, P, xi, rc, perp, para, rv)
+    ref = np.real(eo.field(geo, u, np.full(400, 5.0), 1.0))
+    assert relerr(Au, ref) < 1e-12
+
+
+def test_geometry_errors(gpu):
+    from mpol_hip import Context
+    d = golden("operator_n64_g32.npz")
+    perp, para, rv = tables(float(d["xi"]))
+    c = Context()
+    c.set_tables(perp[:3000], para[:3000], rv[:3000])  # table ends at r = 2.999 < rc
+    h = d["box"] / d["ngrid"]
+    with pytest.raises(ValueError):
+        c.set_geometry(d["x"], d["box"], d["ngrid"], int(d["P"]), float(d["xi"]),
+                       int(d["P"]) * (h * float(d["xi"])) ** 2 / np.pi, float(d["rc"]))
+    with pytest.raises(Exception):
+        c.matvec(np.zeros((1, 64, 3)))

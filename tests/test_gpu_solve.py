@@ -1,0 +1,100 @@
+"""Parity of the device-resident GMRES and the capacitance workflow with the
+reference's golden solves and the FP64 oracle.
+
+FP64 mode: dipoles within 1e-10 (relative to max |p|) of the reference solved
+at tol 1e-12.  Mixed mode (FP32 operator, FP64 refinement): within 1e-6 of the
+FP64 solution.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, golden
+from helpers import relerr, rsa, tables
+
+pytestmark = pytest.mark.gpu
+
+SOLVE_FILES = sorted(glob.glob(os.path.join(GOLDEN, "solve_*.npz")))
+
+
+def solve_ctx(d, op):
+    from mpol_hip import Context
+    xi = float(op["xi"])
+    perp, para, rv = tables(xi)
+    c = Context()
+    c.set_tables(perp, para, rv)
+    h = d["box"] / op["ngrid"]
+    c.set_geometry(d["x"], d["box"], op["ngrid"], int(op["P"]), xi, int(op["P"]) * (h * xi) ** 2 / np.pi,
+                   float(op["rc"]))
+    c.set_permittivity(d["eps"])
+    return c
+
+
+@pytest.mark.parametrize("path", SOLVE_FILES, ids=os.path.basename)
+def test_solve_fp64_matches_reference(gpu, path):
+    d = np.load(path)
+    op = golden("operator_n64_g21.npz")
+    print("case", os.path.basename(path), flush=True)
+    c = solve_ctx(d, op)
+    slab = 1.0 if "slab1" in path else 0.0
+    n = d["x"].shape[0]
+    b = np.tile(d["H"], n).reshape(1, n, 3)
+    res = c.solve(b, d["guess"][None], 1e-12, restart=10, maxit=100, slab_factor=slab)
+    p = res.x.cpu().numpy()[0]
+    assert res.flag == 0
+    assert res.max_relres <= 1e-12
+    assert relerr(p, np.real(d["p_scipy"])) < 1e-10
+    assert relerr(p, np.real(d["p_python"])) < 1e-10
+
+
+@pytest.mark.parametrize("path", SOLVE_FILES, ids=os.path.basename)
+def test_solve_mixed_within_1e6(gpu, path):
+    d = np.load(path)
+    op = golden("operator_n64_g21.npz")
+    print("case", os.path.basename(path), flush=True)
+    c = solve_ctx(d, op)
+    slab = 1.0 if "slab1" in path else 0.0
+    n = d["x"].shape[0]
+    b = np.tile(d["H"], n).reshape(1, n, 3)
+    res = c.solve(b, d["guess"][None], 1e-8, restart=10, maxit=100, mixed=True, slab_factor=slab)
+    p = res.x.cpu().numpy()[0]
+    assert res.flag == 0
+    assert res.matvecs_fp32 > 0
+    assert relerr(p, np.real(d["p_scipy"])) < 1e-6
+
+
+def test_dropin_magnetic_dipole_and_run_capacitance(gpu):
+    import capacitance
+    for slab in (0, 1):
+        d = golden(f"run_n64_slab{slab}.npz")
+        C, p = capacitance.run_capacitance(d["x"], d["box"], d["eps"], float(d["xi"]), d["H0"],
+                                           errortol=float(d["errortol"]), slab_factor=float(slab))
+        # both stop GMRES at tol 1e-5 from the same start; solutions agree to
+        # the tolerance times the conditioning
+        assert relerr(p, d["p"]) < 1e-4
+        assert abs(C[0] - d["C"][0]) / abs(d["C"][0]) < 1e-5
+        info = capacitance.magnetic_dipole.last_info
+        assert info["flag"] == 0 and info["relres"] <= 1e-5
+
+
+def test_tensor_spectrum_batched_equals_single_solves(gpu):
+    import capacitance
+    x, box = rsa(200, 0.25, 5)
+    rng = np.random.default_rng(1)
+    eps = np.stack((np.full(200, 4.0), 1.0 + 30 * rng.random(200)), axis=1)
+    C, p, info = capacitance.capacitance_tensor_spectrum(x, box, eps, 0.9, errortol=1e-8, return_info=True)
+    assert info["flag"] == 0
+    # symmetric tensor for real permittivities
+    for k in range(2):
+        assert np.max(np.abs(C[:, :, k] - C[:, :, k].T)) < 1e-6 * np.max(np.abs(C[:, :, k]))
+    # column k, direction 2 equals a single run_capacitance-style solve
+    import ewald
+    rc, _, ng, h, P, eta = capacitance.ewald_parameters(box, 0.9, 1e-8)
+    perp, para, rv = tables(0.9)
+    phi = 4 * np.pi * 200 / (3 * np.prod(box))
+    guess = capacitance.maxwell_garnett_guess(eps[:, 1], phi, np.array([0, 0, 1.0]))
+    S = capacitance.magnetic_dipole(x, eps[:, 1], np.array([0, 0, 1.0]), box, None, None, ng, h, P, 0.9, eta, rc,
+                                    guess, None, None, perp, para, rv, 1e-10, slab_factor=0.0)
+    assert relerr(p[:, :, 2, 1], S.real) < 1e-6
