@@ -112,6 +112,15 @@ int mpol_set_permittivity(mpol_ctx* ctx, const double* eps_dev, int32_t nsets);
 int mpol_matvec(mpol_ctx* ctx, const double* m_dev, double* out_dev, int32_t nrhs, int32_t precision,
                 double slab_factor);
 
+/* Diagnostic timing of mpol_matvec (no counterpart in the reference): runs it
+ * `repeats` times and returns in stage_ms (host, 7 entries) the mean
+ * milliseconds of each stage measured with HIP events on the context stream:
+ * [0] slab sum, [1] real-space sum, [2] spread + forward x-FFT,
+ * [3] forward y-FFT, [4] z-FFT + k-space scale + inverse z-FFT,
+ * [5] inverse y-FFT, [6] inverse x-FFT + contraction. */
+int mpol_matvec_timed(mpol_ctx* ctx, const double* m_dev, double* out_dev, int32_t nrhs, int32_t precision,
+                      double slab_factor, int32_t repeats, double* stage_ms_host);
+
 /* The three parts separately: hk (wave space), hr (real space incl. the
  * dielectric and self terms) and hslab (= -slab_factor sum(m_z)/V on z, so
  * A m = hk + hr - hslab).

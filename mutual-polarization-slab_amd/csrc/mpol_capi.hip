@@ -236,8 +236,12 @@ const void* fac_ptr(mpol_ctx* c) {
 }
 
 // out = A m (or its three parts when hr/hslab are given) in precision T.
+// Stage boundaries recorded when `ev` is given (kStages + 1 events).
+constexpr int kStages = 7;
+
 template <typename T>
-int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* hslab, int nrhs, double slab) {
+int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* hslab, int nrhs, double slab,
+                 hipEvent_t* ev = nullptr) {
   const size_t cs = sizeof(cplx<T>);
   const size_t spec_bytes = (size_t)3 * nrhs * c->ng[2] * c->ng[1] * c->kx * cs;
   HIPALLOC(c->spec, spec_bytes);
@@ -246,7 +250,9 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   const int n = c->n;
   hipStream_t st = c->stream;
 
+  if (ev) HIPCHK(hipEventRecord(ev[0], st));
   hipLaunchKernelGGL(k_sum_z, dim3(nrhs), dim3(256), 0, st, m, n, c->zsum.p);
+  if (ev) HIPCHK(hipEventRecord(ev[1], st));
   const double V = c->box[0] * c->box[1] * c->box[2];
   const T* pA;
   const T* pBA;
@@ -259,6 +265,7 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   hipLaunchKernelGGL((k_real_space<T>), dim3(cdiv(n, 256), nrhs), dim3(256), 0, st, n, nrhs, c->nsets, m, out,
                      c->diag.p, c->rs_off.p, c->rs_j.p, pA, pBA, prh, c->zsum.p, slab / V, hr != nullptr ? 1 : 0, hr,
                      hslab);
+  if (ev) HIPCHK(hipEventRecord(ev[2], st));
 
   // tile (x axis) kernels
   const TileCfg tc = tile_cfg(c->ng[0], c->plan[0].e, cs, kLdsTile);
@@ -277,6 +284,7 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
     hipLaunchKernelGGL((k_spread_xfft<T, E>), tgrid, dim3(256), tc.lds, st, ta, m, spec);       \
   } while (0)
   DISPATCH_E(ta.plan.e, LAUNCH_SPREAD, 0);
+  if (ev) HIPCHK(hipEventRecord(ev[3], st));
 
   // y axis forward
   const ColCfg cy = col_cfg(c->ng[1], c->plan[1].e, c->kx, 1, cs, kLdsCol, sizeof(T) == 8 ? 8 : 16);
@@ -290,6 +298,7 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
     hipLaunchKernelGGL((k_fft_y<T, E>), ygrid, dim3(256), cy.lds, st, ay, spec, INV);          \
   } while (0)
   DISPATCH_E(ay.plan.e, LAUNCH_FFTY, 0);
+  if (ev) HIPCHK(hipEventRecord(ev[4], st));
 
   // z axis forward + scale + inverse
   const ColCfg cz = col_cfg(c->ng[2], c->plan[2].e, c->kx, 3, cs, kLdsCol, sizeof(T) == 8 ? 8 : 16);
@@ -310,8 +319,10 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
     hipLaunchKernelGGL((k_fft_z_scale<T, E>), zgrid, dim3(256), cz.lds, st, az, sp, spec);     \
   } while (0)
   DISPATCH_E(az.plan.e, LAUNCH_FFTZ, 0);
+  if (ev) HIPCHK(hipEventRecord(ev[5], st));
 
   DISPATCH_E(ay.plan.e, LAUNCH_FFTY, 1);
+  if (ev) HIPCHK(hipEventRecord(ev[6], st));
 
 #define LAUNCH_CONTRACT(E, ...)                                                                 \
   do {                                                                                          \
@@ -319,6 +330,7 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
     hipLaunchKernelGGL((k_xifft_contract<T, E>), tgrid, dim3(256), tc.lds, st, ta, spec, out);  \
   } while (0)
   DISPATCH_E(ta.plan.e, LAUNCH_CONTRACT, 0);
+  if (ev) HIPCHK(hipEventRecord(ev[7], st));
 #undef LAUNCH_SPREAD
 #undef LAUNCH_FFTY
 #undef LAUNCH_FFTZ
@@ -688,6 +700,34 @@ int mpol_matvec(mpol_ctx* c, const double* m, double* out, int32_t nrhs, int32_t
   if (precision != MPOL_FP64 && precision != MPOL_FP32) return fail(MPOL_EINVAL, "unknown precision");
   HIPCHK(hipSetDevice(c->device));
   return apply(c, m, out, nrhs, precision, slab);
+}
+
+int mpol_matvec_timed(mpol_ctx* c, const double* m, double* out, int32_t nrhs, int32_t precision, double slab,
+                      int32_t repeats, double* stage_ms) {
+  CHECK_STATUS(check_ready(c));
+  if (!m || !out || nrhs < 1 || repeats < 1 || !stage_ms) return fail(MPOL_EINVAL, "bad matvec arguments");
+  if (c->nsets != 1 && c->nsets != nrhs) return fail(MPOL_EINVAL, "permittivity sets must be 1 or nrhs");
+  HIPCHK(hipSetDevice(c->device));
+  hipEvent_t ev[kStages + 1];
+  for (int k = 0; k <= kStages; ++k) HIPCHK(hipEventCreate(&ev[k]));
+  for (int k = 0; k < kStages; ++k) stage_ms[k] = 0.0;
+  int status = MPOL_OK;
+  for (int r = 0; r < repeats && status == MPOL_OK; ++r) {
+    status = precision == MPOL_FP32 ? run_operator<float>(c, m, out, nullptr, nullptr, nrhs, slab, ev)
+                                    : run_operator<double>(c, m, out, nullptr, nullptr, nrhs, slab, ev);
+    if (status != MPOL_OK) break;
+    if (hipEventSynchronize(ev[kStages]) != hipSuccess) {
+      status = fail(MPOL_EHIP, "hipEventSynchronize failed");
+      break;
+    }
+    for (int k = 0; k < kStages; ++k) {
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+      stage_ms[k] += ms / repeats;
+    }
+  }
+  for (int k = 0; k <= kStages; ++k) (void)hipEventDestroy(ev[k]);
+  return status;
 }
 
 int mpol_matvec_components(mpol_ctx* c, const double* m, double* hk, double* hr, double* hslab, int32_t nrhs,

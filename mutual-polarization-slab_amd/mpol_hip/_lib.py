@@ -48,6 +48,8 @@ SIGNATURES = [
                                          _DP, ctypes.c_double]),
     ("mpol_set_permittivity", ctypes.c_int, [_VP, _VP, ctypes.c_int32]),
     ("mpol_matvec", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_double]),
+    ("mpol_matvec_timed", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                                         ctypes.c_int32, _DP]),
     ("mpol_matvec_components", ctypes.c_int, [_VP, _VP, _VP, _VP, _VP, ctypes.c_int32, ctypes.c_int32,
                                               ctypes.c_double]),
     ("mpol_solve", ctypes.c_int, [_VP, _VP, _VP, _VP, ctypes.c_int32, ctypes.POINTER(SolveOpts),

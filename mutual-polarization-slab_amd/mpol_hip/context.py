@@ -159,6 +159,19 @@ class Context:
                                          mt.shape[0], int(precision), float(slab_factor)))
         return out
 
+    STAGES = ("slab_sum", "real_space", "spread_xfft", "fft_y_fwd", "fft_z_scale", "fft_y_inv", "xifft_contract")
+
+    def matvec_timed(self, m, precision=FP64, slab_factor=1.0, repeats=10):
+        """Mean milliseconds of each operator stage (HIP events on the launch stream)."""
+        mt = self._as_rhs(m)
+        out = torch.empty_like(mt)
+        ms = np.zeros(len(self.STAGES), dtype=np.float64)
+        self._sync_stream()
+        _lib.check(self._lib.mpol_matvec_timed(self._ctx, ctypes.c_void_p(mt.data_ptr()),
+                                               ctypes.c_void_p(out.data_ptr()), mt.shape[0], int(precision),
+                                               float(slab_factor), int(repeats), _lib.dptr(ms)))
+        return dict(zip(self.STAGES, ms.tolist())), out
+
     def components(self, m, precision=FP64, slab_factor=1.0):
         mt = self._as_rhs(m)
         hk = torch.empty_like(mt)
