@@ -76,6 +76,10 @@ def cpu_baseline(cfg, perp, para, rv, gpu_fp64_matvec):
     return {"precompute_s": t1 - t0, "matvec_s": t2 - t1, "gpu_fp64_matvec_relerr": err}
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,7 +99,9 @@ def main():
     from mpol_hip import FP32, FP64, Context, default_tables
 
     dev = torch.device("cuda", 0)
+    log("building workload")
     cfg = workloads.configuration(WORKLOAD, sweeps=args.sweeps)
+    log("geometry precompute on the GPU")
     n = cfg["x"].shape[0]
     perp, para, rv = default_tables(cfg["xi"])
     ctx = Context(dev)
@@ -118,7 +124,9 @@ def main():
         return ctx.solve(b, x0, t, restart=10, maxit=100, mixed=mixed, slab_factor=slab)
 
     # reference solution: FP64 at tight tolerance
+    log("FP64 reference solve (tol 1e-10)")
     ref = solve(False, 1e-10)
+    log(f"timed mixed solves: {args.warmup} warmup + {args.steps}")
     p_ref = ref.x.cpu().numpy()
 
     for _ in range(args.warmup):
@@ -133,6 +141,7 @@ def main():
     ms_step = 1e3 * dt / args.steps
     mixed_err = float(np.max(np.abs(last.x.cpu().numpy() - p_ref)) / np.max(np.abs(p_ref)))
 
+    log("plain FP64 solves")
     # plain FP64 path
     n64 = max(1, min(3, args.steps))
     solve(False)
@@ -171,6 +180,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
+        log("CPU baseline: FP64 NumPy oracle (one matvec, ~40 s)")
         def gpu_mv(mm):
             return ctx.matvec(mm[None], precision=FP64, slab_factor=0.0).cpu().numpy()[0]
         cb = cpu_baseline(cfg, perp, para, rv, gpu_mv)

@@ -12,7 +12,7 @@
  *                                  3N interleaved vector [p1x p1y p1z p2x ...]
  *                                  of the reference solver.
  *
- * A context owns one HIP stream (or borrows the caller's), the geometry
+ * A context launches on the caller's HIP stream and owns the geometry
  * precompute and all workspaces.  A context is not re-entrant: calls on one
  * context must not overlap.
  */
@@ -75,11 +75,11 @@ const char* mpol_last_error(void);
 /* Number of visible HIP devices. */
 int mpol_device_count(int* count);
 
-/* Create a context on `device`.  `stream` is a hipStream_t to launch on, or
- * NULL for a context-owned stream. */
+/* Create a context on `device`.  `stream` is the hipStream_t to launch on
+ * (NULL: the device's default stream). */
 int mpol_create(mpol_ctx** ctx, int device, void* stream);
 int mpol_destroy(mpol_ctx* ctx);
-/* Change the stream later calls launch on (NULL: the context's own). */
+/* Change the stream later calls launch on (NULL: the default stream). */
 int mpol_set_stream(mpol_ctx* ctx, void* stream);
 /* Wait for all work the context has queued. */
 int mpol_synchronize(mpol_ctx* ctx);
@@ -120,6 +120,14 @@ int mpol_matvec(mpol_ctx* ctx, const double* m_dev, double* out_dev, int32_t nrh
  * [5] inverse y-FFT, [6] inverse x-FFT + contraction. */
 int mpol_matvec_timed(mpol_ctx* ctx, const double* m_dev, double* out_dev, int32_t nrhs, int32_t precision,
                       double slab_factor, int32_t repeats, double* stage_ms_host);
+
+/* Diagnostic (no counterpart in the reference): run the operator on m up to
+ * and including `stage` (2 = spread + forward x-FFT, 3 = forward y-FFT,
+ * 4 = z-FFT + scale + inverse z-FFT, 5 = inverse y-FFT) and copy the grid
+ * spectrum spec[rhs*3+c][z][y][kx] (kx = nx/2+1, complex FP64 or FP32 by
+ * precision) into spec_out (device). */
+int mpol_debug_spectrum(mpol_ctx* ctx, const double* m_dev, int32_t nrhs, int32_t precision, int32_t stage,
+                        void* spec_out_dev);
 
 /* The three parts separately: hk (wave space), hr (real space incl. the
  * dielectric and self terms) and hslab (= -slab_factor sum(m_z)/V on z, so

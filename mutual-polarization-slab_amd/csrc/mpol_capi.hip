@@ -173,8 +173,7 @@ constexpr size_t kLdsCol = 112 * 1024;
 
 struct mpol_ctx {
   int device = 0;
-  hipStream_t stream = nullptr;
-  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;  // launch stream; nullptr is the default stream
   // tables
   int ntab = 0;
   double self_perp = 0.0;
@@ -241,7 +240,7 @@ constexpr int kStages = 7;
 
 template <typename T>
 int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* hslab, int nrhs, double slab,
-                 hipEvent_t* ev = nullptr) {
+                 hipEvent_t* ev = nullptr, int stop_after = kStages) {
   const size_t cs = sizeof(cplx<T>);
   const size_t spec_bytes = (size_t)3 * nrhs * c->ng[2] * c->ng[1] * c->kx * cs;
   HIPALLOC(c->spec, spec_bytes);
@@ -285,6 +284,7 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   } while (0)
   DISPATCH_E(ta.plan.e, LAUNCH_SPREAD, 0);
   if (ev) HIPCHK(hipEventRecord(ev[3], st));
+  if (stop_after <= 2) return MPOL_OK;
 
   // y axis forward
   const ColCfg cy = col_cfg(c->ng[1], c->plan[1].e, c->kx, 1, cs, kLdsCol, sizeof(T) == 8 ? 8 : 16);
@@ -299,6 +299,7 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   } while (0)
   DISPATCH_E(ay.plan.e, LAUNCH_FFTY, 0);
   if (ev) HIPCHK(hipEventRecord(ev[4], st));
+  if (stop_after <= 3) return MPOL_OK;
 
   // z axis forward + scale + inverse
   const ColCfg cz = col_cfg(c->ng[2], c->plan[2].e, c->kx, 3, cs, kLdsCol, sizeof(T) == 8 ? 8 : 16);
@@ -320,9 +321,11 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   } while (0)
   DISPATCH_E(az.plan.e, LAUNCH_FFTZ, 0);
   if (ev) HIPCHK(hipEventRecord(ev[5], st));
+  if (stop_after <= 4) return MPOL_OK;
 
   DISPATCH_E(ay.plan.e, LAUNCH_FFTY, 1);
   if (ev) HIPCHK(hipEventRecord(ev[6], st));
+  if (stop_after <= 5) return MPOL_OK;
 
 #define LAUNCH_CONTRACT(E, ...)                                                                 \
   do {                                                                                          \
@@ -479,16 +482,7 @@ int mpol_create(mpol_ctx** out, int device, void* stream) {
   HIPCHK(hipSetDevice(device));
   mpol_ctx* c = new mpol_ctx();
   c->device = device;
-  if (stream) {
-    c->stream = reinterpret_cast<hipStream_t>(stream);
-  } else {
-    hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
-    if (e != hipSuccess) {
-      delete c;
-      return fail(MPOL_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
-    }
-    c->stream = c->own_stream;
-  }
+  c->stream = reinterpret_cast<hipStream_t>(stream);  // NULL: the device's default stream
   *out = c;
   return MPOL_OK;
 }
@@ -496,20 +490,14 @@ int mpol_create(mpol_ctx** out, int device, void* stream) {
 int mpol_destroy(mpol_ctx* c) {
   if (!c) return MPOL_OK;
   (void)hipSetDevice(c->device);
-  if (c->stream) (void)hipStreamSynchronize(c->stream);
-  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+  (void)hipStreamSynchronize(c->stream);
   delete c;
   return MPOL_OK;
 }
 
 int mpol_set_stream(mpol_ctx* c, void* stream) {
   if (!c) return fail(MPOL_EINVAL, "null context");
-  c->stream = stream ? reinterpret_cast<hipStream_t>(stream) : c->own_stream;
-  if (!c->stream) {
-    HIPCHK(hipSetDevice(c->device));
-    HIPCHK(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
-    c->stream = c->own_stream;
-  }
+  c->stream = reinterpret_cast<hipStream_t>(stream);
   return MPOL_OK;
 }
 
@@ -728,6 +716,23 @@ int mpol_matvec_timed(mpol_ctx* c, const double* m, double* out, int32_t nrhs, i
   }
   for (int k = 0; k <= kStages; ++k) (void)hipEventDestroy(ev[k]);
   return status;
+}
+
+int mpol_debug_spectrum(mpol_ctx* c, const double* m, int32_t nrhs, int32_t precision, int32_t stage,
+                        void* spec_out) {
+  CHECK_STATUS(check_ready(c));
+  if (!m || !spec_out || nrhs < 1 || stage < 2 || stage > 5) return fail(MPOL_EINVAL, "bad debug arguments");
+  HIPCHK(hipSetDevice(c->device));
+  DevBuf<double> tmp;
+  HIPALLOC(tmp, (size_t)nrhs * 3 * c->n);
+  const int st = precision == MPOL_FP32 ? run_operator<float>(c, m, tmp.p, nullptr, nullptr, nrhs, 0.0, nullptr, stage)
+                                        : run_operator<double>(c, m, tmp.p, nullptr, nullptr, nrhs, 0.0, nullptr, stage);
+  CHECK_STATUS(st);
+  const size_t bytes = (size_t)3 * nrhs * c->ng[2] * c->ng[1] * c->kx *
+                       (precision == MPOL_FP32 ? sizeof(cplx<float>) : sizeof(cplx<double>));
+  HIPCHK(hipMemcpyAsync(spec_out, c->spec.p, bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MPOL_OK;
 }
 
 int mpol_matvec_components(mpol_ctx* c, const double* m, double* hk, double* hr, double* hslab, int32_t nrhs,

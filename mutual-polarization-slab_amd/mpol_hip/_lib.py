@@ -50,6 +50,7 @@ SIGNATURES = [
     ("mpol_matvec", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_double]),
     ("mpol_matvec_timed", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                          ctypes.c_int32, _DP]),
+    ("mpol_debug_spectrum", ctypes.c_int, [_VP, _VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _VP]),
     ("mpol_matvec_components", ctypes.c_int, [_VP, _VP, _VP, _VP, _VP, ctypes.c_int32, ctypes.c_int32,
                                               ctypes.c_double]),
     ("mpol_solve", ctypes.c_int, [_VP, _VP, _VP, _VP, ctypes.c_int32, ctypes.POINTER(SolveOpts),

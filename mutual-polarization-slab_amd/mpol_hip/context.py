@@ -172,6 +172,17 @@ class Context:
                                                float(slab_factor), int(repeats), _lib.dptr(ms)))
         return dict(zip(self.STAGES, ms.tolist())), out
 
+    def debug_spectrum(self, m, stage, precision=FP64):
+        """Grid spectrum after an operator stage, shape (nrhs, 3, nz, ny, nx//2+1)."""
+        mt = self._as_rhs(m)
+        g = self.geometry_info()["ngrid"]
+        dt = torch.complex128 if precision == FP64 else torch.complex64
+        spec = torch.empty((mt.shape[0], 3, g[2], g[1], g[0] // 2 + 1), dtype=dt, device=self.device)
+        self._sync_stream()
+        _lib.check(self._lib.mpol_debug_spectrum(self._ctx, ctypes.c_void_p(mt.data_ptr()), mt.shape[0],
+                                                 int(precision), int(stage), ctypes.c_void_p(spec.data_ptr())))
+        return spec.cpu().numpy()
+
     def components(self, m, precision=FP64, slab_factor=1.0):
         mt = self._as_rhs(m)
         hk = torch.empty_like(mt)
