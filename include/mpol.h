@@ -143,6 +143,13 @@ int mpol_matvec_components(mpol_ctx* ctx, const double* m_dev, double* hk_dev, d
 int mpol_solve(mpol_ctx* ctx, const double* b_dev, const double* x0_dev, double* x_dev, int32_t nrhs,
                const mpol_solve_opts* opts, mpol_solve_info* info);
 
+/* Launch-shape tuning (no counterpart in the reference; used by the
+ * autotuner): knobs[0..5] = spread/contract tile rows along y and z (ty*tz a
+ * power of two, each <= 4), particles staged per chunk (32..256), y-pass and
+ * z-pass columns per workgroup (<= 64), z-pass threads (multiple of 64).
+ * 0 keeps the automatic choice; nknobs may be shorter than 6. */
+int mpol_set_tuning(mpol_ctx* ctx, const int32_t* knobs, int32_t nknobs);
+
 /* Geometry summary (host). */
 int mpol_geometry_info(mpol_ctx* ctx, mpol_geometry_summary* info);
 

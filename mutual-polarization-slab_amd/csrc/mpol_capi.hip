@@ -128,12 +128,14 @@ void make_twiddles(int n, std::vector<double>& tw) {
 }
 
 struct TileCfg {
-  int ty = 1, tz = 1, rows = 1, ncl = 2, lstride = 1, gs = 1;
+  int ty = 1, tz = 1, rows = 1, ncl = 2, lstride = 1, ch = 64;
   size_t lds = 0;
 };
 
-TileCfg tile_cfg(int nx, int e, size_t csize, size_t budget) {
+// Largest tile (rows of nx nodes) and particle chunk that fit the LDS budget.
+TileCfg tile_cfg(int nx, bool generic, size_t csize, int P, size_t budget) {
   const int opts[][2] = {{4, 4}, {4, 2}, {2, 2}, {2, 1}, {1, 1}};
+  const size_t rsize = csize / 2;
   TileCfg c;
   for (const auto& o : opts) {
     c.ty = o[0];
@@ -141,11 +143,13 @@ TileCfg tile_cfg(int nx, int e, size_t csize, size_t budget) {
     c.rows = c.ty * c.tz;
     c.ncl = (3 * c.rows + 1) / 2;
     c.lstride = padded_len(nx);
-    c.lds = ((size_t)nx + (size_t)c.ncl * c.lstride * (e == 0 ? 2 : 1)) * csize;
-    if (c.lds <= budget) break;
+    const size_t grid = ((size_t)nx + (size_t)c.ncl * c.lstride * (generic ? 2 : 1)) * csize;
+    const size_t per = (3 + (size_t)P + c.rows) * rsize + 2 * sizeof(int);
+    for (c.ch = 256; c.ch >= 64; c.ch /= 2) {
+      c.lds = grid + c.ch * per;
+      if (c.lds <= budget) return c;
+    }
   }
-  c.gs = 1;
-  while (c.gs < c.rows) c.gs <<= 1;
   return c;
 }
 
@@ -154,19 +158,20 @@ struct ColCfg {
   size_t lds = 0;
 };
 
-ColCfg col_cfg(int n, int e, int kx, int lines_per_col, size_t csize, size_t budget, int tx_pref) {
+ColCfg col_cfg(int n, bool generic, int kx, int lines_per_col, size_t csize, size_t budget, int tx_pref) {
   ColCfg c;
   int tx = std::min(tx_pref, kx);
   for (;; tx = std::max(1, tx / 2)) {
     c.tx = tx;
     c.lstride = padded_len(n);
-    c.lds = ((size_t)n + (size_t)lines_per_col * tx * c.lstride * (e == 0 ? 2 : 1)) * csize;
+    c.lds = ((size_t)n + (size_t)lines_per_col * tx * c.lstride * (generic ? 2 : 1)) * csize;
     if (c.lds <= budget || tx == 1) break;
   }
   return c;
 }
 
-constexpr size_t kLdsTile = 80 * 1024;
+constexpr size_t kLdsTile = 48 * 1024;
+constexpr size_t kLdsMax = 150 * 1024;  // dynamic LDS per workgroup (gfx950 has 160 KB per CU)
 constexpr size_t kLdsCol = 112 * 1024;
 
 }  // namespace
@@ -199,6 +204,9 @@ struct mpol_ctx {
   // workspaces
   DevBuf<unsigned char> spec;
   DevBuf<double> zsum;
+  // launch-shape overrides (0 = automatic): tile ty, tile tz, particle chunk,
+  // y-pass columns, z-pass columns, z-pass threads
+  int tune[6] = {0, 0, 0, 0, 0, 0};
   // gmres
   DevBuf<double> gV, gW, gR, gAx, gH, gcs, gsn, gg, gy, gh1, gh2, gpart, gbn, gbeta, grel, grn2, gx, gd;
   DevBuf<int> gstate, gkdim;
@@ -212,15 +220,31 @@ void set_lds(K kernel, size_t bytes) {
                             (int)bytes);
 }
 
-#define DISPATCH_E(e, KERNEL, ...)                     \
-  switch (e) {                                         \
-    case 0: KERNEL(0, __VA_ARGS__); break;             \
-    case 12: KERNEL(12, __VA_ARGS__); break;           \
-    case 16: KERNEL(16, __VA_ARGS__); break;           \
-    case 20: KERNEL(20, __VA_ARGS__); break;           \
-    case 24: KERNEL(24, __VA_ARGS__); break;           \
-    default: return fail(MPOL_EINVAL, "unsupported FFT plan"); \
+// Launch KERNEL(E, N, ...) with the compile-time plan for length n when there
+// is one, else with the run-time plan of E elements per thread.
+#define DISPATCH_FFT(n, e, KERNEL, ...)                          \
+  switch (n) {                                                   \
+    case 32: KERNEL(0, 32, __VA_ARGS__); break;                  \
+    case 64: KERNEL(0, 64, __VA_ARGS__); break;                  \
+    case 96: KERNEL(0, 96, __VA_ARGS__); break;                  \
+    case 128: KERNEL(0, 128, __VA_ARGS__); break;                \
+    case 160: KERNEL(0, 160, __VA_ARGS__); break;                \
+    case 192: KERNEL(0, 192, __VA_ARGS__); break;                \
+    case 256: KERNEL(0, 256, __VA_ARGS__); break;                \
+    case 320: KERNEL(0, 320, __VA_ARGS__); break;                \
+    case 512: KERNEL(0, 512, __VA_ARGS__); break;                \
+    default:                                                     \
+      switch (e) {                                               \
+        case 0: KERNEL(0, 0, __VA_ARGS__); break;                \
+        case 12: KERNEL(12, 0, __VA_ARGS__); break;              \
+        case 16: KERNEL(16, 0, __VA_ARGS__); break;              \
+        case 20: KERNEL(20, 0, __VA_ARGS__); break;              \
+        case 24: KERNEL(24, 0, __VA_ARGS__); break;              \
+        default: return fail(MPOL_EINVAL, "unsupported FFT plan"); \
+      }                                                          \
   }
+
+inline bool generic_fft(const FftPlan& p) { return !is_ct_size(p.n) && p.e == 0; }
 
 template <typename T>
 const void* tw_ptr(mpol_ctx* c, int axis) {
@@ -237,6 +261,7 @@ const void* fac_ptr(mpol_ctx* c) {
 // out = A m (or its three parts when hr/hslab are given) in precision T.
 // Stage boundaries recorded when `ev` is given (kStages + 1 events).
 constexpr int kStages = 7;
+constexpr int kZsumBlocks = 64;
 
 template <typename T>
 int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* hslab, int nrhs, double slab,
@@ -244,13 +269,15 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   const size_t cs = sizeof(cplx<T>);
   const size_t spec_bytes = (size_t)3 * nrhs * c->ng[2] * c->ng[1] * c->kx * cs;
   HIPALLOC(c->spec, spec_bytes);
-  HIPALLOC(c->zsum, (size_t)nrhs);
+  HIPALLOC(c->zsum, (size_t)nrhs * (kZsumBlocks + 1));
   cplx<T>* spec = reinterpret_cast<cplx<T>*>(c->spec.p);
   const int n = c->n;
   hipStream_t st = c->stream;
 
   if (ev) HIPCHK(hipEventRecord(ev[0], st));
-  hipLaunchKernelGGL(k_sum_z, dim3(nrhs), dim3(256), 0, st, m, n, c->zsum.p);
+  double* zpart = c->zsum.p + nrhs;
+  hipLaunchKernelGGL(k_sum_z, dim3(kZsumBlocks, nrhs), dim3(256), 0, st, m, n, zpart);
+  hipLaunchKernelGGL(k_sum_finish, dim3(cdiv(nrhs, 256)), dim3(256), 0, st, zpart, kZsumBlocks, nrhs, c->zsum.p);
   if (ev) HIPCHK(hipEventRecord(ev[1], st));
   const double V = c->box[0] * c->box[1] * c->box[2];
   const T* pA;
@@ -267,42 +294,72 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   if (ev) HIPCHK(hipEventRecord(ev[2], st));
 
   // tile (x axis) kernels
-  const TileCfg tc = tile_cfg(c->ng[0], c->plan[0].e, cs, kLdsTile);
+  // the spread tile accumulates in FP64 in both precisions
+  using TA = double;
+  const TileCfg tca = tile_cfg(c->ng[0], generic_fft(c->plan[0]), sizeof(cplx<TA>), c->P, kLdsTile);
+  TileCfg tc = tile_cfg(c->ng[0], generic_fft(c->plan[0]), cs, c->P, kLdsTile);
+  if (c->tune[0] > 0 && c->tune[1] > 0) {
+    tc.ty = c->tune[0];
+    tc.tz = c->tune[1];
+    tc.rows = tc.ty * tc.tz;
+    tc.ncl = (3 * tc.rows + 1) / 2;
+  }
+  if (c->tune[2] > 0) tc.ch = c->tune[2];
+  tc.lds = ((size_t)c->ng[0] + (size_t)tc.ncl * tc.lstride * (generic_fft(c->plan[0]) ? 2 : 1)) * cs +
+           (size_t)tc.ch * ((3 + (size_t)c->P + tc.rows) * sizeof(T) + 2 * sizeof(int));
   TileArgs ta{};
   ta.nx = c->ng[0]; ta.ny = c->ng[1]; ta.nz = c->ng[2]; ta.kx = c->kx;
-  ta.ty = tc.ty; ta.tz = tc.tz; ta.rows = tc.rows; ta.ncl = tc.ncl; ta.lstride = tc.lstride; ta.gs = tc.gs;
+  ta.ty = tc.ty; ta.tz = tc.tz; ta.rows = tc.rows; ta.ncl = tc.ncl; ta.lstride = tc.lstride; ta.ch = tc.ch;
   ta.P = c->P; ta.n = n; ta.nrhs = nrhs;
   ta.perm = c->perm.p; ta.g0 = c->g0s.p; ta.fac = fac_ptr<T>(c); ta.bin_start = c->bin_start.p;
   ta.tw = tw_ptr<T>(c, 0); ta.plan = c->plan[0];
   ta.hvol = c->h[0] * c->h[1] * c->h[2];
   const dim3 tgrid(cdiv(c->ng[1], tc.ty), cdiv(c->ng[2], tc.tz), nrhs);
+  if (tc.lds > kLdsMax) return fail(MPOL_EINVAL, "spread/contract tile does not fit in LDS (reduce tile or chunk)");
 
-#define LAUNCH_SPREAD(E, ...)                                                                   \
+  TileArgs tsa = ta;
+  tsa.ty = tca.ty; tsa.tz = tca.tz; tsa.rows = tca.rows; tsa.ncl = tca.ncl; tsa.lstride = tca.lstride;
+  tsa.ch = tca.ch;
+  tsa.fac = c->fac64.p;
+  tsa.tw = c->tw64[0].p;
+  size_t slds = tca.lds;
+  if (c->tune[0] > 0 && c->tune[1] > 0) {
+    tsa.ty = tc.ty; tsa.tz = tc.tz; tsa.rows = tc.rows; tsa.ncl = tc.ncl;
+  }
+  if (c->tune[2] > 0) tsa.ch = tc.ch;
+  slds = ((size_t)c->ng[0] + (size_t)tsa.ncl * tsa.lstride * (generic_fft(c->plan[0]) ? 2 : 1)) * sizeof(cplx<TA>) +
+         (size_t)tsa.ch * ((3 + (size_t)c->P + tsa.rows) * sizeof(TA) + 2 * sizeof(int));
+  if (slds > kLdsMax) return fail(MPOL_EINVAL, "spread tile does not fit in LDS (reduce tile or chunk)");
+  const dim3 sgrid(cdiv(c->ng[1], tsa.ty), cdiv(c->ng[2], tsa.tz), nrhs);
+#define LAUNCH_SPREAD(E, N, ...)                                                                \
   do {                                                                                          \
-    set_lds(k_spread_xfft<T, E>, tc.lds);                                                       \
-    hipLaunchKernelGGL((k_spread_xfft<T, E>), tgrid, dim3(256), tc.lds, st, ta, m, spec);       \
+    set_lds(k_spread_xfft<TA, E, N, T>, slds);                                                  \
+    hipLaunchKernelGGL((k_spread_xfft<TA, E, N, T>), sgrid, dim3(256), slds, st, tsa, m, spec); \
   } while (0)
-  DISPATCH_E(ta.plan.e, LAUNCH_SPREAD, 0);
+  DISPATCH_FFT(c->ng[0], ta.plan.e, LAUNCH_SPREAD, 0);
   if (ev) HIPCHK(hipEventRecord(ev[3], st));
   if (stop_after <= 2) return MPOL_OK;
 
   // y axis forward
-  const ColCfg cy = col_cfg(c->ng[1], c->plan[1].e, c->kx, 1, cs, kLdsCol, sizeof(T) == 8 ? 8 : 16);
+  const ColCfg cy = col_cfg(c->ng[1], generic_fft(c->plan[1]), c->kx, 1, cs, kLdsCol,
+                            c->tune[3] > 0 ? c->tune[3] : (sizeof(T) == 8 ? 8 : 16));  // 128-byte rows
   ColArgs ay{};
   ay.nx = c->ng[0]; ay.ny = c->ng[1]; ay.nz = c->ng[2]; ay.kx = c->kx; ay.tx = cy.tx; ay.lstride = cy.lstride;
   ay.plan = c->plan[1]; ay.tw = tw_ptr<T>(c, 1);
   const dim3 ygrid(cdiv(c->kx, cy.tx), c->ng[2], 3 * nrhs);
-#define LAUNCH_FFTY(E, INV)                                                                     \
+  if (cy.lds > kLdsMax) return fail(MPOL_EINVAL, "y-pass columns do not fit in LDS");
+#define LAUNCH_FFTY(E, N, INV)                                                                  \
   do {                                                                                          \
-    set_lds(k_fft_y<T, E>, cy.lds);                                                             \
-    hipLaunchKernelGGL((k_fft_y<T, E>), ygrid, dim3(256), cy.lds, st, ay, spec, INV);          \
+    set_lds(k_fft_y<T, E, N>, cy.lds);                                                          \
+    hipLaunchKernelGGL((k_fft_y<T, E, N>), ygrid, dim3(256), cy.lds, st, ay, spec, INV);       \
   } while (0)
-  DISPATCH_E(ay.plan.e, LAUNCH_FFTY, 0);
+  DISPATCH_FFT(c->ng[1], ay.plan.e, LAUNCH_FFTY, 0);
   if (ev) HIPCHK(hipEventRecord(ev[4], st));
   if (stop_after <= 3) return MPOL_OK;
 
   // z axis forward + scale + inverse
-  const ColCfg cz = col_cfg(c->ng[2], c->plan[2].e, c->kx, 3, cs, kLdsCol, sizeof(T) == 8 ? 8 : 16);
+  const ColCfg cz = col_cfg(c->ng[2], generic_fft(c->plan[2]), c->kx, 3, cs, kLdsCol,
+                            c->tune[4] > 0 ? c->tune[4] : (sizeof(T) == 8 ? 4 : 8));
   ColArgs az{};
   az.nx = c->ng[0]; az.ny = c->ng[1]; az.nz = c->ng[2]; az.kx = c->kx; az.tx = cz.tx; az.lstride = cz.lstride;
   az.plan = c->plan[2]; az.tw = tw_ptr<T>(c, 2);
@@ -314,25 +371,34 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   sp.inv_4xi2 = 1.0 / (4.0 * c->xi * c->xi);
   sp.inv_g = 1.0 / ((double)c->ng[0] * (double)c->ng[1] * (double)c->ng[2]);
   const dim3 zgrid(cdiv(c->kx, cz.tx), c->ng[1], nrhs);
-#define LAUNCH_FFTZ(E, ...)                                                                     \
+  if (cz.lds > kLdsMax) return fail(MPOL_EINVAL, "z-pass columns do not fit in LDS");
+  // one thread group per line of the z transform, all 3*tx lines at once
+  int zthreads = 256;
+  {
+    const int e = is_ct_size(c->ng[2]) ? ct_plan(c->ng[2]).e : c->plan[2].e;
+    if (e > 0) zthreads = std::min(1024, std::max(512, 3 * cz.tx * (c->ng[2] / e)));
+    zthreads = (zthreads + 63) / 64 * 64;
+    if (c->tune[5] > 0) zthreads = c->tune[5];
+  }
+#define LAUNCH_FFTZ(E, N, ...)                                                                  \
   do {                                                                                          \
-    set_lds(k_fft_z_scale<T, E>, cz.lds);                                                       \
-    hipLaunchKernelGGL((k_fft_z_scale<T, E>), zgrid, dim3(256), cz.lds, st, az, sp, spec);     \
+    set_lds(k_fft_z_scale<T, E, N>, cz.lds);                                                    \
+    hipLaunchKernelGGL((k_fft_z_scale<T, E, N>), zgrid, dim3(zthreads), cz.lds, st, az, sp, spec); \
   } while (0)
-  DISPATCH_E(az.plan.e, LAUNCH_FFTZ, 0);
+  DISPATCH_FFT(c->ng[2], az.plan.e, LAUNCH_FFTZ, 0);
   if (ev) HIPCHK(hipEventRecord(ev[5], st));
   if (stop_after <= 4) return MPOL_OK;
 
-  DISPATCH_E(ay.plan.e, LAUNCH_FFTY, 1);
+  DISPATCH_FFT(c->ng[1], ay.plan.e, LAUNCH_FFTY, 1);
   if (ev) HIPCHK(hipEventRecord(ev[6], st));
   if (stop_after <= 5) return MPOL_OK;
 
-#define LAUNCH_CONTRACT(E, ...)                                                                 \
+#define LAUNCH_CONTRACT(E, N, ...)                                                              \
   do {                                                                                          \
-    set_lds(k_xifft_contract<T, E>, tc.lds);                                                    \
-    hipLaunchKernelGGL((k_xifft_contract<T, E>), tgrid, dim3(256), tc.lds, st, ta, spec, out);  \
+    set_lds(k_xifft_contract<T, E, N>, tc.lds);                                                 \
+    hipLaunchKernelGGL((k_xifft_contract<T, E, N>), tgrid, dim3(256), tc.lds, st, ta, spec, out); \
   } while (0)
-  DISPATCH_E(ta.plan.e, LAUNCH_CONTRACT, 0);
+  DISPATCH_FFT(c->ng[0], ta.plan.e, LAUNCH_CONTRACT, 0);
   if (ev) HIPCHK(hipEventRecord(ev[7], st));
 #undef LAUNCH_SPREAD
 #undef LAUNCH_FFTY
@@ -818,6 +884,21 @@ int mpol_solve(mpol_ctx* c, const double* b, const double* x0, double* x, int32_
   return MPOL_OK;
 }
 
+int mpol_set_tuning(mpol_ctx* c, const int32_t* knobs, int32_t nknobs) {
+  if (!c || (nknobs > 0 && !knobs) || nknobs < 0 || nknobs > 6) return fail(MPOL_EINVAL, "bad tuning arguments");
+  int t[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = 0; i < nknobs; ++i) t[i] = knobs[i];
+  const bool pow2_rows = (t[0] == 0 && t[1] == 0) ||
+                         (t[0] >= 1 && t[0] <= 4 && t[1] >= 1 && t[1] <= 4 && ((t[0] * t[1]) & (t[0] * t[1] - 1)) == 0);
+  if (!pow2_rows) return fail(MPOL_EINVAL, "tile rows ty*tz must be a power of two with ty, tz in [1, 4]");
+  if (t[2] != 0 && (t[2] < 32 || t[2] > 256)) return fail(MPOL_EINVAL, "particle chunk must be in [32, 256]");
+  if (t[3] < 0 || t[3] > 64 || t[4] < 0 || t[4] > 64) return fail(MPOL_EINVAL, "column widths must be in [1, 64]");
+  if (t[5] != 0 && (t[5] < 64 || t[5] > 1024 || t[5] % 64 != 0))
+    return fail(MPOL_EINVAL, "z-pass threads must be a multiple of 64 in [64, 1024]");
+  for (int i = 0; i < 6; ++i) c->tune[i] = t[i];
+  return MPOL_OK;
+}
+
 int mpol_geometry_info(mpol_ctx* c, mpol_geometry_summary* info) {
   if (!c || !info) return fail(MPOL_EINVAL, "null argument");
   if (!c->have_geo) return fail(MPOL_ESTATE, "geometry not set");
@@ -829,7 +910,7 @@ int mpol_geometry_info(mpol_ctx* c, mpol_geometry_summary* info) {
     info->fft_e[d] = c->plan[d].e;
   }
   info->P = c->P;
-  info->tile_rows = tile_cfg(c->ng[0], c->plan[0].e, sizeof(cplx<double>), kLdsTile).rows;
+  info->tile_rows = tile_cfg(c->ng[0], generic_fft(c->plan[0]), sizeof(cplx<double>), c->P, kLdsTile).rows;
   return MPOL_OK;
 }
 

@@ -244,4 +244,106 @@ __device__ void fft_lines(cplx<T>* buf, int lstride, int nlines, const FftPlan& 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Compile-time plans: the same Stockham stages with the radix sequence, the
+// stride products and the elements per thread fixed at compile time, so every
+// stage is straight-line code (about a third of the registers of the
+// run-time plan).  Used for the grid sizes listed in is_ct_size().
+// ---------------------------------------------------------------------------
+struct CtPlan {
+  int e, nst;
+  int r[kMaxStages];
+  int ns[kMaxStages];
+};
+
+constexpr CtPlan ct_plan(int n) {
+  CtPlan p{};
+  p.e = (n % 5 == 0) ? 20 : ((n % 3 == 0) ? 24 : 16);
+  const int cands[6] = {16, 8, 4, 2, 3, 5};
+  int rem = n, ns = 1, s = 0;
+  while (rem > 1 && s < kMaxStages) {
+    int r = 0;
+    for (int c = 0; c < 6; ++c)
+      if (rem % cands[c] == 0 && p.e % cands[c] == 0) {
+        r = cands[c];
+        break;
+      }
+    if (r == 0) break;
+    p.r[s] = r;
+    p.ns[s] = ns;
+    ns *= r;
+    rem /= r;
+    ++s;
+  }
+  p.nst = s;
+  return p;
+}
+
+__host__ __device__ constexpr bool is_ct_size(int n) {
+  return n == 32 || n == 64 || n == 128 || n == 256 || n == 512 || n == 96 || n == 192 || n == 160 || n == 320;
+}
+
+template <typename T, int N, int S>
+__device__ __forceinline__ void ct_stage(cplx<T>* buf, int lstride, int nlines, const cplx<T>* tw, bool cin,
+                                         bool cout) {
+  constexpr CtPlan p = ct_plan(N);
+  constexpr int E = p.e, R = p.r[S], NS = p.ns[S], TL = N / E;
+  const int LPP = (int)blockDim.x / TL;
+  static_assert(E % R == 0, "radix must divide the elements per thread");
+  cplx<T> v[E];
+  for (int l0 = 0; l0 < nlines; l0 += LPP) {
+    const int l = l0 + (int)threadIdx.x / TL;
+    const int t = (int)threadIdx.x % TL;
+    const bool act = ((int)threadIdx.x < LPP * TL) && (l < nlines);
+    cplx<T>* line = buf + l * lstride;
+    if (act) {
+#pragma unroll
+      for (int b = 0; b < E / R; ++b) {
+        const int j = t + TL * b;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+          const cplx<T> a = line[pidx(j + k * (N / R))];
+          v[b * R + k] = cin ? cconj(a) : a;
+        }
+      }
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int b = 0; b < E / R; ++b) {
+        const int j = t + TL * b;
+        const int jm = (NS == 1) ? 0 : j % NS;
+        cplx<T>* w = v + b * R;
+        if constexpr (NS > 1) {
+#pragma unroll
+          for (int k = 1; k < R; ++k) w[k] = cmul(w[k], tw[jm * k * (N / (NS * R))]);
+        }
+        dft_small<T, R>(w);
+        const int base = (j - jm) * R + jm;
+#pragma unroll
+        for (int k = 0; k < R; ++k) line[pidx(base + k * NS)] = cout ? cconj(w[k]) : w[k];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, int N, int S = 0>
+__device__ __forceinline__ void fft_ct(cplx<T>* buf, int lstride, int nlines, const cplx<T>* tw, bool inverse) {
+  constexpr CtPlan p = ct_plan(N);
+  if constexpr (S < p.nst) {
+    ct_stage<T, N, S>(buf, lstride, nlines, tw, inverse && S == 0, inverse && S == p.nst - 1);
+    fft_ct<T, N, S + 1>(buf, lstride, nlines, tw, inverse);
+  }
+}
+
+// Kernels are templated on <E, N>: N > 0 selects the compile-time plan for
+// that length, N == 0 the run-time plan with E elements per thread.
+template <typename T, int E, int N>
+__device__ __forceinline__ void fft_any(cplx<T>* buf, int lstride, int nlines, const FftPlan& p, const cplx<T>* tw,
+                                        bool inverse, cplx<T>* scratch) {
+  if constexpr (N > 0) fft_ct<T, N>(buf, lstride, nlines, tw, inverse);
+  else fft_lines<T, E>(buf, lstride, nlines, p, tw, inverse, scratch);
+}
+
 }  // namespace mpol

@@ -13,6 +13,7 @@
 namespace mpol {
 
 constexpr int kMaxRanges = 160;  // candidate particle ranges per tile (2 per z row)
+constexpr int kPUnroll = 16;    // stencil widths up to this are unrolled in registers
 constexpr double kPi = 3.14159265358979323846264338327950288;
 
 // ---------------------------------------------------------------------------
@@ -229,20 +230,29 @@ __global__ void k_pairs(PairParams p, const double* __restrict__ x, const int* _
 // ---------------------------------------------------------------------------
 // per-matvec kernels
 // ---------------------------------------------------------------------------
-// Sum of the z components per right-hand side (slab term), deterministic.
+// Partial sums of the z components per right-hand side (slab term);
+// partial[rhs*gridDim.x + block], deterministic.
 __global__ void k_sum_z(const double* __restrict__ m, int n, double* __restrict__ out) {
   __shared__ double red[256];
-  const int rhs = blockIdx.x;
+  const int rhs = blockIdx.y;
   const double* mr = m + (size_t)rhs * n * 3;
   double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) s += mr[3 * i + 2];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += mr[3 * i + 2];
   red[threadIdx.x] = s;
   __syncthreads();
   for (int o = blockDim.x / 2; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[rhs] = red[0];
+  if (threadIdx.x == 0) out[(size_t)rhs * gridDim.x + blockIdx.x] = red[0];
+}
+
+__global__ void k_sum_finish(const double* __restrict__ partial, int nblk, int nrhs, double* __restrict__ out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrhs) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += partial[(size_t)r * nblk + b];
+  out[r] = s;
 }
 
 // Real-space sum + dielectric/self diagonal + slab term.  Writes out (the
@@ -291,14 +301,22 @@ __global__ void k_real_space(int n, int nrhs, int nsets, const double* __restric
 
 // ---------------------------------------------------------------------------
 // spread + forward x-FFT (fused), and inverse x-FFT + contract (fused)
+//
+// A workgroup owns a tile of ty*tz grid rows along x (all nx nodes of each
+// row, 3 components), held in LDS as complex lines that pack two real rows
+// (row 2q in .x, row 2q+1 in .y).  The particles whose stencils reach the tile
+// are staged in chunks of `ch`: phase 1 loads, with one thread per particle
+// and no dependent loads between particles, the stencil start, the dipole, the
+// x factors and the weight of every tile row (product of the y and z
+// factors); phase 2 walks (particle, row) items.
 // ---------------------------------------------------------------------------
 struct TileArgs {
   int nx, ny, nz, kx;
   int ty, tz;        // grid rows (y, z) per tile
-  int rows;          // ty*tz
+  int rows;          // ty*tz, a power of two <= 16
   int ncl;           // complex LDS lines = ceil(3*rows/2)
   int lstride;       // complex elements per LDS line (padded)
-  int gs;            // lanes per particle group (power of two >= rows, <= 64)
+  int ch;            // particles staged per chunk
   int P;
   int n;             // particles
   int nrhs;
@@ -361,59 +379,136 @@ __device__ inline T axis_weight(const T* f, int g, int g0, int n, int P) {
   return w;
 }
 
-template <typename T, int E>
+template <typename T>
+struct Stage {
+  int* gx;     // [ch] first x node of the stencil
+  int* orig;   // [ch] caller's particle index
+  T* m;        // [ch][3] dipole (spread)
+  T* fx;       // [ch][P] x factors
+  T* w;        // [ch][rows] y*z factor of each tile row
+};
+
+template <typename T>
+__device__ inline Stage<T> stage_layout(unsigned char* base, int ch, int P, int rows) {
+  Stage<T> st;
+  T* tb = reinterpret_cast<T*>(base);
+  st.m = tb;
+  st.fx = st.m + 3 * ch;
+  st.w = st.fx + (size_t)P * ch;
+  st.gx = reinterpret_cast<int*>(st.w + (size_t)rows * ch);
+  st.orig = st.gx + ch;
+  return st;
+}
+
+// Phase 1: thread t stages candidate base+t (if any).  Returns nothing; the
+// caller synchronises.
+template <typename T, bool WITH_M>
+__device__ inline void stage_candidates(const TileArgs& a, const Stage<T>& st, int base, int total, const int* s_start,
+                                        const int* s_cum, int nr, int ty0, int tz0, const double* mr) {
+  const int t = threadIdx.x;
+  if (t >= a.ch) return;
+  const int q = base + t;
+  if (q >= total) {
+    for (int r = 0; r < a.rows; ++r) st.w[t * a.rows + r] = (T)0;
+    st.orig[t] = -1;
+    return;
+  }
+  int lo = 0, hi = nr;  // largest r with s_cum[r] <= q
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (s_cum[mid] <= q) lo = mid;
+    else hi = mid;
+  }
+  const int s = s_start[lo] + (q - s_cum[lo]);
+  const int gx0 = a.g0[3 * s], gy0 = a.g0[3 * s + 1], gz0 = a.g0[3 * s + 2];
+  const int orig = a.perm[s];
+  const T* fs = reinterpret_cast<const T*>(a.fac) + (size_t)s * 3 * a.P;
+  st.gx[t] = gx0;
+  st.orig[t] = orig;
+  if (WITH_M) {
+    st.m[3 * t] = (T)mr[3 * (size_t)orig];
+    st.m[3 * t + 1] = (T)mr[3 * (size_t)orig + 1];
+    st.m[3 * t + 2] = (T)mr[3 * (size_t)orig + 2];
+  }
+  for (int o = 0; o < a.P; ++o) st.fx[t * a.P + o] = fs[o];
+  T wy[4], wz[4];
+  for (int l = 0; l < a.ty; ++l) {
+    const int y = ty0 + l;
+    wy[l] = (y < a.ny) ? axis_weight(fs + a.P, y, gy0, a.ny, a.P) : (T)0;
+  }
+  for (int l = 0; l < a.tz; ++l) {
+    const int z = tz0 + l;
+    wz[l] = (z < a.nz) ? axis_weight(fs + 2 * a.P, z, gz0, a.nz, a.P) : (T)0;
+  }
+  for (int r = 0; r < a.rows; ++r) st.w[t * a.rows + r] = wy[r % a.ty] * wz[r / a.ty];
+}
+
+// T: accumulation and x-FFT precision; TS: precision of the stored spectrum.
+// The FP32 operator accumulates in FP64 (LDS FP64 atomics are the fast ones on
+// gfx950) and rounds once when it stores the spectrum.
+template <typename T, int E, int N, typename TS = T>
 __global__ void __launch_bounds__(256) k_spread_xfft(TileArgs a, const double* __restrict__ m,
-                                                     cplx<T>* __restrict__ spec) {
+                                                     cplx<TS>* __restrict__ spec) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ int s_start[kMaxRanges], s_cum[kMaxRanges + 1], s_nr;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* buf = tw + a.nx;
   cplx<T>* scratch = buf + a.ncl * a.lstride;
+  unsigned char* stage_base = reinterpret_cast<unsigned char*>(buf + a.ncl * a.lstride * (E == 0 && N == 0 ? 2 : 1));
+  const Stage<T> st = stage_layout<T>(stage_base, a.ch, a.P, a.rows);
   const int ty0 = blockIdx.x * a.ty, tz0 = blockIdx.y * a.tz, rhs = blockIdx.z;
   const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
   for (int k = threadIdx.x; k < a.nx; k += blockDim.x) tw[k] = twg[k];
   for (int k = threadIdx.x; k < a.ncl * a.lstride; k += blockDim.x) buf[k] = cmk<T>(0, 0);
   tile_ranges(a, ty0, tz0, s_start, s_cum, &s_nr);
-
-  const T* fac = reinterpret_cast<const T*>(a.fac);
   const int total = s_cum[s_nr];
-  const int grp = threadIdx.x / a.gs, lane = threadIdx.x % a.gs, ngrp = blockDim.x / a.gs;
-  const int P = a.P;
   const double* mr = m + (size_t)rhs * 3 * (size_t)a.n;
-  int ri = 0;
-  for (int q = grp; q < total; q += ngrp) {
-    while (q >= s_cum[ri + 1]) ++ri;
-    const int s = s_start[ri] + (q - s_cum[ri]);
-    const int gx0 = a.g0[3 * s], gy0 = a.g0[3 * s + 1], gz0 = a.g0[3 * s + 2];
-    const int orig = a.perm[s];
-    const T* fs = fac + (size_t)s * 3 * P;
-    for (int row = lane; row < a.rows; row += a.gs) {
-      const int ly = row % a.ty, lz = row / a.ty;
-      const int y = ty0 + ly, z = tz0 + lz;
-      if (y >= a.ny || z >= a.nz) continue;
-      const T wy = axis_weight(fs + P, y, gy0, a.ny, P);
-      const T wz = axis_weight(fs + 2 * P, z, gz0, a.nz, P);
-      const T wyz = wy * wz;
+  const int P = a.P;
+  for (int base = 0; base < total; base += a.ch) {
+    stage_candidates<T, true>(a, st, base, total, s_start, s_cum, s_nr, ty0, tz0, mr);
+    __syncthreads();
+    const int nitems = min(a.ch, total - base) * a.rows;
+    for (int item = threadIdx.x; item < nitems; item += blockDim.x) {
+      const T wyz = st.w[item];
       if (wyz == (T)0) continue;
-      const double* mv = mr + 3 * (size_t)orig;
-      const T m0 = (T)mv[0], m1 = (T)mv[1], m2 = (T)mv[2];
+      const int cnd = item / a.rows, row = item - cnd * a.rows;
+      const T m0 = st.m[3 * cnd], m1 = st.m[3 * cnd + 1], m2 = st.m[3 * cnd + 2];
+      const T* fx = st.fx + cnd * P;
       const int l0 = row, l1 = a.rows + row, l2 = 2 * a.rows + row;
       T* p0 = reinterpret_cast<T*>(buf + (l0 >> 1) * a.lstride) + (l0 & 1);
       T* p1 = reinterpret_cast<T*>(buf + (l1 >> 1) * a.lstride) + (l1 & 1);
       T* p2 = reinterpret_cast<T*>(buf + (l2 >> 1) * a.lstride) + (l2 & 1);
-      int x = gx0;
-      for (int o = 0; o < P; ++o) {
-        const T w = wyz * fs[o];
-        const int px = 2 * pidx(x);
-        atomicAdd(p0 + px, w * m0);
-        atomicAdd(p1 + px, w * m1);
-        atomicAdd(p2 + px, w * m2);
-        if (++x == a.nx) x = 0;
+      int x = st.gx[cnd];
+      if (P <= kPUnroll) {
+        // factors into registers first so the LDS atomics issue back to back
+        T f[kPUnroll];
+#pragma unroll
+        for (int o = 0; o < kPUnroll; ++o) f[o] = (o < P) ? fx[o] : (T)0;
+#pragma unroll
+        for (int o = 0; o < kPUnroll; ++o) {
+          if (o < P) {
+            const T w = wyz * f[o];
+            const int px = 2 * pidx(x);
+            atomicAdd(p0 + px, w * m0);
+            atomicAdd(p1 + px, w * m1);
+            atomicAdd(p2 + px, w * m2);
+            if (++x == a.nx) x = 0;
+          }
+        }
+      } else {
+        for (int o = 0; o < P; ++o) {
+          const T w = wyz * fx[o];
+          const int px = 2 * pidx(x);
+          atomicAdd(p0 + px, w * m0);
+          atomicAdd(p1 + px, w * m1);
+          atomicAdd(p2 + px, w * m2);
+          if (++x == a.nx) x = 0;
+        }
       }
     }
+    __syncthreads();
   }
-  __syncthreads();
-  fft_lines<T, E>(buf, a.lstride, a.ncl, a.plan, tw, false, scratch);
+  fft_any<T, E, N>(buf, a.lstride, a.ncl, a.plan, tw, false, scratch);
   // split packed pairs into the two half spectra and store
   const int nreal = 3 * a.rows;
   const int tot = nreal * a.kx;
@@ -425,14 +520,14 @@ __global__ void __launch_bounds__(256) k_spread_xfft(TileArgs a, const double* _
     const cplx<T>* line = buf + (l >> 1) * a.lstride;
     const cplx<T> Z = line[pidx(k)];
     const cplx<T> Zc = cconj(line[pidx(k == 0 ? 0 : a.nx - k)]);
-    cplx<T> v;
-    if ((l & 1) == 0) v = cmk<T>((Z.x + Zc.x) * (T)0.5, (Z.y + Zc.y) * (T)0.5);
-    else v = cmk<T>((Z.y - Zc.y) * (T)0.5, (Zc.x - Z.x) * (T)0.5);
+    cplx<TS> v;
+    if ((l & 1) == 0) v = cmk<TS>((TS)((Z.x + Zc.x) * (T)0.5), (TS)((Z.y + Zc.y) * (T)0.5));
+    else v = cmk<TS>((TS)((Z.y - Zc.y) * (T)0.5), (TS)((Zc.x - Z.x) * (T)0.5));
     spec[(((size_t)(rhs * 3 + c) * a.nz + z) * a.ny + y) * a.kx + k] = v;
   }
 }
 
-template <typename T, int E>
+template <typename T, int E, int N>
 __global__ void __launch_bounds__(256) k_xifft_contract(TileArgs a, const cplx<T>* __restrict__ spec,
                                                         double* __restrict__ out) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -440,6 +535,8 @@ __global__ void __launch_bounds__(256) k_xifft_contract(TileArgs a, const cplx<T
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* buf = tw + a.nx;
   cplx<T>* scratch = buf + a.ncl * a.lstride;
+  unsigned char* stage_base = reinterpret_cast<unsigned char*>(buf + a.ncl * a.lstride * (E == 0 && N == 0 ? 2 : 1));
+  const Stage<T> st = stage_layout<T>(stage_base, a.ch, a.P, a.rows);
   const int ty0 = blockIdx.x * a.ty, tz0 = blockIdx.y * a.tz, rhs = blockIdx.z;
   const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
   for (int k = threadIdx.x; k < a.nx; k += blockDim.x) tw[k] = twg[k];
@@ -470,63 +567,70 @@ __global__ void __launch_bounds__(256) k_xifft_contract(TileArgs a, const cplx<T
     }
   }
   tile_ranges(a, ty0, tz0, s_start, s_cum, &s_nr);  // contains __syncthreads
-  fft_lines<T, E>(buf, a.lstride, a.ncl, a.plan, tw, true, scratch);
-
-  const T* fac = reinterpret_cast<const T*>(a.fac);
+  fft_any<T, E, N>(buf, a.lstride, a.ncl, a.plan, tw, true, scratch);
   const int total = s_cum[s_nr];
-  const int grp = threadIdx.x / a.gs, lane = threadIdx.x % a.gs, ngrp = blockDim.x / a.gs;
   const int P = a.P;
-  const int ngroups_iter = (total + ngrp - 1) / ngrp;
-  int ri = 0;
-  for (int it = 0; it < ngroups_iter; ++it) {
-    const int q = grp + it * ngrp;
-    T acc0 = 0, acc1 = 0, acc2 = 0;
-    int orig = -1;
-    if (q < total) {
-      while (q >= s_cum[ri + 1]) ++ri;
-      const int s = s_start[ri] + (q - s_cum[ri]);
-      const int gx0 = a.g0[3 * s], gy0 = a.g0[3 * s + 1], gz0 = a.g0[3 * s + 2];
-      orig = a.perm[s];
-      const T* fs = fac + (size_t)s * 3 * P;
-      for (int row = lane; row < a.rows; row += a.gs) {
-        const int ly = row % a.ty, lz = row / a.ty;
-        const int y = ty0 + ly, z = tz0 + lz;
-        if (y >= a.ny || z >= a.nz) continue;
-        const T wy = axis_weight(fs + P, y, gy0, a.ny, P);
-        const T wz = axis_weight(fs + 2 * P, z, gz0, a.nz, P);
-        const T wyz = wy * wz;
-        if (wyz == (T)0) continue;
-        const int l0 = row, l1 = a.rows + row, l2 = 2 * a.rows + row;
-        const T* p0 = reinterpret_cast<const T*>(buf + (l0 >> 1) * a.lstride) + (l0 & 1);
-        const T* p1 = reinterpret_cast<const T*>(buf + (l1 >> 1) * a.lstride) + (l1 & 1);
-        const T* p2 = reinterpret_cast<const T*>(buf + (l2 >> 1) * a.lstride) + (l2 & 1);
-        T s0 = 0, s1 = 0, s2 = 0;
-        int x = gx0;
-        for (int o = 0; o < P; ++o) {
-          const T w = fs[o];
-          const int px = 2 * pidx(x);
-          s0 += w * p0[px];
-          s1 += w * p1[px];
-          s2 += w * p2[px];
-          if (++x == a.nx) x = 0;
+  double* outr = out + (size_t)rhs * 3 * (size_t)a.n;
+  for (int base = 0; base < total; base += a.ch) {
+    stage_candidates<T, false>(a, st, base, total, s_start, s_cum, s_nr, ty0, tz0, nullptr);
+    __syncthreads();
+    const int nitems = min(a.ch, total - base) * a.rows;
+    const int iters = (nitems + (int)blockDim.x - 1) / (int)blockDim.x;
+    for (int it = 0; it < iters; ++it) {
+      const int item = it * blockDim.x + threadIdx.x;
+      T acc0 = 0, acc1 = 0, acc2 = 0;
+      const int cnd = item / a.rows, row = item - cnd * a.rows;
+      if (item < nitems) {
+        const T wyz = st.w[item];
+        if (wyz != (T)0) {
+          const T* fx = st.fx + cnd * P;
+          const int l0 = row, l1 = a.rows + row, l2 = 2 * a.rows + row;
+          const T* p0 = reinterpret_cast<const T*>(buf + (l0 >> 1) * a.lstride) + (l0 & 1);
+          const T* p1 = reinterpret_cast<const T*>(buf + (l1 >> 1) * a.lstride) + (l1 & 1);
+          const T* p2 = reinterpret_cast<const T*>(buf + (l2 >> 1) * a.lstride) + (l2 & 1);
+          int x = st.gx[cnd];
+          if (P <= kPUnroll) {
+#pragma unroll
+            for (int o = 0; o < kPUnroll; ++o) {
+              if (o < P) {
+                const T w = fx[o];
+                const int px = 2 * pidx(x);
+                acc0 += w * p0[px];
+                acc1 += w * p1[px];
+                acc2 += w * p2[px];
+                if (++x == a.nx) x = 0;
+              }
+            }
+          } else {
+            for (int o = 0; o < P; ++o) {
+              const T w = fx[o];
+              const int px = 2 * pidx(x);
+              acc0 += w * p0[px];
+              acc1 += w * p1[px];
+              acc2 += w * p2[px];
+              if (++x == a.nx) x = 0;
+            }
+          }
+          acc0 *= wyz;
+          acc1 *= wyz;
+          acc2 *= wyz;
         }
-        acc0 += wyz * s0;
-        acc1 += wyz * s1;
-        acc2 += wyz * s2;
+      }
+      // the rows of one particle are consecutive lanes (rows | 64): reduce them
+      for (int o = a.rows / 2; o > 0; o >>= 1) {
+        acc0 += __shfl_xor(acc0, o, a.rows);
+        acc1 += __shfl_xor(acc1, o, a.rows);
+        acc2 += __shfl_xor(acc2, o, a.rows);
+      }
+      if (item < nitems && row == 0) {
+        const int orig = st.orig[cnd];
+        double* po = outr + 3 * (size_t)orig;
+        atomicAdd(po, (double)acc0 * a.hvol);
+        atomicAdd(po + 1, (double)acc1 * a.hvol);
+        atomicAdd(po + 2, (double)acc2 * a.hvol);
       }
     }
-    // reduce over the lanes of the group (groups never straddle a wavefront)
-    for (int o = a.gs / 2; o > 0; o >>= 1) {
-      acc0 += __shfl_xor(acc0, o, a.gs);
-      acc1 += __shfl_xor(acc1, o, a.gs);
-      acc2 += __shfl_xor(acc2, o, a.gs);
-    }
-    if (lane == 0 && orig >= 0) {
-      double* po = out + ((size_t)rhs * a.n + (size_t)orig) * 3;
-      atomicAdd(po, (double)acc0 * a.hvol);
-      atomicAdd(po + 1, (double)acc1 * a.hvol);
-      atomicAdd(po + 2, (double)acc2 * a.hvol);
-    }
+    __syncthreads();
   }
 }
 
@@ -541,7 +645,7 @@ struct ColArgs {
   const void* tw;   // twiddles for the transformed axis
 };
 
-template <typename T, int E>
+template <typename T, int E, int N>
 __global__ void __launch_bounds__(256) k_fft_y(ColArgs a, cplx<T>* __restrict__ spec, int inverse) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
@@ -558,7 +662,7 @@ __global__ void __launch_bounds__(256) k_fft_y(ColArgs a, cplx<T>* __restrict__ 
     buf[c * a.lstride + pidx(y)] = (kx < a.kx) ? base[(size_t)y * a.kx + kx] : cmk<T>(0, 0);
   }
   __syncthreads();
-  fft_lines<T, E>(buf, a.lstride, a.tx, a.plan, tw, inverse != 0, scratch);
+  fft_any<T, E, N>(buf, a.lstride, a.tx, a.plan, tw, inverse != 0, scratch);
   for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
     const int c = idx % a.tx, y = idx / a.tx;
     const int kx = kx0 + c;
@@ -583,8 +687,8 @@ __device__ inline double kspace_coef(const ScaleParams& s, double k0, double k1,
   return 9.0 * kPi / (2.0 * kn) * (j * j) * exp(-ek * s.inv_4xi2) / kk;
 }
 
-template <typename T, int E>
-__global__ void __launch_bounds__(256) k_fft_z_scale(ColArgs a, ScaleParams sp, cplx<T>* __restrict__ spec) {
+template <typename T, int E, int N>
+__global__ void __launch_bounds__(1024) k_fft_z_scale(ColArgs a, ScaleParams sp, cplx<T>* __restrict__ spec) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* buf = tw + a.nz;
@@ -604,7 +708,7 @@ __global__ void __launch_bounds__(256) k_fft_z_scale(ColArgs a, ScaleParams sp, 
     buf[(c * a.tx + cc) * a.lstride + pidx(z)] = v;
   }
   __syncthreads();
-  fft_lines<T, E>(buf, a.lstride, 3 * a.tx, a.plan, tw, false, scratch);
+  fft_any<T, E, N>(buf, a.lstride, 3 * a.tx, a.plan, tw, false, scratch);
   // k-space scale: G = c(k) M F with M = khat khat^T, cross terms with an
   // even-grid Nyquist axis removed (the Hermitian part of the reference's
   // fftshift-ordered operator, so real grids stay real)
@@ -639,7 +743,7 @@ __global__ void __launch_bounds__(256) k_fft_z_scale(ColArgs a, ScaleParams sp, 
     *f2 = cmk<T>(c * (x02 * F0.x + x12 * F1.x + d22 * F2.x), c * (x02 * F0.y + x12 * F1.y + d22 * F2.y));
   }
   __syncthreads();
-  fft_lines<T, E>(buf, a.lstride, 3 * a.tx, a.plan, tw, true, scratch);
+  fft_any<T, E, N>(buf, a.lstride, 3 * a.tx, a.plan, tw, true, scratch);
   for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
     const int c = idx / (a.tx * a.nz);
     const int rem = idx - c * a.tx * a.nz;

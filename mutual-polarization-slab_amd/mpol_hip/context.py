@@ -135,6 +135,16 @@ class Context:
         self.nsets = et.shape[0]
         self._keep = [et]
 
+    TUNING_KNOBS = ("tile_ty", "tile_tz", "chunk", "y_cols", "z_cols", "z_threads")
+
+    def set_tuning(self, **knobs):
+        """Override launch shapes (0 = automatic); see TUNING_KNOBS."""
+        unknown = set(knobs) - set(self.TUNING_KNOBS)
+        if unknown:
+            raise ValueError(f"unknown tuning knobs {sorted(unknown)}")
+        vals = np.array([int(knobs.get(k, 0)) for k in self.TUNING_KNOBS], dtype=np.int32)
+        _lib.check(self._lib.mpol_set_tuning(self._ctx, _lib.i32ptr(vals), vals.size))
+
     def geometry_info(self) -> dict:
         s = _lib.GeometrySummary()
         _lib.check(self._lib.mpol_geometry_info(self._ctx, ctypes.byref(s)))
