@@ -133,7 +133,7 @@ struct TileCfg {
 };
 
 // Largest tile (rows of nx nodes) and particle chunk that fit the LDS budget.
-TileCfg tile_cfg(int nx, bool generic, size_t csize, int P, size_t budget) {
+TileCfg tile_cfg(int nx, bool generic, size_t csize, int P, size_t budget, bool sort_arrays = false) {
   const int opts[][2] = {{4, 4}, {4, 2}, {2, 2}, {2, 1}, {1, 1}};
   const size_t rsize = csize / 2;
   TileCfg c;
@@ -145,8 +145,9 @@ TileCfg tile_cfg(int nx, bool generic, size_t csize, int P, size_t budget) {
     c.lstride = padded_len(nx);
     const size_t grid = ((size_t)nx + (size_t)c.ncl * c.lstride * (generic ? 2 : 1)) * csize;
     const size_t per = (3 + (size_t)P + c.rows) * rsize + 2 * sizeof(int);
+    const size_t fixed = sort_arrays ? ((size_t)nx + 1) * sizeof(int) : 0;
     for (c.ch = 256; c.ch >= 64; c.ch /= 2) {
-      c.lds = grid + c.ch * per;
+      c.lds = grid + fixed + c.ch * per;
       if (c.lds <= budget) return c;
     }
   }
@@ -192,9 +193,14 @@ struct mpol_ctx {
   int64_t npairs = 0;
   DevBuf<double> x, base;
   DevBuf<int> bin_key, bin_count, bin_start, bin_cursor, cell_key, cell_count, cell_start, cell_cursor,
-      cell_members, g0u, perm, g0s, rs_count, rs_off, rs_j, err;
-  DevBuf<double> fac64, rs_A, rs_BA, rs_rh;
-  DevBuf<float> fac32, rs_A32, rs_BA32, rs_rh32;
+      cell_members, g0u, perm, rs_count, rs_off, rs_j, err;
+  DevBuf<int4> rec;
+  DevBuf<double4> base64;
+  DevBuf<float4> base32;
+  DevBuf<double> ms;
+  double norm = 0.0;
+  DevBuf<double> rs_A, rs_BA, rs_rh;
+  DevBuf<float> rs_A32, rs_BA32, rs_rh32;
   FftPlan plan[3];
   DevBuf<double> tw64[3];
   DevBuf<float> tw32[3];
@@ -246,6 +252,21 @@ void set_lds(K kernel, size_t bytes) {
 
 inline bool generic_fft(const FftPlan& p) { return !is_ct_size(p.n) && p.e == 0; }
 
+// Launch KERNEL(N, ...) for a length with a compile-time FFT plan.
+#define DISPATCH_CT(n, KERNEL, ...)                                  \
+  switch (n) {                                                       \
+    case 32: KERNEL(32, __VA_ARGS__); break;                         \
+    case 64: KERNEL(64, __VA_ARGS__); break;                         \
+    case 96: KERNEL(96, __VA_ARGS__); break;                         \
+    case 128: KERNEL(128, __VA_ARGS__); break;                       \
+    case 160: KERNEL(160, __VA_ARGS__); break;                       \
+    case 192: KERNEL(192, __VA_ARGS__); break;                       \
+    case 256: KERNEL(256, __VA_ARGS__); break;                       \
+    case 320: KERNEL(320, __VA_ARGS__); break;                       \
+    case 512: KERNEL(512, __VA_ARGS__); break;                       \
+    default: return fail(MPOL_EINVAL, "no compile-time FFT plan");   \
+  }
+
 template <typename T>
 const void* tw_ptr(mpol_ctx* c, int axis) {
   if constexpr (sizeof(T) == 8) return c->tw64[axis].p;
@@ -253,9 +274,9 @@ const void* tw_ptr(mpol_ctx* c, int axis) {
 }
 
 template <typename T>
-const void* fac_ptr(mpol_ctx* c) {
-  if constexpr (sizeof(T) == 8) return c->fac64.p;
-  else return c->fac32.p;
+const void* base_ptr(mpol_ctx* c) {
+  if constexpr (sizeof(T) == 8) return c->base64.p;
+  else return c->base32.p;
 }
 
 // out = A m (or its three parts when hr/hslab are given) in precision T.
@@ -291,6 +312,8 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   hipLaunchKernelGGL((k_real_space<T>), dim3(cdiv(n, 256), nrhs), dim3(256), 0, st, n, nrhs, c->nsets, m, out,
                      c->diag.p, c->rs_off.p, c->rs_j.p, pA, pBA, prh, c->zsum.p, slab / V, hr != nullptr ? 1 : 0, hr,
                      hslab);
+  HIPALLOC(c->ms, (size_t)nrhs * 3 * n);
+  hipLaunchKernelGGL(k_permute_m, dim3(cdiv(n, 256), nrhs), dim3(256), 0, st, m, c->perm.p, n, c->ms.p);
   if (ev) HIPCHK(hipEventRecord(ev[2], st));
 
   // tile (x axis) kernels
@@ -310,17 +333,43 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   TileArgs ta{};
   ta.nx = c->ng[0]; ta.ny = c->ng[1]; ta.nz = c->ng[2]; ta.kx = c->kx;
   ta.ty = tc.ty; ta.tz = tc.tz; ta.rows = tc.rows; ta.ncl = tc.ncl; ta.lstride = tc.lstride; ta.ch = tc.ch;
-  ta.P = c->P; ta.n = n; ta.nrhs = nrhs;
-  ta.perm = c->perm.p; ta.g0 = c->g0s.p; ta.fac = fac_ptr<T>(c); ta.bin_start = c->bin_start.p;
+  ta.P = c->P; ta.half = c->half; ta.n = n; ta.nrhs = nrhs;
+  ta.rec = c->rec.p; ta.base = base_ptr<T>(c); ta.ms = c->ms.p; ta.bin_start = c->bin_start.p;
+  for (int d = 0; d < 3; ++d) {
+    ta.h[d] = c->h[d];
+    ta.cexp[d] = 2.0 * c->xi * c->xi / c->eta[d];
+  }
+  ta.norm = c->norm;
   ta.tw = tw_ptr<T>(c, 0); ta.plan = c->plan[0];
   ta.hvol = c->h[0] * c->h[1] * c->h[2];
   const dim3 tgrid(cdiv(c->ng[1], tc.ty), cdiv(c->ng[2], tc.tz), nrhs);
   if (tc.lds > kLdsMax) return fail(MPOL_EINVAL, "spread/contract tile does not fit in LDS (reduce tile or chunk)");
 
+  const bool gather = is_ct_size(c->ng[0]) && c->P + 3 <= c->ng[0];
+  if (gather) {
+    TileCfg tg = tile_cfg(c->ng[0], false, cs, c->P, kLdsTile, true);
+    if (c->tune[0] > 0 && c->tune[1] > 0) {
+      tg.ty = c->tune[0]; tg.tz = c->tune[1]; tg.rows = tg.ty * tg.tz; tg.ncl = (3 * tg.rows + 1) / 2;
+    }
+    if (c->tune[2] > 0) tg.ch = c->tune[2];
+    tg.lds = ((size_t)c->ng[0] + (size_t)tg.ncl * tg.lstride) * cs + ((size_t)c->ng[0] + 1) * sizeof(int) +
+             (size_t)tg.ch * ((3 + (size_t)c->P + tg.rows) * sizeof(T) + 2 * sizeof(int));
+    if (tg.lds > kLdsMax) return fail(MPOL_EINVAL, "spread tile does not fit in LDS (reduce tile or chunk)");
+    TileArgs tga = ta;
+    tga.ty = tg.ty; tga.tz = tg.tz; tga.rows = tg.rows; tga.ncl = tg.ncl; tga.lstride = tg.lstride; tga.ch = tg.ch;
+    const dim3 ggrid(cdiv(c->ng[1], tg.ty), cdiv(c->ng[2], tg.tz), nrhs);
+#define LAUNCH_GATHER(N, ...)                                                                   \
+  do {                                                                                          \
+    set_lds(k_spread_gather_xfft<T, 0, N, 4>, tg.lds);                                          \
+    hipLaunchKernelGGL((k_spread_gather_xfft<T, 0, N, 4>), ggrid, dim3(256), tg.lds, st, tga, m, spec); \
+  } while (0)
+    DISPATCH_CT(c->ng[0], LAUNCH_GATHER, 0);
+#undef LAUNCH_GATHER
+  } else {
   TileArgs tsa = ta;
   tsa.ty = tca.ty; tsa.tz = tca.tz; tsa.rows = tca.rows; tsa.ncl = tca.ncl; tsa.lstride = tca.lstride;
   tsa.ch = tca.ch;
-  tsa.fac = c->fac64.p;
+  tsa.base = c->base64.p;
   tsa.tw = c->tw64[0].p;
   size_t slds = tca.lds;
   if (c->tune[0] > 0 && c->tune[1] > 0) {
@@ -337,6 +386,7 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
     hipLaunchKernelGGL((k_spread_xfft<TA, E, N, T>), sgrid, dim3(256), slds, st, tsa, m, spec); \
   } while (0)
   DISPATCH_FFT(c->ng[0], ta.plan.e, LAUNCH_SPREAD, 0);
+  }
   if (ev) HIPCHK(hipEventRecord(ev[3], st));
   if (stop_after <= 2) return MPOL_OK;
 
@@ -637,9 +687,9 @@ int mpol_set_geometry(mpol_ctx* c, const double* x, int64_t n, const double box[
   HIPALLOC(c->cell_members, (size_t)n);
   HIPALLOC(c->g0u, 3 * (size_t)n);
   HIPALLOC(c->perm, (size_t)n);
-  HIPALLOC(c->g0s, 3 * (size_t)n);
-  HIPALLOC(c->fac64, 3 * (size_t)n * P);
-  HIPALLOC(c->fac32, 3 * (size_t)n * P);
+  HIPALLOC(c->rec, (size_t)n);
+  HIPALLOC(c->base64, (size_t)n);
+  HIPALLOC(c->base32, (size_t)n);
   HIPALLOC(c->rs_count, (size_t)n);
   HIPALLOC(c->rs_off, (size_t)n + 1);
   HIPALLOC(c->err, 1);
@@ -664,14 +714,15 @@ int mpol_set_geometry(mpol_ctx* c, const double* x, int64_t n, const double box[
   peta = (eta[0] * eta[1]) * eta[2];
   g.xi = xi;
   g.norm = std::pow(2.0 * xi * xi / 3.14159265358979323846, 1.5) * std::sqrt(1.0 / peta);
+  c->norm = g.norm;
   const int pg = cdiv(n, 256);
   hipLaunchKernelGGL(k_geo_keys, dim3(pg), dim3(256), 0, st, g, c->x.p, c->bin_key.p, c->bin_count.p,
                      c->cell_key.p, c->cell_count.p, c->base.p, c->g0u.p);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, c->bin_count.p, c->bin_start.p, nbins);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, c->cell_count.p, c->cell_start.p, (int)ncells);
   hipLaunchKernelGGL(k_geo_scatter, dim3(pg), dim3(256), 0, st, g, c->bin_key.p, c->bin_start.p, c->bin_cursor.p,
-                     c->cell_key.p, c->cell_start.p, c->cell_cursor.p, c->base.p, c->g0u.p, c->perm.p, c->g0s.p,
-                     c->fac64.p, c->fac32.p, c->cell_members.p);
+                     c->cell_key.p, c->cell_start.p, c->cell_cursor.p, c->base.p, c->g0u.p, c->perm.p, c->rec.p,
+                     c->base64.p, c->base32.p, c->cell_members.p);
   PairParams pp{};
   pp.n = (int)n;
   for (int d = 0; d < 3; ++d) {

@@ -108,38 +108,40 @@ __global__ void k_scan(const int* __restrict__ in, int* __restrict__ out, int n)
   if (t == nt - 1) out[n] = part[t];
 }
 
-// Pass 2: scatter into bin order and write the separable stencil factors
-//   fac[s][d][o] = exp(-2 xi^2 (base_d + (o - half) h_d)^2 / eta_d)   (x row also * norm)
-template <typename T>
-__device__ inline void write_fac(T* dst, double v) { *dst = (T)v; }
-
+// Pass 2: scatter into bin order.  Per sorted particle s: rec[s] = (first
+// stencil node on x, y, z, caller's index) and base[s] = node0*h - x on each
+// axis (FP64 and FP32 copies), from which the kernels recompute the Gaussian
+// factors exp(-2 xi^2 (base_d + (o - half) h_d)^2 / eta_d).
 __global__ void k_geo_scatter(GeoParams g, const int* __restrict__ bin_key, const int* __restrict__ bin_start,
                               int* __restrict__ bin_cursor, const int* __restrict__ cell_key,
                               const int* __restrict__ cell_start, int* __restrict__ cell_cursor,
                               const double* __restrict__ base, const int* __restrict__ g0_unsorted,
-                              int* __restrict__ perm, int* __restrict__ g0s, double* __restrict__ fac64,
-                              float* __restrict__ fac32, int* __restrict__ cell_members) {
-#pragma clang fp contract(off)
+                              int* __restrict__ perm, int4* __restrict__ rec, double4* __restrict__ base64,
+                              float4* __restrict__ base32, int* __restrict__ cell_members) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= g.n) return;
   const int key = bin_key[i];
   const int s = bin_start[key] + atomicAdd(&bin_cursor[key], 1);
   perm[s] = i;
-  const int P = g.P;
-  const double two_xi2 = 2.0 * g.xi * g.xi;
-  for (int d = 0; d < 3; ++d) {
-    g0s[3 * s + d] = g0_unsorted[3 * i + d];
-    const double b = base[3 * i + d];
-    for (int o = 0; o < P; ++o) {
-      const double r = b + (double)(o - g.half) * g.h[d];
-      double w = exp(-two_xi2 * ((r * r) / g.eta[d]));
-      if (d == 0) w *= g.norm;
-      fac64[((size_t)s * 3 + d) * P + o] = w;
-      fac32[((size_t)s * 3 + d) * P + o] = (float)w;
-    }
-  }
+  rec[s] = make_int4(g0_unsorted[3 * i], g0_unsorted[3 * i + 1], g0_unsorted[3 * i + 2], i);
+  const double b0 = base[3 * i], b1 = base[3 * i + 1], b2 = base[3 * i + 2];
+  base64[s] = make_double4(b0, b1, b2, 0.0);
+  base32[s] = make_float4((float)b0, (float)b1, (float)b2, 0.0f);
   const int ck = cell_key[i];
   cell_members[cell_start[ck] + atomicAdd(&cell_cursor[ck], 1)] = i;
+}
+
+// ms[rhs][s] = m[rhs][perm[s]]: dipoles in stencil-bin order, once per matvec.
+__global__ void k_permute_m(const double* __restrict__ m, const int* __restrict__ perm, int n,
+                            double* __restrict__ ms) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int rhs = blockIdx.y;
+  if (s >= n) return;
+  const size_t o = (size_t)rhs * n;
+  const int i = perm[s];
+  ms[3 * (o + s)] = m[3 * (o + i)];
+  ms[3 * (o + s) + 1] = m[3 * (o + i) + 1];
+  ms[3 * (o + s) + 2] = m[3 * (o + i) + 2];
 }
 
 // ---------------------------------------------------------------------------
@@ -318,12 +320,16 @@ struct TileArgs {
   int lstride;       // complex elements per LDS line (padded)
   int ch;            // particles staged per chunk
   int P;
+  int half;          // (P-1)/2
   int n;             // particles
   int nrhs;
-  const int* perm;
-  const int* g0;
-  const void* fac;
+  const int4* rec;   // sorted: first stencil node x, y, z; caller's index
+  const void* base;  // sorted: node0*h - x per axis (double4 or float4)
+  const double* ms;  // sorted dipoles [nrhs][n][3]
   const int* bin_start;
+  double h[3];
+  double cexp[3];    // 2 xi^2 / eta_d
+  double norm;       // (2 xi^2/pi)^{3/2} / sqrt(prod eta)
   const void* tw;    // nx twiddles
   FftPlan plan;
   double hvol;       // prod(h), trapezoid factor of the contraction
@@ -368,24 +374,13 @@ __device__ inline void tile_ranges(const TileArgs& a, int ty0, int tz0, int* s_s
   __syncthreads();
 }
 
-// Sum of the stencil factors of one axis that land on grid row `g`
-// (more than one term only when the stencil is wider than the grid).
-template <typename T>
-__device__ inline T axis_weight(const T* f, int g, int g0, int n, int P) {
-  int o = g - g0;
-  if (o < 0) o += n;
-  T w = 0;
-  for (; o < P; o += n) w += f[o];
-  return w;
-}
-
 template <typename T>
 struct Stage {
   int* gx;     // [ch] first x node of the stencil
   int* orig;   // [ch] caller's particle index
   T* m;        // [ch][3] dipole (spread)
   T* fx;       // [ch][P] x factors
-  T* w;        // [ch][rows] y*z factor of each tile row
+  T* w;        // [rows][ch] y*z factor of each tile row
 };
 
 template <typename T>
@@ -400,16 +395,78 @@ __device__ inline Stage<T> stage_layout(unsigned char* base, int ch, int P, int 
   return st;
 }
 
-// Phase 1: thread t stages candidate base+t (if any).  Returns nothing; the
-// caller synchronises.
+template <typename T>
+struct Vec4Of;
+template <>
+struct Vec4Of<double> { using type = double4; };
+template <>
+struct Vec4Of<float> { using type = float4; };
+
+// Gaussian factor of stencil offset o on axis d (same expression as the
+// precompute of the reference weights).
+template <typename T>
+__device__ inline T stencil_factor(const TileArgs& a, int d, T b, int o) {
+  const T r = b + (T)(o - a.half) * (T)a.h[d];
+  return exp(-(T)a.cexp[d] * (r * r));
+}
+
+template <typename T>
+__device__ inline T axis_weight_fn(const TileArgs& a, int d, T b, int g, int g0, int n) {
+  int o = g - g0;
+  if (o < 0) o += n;
+  T w = 0;
+  for (; o < a.P; o += n) w += stencil_factor<T>(a, d, b, o);
+  return w;
+}
+
+// Locate candidate q of a tile: the sorted particle index.
+__device__ inline int candidate_particle(const int* s_start, const int* s_cum, int nr, int q) {
+  int lo = 0, hi = nr;  // largest r with s_cum[r] <= q
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (s_cum[mid] <= q) lo = mid;
+    else hi = mid;
+  }
+  return s_start[lo] + (q - s_cum[lo]);
+}
+
+// Write the staged data of sorted particle s (record rc, node offsets b) into
+// stage slot t.
+template <typename T, bool WITH_M, typename V4>
+__device__ inline void stage_at(const TileArgs& a, const Stage<T>& st, int t, int s, int4 rc, V4 b, int ty0, int tz0,
+                                int rhs) {
+  st.gx[t] = rc.x;
+  st.orig[t] = rc.w;
+  if (WITH_M) {
+    const double* mp = a.ms + 3 * ((size_t)rhs * a.n + s);
+    st.m[3 * t] = (T)mp[0];
+    st.m[3 * t + 1] = (T)mp[1];
+    st.m[3 * t + 2] = (T)mp[2];
+  }
+  for (int o = 0; o < a.P; ++o) st.fx[t * a.P + o] = (T)a.norm * stencil_factor<T>(a, 0, b.x, o);
+  T wy[4], wz[4];
+  for (int l = 0; l < a.ty; ++l) {
+    const int y = ty0 + l;
+    wy[l] = (y < a.ny) ? axis_weight_fn<T>(a, 1, b.y, y, rc.y, a.ny) : (T)0;
+  }
+  for (int l = 0; l < a.tz; ++l) {
+    const int z = tz0 + l;
+    wz[l] = (z < a.nz) ? axis_weight_fn<T>(a, 2, b.z, z, rc.z, a.nz) : (T)0;
+  }
+  for (int r = 0; r < a.rows; ++r) st.w[r * a.ch + t] = wy[r % a.ty] * wz[r / a.ty];
+}
+
+// Phase 1: thread t stages candidate base+t (if any): one 16-byte record, the
+// node offsets and (spread) the sorted dipole -- independent loads -- then
+// the P x factors and the weight of every tile row from the Gaussians.
 template <typename T, bool WITH_M>
 __device__ inline void stage_candidates(const TileArgs& a, const Stage<T>& st, int base, int total, const int* s_start,
-                                        const int* s_cum, int nr, int ty0, int tz0, const double* mr) {
+                                        const int* s_cum, int nr, int ty0, int tz0, int rhs) {
   const int t = threadIdx.x;
   if (t >= a.ch) return;
   const int q = base + t;
   if (q >= total) {
-    for (int r = 0; r < a.rows; ++r) st.w[t * a.rows + r] = (T)0;
+    for (int r = 0; r < a.rows; ++r) st.w[r * a.ch + t] = (T)0;
     st.orig[t] = -1;
     return;
   }
@@ -420,32 +477,29 @@ __device__ inline void stage_candidates(const TileArgs& a, const Stage<T>& st, i
     else hi = mid;
   }
   const int s = s_start[lo] + (q - s_cum[lo]);
-  const int gx0 = a.g0[3 * s], gy0 = a.g0[3 * s + 1], gz0 = a.g0[3 * s + 2];
-  const int orig = a.perm[s];
-  const T* fs = reinterpret_cast<const T*>(a.fac) + (size_t)s * 3 * a.P;
-  st.gx[t] = gx0;
-  st.orig[t] = orig;
+  const int4 rc = a.rec[s];
+  const typename Vec4Of<T>::type b = reinterpret_cast<const typename Vec4Of<T>::type*>(a.base)[s];
+  st.gx[t] = rc.x;
+  st.orig[t] = rc.w;
   if (WITH_M) {
-    st.m[3 * t] = (T)mr[3 * (size_t)orig];
-    st.m[3 * t + 1] = (T)mr[3 * (size_t)orig + 1];
-    st.m[3 * t + 2] = (T)mr[3 * (size_t)orig + 2];
+    const double* mp = a.ms + 3 * ((size_t)rhs * a.n + s);
+    st.m[3 * t] = (T)mp[0];
+    st.m[3 * t + 1] = (T)mp[1];
+    st.m[3 * t + 2] = (T)mp[2];
   }
-  for (int o = 0; o < a.P; ++o) st.fx[t * a.P + o] = fs[o];
+  for (int o = 0; o < a.P; ++o) st.fx[t * a.P + o] = (T)a.norm * stencil_factor<T>(a, 0, b.x, o);
   T wy[4], wz[4];
   for (int l = 0; l < a.ty; ++l) {
     const int y = ty0 + l;
-    wy[l] = (y < a.ny) ? axis_weight(fs + a.P, y, gy0, a.ny, a.P) : (T)0;
+    wy[l] = (y < a.ny) ? axis_weight_fn<T>(a, 1, b.y, y, rc.y, a.ny) : (T)0;
   }
   for (int l = 0; l < a.tz; ++l) {
     const int z = tz0 + l;
-    wz[l] = (z < a.nz) ? axis_weight(fs + 2 * a.P, z, gz0, a.nz, a.P) : (T)0;
+    wz[l] = (z < a.nz) ? axis_weight_fn<T>(a, 2, b.z, z, rc.z, a.nz) : (T)0;
   }
-  for (int r = 0; r < a.rows; ++r) st.w[t * a.rows + r] = wy[r % a.ty] * wz[r / a.ty];
+  for (int r = 0; r < a.rows; ++r) st.w[r * a.ch + t] = wy[r % a.ty] * wz[r / a.ty];
 }
 
-// T: accumulation and x-FFT precision; TS: precision of the stored spectrum.
-// The FP32 operator accumulates in FP64 (LDS FP64 atomics are the fast ones on
-// gfx950) and rounds once when it stores the spectrum.
 template <typename T, int E, int N, typename TS = T>
 __global__ void __launch_bounds__(256) k_spread_xfft(TileArgs a, const double* __restrict__ m,
                                                      cplx<TS>* __restrict__ spec) {
@@ -462,16 +516,15 @@ __global__ void __launch_bounds__(256) k_spread_xfft(TileArgs a, const double* _
   for (int k = threadIdx.x; k < a.ncl * a.lstride; k += blockDim.x) buf[k] = cmk<T>(0, 0);
   tile_ranges(a, ty0, tz0, s_start, s_cum, &s_nr);
   const int total = s_cum[s_nr];
-  const double* mr = m + (size_t)rhs * 3 * (size_t)a.n;
   const int P = a.P;
   for (int base = 0; base < total; base += a.ch) {
-    stage_candidates<T, true>(a, st, base, total, s_start, s_cum, s_nr, ty0, tz0, mr);
+    stage_candidates<T, true>(a, st, base, total, s_start, s_cum, s_nr, ty0, tz0, rhs);
     __syncthreads();
     const int nitems = min(a.ch, total - base) * a.rows;
     for (int item = threadIdx.x; item < nitems; item += blockDim.x) {
-      const T wyz = st.w[item];
-      if (wyz == (T)0) continue;
       const int cnd = item / a.rows, row = item - cnd * a.rows;
+      const T wyz = st.w[row * a.ch + cnd];
+      if (wyz == (T)0) continue;
       const T m0 = st.m[3 * cnd], m1 = st.m[3 * cnd + 1], m2 = st.m[3 * cnd + 2];
       const T* fx = st.fx + cnd * P;
       const int l0 = row, l1 = a.rows + row, l2 = 2 * a.rows + row;
@@ -527,6 +580,139 @@ __global__ void __launch_bounds__(256) k_spread_xfft(TileArgs a, const double* _
   }
 }
 
+// Exclusive prefix sum of a[0..n) in place (whole workgroup; part has
+// blockDim entries).
+__device__ inline void block_exclusive_scan(int* a, int n, int* part) {
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int chunk = (n + nt - 1) / nt;
+  const int lo = min(n, t * chunk), hi = min(n, lo + chunk);
+  int s = 0;
+  for (int k = lo; k < hi; ++k) s += a[k];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < nt; off <<= 1) {
+    const int v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = part[t] - s;
+  for (int k = lo; k < hi; ++k) {
+    const int c = a[k];
+    a[k] = run;
+    run += c;
+  }
+  __syncthreads();
+}
+
+// Spread + forward x-FFT without atomics: the staged particles of each chunk
+// are counting-sorted by stencil start gx, then every thread owns XS
+// consecutive nodes of one tile row and sums the particles whose x window
+// [gx, gx+P) covers them (a contiguous bucket range of the sorted order).
+// Needs P + XS - 1 <= nx.
+template <typename T, int E, int N, int XS>
+__global__ void __launch_bounds__(256) k_spread_gather_xfft(TileArgs a, const double* __restrict__ m,
+                                                            cplx<T>* __restrict__ spec) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ int s_start[kMaxRanges], s_cum[kMaxRanges + 1], s_nr;
+  __shared__ int s_part[256];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* buf = tw + a.nx;
+  cplx<T>* scratch = buf + a.ncl * a.lstride;
+  unsigned char* stage_base = reinterpret_cast<unsigned char*>(buf + a.ncl * a.lstride * (E == 0 && N == 0 ? 2 : 1));
+  const Stage<T> st = stage_layout<T>(stage_base, a.ch, a.P, a.rows);
+  int* cnt = st.orig + a.ch;     // nx + 1 bucket starts
+  const int ty0 = blockIdx.x * a.ty, tz0 = blockIdx.y * a.tz, rhs = blockIdx.z;
+  const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
+  for (int k = threadIdx.x; k < a.nx; k += blockDim.x) tw[k] = twg[k];
+  for (int k = threadIdx.x; k < a.ncl * a.lstride; k += blockDim.x) buf[k] = cmk<T>(0, 0);
+  tile_ranges(a, ty0, tz0, s_start, s_cum, &s_nr);
+  const int total = s_cum[s_nr];
+  const int P = a.P, nx = a.nx;
+  const int nseg = nx / XS;
+  const int nitems = a.rows * nseg;
+  using V4 = typename Vec4Of<T>::type;
+  for (int base = 0; base < total; base += a.ch) {
+    const int nc = min(a.ch, total - base);
+    // 1a: records into registers, bucket counts by stencil start gx
+    const int t = threadIdx.x;
+    int s = 0, rk = 0;
+    int4 rc = make_int4(0, 0, 0, 0);
+    V4 bv;
+    for (int k = threadIdx.x; k <= nx; k += blockDim.x) cnt[k] = 0;
+    __syncthreads();
+    if (t < nc) {
+      s = candidate_particle(s_start, s_cum, s_nr, base + t);
+      rc = a.rec[s];
+      bv = reinterpret_cast<const V4*>(a.base)[s];
+      rk = atomicAdd(&cnt[rc.x], 1);
+    }
+    __syncthreads();
+    block_exclusive_scan(cnt, nx + 1, s_part);
+    // 1b: stage straight into the sorted slot
+    if (t < nc) stage_at<T, true>(a, st, cnt[rc.x] + rk, s, rc, bv, ty0, tz0, rhs);
+    __syncthreads();
+    for (int item = threadIdx.x; item < nitems; item += blockDim.x) {
+      const int row = item / nseg, x0 = (item - row * nseg) * XS;
+      T acc[XS][3];
+#pragma unroll
+      for (int k = 0; k < XS; ++k) acc[k][0] = acc[k][1] = acc[k][2] = (T)0;
+      const int lo = x0 - P + 1, hi = x0 + XS;  // buckets [lo, hi)
+      int q0[2], q1[2];
+      if (lo >= 0) {
+        q0[0] = cnt[lo]; q1[0] = cnt[hi]; q0[1] = 0; q1[1] = 0;
+      } else {
+        q0[0] = cnt[lo + nx]; q1[0] = cnt[nx]; q0[1] = cnt[0]; q1[1] = cnt[hi];
+      }
+      for (int part = 0; part < 2; ++part) {
+        for (int c = q0[part]; c < q1[part]; ++c) {
+          const T wv = st.w[row * a.ch + c];
+          if (wv == (T)0) continue;
+          int o0 = x0 - st.gx[c];
+          if (o0 < 0) o0 += nx;
+          const T m0 = st.m[3 * c], m1 = st.m[3 * c + 1], m2 = st.m[3 * c + 2];
+          const T* fx = st.fx + c * P;
+#pragma unroll
+          for (int k = 0; k < XS; ++k) {
+            int o = o0 + k;
+            if (o >= nx) o -= nx;
+            if (o < P) {
+              const T f = wv * fx[o];
+              acc[k][0] += f * m0;
+              acc[k][1] += f * m1;
+              acc[k][2] += f * m2;
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int comp = 0; comp < 3; ++comp) {
+        const int l = comp * a.rows + row;
+        T* pl = reinterpret_cast<T*>(buf + (l >> 1) * a.lstride) + (l & 1);
+#pragma unroll
+        for (int k = 0; k < XS; ++k) pl[2 * pidx(x0 + k)] += acc[k][comp];
+      }
+    }
+    __syncthreads();
+  }
+  fft_any<T, E, N>(buf, a.lstride, a.ncl, a.plan, tw, false, scratch);
+  const int nreal = 3 * a.rows;
+  const int tot = nreal * a.kx;
+  for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+    const int l = idx / a.kx, k = idx - l * a.kx;
+    const int c = l / a.rows, row = l - c * a.rows;
+    const int y = ty0 + row % a.ty, z = tz0 + row / a.ty;
+    if (y >= a.ny || z >= a.nz) continue;
+    const cplx<T>* line = buf + (l >> 1) * a.lstride;
+    const cplx<T> Z = line[pidx(k)];
+    const cplx<T> Zc = cconj(line[pidx(k == 0 ? 0 : a.nx - k)]);
+    cplx<T> v;
+    if ((l & 1) == 0) v = cmk<T>((Z.x + Zc.x) * (T)0.5, (Z.y + Zc.y) * (T)0.5);
+    else v = cmk<T>((Z.y - Zc.y) * (T)0.5, (Zc.x - Z.x) * (T)0.5);
+    spec[(((size_t)(rhs * 3 + c) * a.nz + z) * a.ny + y) * a.kx + k] = v;
+  }
+}
+
 template <typename T, int E, int N>
 __global__ void __launch_bounds__(256) k_xifft_contract(TileArgs a, const cplx<T>* __restrict__ spec,
                                                         double* __restrict__ out) {
@@ -572,7 +758,7 @@ __global__ void __launch_bounds__(256) k_xifft_contract(TileArgs a, const cplx<T
   const int P = a.P;
   double* outr = out + (size_t)rhs * 3 * (size_t)a.n;
   for (int base = 0; base < total; base += a.ch) {
-    stage_candidates<T, false>(a, st, base, total, s_start, s_cum, s_nr, ty0, tz0, nullptr);
+    stage_candidates<T, false>(a, st, base, total, s_start, s_cum, s_nr, ty0, tz0, rhs);
     __syncthreads();
     const int nitems = min(a.ch, total - base) * a.rows;
     const int iters = (nitems + (int)blockDim.x - 1) / (int)blockDim.x;
@@ -581,7 +767,7 @@ __global__ void __launch_bounds__(256) k_xifft_contract(TileArgs a, const cplx<T
       T acc0 = 0, acc1 = 0, acc2 = 0;
       const int cnd = item / a.rows, row = item - cnd * a.rows;
       if (item < nitems) {
-        const T wyz = st.w[item];
+        const T wyz = st.w[row * a.ch + cnd];
         if (wyz != (T)0) {
           const T* fx = st.fx + cnd * P;
           const int l0 = row, l1 = a.rows + row, l2 = 2 * a.rows + row;
