@@ -202,6 +202,8 @@ struct mpol_ctx {
   DevBuf<double> rs_A, rs_BA, rs_rh;
   DevBuf<float> rs_A32, rs_BA32, rs_rh32;
   FftPlan plan[3];
+  DevBuf<double> ktab;      // e_x, e_y, e_z, then (cubic boxes) the Bessel factor table
+  bool cubic = false;
   DevBuf<double> tw64[3];
   DevBuf<float> tw32[3];
   // permittivity
@@ -345,7 +347,8 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   const dim3 tgrid(cdiv(c->ng[1], tc.ty), cdiv(c->ng[2], tc.tz), nrhs);
   if (tc.lds > kLdsMax) return fail(MPOL_EINVAL, "spread/contract tile does not fit in LDS (reduce tile or chunk)");
 
-  const bool gather = is_ct_size(c->ng[0]) && c->P + 3 <= c->ng[0];
+  // FP32: atomic-free gather spread; FP64: LDS FP64 atomics (faster there)
+  const bool gather = sizeof(T) == 4 && is_ct_size(c->ng[0]) && c->P + 3 <= c->ng[0];
   if (gather) {
     TileCfg tg = tile_cfg(c->ng[0], false, cs, c->P, kLdsTile, true);
     if (c->tune[0] > 0 && c->tune[1] > 0) {
@@ -420,6 +423,10 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   }
   sp.inv_4xi2 = 1.0 / (4.0 * c->xi * c->xi);
   sp.inv_g = 1.0 / ((double)c->ng[0] * (double)c->ng[1] * (double)c->ng[2]);
+  sp.ex = c->ktab.p;
+  sp.ey = sp.ex + c->ng[0];
+  sp.ez = sp.ey + c->ng[1];
+  sp.jt = c->cubic ? sp.ez + c->ng[2] : nullptr;
   const dim3 zgrid(cdiv(c->kx, cz.tx), c->ng[1], nrhs);
   if (cz.lds > kLdsMax) return fail(MPOL_EINVAL, "z-pass columns do not fit in LDS");
   // one thread group per line of the z transform, all 3*tx lines at once
@@ -767,6 +774,34 @@ int mpol_set_geometry(mpol_ctx* c, const double* x, int64_t n, const double box[
     HIPALLOC(c->tw32[d], tw32.size());
     HIPCHK(hipMemcpyAsync(c->tw64[d].p, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(c->tw32[d].p, tw32.data(), tw32.size() * sizeof(float), hipMemcpyHostToDevice, st));
+  }
+  // k-space coefficient tables
+  {
+    c->cubic = (box[0] == box[1] && box[1] == box[2]);
+    std::vector<double> kt;
+    for (int d = 0; d < 3; ++d) {
+      for (int i = 0; i < c->ng[d]; ++i) {
+        const double k = fft_freq(i, c->ng[d]) * (2.0 * kPi / box[d]);
+        kt.push_back(std::exp(-(1.0 - eta[d]) * k * k / (4.0 * xi * xi)));
+      }
+    }
+    if (c->cubic) {
+      int64_t nmax = 0;
+      for (int d = 0; d < 3; ++d) nmax += (int64_t)(c->ng[d] / 2) * (c->ng[d] / 2);
+      const double dk = 2.0 * kPi / box[0];
+      for (int64_t q = 0; q <= nmax; ++q) {
+        if (q == 0) {
+          kt.push_back(0.0);
+          continue;
+        }
+        const double kk = dk * dk * (double)q;
+        const double kn = std::sqrt(kk);
+        const double j = std::sqrt(2.0 / (kPi * kn)) * (std::sin(kn) / kn - std::cos(kn));
+        kt.push_back(9.0 * kPi / (2.0 * kn) * (j * j) / kk);
+      }
+    }
+    HIPALLOC(c->ktab, kt.size());
+    HIPCHK(hipMemcpyAsync(c->ktab.p, kt.data(), kt.size() * sizeof(double), hipMemcpyHostToDevice, st));
   }
   HIPCHK(hipStreamSynchronize(st));
   HIPCHK(hipGetLastError());

@@ -861,6 +861,13 @@ struct ScaleParams {
   double one_minus_eta[3];
   double inv_4xi2;
   double inv_g;     // 1/(nx ny nz): inverse-FFT normalisation folded into the scale
+  // Separable tables (natural FFT index): e_d[i] = exp(-(1-eta_d) k_d^2/(4 xi^2)).
+  const double* ex;
+  const double* ey;
+  const double* ez;
+  // Cubic boxes: jt[mx^2+my^2+mz^2] = 9 pi/(2|k|) J_{3/2}(|k|)^2/|k|^2 (0 at k=0);
+  // null for other boxes (then computed per k).
+  const double* jt;
 };
 
 // c(k) = 9 pi/(2|k|) J_{3/2}(|k|)^2 exp(-sum (1-eta_d) k_d^2 / (4 xi^2)) / |k|^2
@@ -908,7 +915,9 @@ __global__ void __launch_bounds__(1024) k_fft_z_scale(ColArgs a, ScaleParams sp,
     const bool nyq_x = (a.nx % 2 == 0) && (2 * mx == -a.nx);
     const bool nyq_z = (a.nz % 2 == 0) && (2 * mz == -a.nz);
     const double k0 = mx * sp.twopi_over_l[0], k1 = my * sp.twopi_over_l[1], k2 = mz * sp.twopi_over_l[2];
-    const double cf = kspace_coef(sp, k0, k1, k2) * sp.inv_g;
+    double cf;
+    if (sp.jt) cf = sp.jt[mx * mx + my * my + mz * mz] * (sp.ex[kx] * sp.ey[y] * sp.ez[z]) * sp.inv_g;
+    else cf = kspace_coef(sp, k0, k1, k2) * sp.inv_g;
     cplx<T>* f0 = buf + (0 * a.tx + cc) * a.lstride + pidx(z);
     cplx<T>* f1 = buf + (1 * a.tx + cc) * a.lstride + pidx(z);
     cplx<T>* f2 = buf + (2 * a.tx + cc) * a.lstride + pidx(z);

@@ -135,3 +135,30 @@ def test_geometry_errors(gpu):
                        int(d["P"]) * (h * float(d["xi"])) ** 2 / np.pi, float(d["rc"]))
     with pytest.raises(Exception):
         c.matvec(np.zeros((1, 64, 3)))
+
+
+def test_operator_noncubic_box(gpu):
+    """Orthorhombic box (per-k coefficient path) against the oracle."""
+    from mpol_hip import Context
+    rng = np.random.default_rng(21)
+    box = np.array([14.0, 12.5, 16.0])
+    pts = []
+    while len(pts) < 60:
+        c = rng.random(3) * box
+        if all(np.sum(((c - p) - box * np.rint((c - p) / box)) ** 2) >= 4.0 for p in pts):
+            pts.append(c)
+    x = np.array(pts)
+    xi, P, ng = 0.9, 8, np.array([32, 30, 36])
+    rc = 5.0
+    perp, para, rv = tables(xi)
+    eps = 2.0 + 10 * rng.random(60)
+    c = Context()
+    c.set_tables(perp, para, rv)
+    h = box / ng
+    c.set_geometry(x, box, ng, P, xi, P * (h * xi) ** 2 / np.pi, rc)
+    c.set_permittivity(eps)
+    m = rng.standard_normal((60, 3))
+    out = c.matvec(m[None], slab_factor=1.0).cpu().numpy()[0]
+    geo = eo.OperatorGeometry(x, box, ng, P, xi, rc, perp, para, rv)
+    ref = np.real(eo.field(geo, m, eps, 1.0))
+    assert relerr(out, ref) < 1e-12
