@@ -160,6 +160,22 @@ int mpol_geometry_info(mpol_ctx* ctx, mpol_geometry_summary* info);
  * ("CPU - Ewald test/ewald.py":604-931). */
 int mpol_real_space_table(double xi, const double* r_host, int64_t n, double* out_host);
 
+/* Truncated point-dipole forces and per-particle energies of fixed dipoles:
+ * pos, dip (n x 3) and the outputs forces (n x 3), pe (n) are FP64 device
+ * arrays; box (3 lengths) is host memory.  x and y are periodic by minimum
+ * image (rint), z when periodic_z != 0; pairs with r^2 > r_cut^2 or
+ * r^2 < 1e-24 are skipped; each particle receives half of every pair energy.
+ * use_cell_list selects the cell-list neighbour search (cells of size
+ * L / floor(L / r_cut)) over the all-pairs loop; both give the same pairs.
+ * Rejects the arguments the reference rejects, with its messages.  Only the
+ * context's device, stream and cell-list buffers are used (no tables or
+ * geometry needed).  Replaces compute_forces
+ * ("CPU - HOOMD simulation test/dipole_forces_ext/_dipole_forces_ext.cpp":119-497,
+ * binding at :454-497). */
+int mpol_compute_forces(mpol_ctx* ctx, const double* pos_dev, const double* dip_dev, int64_t n,
+                        const double box_host[3], double r_cut, double prefactor, int32_t periodic_z,
+                        int32_t use_cell_list, double* forces_dev, double* pe_dev);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include "mpol_forces.h"
 #include "mpol_gmres.h"
 #include "mpol_kernels.h"
 
@@ -218,6 +219,8 @@ struct mpol_ctx {
   // gmres
   DevBuf<double> gV, gW, gR, gAx, gH, gcs, gsn, gg, gy, gh1, gh2, gpart, gbn, gbeta, grel, grn2, gx, gd;
   DevBuf<int> gstate, gkdim;
+  // truncated fixed-dipole forces: cell list
+  DevBuf<int> f_cell, f_count, f_start, f_cursor, f_members;
 };
 
 namespace {
@@ -1006,6 +1009,57 @@ int mpol_real_space_table(double xi, const double* r, int64_t n, double* out) {
   for (int64_t i = 0; i < n; ++i)
     if (!(r[i] > 0.0)) return fail(MPOL_EINVAL, "table separations must be positive");
   real_space_tables(xi, r, n, out);
+  return MPOL_OK;
+}
+
+int mpol_compute_forces(mpol_ctx* c, const double* pos, const double* dip, int64_t n, const double box[3],
+                        double r_cut, double prefactor, int32_t periodic_z, int32_t use_cell_list, double* forces,
+                        double* pe) {
+  if (!c || !box) return fail(MPOL_EINVAL, "null argument");
+  if (n < 0 || n > (1LL << 30)) return fail(MPOL_EINVAL, "Invalid shapes: pos(N,3), dip(N,3), box(3) required.");
+  if (n > 0 && (!pos || !dip || !forces || !pe)) return fail(MPOL_EINVAL, "null argument");
+  if (!(box[0] > 0.0 && box[1] > 0.0 && box[2] > 0.0)) return fail(MPOL_EINVAL, "Box lengths must be positive.");
+  if (!(r_cut > 0.0)) return fail(MPOL_EINVAL, "r_cut must be > 0.");
+  if (r_cut >= std::min(0.5 * box[0], 0.5 * box[1]))
+    return fail(MPOL_EINVAL, "r_cut too large for minimum-image periodic x,y. Require r_cut < 0.5*min(Lx,Ly).");
+  if (periodic_z && r_cut >= 0.5 * box[2])
+    return fail(MPOL_EINVAL,
+                "r_cut too large for minimum-image periodic z. Require r_cut < 0.5*Lz when periodic_z=True.");
+  if (n == 0) return MPOL_OK;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = c->stream;
+  ForceParams p{};
+  p.n = (int)n;
+  for (int d = 0; d < 3; ++d) {
+    p.box[d] = box[d];
+    // cells no smaller than r_cut, so every neighbour within r_cut is in an adjacent cell
+    const int nc = (int)std::floor(box[d] / r_cut);
+    p.nc[d] = nc < 1 ? 1 : nc;
+  }
+  p.rc2 = r_cut * r_cut;
+  p.pref = prefactor;
+  p.periodic_z = periodic_z ? 1 : 0;
+  const int64_t ncells = (int64_t)p.nc[0] * p.nc[1] * p.nc[2];
+  // very dilute systems with tiny r_cut: fall back to the all-pairs loop
+  // rather than allocate more cells than a few per particle
+  p.use_cells = (use_cell_list && ncells <= std::max<int64_t>(1024, 8 * n) && ncells < (1LL << 30)) ? 1 : 0;
+  const int blocks = cdiv(n, 256);
+  if (p.use_cells) {
+    HIPALLOC(c->f_cell, (size_t)n);
+    HIPALLOC(c->f_members, (size_t)n);
+    HIPALLOC(c->f_count, (size_t)ncells);
+    HIPALLOC(c->f_cursor, (size_t)ncells);
+    HIPALLOC(c->f_start, (size_t)ncells + 1);
+    HIPCHK(hipMemsetAsync(c->f_count.p, 0, ncells * sizeof(int), st));
+    HIPCHK(hipMemsetAsync(c->f_cursor.p, 0, ncells * sizeof(int), st));
+    hipLaunchKernelGGL(k_force_cells, dim3(blocks), dim3(256), 0, st, p, pos, c->f_cell.p, c->f_count.p);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, c->f_count.p, c->f_start.p, (int)ncells);
+    hipLaunchKernelGGL(k_force_scatter, dim3(blocks), dim3(256), 0, st, (int)n, c->f_cell.p, c->f_start.p,
+                       c->f_cursor.p, c->f_members.p);
+  }
+  hipLaunchKernelGGL(k_forces, dim3(blocks), dim3(256), 0, st, p, pos, dip, c->f_cell.p, c->f_start.p,
+                     c->f_members.p, forces, pe);
+  HIPCHK(hipGetLastError());
   return MPOL_OK;
 }
 

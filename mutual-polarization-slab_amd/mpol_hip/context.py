@@ -220,3 +220,23 @@ class Context:
                                         ctypes.byref(info)))
         return SolveResult(x, info.matvecs_fp64, info.matvecs_fp32, info.converged, info.refinements,
                            info.max_relres)
+
+    # -- fixed-dipole forces ----------------------------------------------
+    def compute_forces(self, pos, dip, box, r_cut, prefactor=1.0, periodic_z=True, use_cell_list=True):
+        """Truncated dipole-dipole forces (N, 3) and per-particle energies (N,)
+        as FP64 device tensors.  Needs no tables or geometry."""
+        pt = self._dev(pos)
+        dt = self._dev(dip)
+        box = np.ascontiguousarray(np.asarray(box, dtype=np.float64).reshape(-1))
+        n = pt.shape[0] if pt.dim() == 2 else -1
+        if pt.dim() != 2 or pt.shape[1] != 3 or dt.shape != pt.shape or box.size != 3:
+            raise _lib.MpolError("Invalid shapes: pos(N,3), dip(N,3), box(3) required.")
+        f = torch.empty((n, 3), dtype=torch.float64, device=self.device)
+        pe = torch.empty((n,), dtype=torch.float64, device=self.device)
+        self._sync_stream()
+        _lib.check(self._lib.mpol_compute_forces(self._ctx, ctypes.c_void_p(pt.data_ptr()),
+                                                 ctypes.c_void_p(dt.data_ptr()), n, _lib.dptr(box), float(r_cut),
+                                                 float(prefactor), 1 if periodic_z else 0,
+                                                 1 if use_cell_list else 0, ctypes.c_void_p(f.data_ptr()),
+                                                 ctypes.c_void_p(pe.data_ptr())))
+        return f, pe

@@ -28,9 +28,10 @@ def _validate(pos, dip, box, r_cut, periodic_z):
     if not r_cut > 0.0:
         raise RuntimeError("r_cut must be > 0.")
     if r_cut >= min(0.5 * box[0], 0.5 * box[1]):
-        raise RuntimeError("r_cut too large for minimum-image periodic x,y.")
+        raise RuntimeError("r_cut too large for minimum-image periodic x,y. Require r_cut < 0.5*min(Lx,Ly).")
     if periodic_z and r_cut >= 0.5 * box[2]:
-        raise RuntimeError("r_cut too large for minimum-image periodic z.")
+        raise RuntimeError("r_cut too large for minimum-image periodic z. Require r_cut < 0.5*Lz when "
+                           "periodic_z=True.")
     return pos, dip, box
 
 
