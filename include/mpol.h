@@ -11,6 +11,14 @@
  *   dipoles/fields m[nrhs][N][3]   FP64, device -- per right-hand side the
  *                                  3N interleaved vector [p1x p1y p1z p2x ...]
  *                                  of the reference solver.
+ *   dipole + quadrupole moments m[nrhs][N][8] = (S_x, S_y, S_z, Q_xx, Q_xy,
+ *                                  Q_xz, Q_yy, Q_yz) per particle; the
+ *                                  operator returns (E_x, E_y, E_z, G_xx, G_xy,
+ *                                  G_xz, G_yy, G_yz) in the same layout.
+ *   complex permittivities         (after mpol_set_permittivity_complex) every
+ *                                  right-hand side is a complex vector stored
+ *                                  as its real part followed by its imaginary
+ *                                  part: m[nrhs][2][N][C], C = 3 or 8.
  *
  * A context launches on the caller's HIP stream and owns the geometry
  * precompute and all workspaces.  A context is not re-entrant: calls on one
@@ -107,6 +115,17 @@ int mpol_set_geometry(mpol_ctx* ctx, const double* x_dev, int64_t n, const doubl
  * (nsets == nrhs) or to every rhs (nsets == 1). */
 int mpol_set_permittivity(mpol_ctx* ctx, const double* eps_dev, int32_t nsets);
 
+/* Complex particle permittivities eps_re + i eps_im, nsets x N each (device
+ * FP64).  From then on every right-hand side of the operator and the solves
+ * is complex (layout above), as in the MATLAB spectrum drivers, which take
+ * the dielectric function at each frequency
+ * (Capacitance_Hetero/CapacitanceTensorSpectrum.m:16-62,
+ * Capacitance_Quadrupole/CapacitanceTensorSpectrum.m:1-64).  The operator off
+ * the diagonal is real, so it acts on both parts; the diagonal couples them.
+ * mpol_set_permittivity returns to real right-hand sides. */
+int mpol_set_permittivity_complex(mpol_ctx* ctx, const double* eps_re_dev, const double* eps_im_dev,
+                                  int32_t nsets);
+
 /* out = A m for nrhs right-hand sides: wave-space + real-space + slab terms.
  * Replaces magnetic_field ("CPU - Ewald test/ewald.py":1146-1236). */
 int mpol_matvec(mpol_ctx* ctx, const double* m_dev, double* out_dev, int32_t nrhs, int32_t precision,
@@ -143,6 +162,32 @@ int mpol_matvec_components(mpol_ctx* ctx, const double* m_dev, double* hk_dev, d
 int mpol_solve(mpol_ctx* ctx, const double* b_dev, const double* x0_dev, double* x_dev, int32_t nrhs,
                const mpol_solve_opts* opts, mpol_solve_info* info);
 
+/* Dipole + quadrupole operator (8 unknowns per particle).  The seven
+ * quadrupole tables (host FP64, 7 x ntab, on the rvals of mpol_set_tables, in
+ * the order field_quad_1, field_quad_2, field_quad_3, grad_quad_1,
+ * grad_quad_2, grad_quad_3, grad_quad_4, entry 0 the self term) complete the
+ * field_dip_1 / field_dip_2 pair given to mpol_set_tables.
+ * Replaces the table arguments of MatrixVectorMultiply
+ * (Capacitance_Quadrupole/MatrixVectorMultiply.m:1-3). */
+int mpol_set_quad_tables(mpol_ctx* ctx, const double* qtab_host, int64_t ntab);
+
+/* out = A m, the fields E and gradients G at the particles from their dipoles
+ * S and quadrupoles Q: wave space (four k-space blocks) + real space (nine
+ * tables, dielectric terms 3/(4 pi (eps-1)) and 15/(8 pi (eps-1))).
+ * Replaces MatrixVectorMultiply (Capacitance_Quadrupole/MatrixVectorMultiply.m:31-77). */
+int mpol_quad_matvec(mpol_ctx* ctx, const double* m_dev, double* out_dev, int32_t nrhs, int32_t precision);
+
+/* Wave-space part hk and real-space part hr separately (A m = hk + hr); no
+ * counterpart in the reference (used to check the Ewald splitting). */
+int mpol_quad_matvec_components(mpol_ctx* ctx, const double* m_dev, double* hk_dev, double* hr_dev,
+                                int32_t nrhs, int32_t precision);
+
+/* Restarted GMRES for the dipole + quadrupole system (b = [E0; 0] per
+ * particle for the reference's problem).  opts.slab_factor is ignored.
+ * Replaces MatrixVectorSolve (Capacitance_Quadrupole/MatrixVectorSolve.m:35-43). */
+int mpol_quad_solve(mpol_ctx* ctx, const double* b_dev, const double* x0_dev, double* x_dev, int32_t nrhs,
+                    const mpol_solve_opts* opts, mpol_solve_info* info);
+
 /* Launch-shape tuning (no counterpart in the reference; used by the
  * autotuner): knobs[0..5] = spread/contract tile rows along y and z (ty*tz a
  * power of two, each <= 4), particles staged per chunk (32..256), y-pass and
@@ -159,6 +204,13 @@ int mpol_geometry_info(mpol_ctx* ctx, mpol_geometry_summary* info);
  * field_dip_force_2.  Replaces real_space_table
  * ("CPU - Ewald test/ewald.py":604-931). */
 int mpol_real_space_table(double xi, const double* r_host, int64_t n, double* out_host);
+
+/* The eight couplings the quadrupole method adds, at separations r (host, n
+ * values > 0), each with its r = 0 self term first: out (host) is 8 x (n+1)
+ * in the order pot_quad, field_quad_1, field_quad_2, field_quad_3,
+ * grad_quad_1, grad_quad_2, grad_quad_3, grad_quad_4.  Replaces those outputs
+ * of RealSpaceTable (Capacitance_Quadrupole/RealSpaceTable.m:1-380). */
+int mpol_real_space_table_quad(double xi, const double* r_host, int64_t n, double* out_host);
 
 /* Truncated point-dipole forces and per-particle energies of fixed dipoles:
  * pos, dip (n x 3) and the outputs forces (n x 3), pe (n) are FP64 device

@@ -256,10 +256,13 @@ def capacitance_tensor_spectrum(x, box, eps_p, xi, errortol: float = 1e-3, slab_
     Each rhs starts from the Maxwell-Garnett guess of its own field; the
     reference warm-starts directions 2 and 3 from the permuted previous
     solution (Capacitance_Tensor.m:57), which changes only the iterate path,
-    not the converged solution."""
+    not the converged solution.  eps_p may be complex (a dielectric function,
+    as the MATLAB drivers allow): each rhs is then one complex system."""
     x = np.asarray(x, dtype=np.float64)
     box = np.asarray(box, dtype=np.float64).reshape(3)
-    eps = np.asarray(eps_p, dtype=np.float64)
+    eps = np.asarray(eps_p)
+    cplx = np.iscomplexobj(eps) and bool(np.any(np.imag(eps) != 0))
+    eps = eps.astype(np.complex128) if cplx else np.real(eps).astype(np.float64)
     if eps.ndim == 1:
         eps = eps[:, None]
     n, nk = eps.shape
@@ -273,17 +276,26 @@ def capacitance_tensor_spectrum(x, box, eps_p, xi, errortol: float = 1e-3, slab_
     eps_sets = np.repeat(eps.T, 3, axis=0)  # (3*nk, n)
     ctx.set_permittivity(eps_sets)
     ctx._eps_key = None
-    b = np.zeros((3 * nk, n, 3))
-    x0 = np.zeros((3 * nk, n, 3))
+    dt = np.complex128 if cplx else np.float64
+    b = np.zeros((3 * nk, n, 3), dtype=dt)
+    x0 = np.zeros((3 * nk, n, 3), dtype=dt)
     for k in range(nk):
+        beta = (eps[:, k] - 1.0) / (eps[:, k] + 2.0)
+        mg = 4.0 * np.pi * beta / (1.0 - beta * phi)
         for d in range(3):
-            e = np.zeros(3)
-            e[d] = 1.0
-            b[3 * k + d] = e[None, :]
-            x0[3 * k + d] = maxwell_garnett_guess(eps[:, k], phi, e)
+            b[3 * k + d, :, d] = 1.0
+            x0[3 * k + d, :, d] = mg
     restart = min(3 * n, 10)
     maxit = min(3 * n, 100)
-    sol, info = _solve_on(ctx, b, x0, errortol, restart, maxit, slab_factor, precision)
+    if cplx:
+        # complex permittivities couple the real and imaginary parts: one complex system per rhs
+        mixed = str(precision).lower() in ("mixed", "fp32")
+        res = ctx.solve(b, x0, errortol, restart=restart, maxit=maxit, mixed=mixed, slab_factor=slab_factor)
+        sol = res.x.detach().cpu().numpy()
+        info = {"flag": res.flag, "relres": res.max_relres, "matvecs_fp64": res.matvecs_fp64,
+                "matvecs_fp32": res.matvecs_fp32, "refinements": res.refinements}
+    else:
+        sol, info = _solve_on(ctx, b, x0, errortol, restart, maxit, slab_factor, precision)
     p = sol.reshape(nk, 3, n, 3).transpose(2, 3, 1, 0)  # (n, comp, dir, k)
     C = p.mean(axis=0).transpose(1, 0, 2)                # (dir, comp, k)
     return (C, p, info) if return_info else (C, p)

@@ -18,6 +18,7 @@
 
 namespace mpol {
 void real_space_tables(double xi, const double* r, int64_t n, double* out);
+void real_space_tables_quad(double xi, const double* r, int64_t n, double* out);
 }
 
 using namespace mpol;
@@ -133,28 +134,6 @@ struct TileCfg {
   size_t lds = 0;
 };
 
-// Largest tile (rows of nx nodes) and particle chunk that fit the LDS budget.
-TileCfg tile_cfg(int nx, bool generic, size_t csize, int P, size_t budget, bool sort_arrays = false) {
-  const int opts[][2] = {{4, 4}, {4, 2}, {2, 2}, {2, 1}, {1, 1}};
-  const size_t rsize = csize / 2;
-  TileCfg c;
-  for (const auto& o : opts) {
-    c.ty = o[0];
-    c.tz = o[1];
-    c.rows = c.ty * c.tz;
-    c.ncl = (3 * c.rows + 1) / 2;
-    c.lstride = padded_len(nx);
-    const size_t grid = ((size_t)nx + (size_t)c.ncl * c.lstride * (generic ? 2 : 1)) * csize;
-    const size_t per = (3 + (size_t)P + c.rows) * rsize + 2 * sizeof(int);
-    const size_t fixed = sort_arrays ? ((size_t)nx + 1) * sizeof(int) : 0;
-    for (c.ch = 256; c.ch >= 64; c.ch /= 2) {
-      c.lds = grid + fixed + c.ch * per;
-      if (c.lds <= budget) return c;
-    }
-  }
-  return c;
-}
-
 struct ColCfg {
   int tx = 1, lstride = 1;
   size_t lds = 0;
@@ -207,9 +186,21 @@ struct mpol_ctx {
   bool cubic = false;
   DevBuf<double> tw64[3];
   DevBuf<float> tw32[3];
-  // permittivity
+  // permittivity: diagonal of the dipole rows (and, when quadrupole tables are
+  // set, of the quadrupole rows); imaginary parts when complex
   int nsets = 0;
-  DevBuf<double> diag;
+  bool cplx_eps = false;
+  DevBuf<double> eps_re, eps_im;
+  DevBuf<double> diag, diag_im, diag_q, diag_qim;
+  bool diag_q_valid = false;
+  // dipole + quadrupole operator: 7 tables on the rvals grid, per-pair coefficients
+  bool have_qtab = false;
+  double self_gq2 = 0.0;
+  DevBuf<double> qtab;
+  bool quad_pairs_valid = false;
+  DevBuf<double> rs_Q;
+  DevBuf<float> rs_Q32;
+  int64_t jq_off = -1;      // offset of the quadrupole Bessel table in ktab (cubic boxes)
   // workspaces
   DevBuf<unsigned char> spec;
   DevBuf<double> zsum;
@@ -284,16 +275,62 @@ const void* base_ptr(mpol_ctx* c) {
   else return c->base32.p;
 }
 
-// out = A m (or its three parts when hr/hslab are given) in precision T.
-// Stage boundaries recorded when `ev` is given (kStages + 1 events).
+// out = A m (or its parts when hr/hslab are given) in precision T, for NC = 3
+// (dipole operator: hr <- real space, hslab <- slab term, out <- wave space)
+// or NC = 8 (dipole + quadrupole operator: hr <- real space, out <- wave
+// space).  Stage boundaries recorded when `ev` is given (kStages + 1 events).
 constexpr int kStages = 7;
 constexpr int kZsumBlocks = 64;
+enum { kOpDipole = 3, kOpQuad = 8 };
 
-template <typename T>
+// Largest tile (rows of nx nodes, NC components) and particle chunk that fit
+// the LDS budget.
+template <int NC>
+TileCfg tile_cfg_nc(int nx, bool generic, size_t csize, int P, size_t budget, bool sort_arrays = false) {
+  const int opts[][2] = {{4, 4}, {4, 2}, {2, 2}, {2, 1}, {1, 1}};
+  const size_t rsize = csize / 2;
+  TileCfg c;
+  for (const auto& o : opts) {
+    c.ty = o[0];
+    c.tz = o[1];
+    c.rows = c.ty * c.tz;
+    c.ncl = (NC * c.rows + 1) / 2;
+    c.lstride = padded_len(nx);
+    const size_t grid = ((size_t)nx + (size_t)c.ncl * c.lstride * (generic ? 2 : 1)) * csize;
+    const size_t per = (NC + (size_t)P + c.rows) * rsize + 2 * sizeof(int);
+    const size_t fixed = sort_arrays ? ((size_t)nx + 1) * sizeof(int) : 0;
+    for (c.ch = 256; c.ch >= 64; c.ch /= 2) {
+      c.lds = grid + fixed + c.ch * per;
+      if (c.lds <= budget) return c;
+    }
+  }
+  return c;
+}
+
+// LDS bytes of a tile kernel with the given shape.
+template <int NC>
+size_t tile_lds(int nx, bool generic, size_t csize, int P, const TileCfg& t, bool sort_arrays) {
+  return ((size_t)nx + (size_t)t.ncl * t.lstride * (generic ? 2 : 1)) * csize +
+         (sort_arrays ? ((size_t)nx + 1) * sizeof(int) : 0) +
+         (size_t)t.ch * ((NC + (size_t)P + t.rows) * (csize / 2) + 2 * sizeof(int));
+}
+
+template <int NC>
+void apply_tile_tuning(const mpol_ctx* c, TileCfg& t) {
+  if (c->tune[0] > 0 && c->tune[1] > 0) {
+    t.ty = c->tune[0];
+    t.tz = c->tune[1];
+    t.rows = t.ty * t.tz;
+    t.ncl = (NC * t.rows + 1) / 2;
+  }
+  if (c->tune[2] > 0) t.ch = c->tune[2];
+}
+
+template <typename T, int NC>
 int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* hslab, int nrhs, double slab,
                  hipEvent_t* ev = nullptr, int stop_after = kStages) {
   const size_t cs = sizeof(cplx<T>);
-  const size_t spec_bytes = (size_t)3 * nrhs * c->ng[2] * c->ng[1] * c->kx * cs;
+  const size_t spec_bytes = (size_t)NC * nrhs * c->ng[2] * c->ng[1] * c->kx * cs;
   HIPALLOC(c->spec, spec_bytes);
   HIPALLOC(c->zsum, (size_t)nrhs * (kZsumBlocks + 1));
   cplx<T>* spec = reinterpret_cast<cplx<T>*>(c->spec.p);
@@ -301,11 +338,6 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   hipStream_t st = c->stream;
 
   if (ev) HIPCHK(hipEventRecord(ev[0], st));
-  double* zpart = c->zsum.p + nrhs;
-  hipLaunchKernelGGL(k_sum_z, dim3(kZsumBlocks, nrhs), dim3(256), 0, st, m, n, zpart);
-  hipLaunchKernelGGL(k_sum_finish, dim3(cdiv(nrhs, 256)), dim3(256), 0, st, zpart, kZsumBlocks, nrhs, c->zsum.p);
-  if (ev) HIPCHK(hipEventRecord(ev[1], st));
-  const double V = c->box[0] * c->box[1] * c->box[2];
   const T* pA;
   const T* pBA;
   const T* prh;
@@ -314,27 +346,34 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   } else {
     pA = c->rs_A32.p; pBA = c->rs_BA32.p; prh = c->rs_rh32.p;
   }
-  hipLaunchKernelGGL((k_real_space<T>), dim3(cdiv(n, 256), nrhs), dim3(256), 0, st, n, nrhs, c->nsets, m, out,
-                     c->diag.p, c->rs_off.p, c->rs_j.p, pA, pBA, prh, c->zsum.p, slab / V, hr != nullptr ? 1 : 0, hr,
-                     hslab);
-  HIPALLOC(c->ms, (size_t)nrhs * 3 * n);
-  hipLaunchKernelGGL(k_permute_m, dim3(cdiv(n, 256), nrhs), dim3(256), 0, st, m, c->perm.p, n, c->ms.p);
+  if constexpr (NC == kOpDipole) {
+    double* zpart = c->zsum.p + nrhs;
+    hipLaunchKernelGGL(k_sum_z, dim3(kZsumBlocks, nrhs), dim3(256), 0, st, m, n, zpart);
+    hipLaunchKernelGGL(k_sum_finish, dim3(cdiv(nrhs, 256)), dim3(256), 0, st, zpart, kZsumBlocks, nrhs, c->zsum.p);
+    if (ev) HIPCHK(hipEventRecord(ev[1], st));
+    const double V = c->box[0] * c->box[1] * c->box[2];
+    hipLaunchKernelGGL((k_real_space<T>), dim3(cdiv(n, 256), nrhs), dim3(256), 0, st, n, nrhs, c->nsets, m, out,
+                       c->diag.p, c->cplx_eps ? c->diag_im.p : nullptr, c->rs_off.p, c->rs_j.p, pA, pBA, prh,
+                       c->zsum.p, slab / V, hr != nullptr ? 1 : 0, hr, hslab);
+  } else {
+    if (ev) HIPCHK(hipEventRecord(ev[1], st));
+    const T* pq;
+    if constexpr (sizeof(T) == 8) pq = c->rs_Q.p;
+    else pq = c->rs_Q32.p;
+    hipLaunchKernelGGL((k_real_space_quad<T>), dim3(cdiv(n, 256), nrhs), dim3(256), 0, st, n, nrhs, c->nsets, m,
+                       out, c->diag.p, c->diag_q.p, c->cplx_eps ? c->diag_im.p : nullptr,
+                       c->cplx_eps ? c->diag_qim.p : nullptr, c->rs_off.p, c->rs_j.p, pA, pBA, prh, pq,
+                       hr != nullptr ? 1 : 0, hr);
+  }
+  HIPALLOC(c->ms, (size_t)nrhs * NC * n);
+  hipLaunchKernelGGL(k_permute_m<NC>, dim3(cdiv(n, 256), nrhs), dim3(256), 0, st, m, c->perm.p, n, c->ms.p);
   if (ev) HIPCHK(hipEventRecord(ev[2], st));
 
   // tile (x axis) kernels
-  // the spread tile accumulates in FP64 in both precisions
-  using TA = double;
-  const TileCfg tca = tile_cfg(c->ng[0], generic_fft(c->plan[0]), sizeof(cplx<TA>), c->P, kLdsTile);
-  TileCfg tc = tile_cfg(c->ng[0], generic_fft(c->plan[0]), cs, c->P, kLdsTile);
-  if (c->tune[0] > 0 && c->tune[1] > 0) {
-    tc.ty = c->tune[0];
-    tc.tz = c->tune[1];
-    tc.rows = tc.ty * tc.tz;
-    tc.ncl = (3 * tc.rows + 1) / 2;
-  }
-  if (c->tune[2] > 0) tc.ch = c->tune[2];
-  tc.lds = ((size_t)c->ng[0] + (size_t)tc.ncl * tc.lstride * (generic_fft(c->plan[0]) ? 2 : 1)) * cs +
-           (size_t)tc.ch * ((3 + (size_t)c->P + tc.rows) * sizeof(T) + 2 * sizeof(int));
+  const bool gen_x = generic_fft(c->plan[0]);
+  TileCfg tc = tile_cfg_nc<NC>(c->ng[0], gen_x, cs, c->P, kLdsTile);
+  apply_tile_tuning<NC>(c, tc);
+  tc.lds = tile_lds<NC>(c->ng[0], gen_x, cs, c->P, tc, false);
   TileArgs ta{};
   ta.nx = c->ng[0]; ta.ny = c->ng[1]; ta.nz = c->ng[2]; ta.kx = c->kx;
   ta.ty = tc.ty; ta.tz = tc.tz; ta.rows = tc.rows; ta.ncl = tc.ncl; ta.lstride = tc.lstride; ta.ch = tc.ch;
@@ -353,45 +392,40 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   // FP32: atomic-free gather spread; FP64: LDS FP64 atomics (faster there)
   const bool gather = sizeof(T) == 4 && is_ct_size(c->ng[0]) && c->P + 3 <= c->ng[0];
   if (gather) {
-    TileCfg tg = tile_cfg(c->ng[0], false, cs, c->P, kLdsTile, true);
-    if (c->tune[0] > 0 && c->tune[1] > 0) {
-      tg.ty = c->tune[0]; tg.tz = c->tune[1]; tg.rows = tg.ty * tg.tz; tg.ncl = (3 * tg.rows + 1) / 2;
-    }
-    if (c->tune[2] > 0) tg.ch = c->tune[2];
-    tg.lds = ((size_t)c->ng[0] + (size_t)tg.ncl * tg.lstride) * cs + ((size_t)c->ng[0] + 1) * sizeof(int) +
-             (size_t)tg.ch * ((3 + (size_t)c->P + tg.rows) * sizeof(T) + 2 * sizeof(int));
+    TileCfg tg = tile_cfg_nc<NC>(c->ng[0], false, cs, c->P, kLdsTile, true);
+    apply_tile_tuning<NC>(c, tg);
+    tg.lds = tile_lds<NC>(c->ng[0], false, cs, c->P, tg, true);
     if (tg.lds > kLdsMax) return fail(MPOL_EINVAL, "spread tile does not fit in LDS (reduce tile or chunk)");
     TileArgs tga = ta;
     tga.ty = tg.ty; tga.tz = tg.tz; tga.rows = tg.rows; tga.ncl = tg.ncl; tga.lstride = tg.lstride; tga.ch = tg.ch;
     const dim3 ggrid(cdiv(c->ng[1], tg.ty), cdiv(c->ng[2], tg.tz), nrhs);
 #define LAUNCH_GATHER(N, ...)                                                                   \
   do {                                                                                          \
-    set_lds(k_spread_gather_xfft<T, 0, N, 4>, tg.lds);                                          \
-    hipLaunchKernelGGL((k_spread_gather_xfft<T, 0, N, 4>), ggrid, dim3(256), tg.lds, st, tga, m, spec); \
+    set_lds(k_spread_gather_xfft<T, 0, N, 4, NC>, tg.lds);                                      \
+    hipLaunchKernelGGL((k_spread_gather_xfft<T, 0, N, 4, NC>), ggrid, dim3(256), tg.lds, st, tga, m, spec); \
   } while (0)
     DISPATCH_CT(c->ng[0], LAUNCH_GATHER, 0);
 #undef LAUNCH_GATHER
   } else {
-  TileArgs tsa = ta;
-  tsa.ty = tca.ty; tsa.tz = tca.tz; tsa.rows = tca.rows; tsa.ncl = tca.ncl; tsa.lstride = tca.lstride;
-  tsa.ch = tca.ch;
-  tsa.base = c->base64.p;
-  tsa.tw = c->tw64[0].p;
-  size_t slds = tca.lds;
-  if (c->tune[0] > 0 && c->tune[1] > 0) {
-    tsa.ty = tc.ty; tsa.tz = tc.tz; tsa.rows = tc.rows; tsa.ncl = tc.ncl;
-  }
-  if (c->tune[2] > 0) tsa.ch = tc.ch;
-  slds = ((size_t)c->ng[0] + (size_t)tsa.ncl * tsa.lstride * (generic_fft(c->plan[0]) ? 2 : 1)) * sizeof(cplx<TA>) +
-         (size_t)tsa.ch * ((3 + (size_t)c->P + tsa.rows) * sizeof(TA) + 2 * sizeof(int));
-  if (slds > kLdsMax) return fail(MPOL_EINVAL, "spread tile does not fit in LDS (reduce tile or chunk)");
-  const dim3 sgrid(cdiv(c->ng[1], tsa.ty), cdiv(c->ng[2], tsa.tz), nrhs);
+    // the spread tile accumulates in FP64 in both precisions
+    using TA = double;
+    TileCfg tsc = tile_cfg_nc<NC>(c->ng[0], gen_x, sizeof(cplx<TA>), c->P, kLdsTile);
+    apply_tile_tuning<NC>(c, tsc);
+    const size_t slds = tile_lds<NC>(c->ng[0], gen_x, sizeof(cplx<TA>), c->P, tsc, false);
+    if (slds > kLdsMax) return fail(MPOL_EINVAL, "spread tile does not fit in LDS (reduce tile or chunk)");
+    TileArgs tsa = ta;
+    tsa.ty = tsc.ty; tsa.tz = tsc.tz; tsa.rows = tsc.rows; tsa.ncl = tsc.ncl; tsa.lstride = tsc.lstride;
+    tsa.ch = tsc.ch;
+    tsa.base = c->base64.p;
+    tsa.tw = c->tw64[0].p;
+    const dim3 sgrid(cdiv(c->ng[1], tsa.ty), cdiv(c->ng[2], tsa.tz), nrhs);
 #define LAUNCH_SPREAD(E, N, ...)                                                                \
   do {                                                                                          \
-    set_lds(k_spread_xfft<TA, E, N, T>, slds);                                                  \
-    hipLaunchKernelGGL((k_spread_xfft<TA, E, N, T>), sgrid, dim3(256), slds, st, tsa, m, spec); \
+    set_lds(k_spread_xfft<TA, E, N, T, NC>, slds);                                              \
+    hipLaunchKernelGGL((k_spread_xfft<TA, E, N, T, NC>), sgrid, dim3(256), slds, st, tsa, m, spec); \
   } while (0)
-  DISPATCH_FFT(c->ng[0], ta.plan.e, LAUNCH_SPREAD, 0);
+    DISPATCH_FFT(c->ng[0], ta.plan.e, LAUNCH_SPREAD, 0);
+#undef LAUNCH_SPREAD
   }
   if (ev) HIPCHK(hipEventRecord(ev[3], st));
   if (stop_after <= 2) return MPOL_OK;
@@ -402,7 +436,7 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   ColArgs ay{};
   ay.nx = c->ng[0]; ay.ny = c->ng[1]; ay.nz = c->ng[2]; ay.kx = c->kx; ay.tx = cy.tx; ay.lstride = cy.lstride;
   ay.plan = c->plan[1]; ay.tw = tw_ptr<T>(c, 1);
-  const dim3 ygrid(cdiv(c->kx, cy.tx), c->ng[2], 3 * nrhs);
+  const dim3 ygrid(cdiv(c->kx, cy.tx), c->ng[2], NC * nrhs);
   if (cy.lds > kLdsMax) return fail(MPOL_EINVAL, "y-pass columns do not fit in LDS");
 #define LAUNCH_FFTY(E, N, INV)                                                                  \
   do {                                                                                          \
@@ -414,7 +448,7 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   if (stop_after <= 3) return MPOL_OK;
 
   // z axis forward + scale + inverse
-  const ColCfg cz = col_cfg(c->ng[2], generic_fft(c->plan[2]), c->kx, 3, cs, kLdsCol,
+  const ColCfg cz = col_cfg(c->ng[2], generic_fft(c->plan[2]), c->kx, NC, cs, kLdsCol,
                             c->tune[4] > 0 ? c->tune[4] : (sizeof(T) == 8 ? 4 : 8));
   ColArgs az{};
   az.nx = c->ng[0]; az.ny = c->ng[1]; az.nz = c->ng[2]; az.kx = c->kx; az.tx = cz.tx; az.lstride = cz.lstride;
@@ -430,20 +464,21 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   sp.ey = sp.ex + c->ng[0];
   sp.ez = sp.ey + c->ng[1];
   sp.jt = c->cubic ? sp.ez + c->ng[2] : nullptr;
+  sp.jq = (c->cubic && c->jq_off >= 0) ? c->ktab.p + c->jq_off : nullptr;
   const dim3 zgrid(cdiv(c->kx, cz.tx), c->ng[1], nrhs);
   if (cz.lds > kLdsMax) return fail(MPOL_EINVAL, "z-pass columns do not fit in LDS");
-  // one thread group per line of the z transform, all 3*tx lines at once
+  // one thread group per line of the z transform, all NC*tx lines at once
   int zthreads = 256;
   {
     const int e = is_ct_size(c->ng[2]) ? ct_plan(c->ng[2]).e : c->plan[2].e;
-    if (e > 0) zthreads = std::min(1024, std::max(512, 3 * cz.tx * (c->ng[2] / e)));
+    if (e > 0) zthreads = std::min(1024, std::max(512, NC * cz.tx * (c->ng[2] / e)));
     zthreads = (zthreads + 63) / 64 * 64;
     if (c->tune[5] > 0) zthreads = c->tune[5];
   }
 #define LAUNCH_FFTZ(E, N, ...)                                                                  \
   do {                                                                                          \
-    set_lds(k_fft_z_scale<T, E, N>, cz.lds);                                                    \
-    hipLaunchKernelGGL((k_fft_z_scale<T, E, N>), zgrid, dim3(zthreads), cz.lds, st, az, sp, spec); \
+    set_lds(k_fft_z_scale<T, E, N, NC>, cz.lds);                                                \
+    hipLaunchKernelGGL((k_fft_z_scale<T, E, N, NC>), zgrid, dim3(zthreads), cz.lds, st, az, sp, spec); \
   } while (0)
   DISPATCH_FFT(c->ng[2], az.plan.e, LAUNCH_FFTZ, 0);
   if (ev) HIPCHK(hipEventRecord(ev[5], st));
@@ -455,12 +490,11 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
 
 #define LAUNCH_CONTRACT(E, N, ...)                                                              \
   do {                                                                                          \
-    set_lds(k_xifft_contract<T, E, N>, tc.lds);                                                 \
-    hipLaunchKernelGGL((k_xifft_contract<T, E, N>), tgrid, dim3(256), tc.lds, st, ta, spec, out); \
+    set_lds(k_xifft_contract<T, E, N, NC>, tc.lds);                                             \
+    hipLaunchKernelGGL((k_xifft_contract<T, E, N, NC>), tgrid, dim3(256), tc.lds, st, ta, spec, out); \
   } while (0)
   DISPATCH_FFT(c->ng[0], ta.plan.e, LAUNCH_CONTRACT, 0);
   if (ev) HIPCHK(hipEventRecord(ev[7], st));
-#undef LAUNCH_SPREAD
 #undef LAUNCH_FFTY
 #undef LAUNCH_FFTZ
 #undef LAUNCH_CONTRACT
@@ -468,9 +502,18 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   return MPOL_OK;
 }
 
-int apply(mpol_ctx* c, const double* m, double* out, int nrhs, int precision, double slab) {
-  if (precision == MPOL_FP32) return run_operator<float>(c, m, out, nullptr, nullptr, nrhs, slab);
-  return run_operator<double>(c, m, out, nullptr, nullptr, nrhs, slab);
+// Right-hand sides per system: 2 (real, imaginary part) with complex permittivities.
+inline int rhs_per_system(const mpol_ctx* c) { return c->cplx_eps ? 2 : 1; }
+
+// Operator `kind` (kOpDipole / kOpQuad) on nsys systems.
+int apply(mpol_ctx* c, int kind, const double* m, double* out, int nsys, int precision, double slab) {
+  const int nrhs = nsys * rhs_per_system(c);
+  if (kind == kOpQuad) {
+    if (precision == MPOL_FP32) return run_operator<float, kOpQuad>(c, m, out, nullptr, nullptr, nrhs, 0.0);
+    return run_operator<double, kOpQuad>(c, m, out, nullptr, nullptr, nrhs, 0.0);
+  }
+  if (precision == MPOL_FP32) return run_operator<float, kOpDipole>(c, m, out, nullptr, nullptr, nrhs, slab);
+  return run_operator<double, kOpDipole>(c, m, out, nullptr, nullptr, nrhs, slab);
 }
 
 int check_ready(mpol_ctx* c) {
@@ -490,9 +533,11 @@ int dots(mpol_ctx* c, const double* A, size_t stride_i, int cnt, const double* w
 }
 
 // Restarted GMRES on x (in: initial guess, out: solution) for A x = b.
-int gmres_core(mpol_ctx* c, const double* b, double* x, int nrhs, double tol, int restart, int maxit, int precision,
-               double slab, int64_t* nmv, double* relres_host, int* converged_host) {
-  const int64_t n3 = 3LL * c->n;
+// nrhs here counts systems; a system's vector holds all its right-hand sides
+// (kind components per particle, times 2 with complex permittivities).
+int gmres_core(mpol_ctx* c, int kind, const double* b, double* x, int nrhs, double tol, int restart, int maxit,
+               int precision, double slab, int64_t* nmv, double* relres_host, int* converged_host) {
+  const int64_t n3 = (int64_t)kind * c->n * rhs_per_system(c);
   const size_t tot = (size_t)nrhs * n3;
   const size_t sv = tot;  // stride between basis vectors
   HIPALLOC(c->gV, (size_t)(restart + 1) * tot);
@@ -535,7 +580,7 @@ int gmres_core(mpol_ctx* c, const double* b, double* x, int nrhs, double tol, in
   bool all_done = false;
   for (int outer = 0; outer <= maxit; ++outer) {
     // true residual at the start of each cycle (and once after the last)
-    CHECK_STATUS(apply(c, x, c->gAx.p, nrhs, precision, slab));
+    CHECK_STATUS(apply(c, kind, x, c->gAx.p, nrhs, precision, slab));
     ++*nmv;
     hipLaunchKernelGGL(k_residual, dim3(cdiv((int64_t)tot, 256)), dim3(256), 0, st, b, c->gAx.p, c->gR.p,
                        (int64_t)tot);
@@ -550,7 +595,7 @@ int gmres_core(mpol_ctx* c, const double* b, double* x, int nrhs, double tol, in
     hipLaunchKernelGGL(k_scale_into, vgrid, dim3(256), 0, st, c->gR.p, c->gbeta.p, c->gV.p, n3, c->gstate.p);
     for (int j = 0; j < restart; ++j) {
       double* vj = c->gV.p + (size_t)j * sv;
-      CHECK_STATUS(apply(c, vj, c->gW.p, nrhs, precision, slab));
+      CHECK_STATUS(apply(c, kind, vj, c->gW.p, nrhs, precision, slab));
       ++*nmv;
       // classical Gram-Schmidt, twice
       dots(c, c->gV.p, sv, j + 1, c->gW.p, n3, nrhs, c->gh1.p);
@@ -648,7 +693,8 @@ int mpol_set_tables(mpol_ctx* c, const double* perp, const double* para, const d
   HIPCHK(hipStreamSynchronize(c->stream));
   c->ntab = (int)ntab;
   c->self_perp = perp[0];
-  c->have_geo = false;  // pair data depends on the tables
+  c->have_geo = false;   // pair data depends on the tables
+  c->have_qtab = false;  // quadrupole tables live on the same rvals
   return MPOL_OK;
 }
 
@@ -802,6 +848,22 @@ int mpol_set_geometry(mpol_ctx* c, const double* x, int64_t n, const double box[
         const double j = std::sqrt(2.0 / (kPi * kn)) * (std::sin(kn) / kn - std::cos(kn));
         kt.push_back(9.0 * kPi / (2.0 * kn) * (j * j) / kk);
       }
+      // (j1^2, j1 j2, j2^2)/k^2 of the dipole + quadrupole blocks
+      c->jq_off = (int64_t)kt.size();
+      for (int64_t q = 0; q <= nmax; ++q) {
+        if (q == 0) {
+          kt.push_back(0.0); kt.push_back(0.0); kt.push_back(0.0);
+          continue;
+        }
+        const double kk = dk * dk * (double)q;
+        double j1, j2;
+        sph_j12(std::sqrt(kk), j1, j2);
+        kt.push_back(j1 * j1 / kk);
+        kt.push_back(j1 * j2 / kk);
+        kt.push_back(j2 * j2 / kk);
+      }
+    } else {
+      c->jq_off = -1;
     }
     HIPALLOC(c->ktab, kt.size());
     HIPCHK(hipMemcpyAsync(c->ktab.p, kt.data(), kt.size() * sizeof(double), hipMemcpyHostToDevice, st));
@@ -809,6 +871,7 @@ int mpol_set_geometry(mpol_ctx* c, const double* x, int64_t n, const double box[
   HIPCHK(hipStreamSynchronize(st));
   HIPCHK(hipGetLastError());
   c->have_geo = true;
+  c->quad_pairs_valid = false;
   return MPOL_OK;
 }
 
@@ -818,6 +881,148 @@ namespace {
 __global__ void k_diag(const double* __restrict__ eps, int64_t total, double self_perp, double* __restrict__ diag) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < total) diag[k] = -3.0 / (4.0 * kPi * (1.0 - eps[k])) + self_perp;
+}
+
+// Complex permittivity eps = er + i ei: c/(1 - eps) = c (a + i b)/(a^2 + b^2)
+// with a = 1 - er, b = ei; diag = -coef/(1 - eps) + self (self term real).
+__global__ void k_diag_cplx(const double* __restrict__ er, const double* __restrict__ ei, int64_t total,
+                            double coef, double self, double* __restrict__ dre, double* __restrict__ dim) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= total) return;
+  const double a = 1.0 - er[k], b = ei[k];
+  const double inv = 1.0 / (a * a + b * b);
+  dre[k] = -coef * a * inv + self;
+  dim[k] = -coef * b * inv;
+}
+
+// Quadrupole rows: 15/(8 pi (eps - 1)) + grad_quad_2(0)/2.
+__global__ void k_diag_q(const double* __restrict__ eps, int64_t total, double self, double* __restrict__ diag) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < total) diag[k] = -15.0 / (8.0 * kPi * (1.0 - eps[k])) + self;
+}
+
+int check_sets(mpol_ctx* c, int nsys) {
+  if (c->nsets != 1 && c->nsets != nsys)
+    return fail(MPOL_EINVAL, "permittivity sets must be 1 or one per right-hand side");
+  return MPOL_OK;
+}
+
+// Quadrupole pair coefficients and diagonal, (re)built when the geometry,
+// the quadrupole tables or the permittivities changed.
+int prepare_quad(mpol_ctx* c) {
+  if (!c->have_qtab) return fail(MPOL_ESTATE, "quadrupole tables not set (call mpol_set_quad_tables first)");
+  hipStream_t st = c->stream;
+  if (!c->quad_pairs_valid) {
+    const int64_t np = c->npairs;
+    HIPALLOC(c->rs_Q, (size_t)std::max<int64_t>(1, np) * kQuadTabs);
+    HIPALLOC(c->rs_Q32, (size_t)std::max<int64_t>(1, np) * kQuadTabs);
+    PairParams pp{};
+    pp.n = c->n;
+    for (int d = 0; d < 3; ++d) {
+      pp.box[d] = c->box[d];
+      pp.nc[d] = c->nc[d];
+    }
+    pp.rc = c->rc;
+    pp.ntab = c->ntab;
+    hipLaunchKernelGGL(k_pairs_quad, dim3(cdiv(c->n, 256)), dim3(256), 0, st, pp, c->x.p, c->rs_off.p, c->rs_j.p,
+                       c->rvals.p, c->qtab.p, c->rs_Q.p, c->rs_Q32.p);
+    HIPCHK(hipGetLastError());
+    c->quad_pairs_valid = true;
+  }
+  if (!c->diag_q_valid) {
+    const int64_t total = (int64_t)c->nsets * c->n;
+    HIPALLOC(c->diag_q, (size_t)total);
+    if (c->cplx_eps) {
+      HIPALLOC(c->diag_qim, (size_t)total);
+      hipLaunchKernelGGL(k_diag_cplx, dim3(cdiv(total, 256)), dim3(256), 0, st, c->eps_re.p, c->eps_im.p, total,
+                         15.0 / (8.0 * kPi), 0.5 * c->self_gq2, c->diag_q.p, c->diag_qim.p);
+    } else {
+      hipLaunchKernelGGL(k_diag_q, dim3(cdiv(total, 256)), dim3(256), 0, st, c->eps_re.p, total, 0.5 * c->self_gq2,
+                         c->diag_q.p);
+    }
+    HIPCHK(hipGetLastError());
+    c->diag_q_valid = true;
+  }
+  return MPOL_OK;
+}
+
+// Restarted GMRES, plain FP64 or FP64 iterative refinement around an
+// FP32-operator GMRES, for operator `kind` on nsys systems.
+int solve_impl(mpol_ctx* c, int kind, const double* b, const double* x0, double* x, int nsys,
+               const mpol_solve_opts* o, mpol_solve_info* info) {
+  hipStream_t st = c->stream;
+  const int64_t n3 = (int64_t)kind * c->n * rhs_per_system(c);
+  const int64_t tot = n3 * nsys;
+  const int nrhs = nsys;
+  const double slab = kind == kOpDipole ? o->slab_factor : 0.0;
+  mpol_solve_info inf{};
+  std::vector<double> rel(nrhs, 0.0);
+  int conv = 0;
+  if (x != x0) HIPCHK(hipMemcpyAsync(x, x0, tot * sizeof(double), hipMemcpyDeviceToDevice, st));
+  if (!o->mixed) {
+    int64_t nmv = 0;
+    CHECK_STATUS(gmres_core(c, kind, b, x, nrhs, o->tol, o->restart, o->maxit, MPOL_FP64, slab, &nmv, rel.data(),
+                            &conv));
+    inf.matvecs_fp64 = nmv;
+  } else {
+    HIPALLOC(c->gd, (size_t)tot);
+    HIPALLOC(c->gx, (size_t)tot);
+    HIPALLOC(c->gR, (size_t)tot);
+    HIPALLOC(c->gAx, (size_t)tot);
+    HIPALLOC(c->grn2, (size_t)nrhs);
+    HIPALLOC(c->gpart, (size_t)nrhs * (o->restart + 1) * kDotBlocks);
+    std::vector<double> bn2(nrhs), rn2(nrhs);
+    dots(c, b, 0, 1, b, n3, nrhs, c->grn2.p);
+    HIPCHK(hipMemcpyAsync(bn2.data(), c->grn2.p, nrhs * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    int64_t nmv64 = 0, nmv32 = 0;
+    int ref = 0;
+    for (;; ++ref) {
+      CHECK_STATUS(apply(c, kind, x, c->gAx.p, nrhs, MPOL_FP64, slab));
+      ++nmv64;
+      hipLaunchKernelGGL(k_residual, dim3(cdiv(tot, 256)), dim3(256), 0, st, b, c->gAx.p, c->gd.p, tot);
+      dots(c, c->gd.p, 0, 1, c->gd.p, n3, nrhs, c->grn2.p);
+      HIPCHK(hipMemcpyAsync(rn2.data(), c->grn2.p, nrhs * sizeof(double), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      double ratio = 1e300;
+      conv = 0;
+      for (int r = 0; r < nrhs; ++r) {
+        rel[r] = bn2[r] > 0.0 ? std::sqrt(rn2[r] / bn2[r]) : 0.0;
+        if (rel[r] <= o->tol) ++conv;
+        else if (rn2[r] > 0.0) ratio = std::min(ratio, std::sqrt(bn2[r] / rn2[r]));
+      }
+      if (conv == nrhs || ref >= o->max_refine) break;
+      // inner tolerance relative to the current residual
+      double tin = 0.3 * o->tol * ratio;
+      tin = std::min(0.1, std::max(tin, 2e-6));
+      // residual -> rhs of the correction solve (copy into gx), correction into gd
+      hipLaunchKernelGGL(k_copy, dim3(cdiv(tot, 256)), dim3(256), 0, st, c->gd.p, c->gx.p, tot);
+      HIPCHK(hipMemsetAsync(c->gd.p, 0, tot * sizeof(double), st));
+      int64_t nmv = 0;
+      CHECK_STATUS(gmres_core(c, kind, c->gx.p, c->gd.p, nrhs, tin, o->restart, o->maxit, MPOL_FP32, slab, &nmv,
+                              nullptr, nullptr));
+      nmv32 += nmv;
+      hipLaunchKernelGGL(k_axpy_all, dim3(cdiv(tot, 256)), dim3(256), 0, st, 1.0, c->gd.p, x, tot);
+    }
+    inf.matvecs_fp64 = nmv64;
+    inf.matvecs_fp32 = nmv32;
+    inf.refinements = ref;
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  inf.converged = conv;
+  inf.max_relres = 0.0;
+  for (double v : rel) inf.max_relres = std::max(inf.max_relres, v);
+  if (info) *info = inf;
+  return MPOL_OK;
+}
+
+int check_solve_args(mpol_ctx* c, const double* b, const double* x0, double* x, int32_t nrhs,
+                     const mpol_solve_opts* o) {
+  if (!b || !x0 || !x || !o || nrhs < 1) return fail(MPOL_EINVAL, "bad solve arguments");
+  CHECK_STATUS(check_sets(c, nrhs));
+  if (o->restart < 1 || o->restart > 200 || o->maxit < 0) return fail(MPOL_EINVAL, "bad restart/maxit");
+  if (!(o->tol > 0.0)) return fail(MPOL_EINVAL, "tolerance must be positive");
+  return MPOL_OK;
 }
 }  // namespace
 
@@ -830,34 +1035,60 @@ int mpol_set_permittivity(mpol_ctx* c, const double* eps, int32_t nsets) {
   HIPCHK(hipSetDevice(c->device));
   const int64_t total = (int64_t)nsets * c->n;
   HIPALLOC(c->diag, (size_t)total);
+  HIPALLOC(c->eps_re, (size_t)total);
+  HIPCHK(hipMemcpyAsync(c->eps_re.p, eps, total * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
   hipLaunchKernelGGL(k_diag, dim3(cdiv(total, 256)), dim3(256), 0, c->stream, eps, total, c->self_perp, c->diag.p);
   HIPCHK(hipGetLastError());
   c->nsets = nsets;
+  c->cplx_eps = false;
+  c->diag_q_valid = false;
+  return MPOL_OK;
+}
+
+int mpol_set_permittivity_complex(mpol_ctx* c, const double* eps_re, const double* eps_im, int32_t nsets) {
+  if (!c || !eps_re || !eps_im) return fail(MPOL_EINVAL, "null argument");
+  if (!c->have_geo) return fail(MPOL_ESTATE, "geometry not set (call mpol_set_geometry first)");
+  if (nsets < 1) return fail(MPOL_EINVAL, "nsets must be >= 1");
+  HIPCHK(hipSetDevice(c->device));
+  const int64_t total = (int64_t)nsets * c->n;
+  HIPALLOC(c->diag, (size_t)total);
+  HIPALLOC(c->diag_im, (size_t)total);
+  HIPALLOC(c->eps_re, (size_t)total);
+  HIPALLOC(c->eps_im, (size_t)total);
+  HIPCHK(hipMemcpyAsync(c->eps_re.p, eps_re, total * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->eps_im.p, eps_im, total * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  hipLaunchKernelGGL(k_diag_cplx, dim3(cdiv(total, 256)), dim3(256), 0, c->stream, eps_re, eps_im, total,
+                     3.0 / (4.0 * kPi), c->self_perp, c->diag.p, c->diag_im.p);
+  HIPCHK(hipGetLastError());
+  c->nsets = nsets;
+  c->cplx_eps = true;
+  c->diag_q_valid = false;
   return MPOL_OK;
 }
 
 int mpol_matvec(mpol_ctx* c, const double* m, double* out, int32_t nrhs, int32_t precision, double slab) {
   CHECK_STATUS(check_ready(c));
   if (!m || !out || nrhs < 1) return fail(MPOL_EINVAL, "bad matvec arguments");
-  if (c->nsets != 1 && c->nsets != nrhs) return fail(MPOL_EINVAL, "permittivity sets must be 1 or nrhs");
+  CHECK_STATUS(check_sets(c, nrhs));
   if (precision != MPOL_FP64 && precision != MPOL_FP32) return fail(MPOL_EINVAL, "unknown precision");
   HIPCHK(hipSetDevice(c->device));
-  return apply(c, m, out, nrhs, precision, slab);
+  return apply(c, kOpDipole, m, out, nrhs, precision, slab);
 }
 
 int mpol_matvec_timed(mpol_ctx* c, const double* m, double* out, int32_t nrhs, int32_t precision, double slab,
                       int32_t repeats, double* stage_ms) {
   CHECK_STATUS(check_ready(c));
   if (!m || !out || nrhs < 1 || repeats < 1 || !stage_ms) return fail(MPOL_EINVAL, "bad matvec arguments");
-  if (c->nsets != 1 && c->nsets != nrhs) return fail(MPOL_EINVAL, "permittivity sets must be 1 or nrhs");
+  CHECK_STATUS(check_sets(c, nrhs));
   HIPCHK(hipSetDevice(c->device));
+  const int nr = nrhs * rhs_per_system(c);
   hipEvent_t ev[kStages + 1];
   for (int k = 0; k <= kStages; ++k) HIPCHK(hipEventCreate(&ev[k]));
   for (int k = 0; k < kStages; ++k) stage_ms[k] = 0.0;
   int status = MPOL_OK;
   for (int r = 0; r < repeats && status == MPOL_OK; ++r) {
-    status = precision == MPOL_FP32 ? run_operator<float>(c, m, out, nullptr, nullptr, nrhs, slab, ev)
-                                    : run_operator<double>(c, m, out, nullptr, nullptr, nrhs, slab, ev);
+    status = precision == MPOL_FP32 ? run_operator<float, kOpDipole>(c, m, out, nullptr, nullptr, nr, slab, ev)
+                                    : run_operator<double, kOpDipole>(c, m, out, nullptr, nullptr, nr, slab, ev);
     if (status != MPOL_OK) break;
     if (hipEventSynchronize(ev[kStages]) != hipSuccess) {
       status = fail(MPOL_EHIP, "hipEventSynchronize failed");
@@ -877,11 +1108,13 @@ int mpol_debug_spectrum(mpol_ctx* c, const double* m, int32_t nrhs, int32_t prec
                         void* spec_out) {
   CHECK_STATUS(check_ready(c));
   if (!m || !spec_out || nrhs < 1 || stage < 2 || stage > 5) return fail(MPOL_EINVAL, "bad debug arguments");
+  if (c->cplx_eps) return fail(MPOL_ESTATE, "spectrum dump needs real permittivities");
   HIPCHK(hipSetDevice(c->device));
   DevBuf<double> tmp;
   HIPALLOC(tmp, (size_t)nrhs * 3 * c->n);
-  const int st = precision == MPOL_FP32 ? run_operator<float>(c, m, tmp.p, nullptr, nullptr, nrhs, 0.0, nullptr, stage)
-                                        : run_operator<double>(c, m, tmp.p, nullptr, nullptr, nrhs, 0.0, nullptr, stage);
+  const int st = precision == MPOL_FP32
+                     ? run_operator<float, kOpDipole>(c, m, tmp.p, nullptr, nullptr, nrhs, 0.0, nullptr, stage)
+                     : run_operator<double, kOpDipole>(c, m, tmp.p, nullptr, nullptr, nrhs, 0.0, nullptr, stage);
   CHECK_STATUS(st);
   const size_t bytes = (size_t)3 * nrhs * c->ng[2] * c->ng[1] * c->kx *
                        (precision == MPOL_FP32 ? sizeof(cplx<float>) : sizeof(cplx<double>));
@@ -894,83 +1127,66 @@ int mpol_matvec_components(mpol_ctx* c, const double* m, double* hk, double* hr,
                            int32_t precision, double slab) {
   CHECK_STATUS(check_ready(c));
   if (!m || !hk || !hr || !hslab || nrhs < 1) return fail(MPOL_EINVAL, "bad matvec arguments");
-  if (c->nsets != 1 && c->nsets != nrhs) return fail(MPOL_EINVAL, "permittivity sets must be 1 or nrhs");
+  CHECK_STATUS(check_sets(c, nrhs));
   HIPCHK(hipSetDevice(c->device));
-  if (precision == MPOL_FP32) return run_operator<float>(c, m, hk, hr, hslab, nrhs, slab);
-  return run_operator<double>(c, m, hk, hr, hslab, nrhs, slab);
+  const int nr = nrhs * rhs_per_system(c);
+  if (precision == MPOL_FP32) return run_operator<float, kOpDipole>(c, m, hk, hr, hslab, nr, slab);
+  return run_operator<double, kOpDipole>(c, m, hk, hr, hslab, nr, slab);
 }
 
 int mpol_solve(mpol_ctx* c, const double* b, const double* x0, double* x, int32_t nrhs, const mpol_solve_opts* o,
                mpol_solve_info* info) {
   CHECK_STATUS(check_ready(c));
-  if (!b || !x0 || !x || !o || nrhs < 1) return fail(MPOL_EINVAL, "bad solve arguments");
-  if (c->nsets != 1 && c->nsets != nrhs) return fail(MPOL_EINVAL, "permittivity sets must be 1 or nrhs");
-  if (o->restart < 1 || o->restart > 200 || o->maxit < 0) return fail(MPOL_EINVAL, "bad restart/maxit");
+  CHECK_STATUS(check_solve_args(c, b, x0, x, nrhs, o));
   HIPCHK(hipSetDevice(c->device));
-  hipStream_t st = c->stream;
-  const int64_t n3 = 3LL * c->n;
-  const int64_t tot = n3 * nrhs;
-  mpol_solve_info inf{};
-  std::vector<double> rel(nrhs, 0.0);
-  int conv = 0;
-  if (x != x0) HIPCHK(hipMemcpyAsync(x, x0, tot * sizeof(double), hipMemcpyDeviceToDevice, st));
-  if (!o->mixed) {
-    int64_t nmv = 0;
-    CHECK_STATUS(gmres_core(c, b, x, nrhs, o->tol, o->restart, o->maxit, MPOL_FP64, o->slab_factor, &nmv,
-                            rel.data(), &conv));
-    inf.matvecs_fp64 = nmv;
-  } else {
-    // FP64 iterative refinement around an FP32-operator GMRES
-    HIPALLOC(c->gd, (size_t)tot);
-    HIPALLOC(c->gx, (size_t)tot);
-    HIPALLOC(c->gR, (size_t)tot);
-    HIPALLOC(c->gAx, (size_t)tot);
-    HIPALLOC(c->grn2, (size_t)nrhs);
-    HIPALLOC(c->gpart, (size_t)nrhs * (o->restart + 1) * kDotBlocks);
-    std::vector<double> bn2(nrhs), rn2(nrhs);
-    dots(c, b, 0, 1, b, n3, nrhs, c->grn2.p);
-    HIPCHK(hipMemcpyAsync(bn2.data(), c->grn2.p, nrhs * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    int64_t nmv64 = 0, nmv32 = 0;
-    int ref = 0;
-    for (;; ++ref) {
-      CHECK_STATUS(apply(c, x, c->gAx.p, nrhs, MPOL_FP64, o->slab_factor));
-      ++nmv64;
-      hipLaunchKernelGGL(k_residual, dim3(cdiv(tot, 256)), dim3(256), 0, st, b, c->gAx.p, c->gd.p, tot);
-      dots(c, c->gd.p, 0, 1, c->gd.p, n3, nrhs, c->grn2.p);
-      HIPCHK(hipMemcpyAsync(rn2.data(), c->grn2.p, nrhs * sizeof(double), hipMemcpyDeviceToHost, st));
-      HIPCHK(hipStreamSynchronize(st));
-      double worst = 0.0, ratio = 1e300;
-      conv = 0;
-      for (int r = 0; r < nrhs; ++r) {
-        rel[r] = bn2[r] > 0.0 ? std::sqrt(rn2[r] / bn2[r]) : 0.0;
-        worst = std::max(worst, rel[r]);
-        if (rel[r] <= o->tol) ++conv;
-        else if (rn2[r] > 0.0) ratio = std::min(ratio, std::sqrt(bn2[r] / rn2[r]));
-      }
-      if (conv == nrhs || ref >= o->max_refine) break;
-      // inner tolerance relative to the current residual
-      double tin = 0.3 * o->tol * ratio;
-      tin = std::min(0.1, std::max(tin, 2e-6));
-      // residual -> rhs of the correction solve (copy into gx), correction into gd
-      hipLaunchKernelGGL(k_copy, dim3(cdiv(tot, 256)), dim3(256), 0, st, c->gd.p, c->gx.p, tot);
-      HIPCHK(hipMemsetAsync(c->gd.p, 0, tot * sizeof(double), st));
-      int64_t nmv = 0;
-      CHECK_STATUS(gmres_core(c, c->gx.p, c->gd.p, nrhs, tin, o->restart, o->maxit, MPOL_FP32, o->slab_factor, &nmv,
-                              nullptr, nullptr));
-      nmv32 += nmv;
-      hipLaunchKernelGGL(k_axpy_all, dim3(cdiv(tot, 256)), dim3(256), 0, st, 1.0, c->gd.p, x, tot);
-    }
-    inf.matvecs_fp64 = nmv64;
-    inf.matvecs_fp32 = nmv32;
-    inf.refinements = ref;
-  }
-  HIPCHK(hipStreamSynchronize(st));
-  inf.converged = conv;
-  inf.max_relres = 0.0;
-  for (double v : rel) inf.max_relres = std::max(inf.max_relres, v);
-  if (info) *info = inf;
+  return solve_impl(c, kOpDipole, b, x0, x, nrhs, o, info);
+}
+
+int mpol_set_quad_tables(mpol_ctx* c, const double* qtab, int64_t ntab) {
+  if (!c || !qtab) return fail(MPOL_EINVAL, "null argument");
+  if (c->ntab == 0) return fail(MPOL_ESTATE, "tables not set (call mpol_set_tables first)");
+  if (ntab != c->ntab) return fail(MPOL_EINVAL, "quadrupole tables must have the length of rvals");
+  HIPCHK(hipSetDevice(c->device));
+  HIPALLOC(c->qtab, (size_t)kQuadTabs * ntab);
+  HIPCHK(hipMemcpyAsync(c->qtab.p, qtab, (size_t)kQuadTabs * ntab * sizeof(double), hipMemcpyHostToDevice,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->self_gq2 = qtab[4 * ntab];  // grad_quad_2 at r = 0: its self term
+  c->have_qtab = true;
+  c->quad_pairs_valid = false;
+  c->diag_q_valid = false;
   return MPOL_OK;
+}
+
+int mpol_quad_matvec(mpol_ctx* c, const double* m, double* out, int32_t nrhs, int32_t precision) {
+  CHECK_STATUS(check_ready(c));
+  if (!m || !out || nrhs < 1) return fail(MPOL_EINVAL, "bad matvec arguments");
+  CHECK_STATUS(check_sets(c, nrhs));
+  if (precision != MPOL_FP64 && precision != MPOL_FP32) return fail(MPOL_EINVAL, "unknown precision");
+  HIPCHK(hipSetDevice(c->device));
+  CHECK_STATUS(prepare_quad(c));
+  return apply(c, kOpQuad, m, out, nrhs, precision, 0.0);
+}
+
+int mpol_quad_matvec_components(mpol_ctx* c, const double* m, double* hk, double* hr, int32_t nrhs,
+                                int32_t precision) {
+  CHECK_STATUS(check_ready(c));
+  if (!m || !hk || !hr || nrhs < 1) return fail(MPOL_EINVAL, "bad matvec arguments");
+  CHECK_STATUS(check_sets(c, nrhs));
+  HIPCHK(hipSetDevice(c->device));
+  CHECK_STATUS(prepare_quad(c));
+  const int nr = nrhs * rhs_per_system(c);
+  if (precision == MPOL_FP32) return run_operator<float, kOpQuad>(c, m, hk, hr, nullptr, nr, 0.0);
+  return run_operator<double, kOpQuad>(c, m, hk, hr, nullptr, nr, 0.0);
+}
+
+int mpol_quad_solve(mpol_ctx* c, const double* b, const double* x0, double* x, int32_t nrhs,
+                    const mpol_solve_opts* o, mpol_solve_info* info) {
+  CHECK_STATUS(check_ready(c));
+  CHECK_STATUS(check_solve_args(c, b, x0, x, nrhs, o));
+  HIPCHK(hipSetDevice(c->device));
+  CHECK_STATUS(prepare_quad(c));
+  return solve_impl(c, kOpQuad, b, x0, x, nrhs, o, info);
 }
 
 int mpol_set_tuning(mpol_ctx* c, const int32_t* knobs, int32_t nknobs) {
@@ -999,7 +1215,7 @@ int mpol_geometry_info(mpol_ctx* c, mpol_geometry_summary* info) {
     info->fft_e[d] = c->plan[d].e;
   }
   info->P = c->P;
-  info->tile_rows = tile_cfg(c->ng[0], generic_fft(c->plan[0]), sizeof(cplx<double>), c->P, kLdsTile).rows;
+  info->tile_rows = tile_cfg_nc<kOpDipole>(c->ng[0], generic_fft(c->plan[0]), sizeof(cplx<double>), c->P, kLdsTile).rows;
   return MPOL_OK;
 }
 
@@ -1009,6 +1225,15 @@ int mpol_real_space_table(double xi, const double* r, int64_t n, double* out) {
   for (int64_t i = 0; i < n; ++i)
     if (!(r[i] > 0.0)) return fail(MPOL_EINVAL, "table separations must be positive");
   real_space_tables(xi, r, n, out);
+  return MPOL_OK;
+}
+
+int mpol_real_space_table_quad(double xi, const double* r, int64_t n, double* out) {
+  if (!r || !out || n < 0) return fail(MPOL_EINVAL, "bad table arguments");
+  if (!(xi > 0.0)) return fail(MPOL_EINVAL, "xi must be positive");
+  for (int64_t i = 0; i < n; ++i)
+    if (!(r[i] > 0.0)) return fail(MPOL_EINVAL, "table separations must be positive");
+  real_space_tables_quad(xi, r, n, out);
   return MPOL_OK;
 }
 

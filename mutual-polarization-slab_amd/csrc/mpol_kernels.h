@@ -132,6 +132,9 @@ __global__ void k_geo_scatter(GeoParams g, const int* __restrict__ bin_key, cons
 }
 
 // ms[rhs][s] = m[rhs][perm[s]]: dipoles in stencil-bin order, once per matvec.
+// Moments of NC components per particle (3 dipole, or 3 dipole + 5
+// quadrupole) into sorted order.
+template <int NC>
 __global__ void k_permute_m(const double* __restrict__ m, const int* __restrict__ perm, int n,
                             double* __restrict__ ms) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -139,9 +142,8 @@ __global__ void k_permute_m(const double* __restrict__ m, const int* __restrict_
   if (s >= n) return;
   const size_t o = (size_t)rhs * n;
   const int i = perm[s];
-  ms[3 * (o + s)] = m[3 * (o + i)];
-  ms[3 * (o + s) + 1] = m[3 * (o + i) + 1];
-  ms[3 * (o + s) + 2] = m[3 * (o + i) + 2];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) ms[NC * (o + s) + c] = m[NC * (o + i) + c];
 }
 
 // ---------------------------------------------------------------------------
@@ -259,17 +261,22 @@ __global__ void k_sum_finish(const double* __restrict__ partial, int nblk, int n
 
 // Real-space sum + dielectric/self diagonal + slab term.  Writes out (the
 // wave-space contraction is accumulated on top afterwards).
+// Complex permittivities (diag_im != null): right-hand sides come in pairs
+// (real part, imaginary part) of one complex system, and the diagonal
+// couples them: out_re += d_re m_re - d_im m_im, out_im += d_re m_im + d_im m_re.
+// Permittivity sets are per right-hand side (real) or per pair (complex).
 template <typename T>
 __global__ void k_real_space(int n, int nrhs, int nsets, const double* __restrict__ m, double* __restrict__ out,
-                             const double* __restrict__ diag, const int* __restrict__ off,
-                             const int* __restrict__ pj, const T* __restrict__ pA, const T* __restrict__ pBA,
-                             const T* __restrict__ prh, const double* __restrict__ zsum, double slab_over_v,
-                             int write_diag_only_parts, double* __restrict__ hr_out, double* __restrict__ slab_out) {
+                             const double* __restrict__ diag, const double* __restrict__ diag_im,
+                             const int* __restrict__ off, const int* __restrict__ pj, const T* __restrict__ pA,
+                             const T* __restrict__ pBA, const T* __restrict__ prh, const double* __restrict__ zsum,
+                             double slab_over_v, int write_diag_only_parts, double* __restrict__ hr_out,
+                             double* __restrict__ slab_out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int rhs = blockIdx.y;
   if (i >= n || rhs >= nrhs) return;
   const double* mr = m + (size_t)rhs * n * 3;
-  const int set = (nsets == 1) ? 0 : rhs;
+  const int set = (nsets == 1) ? 0 : (diag_im ? rhs >> 1 : rhs);
   const double dg = diag[(size_t)set * n + i];
   T a0 = 0, a1 = 0, a2 = 0;
   for (int q = off[i]; q < off[i + 1]; ++q) {
@@ -282,9 +289,16 @@ __global__ void k_real_space(int n, int nrhs, int nsets, const double* __restric
     a1 += A * m1 + c * r1;
     a2 += A * m2 + c * r2;
   }
-  const double h0 = dg * mr[3 * i] + (double)a0;
-  const double h1 = dg * mr[3 * i + 1] + (double)a1;
-  const double h2 = dg * mr[3 * i + 2] + (double)a2;
+  double h0 = dg * mr[3 * i] + (double)a0;
+  double h1 = dg * mr[3 * i + 1] + (double)a1;
+  double h2 = dg * mr[3 * i + 2] + (double)a2;
+  if (diag_im) {
+    const double di = (rhs & 1) ? diag_im[(size_t)set * n + i] : -diag_im[(size_t)set * n + i];
+    const double* mp = m + (size_t)(rhs ^ 1) * n * 3;
+    h0 += di * mp[3 * i];
+    h1 += di * mp[3 * i + 1];
+    h2 += di * mp[3 * i + 2];
+  }
   const double sz = slab_over_v * zsum[rhs];
   double* o = out + ((size_t)rhs * n + i) * 3;
   if (write_diag_only_parts) {
@@ -298,6 +312,124 @@ __global__ void k_real_space(int n, int nrhs, int nsets, const double* __restric
     o[0] = h0;
     o[1] = h1;
     o[2] = h2 + sz;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// dipole + quadrupole real space (8 moments per particle: S_x, S_y, S_z and
+// the traceless Q_xx, Q_xy, Q_xz, Q_yy, Q_yz; outputs E (3) and G (5))
+// ---------------------------------------------------------------------------
+constexpr int kQuadTabs = 7;  // field_quad_1..3, grad_quad_1..4 (the field/dipole pair is A, B-A)
+
+// Coefficients C..I of every CSR pair from the seven quadrupole tables, with
+// the distance and interval decisions of k_pairs (same FP64 operations).
+__global__ void k_pairs_quad(PairParams p, const double* __restrict__ x, const int* __restrict__ off,
+                             const int* __restrict__ pj, const double* __restrict__ rv,
+                             const double* __restrict__ qtab, double* __restrict__ pq,
+                             float* __restrict__ pq32) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p.n) return;
+  const double xi0 = x[3 * i], xi1 = x[3 * i + 1], xi2 = x[3 * i + 2];
+  for (int w = off[i]; w < off[i + 1]; ++w) {
+    const int j = pj[w];
+    double r0 = xi0 - x[3 * j], r1 = xi1 - x[3 * j + 1], r2 = xi2 - x[3 * j + 2];
+    r0 = r0 - p.box[0] * trunc(2.0 * r0 / p.box[0]);
+    r1 = r1 - p.box[1] * trunc(2.0 * r1 / p.box[1]);
+    r2 = r2 - p.box[2] * trunc(2.0 * r2 / p.box[2]);
+    const double d = sqrt((r0 * r0 + r1 * r1) + r2 * r2);
+    int idx = table_lower_bound(rv, p.ntab, d);
+    if (idx <= 0) idx = 1;
+    if (idx >= p.ntab) idx = p.ntab - 1;
+    const double lo = rv[idx - 1], hi = rv[idx];
+    const double t = (d - lo) / (hi - lo);
+    for (int k = 0; k < kQuadTabs; ++k) {
+      const double* tb = qtab + (size_t)k * p.ntab;
+      const double v = tb[idx - 1] * (1.0 - t) + tb[idx] * t;
+      pq[(size_t)w * kQuadTabs + k] = v;
+      pq32[(size_t)w * kQuadTabs + k] = (float)v;
+    }
+  }
+}
+
+// E_i, G_i from the moments of the neighbours (RealSpace.m:55-96 restated
+// owner-computes) plus the diagonal: dielectric 3/(4 pi (eps-1)) S and
+// 15/(8 pi (eps-1)) Q with the self terms field_dip_1(0) S and
+// grad_quad_2(0) Q / 2 (folded into dg_s, dg_q).
+template <typename T>
+__global__ void k_real_space_quad(int n, int nrhs, int nsets, const double* __restrict__ m, double* __restrict__ out,
+                                  const double* __restrict__ dg_s, const double* __restrict__ dg_q,
+                                  const double* __restrict__ dgi_s, const double* __restrict__ dgi_q,
+                                  const int* __restrict__ off, const int* __restrict__ pj,
+                                  const T* __restrict__ pA, const T* __restrict__ pBA, const T* __restrict__ prh,
+                                  const T* __restrict__ pq, int parts, double* __restrict__ hr_out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int rhs = blockIdx.y;
+  if (i >= n || rhs >= nrhs) return;
+  const double* mr = m + (size_t)rhs * n * 8;
+  T e[3] = {0, 0, 0}, g[5] = {0, 0, 0, 0, 0};
+  const T half = (T)0.5;
+  for (int q = off[i]; q < off[i + 1]; ++q) {
+    const int j = pj[q];
+    const double* mj = mr + 8 * (size_t)j;
+    const T S0 = (T)mj[0], S1 = (T)mj[1], S2 = (T)mj[2];
+    const T q1 = (T)mj[3], q2 = (T)mj[4], q3 = (T)mj[5], q4 = (T)mj[6], q5 = (T)mj[7];
+    const T r0 = prh[3 * (size_t)q], r1 = prh[3 * (size_t)q + 1], r2 = prh[3 * (size_t)q + 2];
+    const T A = pA[q], BA = pBA[q];
+    const T* cq = pq + (size_t)q * kQuadTabs;
+    const T C = cq[0], D = cq[1], E = cq[2], F = cq[3], G = cq[4], H = cq[5], I = cq[6];
+    const T Sr = S0 * r0 + S1 * r1 + S2 * r2;
+    // Q.r (third row uses Q_zz = -(Q_xx + Q_yy)) and Q:rr
+    const T Qr0 = q1 * r0 + q2 * r1 + q3 * r2;
+    const T Qr1 = q2 * r0 + q4 * r1 + q5 * r2;
+    const T Qr2 = q3 * r0 + q5 * r1 - (q1 + q4) * r2;
+    const T qdd = (r0 * r0 - r2 * r2) * q1 + 2 * r0 * r1 * q2 + 2 * r0 * r2 * q3 + (r1 * r1 - r2 * r2) * q4 +
+                  2 * r1 * r2 * q5;
+    // field: A S + (B - A)(S.r) r - (C (Q:rr) r + 2 D Q.r)/2
+    const T ce = BA * Sr - half * C * qdd;
+    e[0] += A * S0 + ce * r0 - D * Qr0;
+    e[1] += A * S1 + ce * r1 - D * Qr1;
+    e[2] += A * S2 + ce * r2 - D * Qr2;
+    // gradient, components (xx, xy, xz, yy, yz) of
+    //   C (S.r) rr + D (S r + r S) + (D + E)(S.r) I
+    //   + (F (Q:rr) I + G Q + H ((Q:rr) I + 2 Q.r r + 2 r Q.r) + I (Q:rr) rr)/2
+    const T rr[5] = {r0 * r0, r0 * r1, r0 * r2, r1 * r1, r1 * r2};
+    const T Sv[5] = {S0 * r0 + r0 * S0, S0 * r1 + r0 * S1, S0 * r2 + r0 * S2, S1 * r1 + r1 * S1,
+                     S1 * r2 + r1 * S2};
+    const T Qv[5] = {Qr0 * r0 + r0 * Qr0, Qr0 * r1 + r0 * Qr1, Qr0 * r2 + r0 * Qr2, Qr1 * r1 + r1 * Qr1,
+                     Qr1 * r2 + r1 * Qr2};
+    const T Qj[5] = {q1, q2, q3, q4, q5};
+    const T cdiag = (D + E) * Sr + half * (F + H) * qdd;  // coefficient of I (xx and yy slots)
+    const T crr = C * Sr + half * I * qdd;
+#pragma unroll
+    for (int p = 0; p < 5; ++p) {
+      T v = crr * rr[p] + D * Sv[p] + half * (G * Qj[p] + 2 * H * Qv[p]);
+      if (p == 0 || p == 3) v += cdiag;
+      g[p] += v;
+    }
+  }
+  const int set = (nsets == 1) ? 0 : (dgi_s ? rhs >> 1 : rhs);
+  const double ds = dg_s[(size_t)set * n + i], dq = dg_q[(size_t)set * n + i];
+  const double* mi = mr + 8 * (size_t)i;
+  double h[8];
+  for (int c = 0; c < 3; ++c) h[c] = ds * mi[c] + (double)e[c];
+  for (int c = 0; c < 5; ++c) h[3 + c] = dq * mi[3 + c] + (double)g[c];
+  if (dgi_s) {
+    const double sgn = (rhs & 1) ? 1.0 : -1.0;
+    const double is = sgn * dgi_s[(size_t)set * n + i], iq = sgn * dgi_q[(size_t)set * n + i];
+    const double* mp = m + (size_t)(rhs ^ 1) * n * 8 + 8 * (size_t)i;
+    for (int c = 0; c < 3; ++c) h[c] += is * mp[c];
+    for (int c = 0; c < 5; ++c) h[3 + c] += iq * mp[3 + c];
+  }
+  double* o = out + ((size_t)rhs * n + i) * 8;
+  if (parts) {
+    double* hr = hr_out + ((size_t)rhs * n + i) * 8;
+    for (int c = 0; c < 8; ++c) {
+      hr[c] = h[c];
+      o[c] = 0.0;
+    }
+  } else {
+    for (int c = 0; c < 8; ++c) o[c] = h[c];
   }
 }
 
@@ -378,17 +510,17 @@ template <typename T>
 struct Stage {
   int* gx;     // [ch] first x node of the stencil
   int* orig;   // [ch] caller's particle index
-  T* m;        // [ch][3] dipole (spread)
+  T* m;        // [ch][NC] moments (spread)
   T* fx;       // [ch][P] x factors
   T* w;        // [rows][ch] y*z factor of each tile row
 };
 
-template <typename T>
+template <typename T, int NC>
 __device__ inline Stage<T> stage_layout(unsigned char* base, int ch, int P, int rows) {
   Stage<T> st;
   T* tb = reinterpret_cast<T*>(base);
   st.m = tb;
-  st.fx = st.m + 3 * ch;
+  st.fx = st.m + NC * ch;
   st.w = st.fx + (size_t)P * ch;
   st.gx = reinterpret_cast<int*>(st.w + (size_t)rows * ch);
   st.orig = st.gx + ch;
@@ -432,16 +564,15 @@ __device__ inline int candidate_particle(const int* s_start, const int* s_cum, i
 
 // Write the staged data of sorted particle s (record rc, node offsets b) into
 // stage slot t.
-template <typename T, bool WITH_M, typename V4>
+template <typename T, bool WITH_M, int NC, typename V4>
 __device__ inline void stage_at(const TileArgs& a, const Stage<T>& st, int t, int s, int4 rc, V4 b, int ty0, int tz0,
                                 int rhs) {
   st.gx[t] = rc.x;
   st.orig[t] = rc.w;
   if (WITH_M) {
-    const double* mp = a.ms + 3 * ((size_t)rhs * a.n + s);
-    st.m[3 * t] = (T)mp[0];
-    st.m[3 * t + 1] = (T)mp[1];
-    st.m[3 * t + 2] = (T)mp[2];
+    const double* mp = a.ms + NC * ((size_t)rhs * a.n + s);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) st.m[NC * t + c] = (T)mp[c];
   }
   for (int o = 0; o < a.P; ++o) st.fx[t * a.P + o] = (T)a.norm * stencil_factor<T>(a, 0, b.x, o);
   T wy[4], wz[4];
@@ -459,7 +590,7 @@ __device__ inline void stage_at(const TileArgs& a, const Stage<T>& st, int t, in
 // Phase 1: thread t stages candidate base+t (if any): one 16-byte record, the
 // node offsets and (spread) the sorted dipole -- independent loads -- then
 // the P x factors and the weight of every tile row from the Gaussians.
-template <typename T, bool WITH_M>
+template <typename T, bool WITH_M, int NC>
 __device__ inline void stage_candidates(const TileArgs& a, const Stage<T>& st, int base, int total, const int* s_start,
                                         const int* s_cum, int nr, int ty0, int tz0, int rhs) {
   const int t = threadIdx.x;
@@ -482,10 +613,9 @@ __device__ inline void stage_candidates(const TileArgs& a, const Stage<T>& st, i
   st.gx[t] = rc.x;
   st.orig[t] = rc.w;
   if (WITH_M) {
-    const double* mp = a.ms + 3 * ((size_t)rhs * a.n + s);
-    st.m[3 * t] = (T)mp[0];
-    st.m[3 * t + 1] = (T)mp[1];
-    st.m[3 * t + 2] = (T)mp[2];
+    const double* mp = a.ms + NC * ((size_t)rhs * a.n + s);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) st.m[NC * t + c] = (T)mp[c];
   }
   for (int o = 0; o < a.P; ++o) st.fx[t * a.P + o] = (T)a.norm * stencil_factor<T>(a, 0, b.x, o);
   T wy[4], wz[4];
@@ -500,70 +630,12 @@ __device__ inline void stage_candidates(const TileArgs& a, const Stage<T>& st, i
   for (int r = 0; r < a.rows; ++r) st.w[r * a.ch + t] = wy[r % a.ty] * wz[r / a.ty];
 }
 
-template <typename T, int E, int N, typename TS = T>
-__global__ void __launch_bounds__(256) k_spread_xfft(TileArgs a, const double* __restrict__ m,
-                                                     cplx<TS>* __restrict__ spec) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  __shared__ int s_start[kMaxRanges], s_cum[kMaxRanges + 1], s_nr;
-  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
-  cplx<T>* buf = tw + a.nx;
-  cplx<T>* scratch = buf + a.ncl * a.lstride;
-  unsigned char* stage_base = reinterpret_cast<unsigned char*>(buf + a.ncl * a.lstride * (E == 0 && N == 0 ? 2 : 1));
-  const Stage<T> st = stage_layout<T>(stage_base, a.ch, a.P, a.rows);
-  const int ty0 = blockIdx.x * a.ty, tz0 = blockIdx.y * a.tz, rhs = blockIdx.z;
-  const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
-  for (int k = threadIdx.x; k < a.nx; k += blockDim.x) tw[k] = twg[k];
-  for (int k = threadIdx.x; k < a.ncl * a.lstride; k += blockDim.x) buf[k] = cmk<T>(0, 0);
-  tile_ranges(a, ty0, tz0, s_start, s_cum, &s_nr);
-  const int total = s_cum[s_nr];
-  const int P = a.P;
-  for (int base = 0; base < total; base += a.ch) {
-    stage_candidates<T, true>(a, st, base, total, s_start, s_cum, s_nr, ty0, tz0, rhs);
-    __syncthreads();
-    const int nitems = min(a.ch, total - base) * a.rows;
-    for (int item = threadIdx.x; item < nitems; item += blockDim.x) {
-      const int cnd = item / a.rows, row = item - cnd * a.rows;
-      const T wyz = st.w[row * a.ch + cnd];
-      if (wyz == (T)0) continue;
-      const T m0 = st.m[3 * cnd], m1 = st.m[3 * cnd + 1], m2 = st.m[3 * cnd + 2];
-      const T* fx = st.fx + cnd * P;
-      const int l0 = row, l1 = a.rows + row, l2 = 2 * a.rows + row;
-      T* p0 = reinterpret_cast<T*>(buf + (l0 >> 1) * a.lstride) + (l0 & 1);
-      T* p1 = reinterpret_cast<T*>(buf + (l1 >> 1) * a.lstride) + (l1 & 1);
-      T* p2 = reinterpret_cast<T*>(buf + (l2 >> 1) * a.lstride) + (l2 & 1);
-      int x = st.gx[cnd];
-      if (P <= kPUnroll) {
-        // factors into registers first so the LDS atomics issue back to back
-        T f[kPUnroll];
-#pragma unroll
-        for (int o = 0; o < kPUnroll; ++o) f[o] = (o < P) ? fx[o] : (T)0;
-#pragma unroll
-        for (int o = 0; o < kPUnroll; ++o) {
-          if (o < P) {
-            const T w = wyz * f[o];
-            const int px = 2 * pidx(x);
-            atomicAdd(p0 + px, w * m0);
-            atomicAdd(p1 + px, w * m1);
-            atomicAdd(p2 + px, w * m2);
-            if (++x == a.nx) x = 0;
-          }
-        }
-      } else {
-        for (int o = 0; o < P; ++o) {
-          const T w = wyz * fx[o];
-          const int px = 2 * pidx(x);
-          atomicAdd(p0 + px, w * m0);
-          atomicAdd(p1 + px, w * m1);
-          atomicAdd(p2 + px, w * m2);
-          if (++x == a.nx) x = 0;
-        }
-      }
-    }
-    __syncthreads();
-  }
-  fft_any<T, E, N>(buf, a.lstride, a.ncl, a.plan, tw, false, scratch);
-  // split packed pairs into the two half spectra and store
-  const int nreal = 3 * a.rows;
+// Write the NC*rows real rows of a tile (packed two per complex LDS line,
+// forward-transformed along x) into the half spectrum.
+template <typename T, typename TS, int NC>
+__device__ inline void store_half_spectrum(const TileArgs& a, const cplx<T>* buf, int ty0, int tz0, int rhs,
+                                           cplx<TS>* __restrict__ spec) {
+  const int nreal = NC * a.rows;
   const int tot = nreal * a.kx;
   for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
     const int l = idx / a.kx, k = idx - l * a.kx;
@@ -576,8 +648,74 @@ __global__ void __launch_bounds__(256) k_spread_xfft(TileArgs a, const double* _
     cplx<TS> v;
     if ((l & 1) == 0) v = cmk<TS>((TS)((Z.x + Zc.x) * (T)0.5), (TS)((Z.y + Zc.y) * (T)0.5));
     else v = cmk<TS>((TS)((Z.y - Zc.y) * (T)0.5), (TS)((Zc.x - Z.x) * (T)0.5));
-    spec[(((size_t)(rhs * 3 + c) * a.nz + z) * a.ny + y) * a.kx + k] = v;
+    spec[(((size_t)(rhs * NC + c) * a.nz + z) * a.ny + y) * a.kx + k] = v;
   }
+}
+
+template <typename T, int E, int N, typename TS, int NC>
+__global__ void __launch_bounds__(256) k_spread_xfft(TileArgs a, const double* __restrict__ m,
+                                                     cplx<TS>* __restrict__ spec) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ int s_start[kMaxRanges], s_cum[kMaxRanges + 1], s_nr;
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* buf = tw + a.nx;
+  cplx<T>* scratch = buf + a.ncl * a.lstride;
+  unsigned char* stage_base = reinterpret_cast<unsigned char*>(buf + a.ncl * a.lstride * (E == 0 && N == 0 ? 2 : 1));
+  const Stage<T> st = stage_layout<T, NC>(stage_base, a.ch, a.P, a.rows);
+  const int ty0 = blockIdx.x * a.ty, tz0 = blockIdx.y * a.tz, rhs = blockIdx.z;
+  const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
+  for (int k = threadIdx.x; k < a.nx; k += blockDim.x) tw[k] = twg[k];
+  for (int k = threadIdx.x; k < a.ncl * a.lstride; k += blockDim.x) buf[k] = cmk<T>(0, 0);
+  tile_ranges(a, ty0, tz0, s_start, s_cum, &s_nr);
+  const int total = s_cum[s_nr];
+  const int P = a.P;
+  for (int base = 0; base < total; base += a.ch) {
+    stage_candidates<T, true, NC>(a, st, base, total, s_start, s_cum, s_nr, ty0, tz0, rhs);
+    __syncthreads();
+    const int nitems = min(a.ch, total - base) * a.rows;
+    for (int item = threadIdx.x; item < nitems; item += blockDim.x) {
+      const int cnd = item / a.rows, row = item - cnd * a.rows;
+      const T wyz = st.w[row * a.ch + cnd];
+      if (wyz == (T)0) continue;
+      T mv[NC];
+      T* pl[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        mv[c] = st.m[NC * cnd + c];
+        const int l = c * a.rows + row;
+        pl[c] = reinterpret_cast<T*>(buf + (l >> 1) * a.lstride) + (l & 1);
+      }
+      const T* fx = st.fx + cnd * P;
+      int x = st.gx[cnd];
+      if (P <= kPUnroll) {
+        // factors into registers first so the LDS atomics issue back to back
+        T f[kPUnroll];
+#pragma unroll
+        for (int o = 0; o < kPUnroll; ++o) f[o] = (o < P) ? fx[o] : (T)0;
+#pragma unroll
+        for (int o = 0; o < kPUnroll; ++o) {
+          if (o < P) {
+            const T w = wyz * f[o];
+            const int px = 2 * pidx(x);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) atomicAdd(pl[c] + px, w * mv[c]);
+            if (++x == a.nx) x = 0;
+          }
+        }
+      } else {
+        for (int o = 0; o < P; ++o) {
+          const T w = wyz * fx[o];
+          const int px = 2 * pidx(x);
+#pragma unroll
+          for (int c = 0; c < NC; ++c) atomicAdd(pl[c] + px, w * mv[c]);
+          if (++x == a.nx) x = 0;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  fft_any<T, E, N>(buf, a.lstride, a.ncl, a.plan, tw, false, scratch);
+  store_half_spectrum<T, TS, NC>(a, buf, ty0, tz0, rhs, spec);
 }
 
 // Exclusive prefix sum of a[0..n) in place (whole workgroup; part has
@@ -610,7 +748,7 @@ __device__ inline void block_exclusive_scan(int* a, int n, int* part) {
 // consecutive nodes of one tile row and sums the particles whose x window
 // [gx, gx+P) covers them (a contiguous bucket range of the sorted order).
 // Needs P + XS - 1 <= nx.
-template <typename T, int E, int N, int XS>
+template <typename T, int E, int N, int XS, int NC>
 __global__ void __launch_bounds__(256) k_spread_gather_xfft(TileArgs a, const double* __restrict__ m,
                                                             cplx<T>* __restrict__ spec) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -620,7 +758,7 @@ __global__ void __launch_bounds__(256) k_spread_gather_xfft(TileArgs a, const do
   cplx<T>* buf = tw + a.nx;
   cplx<T>* scratch = buf + a.ncl * a.lstride;
   unsigned char* stage_base = reinterpret_cast<unsigned char*>(buf + a.ncl * a.lstride * (E == 0 && N == 0 ? 2 : 1));
-  const Stage<T> st = stage_layout<T>(stage_base, a.ch, a.P, a.rows);
+  const Stage<T> st = stage_layout<T, NC>(stage_base, a.ch, a.P, a.rows);
   int* cnt = st.orig + a.ch;     // nx + 1 bucket starts
   const int ty0 = blockIdx.x * a.ty, tz0 = blockIdx.y * a.tz, rhs = blockIdx.z;
   const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
@@ -650,13 +788,15 @@ __global__ void __launch_bounds__(256) k_spread_gather_xfft(TileArgs a, const do
     __syncthreads();
     block_exclusive_scan(cnt, nx + 1, s_part);
     // 1b: stage straight into the sorted slot
-    if (t < nc) stage_at<T, true>(a, st, cnt[rc.x] + rk, s, rc, bv, ty0, tz0, rhs);
+    if (t < nc) stage_at<T, true, NC>(a, st, cnt[rc.x] + rk, s, rc, bv, ty0, tz0, rhs);
     __syncthreads();
     for (int item = threadIdx.x; item < nitems; item += blockDim.x) {
       const int row = item / nseg, x0 = (item - row * nseg) * XS;
-      T acc[XS][3];
+      T acc[XS][NC];
 #pragma unroll
-      for (int k = 0; k < XS; ++k) acc[k][0] = acc[k][1] = acc[k][2] = (T)0;
+      for (int k = 0; k < XS; ++k)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[k][c] = (T)0;
       const int lo = x0 - P + 1, hi = x0 + XS;  // buckets [lo, hi)
       int q0[2], q1[2];
       if (lo >= 0) {
@@ -665,28 +805,29 @@ __global__ void __launch_bounds__(256) k_spread_gather_xfft(TileArgs a, const do
         q0[0] = cnt[lo + nx]; q1[0] = cnt[nx]; q0[1] = cnt[0]; q1[1] = cnt[hi];
       }
       for (int part = 0; part < 2; ++part) {
-        for (int c = q0[part]; c < q1[part]; ++c) {
-          const T wv = st.w[row * a.ch + c];
+        for (int cc = q0[part]; cc < q1[part]; ++cc) {
+          const T wv = st.w[row * a.ch + cc];
           if (wv == (T)0) continue;
-          int o0 = x0 - st.gx[c];
+          int o0 = x0 - st.gx[cc];
           if (o0 < 0) o0 += nx;
-          const T m0 = st.m[3 * c], m1 = st.m[3 * c + 1], m2 = st.m[3 * c + 2];
-          const T* fx = st.fx + c * P;
+          T mv[NC];
+#pragma unroll
+          for (int c = 0; c < NC; ++c) mv[c] = st.m[NC * cc + c];
+          const T* fx = st.fx + cc * P;
 #pragma unroll
           for (int k = 0; k < XS; ++k) {
             int o = o0 + k;
             if (o >= nx) o -= nx;
             if (o < P) {
               const T f = wv * fx[o];
-              acc[k][0] += f * m0;
-              acc[k][1] += f * m1;
-              acc[k][2] += f * m2;
+#pragma unroll
+              for (int c = 0; c < NC; ++c) acc[k][c] += f * mv[c];
             }
           }
         }
       }
 #pragma unroll
-      for (int comp = 0; comp < 3; ++comp) {
+      for (int comp = 0; comp < NC; ++comp) {
         const int l = comp * a.rows + row;
         T* pl = reinterpret_cast<T*>(buf + (l >> 1) * a.lstride) + (l & 1);
 #pragma unroll
@@ -696,24 +837,10 @@ __global__ void __launch_bounds__(256) k_spread_gather_xfft(TileArgs a, const do
     __syncthreads();
   }
   fft_any<T, E, N>(buf, a.lstride, a.ncl, a.plan, tw, false, scratch);
-  const int nreal = 3 * a.rows;
-  const int tot = nreal * a.kx;
-  for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
-    const int l = idx / a.kx, k = idx - l * a.kx;
-    const int c = l / a.rows, row = l - c * a.rows;
-    const int y = ty0 + row % a.ty, z = tz0 + row / a.ty;
-    if (y >= a.ny || z >= a.nz) continue;
-    const cplx<T>* line = buf + (l >> 1) * a.lstride;
-    const cplx<T> Z = line[pidx(k)];
-    const cplx<T> Zc = cconj(line[pidx(k == 0 ? 0 : a.nx - k)]);
-    cplx<T> v;
-    if ((l & 1) == 0) v = cmk<T>((Z.x + Zc.x) * (T)0.5, (Z.y + Zc.y) * (T)0.5);
-    else v = cmk<T>((Z.y - Zc.y) * (T)0.5, (Zc.x - Z.x) * (T)0.5);
-    spec[(((size_t)(rhs * 3 + c) * a.nz + z) * a.ny + y) * a.kx + k] = v;
-  }
+  store_half_spectrum<T, T, NC>(a, buf, ty0, tz0, rhs, spec);
 }
 
-template <typename T, int E, int N>
+template <typename T, int E, int N, int NC>
 __global__ void __launch_bounds__(256) k_xifft_contract(TileArgs a, const cplx<T>* __restrict__ spec,
                                                         double* __restrict__ out) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -722,12 +849,12 @@ __global__ void __launch_bounds__(256) k_xifft_contract(TileArgs a, const cplx<T
   cplx<T>* buf = tw + a.nx;
   cplx<T>* scratch = buf + a.ncl * a.lstride;
   unsigned char* stage_base = reinterpret_cast<unsigned char*>(buf + a.ncl * a.lstride * (E == 0 && N == 0 ? 2 : 1));
-  const Stage<T> st = stage_layout<T>(stage_base, a.ch, a.P, a.rows);
+  const Stage<T> st = stage_layout<T, NC>(stage_base, a.ch, a.P, a.rows);
   const int ty0 = blockIdx.x * a.ty, tz0 = blockIdx.y * a.tz, rhs = blockIdx.z;
   const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
   for (int k = threadIdx.x; k < a.nx; k += blockDim.x) tw[k] = twg[k];
   // pack two half spectra per complex line: Z = A + iB (Hermitian extension)
-  const int nreal = 3 * a.rows;
+  const int nreal = NC * a.rows;
   const int tot = a.ncl * a.kx;
   for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
     const int q = idx / a.kx, k = idx - q * a.kx;
@@ -736,12 +863,12 @@ __global__ void __launch_bounds__(256) k_xifft_contract(TileArgs a, const cplx<T
     {
       const int c = la / a.rows, row = la - c * a.rows;
       const int y = ty0 + row % a.ty, z = tz0 + row / a.ty;
-      if (y < a.ny && z < a.nz) A = spec[(((size_t)(rhs * 3 + c) * a.nz + z) * a.ny + y) * a.kx + k];
+      if (y < a.ny && z < a.nz) A = spec[(((size_t)(rhs * NC + c) * a.nz + z) * a.ny + y) * a.kx + k];
     }
     if (lb < nreal) {
       const int c = lb / a.rows, row = lb - c * a.rows;
       const int y = ty0 + row % a.ty, z = tz0 + row / a.ty;
-      if (y < a.ny && z < a.nz) B = spec[(((size_t)(rhs * 3 + c) * a.nz + z) * a.ny + y) * a.kx + k];
+      if (y < a.ny && z < a.nz) B = spec[(((size_t)(rhs * NC + c) * a.nz + z) * a.ny + y) * a.kx + k];
     }
     cplx<T>* line = buf + q * a.lstride;
     const bool selfmirror = (k == 0) || (2 * k == a.nx);
@@ -756,24 +883,28 @@ __global__ void __launch_bounds__(256) k_xifft_contract(TileArgs a, const cplx<T
   fft_any<T, E, N>(buf, a.lstride, a.ncl, a.plan, tw, true, scratch);
   const int total = s_cum[s_nr];
   const int P = a.P;
-  double* outr = out + (size_t)rhs * 3 * (size_t)a.n;
+  double* outr = out + (size_t)rhs * NC * (size_t)a.n;
   for (int base = 0; base < total; base += a.ch) {
-    stage_candidates<T, false>(a, st, base, total, s_start, s_cum, s_nr, ty0, tz0, rhs);
+    stage_candidates<T, false, NC>(a, st, base, total, s_start, s_cum, s_nr, ty0, tz0, rhs);
     __syncthreads();
     const int nitems = min(a.ch, total - base) * a.rows;
     const int iters = (nitems + (int)blockDim.x - 1) / (int)blockDim.x;
     for (int it = 0; it < iters; ++it) {
       const int item = it * blockDim.x + threadIdx.x;
-      T acc0 = 0, acc1 = 0, acc2 = 0;
+      T acc[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) acc[c] = (T)0;
       const int cnd = item / a.rows, row = item - cnd * a.rows;
       if (item < nitems) {
         const T wyz = st.w[row * a.ch + cnd];
         if (wyz != (T)0) {
           const T* fx = st.fx + cnd * P;
-          const int l0 = row, l1 = a.rows + row, l2 = 2 * a.rows + row;
-          const T* p0 = reinterpret_cast<const T*>(buf + (l0 >> 1) * a.lstride) + (l0 & 1);
-          const T* p1 = reinterpret_cast<const T*>(buf + (l1 >> 1) * a.lstride) + (l1 & 1);
-          const T* p2 = reinterpret_cast<const T*>(buf + (l2 >> 1) * a.lstride) + (l2 & 1);
+          const T* pl[NC];
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            const int l = c * a.rows + row;
+            pl[c] = reinterpret_cast<const T*>(buf + (l >> 1) * a.lstride) + (l & 1);
+          }
           int x = st.gx[cnd];
           if (P <= kPUnroll) {
 #pragma unroll
@@ -781,9 +912,8 @@ __global__ void __launch_bounds__(256) k_xifft_contract(TileArgs a, const cplx<T
               if (o < P) {
                 const T w = fx[o];
                 const int px = 2 * pidx(x);
-                acc0 += w * p0[px];
-                acc1 += w * p1[px];
-                acc2 += w * p2[px];
+#pragma unroll
+                for (int c = 0; c < NC; ++c) acc[c] += w * pl[c][px];
                 if (++x == a.nx) x = 0;
               }
             }
@@ -791,29 +921,25 @@ __global__ void __launch_bounds__(256) k_xifft_contract(TileArgs a, const cplx<T
             for (int o = 0; o < P; ++o) {
               const T w = fx[o];
               const int px = 2 * pidx(x);
-              acc0 += w * p0[px];
-              acc1 += w * p1[px];
-              acc2 += w * p2[px];
+#pragma unroll
+              for (int c = 0; c < NC; ++c) acc[c] += w * pl[c][px];
               if (++x == a.nx) x = 0;
             }
           }
-          acc0 *= wyz;
-          acc1 *= wyz;
-          acc2 *= wyz;
+#pragma unroll
+          for (int c = 0; c < NC; ++c) acc[c] *= wyz;
         }
       }
       // the rows of one particle are consecutive lanes (rows | 64): reduce them
       for (int o = a.rows / 2; o > 0; o >>= 1) {
-        acc0 += __shfl_xor(acc0, o, a.rows);
-        acc1 += __shfl_xor(acc1, o, a.rows);
-        acc2 += __shfl_xor(acc2, o, a.rows);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[c] += __shfl_xor(acc[c], o, a.rows);
       }
       if (item < nitems && row == 0) {
         const int orig = st.orig[cnd];
-        double* po = outr + 3 * (size_t)orig;
-        atomicAdd(po, (double)acc0 * a.hvol);
-        atomicAdd(po + 1, (double)acc1 * a.hvol);
-        atomicAdd(po + 2, (double)acc2 * a.hvol);
+        double* po = outr + NC * (size_t)orig;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) atomicAdd(po + c, (double)acc[c] * a.hvol);
       }
     }
     __syncthreads();
@@ -868,7 +994,49 @@ struct ScaleParams {
   // Cubic boxes: jt[mx^2+my^2+mz^2] = 9 pi/(2|k|) J_{3/2}(|k|)^2/|k|^2 (0 at k=0);
   // null for other boxes (then computed per k).
   const double* jt;
+  // Cubic boxes, dipole + quadrupole operator: jq[3 m^2 + (0,1,2)] =
+  // (j1^2, j1 j2, j2^2)/|k|^2 with the spherical Bessel functions j1, j2 of
+  // |k| (0 at k=0); null otherwise.
+  const double* jq;
 };
+
+// Spherical Bessel functions j1(k), j2(k): power series below k = 0.6 (the
+// closed forms lose digits to cancellation there), closed forms above.
+__host__ __device__ inline void sph_j12(double k, double& j1, double& j2) {
+  if (k < 0.6) {
+    const double q = k * k;
+    j1 = k / 3.0 *
+         (1.0 - q / 10.0 * (1.0 - q / 28.0 * (1.0 - q / 54.0 * (1.0 - q / 88.0 * (1.0 - q / 130.0 * (1.0 - q / 180.0))))));
+    j2 = q / 15.0 *
+         (1.0 - q / 14.0 * (1.0 - q / 36.0 * (1.0 - q / 66.0 * (1.0 - q / 104.0 * (1.0 - q / 150.0 * (1.0 - q / 204.0))))));
+  } else {
+    const double s = sin(k), c = cos(k), ik = 1.0 / k;
+    j1 = (s * ik - c) * ik;
+    j2 = ((3.0 * ik * ik - 1.0) * s - 3.0 * c * ik) * ik;
+  }
+}
+
+// The four k-space blocks of the dipole + quadrupole operator at one wave
+// vector with unit vector u: with sS = u.F_S and sQ = (u u^T - u_z^2 ..):F_Q
+// in the 5-component traceless form,
+//   E   = (ces sS + i ceq sQ) u,    G_p = (i cgs sS + cgq sQ) (u u^T - I/3)_p.
+template <typename T>
+__device__ inline void quad_scale_point(T u0, T u1, T u2, T ces, T ceq, T cgs, T cgq, const cplx<T>* F,
+                                        cplx<T>* G) {
+  const cplx<T> sS = cmk<T>(u0 * F[0].x + u1 * F[1].x + u2 * F[2].x, u0 * F[0].y + u1 * F[1].y + u2 * F[2].y);
+  const T d0 = u0 * u0 - u2 * u2, d1 = 2 * u0 * u1, d2 = 2 * u0 * u2, d3 = u1 * u1 - u2 * u2, d4 = 2 * u1 * u2;
+  const cplx<T> sQ = cmk<T>(d0 * F[3].x + d1 * F[4].x + d2 * F[5].x + d3 * F[6].x + d4 * F[7].x,
+                            d0 * F[3].y + d1 * F[4].y + d2 * F[5].y + d3 * F[6].y + d4 * F[7].y);
+  const cplx<T> e = cmk<T>(ces * sS.x - ceq * sQ.y, ces * sS.y + ceq * sQ.x);
+  const cplx<T> g = cmk<T>(cgq * sQ.x - cgs * sS.y, cgq * sQ.y + cgs * sS.x);
+  const T third = (T)(1.0 / 3.0);
+  const T f[5] = {u0 * u0 - third, u0 * u1, u0 * u2, u1 * u1 - third, u1 * u2};
+  G[0] = cmk<T>(e.x * u0, e.y * u0);
+  G[1] = cmk<T>(e.x * u1, e.y * u1);
+  G[2] = cmk<T>(e.x * u2, e.y * u2);
+#pragma unroll
+  for (int p = 0; p < 5; ++p) G[3 + p] = cmk<T>(g.x * f[p], g.y * f[p]);
+}
 
 // c(k) = 9 pi/(2|k|) J_{3/2}(|k|)^2 exp(-sum (1-eta_d) k_d^2 / (4 xi^2)) / |k|^2
 __device__ inline double kspace_coef(const ScaleParams& s, double k0, double k1, double k2) {
@@ -880,31 +1048,33 @@ __device__ inline double kspace_coef(const ScaleParams& s, double k0, double k1,
   return 9.0 * kPi / (2.0 * kn) * (j * j) * exp(-ek * s.inv_4xi2) / kk;
 }
 
-template <typename T, int E, int N>
+template <typename T, int E, int N, int NC>
 __global__ void __launch_bounds__(1024) k_fft_z_scale(ColArgs a, ScaleParams sp, cplx<T>* __restrict__ spec) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* buf = tw + a.nz;
-  cplx<T>* scratch = buf + 3 * a.tx * a.lstride;
+  cplx<T>* scratch = buf + NC * a.tx * a.lstride;
   const int kx0 = blockIdx.x * a.tx, y = blockIdx.y, rhs = blockIdx.z;
   const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
   for (int k = threadIdx.x; k < a.nz; k += blockDim.x) tw[k] = twg[k];
   const size_t plane = (size_t)a.ny * a.kx;
-  const int tot = 3 * a.tx * a.nz;
+  const int tot = NC * a.tx * a.nz;
   for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
     const int c = idx / (a.tx * a.nz);
     const int rem = idx - c * a.tx * a.nz;
     const int z = rem / a.tx, cc = rem - z * a.tx;
     const int kx = kx0 + cc;
     cplx<T> v = cmk<T>(0, 0);
-    if (kx < a.kx) v = spec[((size_t)(rhs * 3 + c) * a.nz + z) * plane + (size_t)y * a.kx + kx];
+    if (kx < a.kx) v = spec[((size_t)(rhs * NC + c) * a.nz + z) * plane + (size_t)y * a.kx + kx];
     buf[(c * a.tx + cc) * a.lstride + pidx(z)] = v;
   }
   __syncthreads();
-  fft_any<T, E, N>(buf, a.lstride, 3 * a.tx, a.plan, tw, false, scratch);
-  // k-space scale: G = c(k) M F with M = khat khat^T, cross terms with an
-  // even-grid Nyquist axis removed (the Hermitian part of the reference's
-  // fftshift-ordered operator, so real grids stay real)
+  fft_any<T, E, N>(buf, a.lstride, NC * a.tx, a.plan, tw, false, scratch);
+  // k-space scale.  Even grids: the Hermitian part of the reference's
+  // operator, whose Nyquist bins carry k = -N/2 on both sides of the
+  // spectrum: the matrix averaged over the sign of each Nyquist component
+  // (for the dipole block that removes the cross terms with a Nyquist axis),
+  // so real grids stay real.
   const int my = fft_freq(y, a.ny);
   const bool nyq_y = (a.ny % 2 == 0) && (2 * my == -a.ny);
   for (int idx = threadIdx.x; idx < a.tx * a.nz; idx += blockDim.x) {
@@ -915,36 +1085,70 @@ __global__ void __launch_bounds__(1024) k_fft_z_scale(ColArgs a, ScaleParams sp,
     const bool nyq_x = (a.nx % 2 == 0) && (2 * mx == -a.nx);
     const bool nyq_z = (a.nz % 2 == 0) && (2 * mz == -a.nz);
     const double k0 = mx * sp.twopi_over_l[0], k1 = my * sp.twopi_over_l[1], k2 = mz * sp.twopi_over_l[2];
-    double cf;
-    if (sp.jt) cf = sp.jt[mx * mx + my * my + mz * mz] * (sp.ex[kx] * sp.ey[y] * sp.ez[z]) * sp.inv_g;
-    else cf = kspace_coef(sp, k0, k1, k2) * sp.inv_g;
-    cplx<T>* f0 = buf + (0 * a.tx + cc) * a.lstride + pidx(z);
-    cplx<T>* f1 = buf + (1 * a.tx + cc) * a.lstride + pidx(z);
-    cplx<T>* f2 = buf + (2 * a.tx + cc) * a.lstride + pidx(z);
-    if (cf == 0.0) {
-      *f0 = cmk<T>(0, 0); *f1 = cmk<T>(0, 0); *f2 = cmk<T>(0, 0);
+    cplx<T>* f[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) f[c] = buf + (c * a.tx + cc) * a.lstride + pidx(z);
+    const double kk = (k0 * k0 + k1 * k1) + k2 * k2;
+    if (kk == 0.0) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) *f[c] = cmk<T>(0, 0);
       continue;
     }
-    const double kn = sqrt((k0 * k0 + k1 * k1) + k2 * k2);
+    const double kn = sqrt(kk);
     const T u0 = (T)(k0 / kn), u1 = (T)(k1 / kn), u2 = (T)(k2 / kn);
-    const T c = (T)cf;
-    const T x01 = (nyq_x || nyq_y) ? (T)0 : u0 * u1;
-    const T x02 = (nyq_x || nyq_z) ? (T)0 : u0 * u2;
-    const T x12 = (nyq_y || nyq_z) ? (T)0 : u1 * u2;
-    const cplx<T> F0 = *f0, F1 = *f1, F2 = *f2;
-    const T d00 = u0 * u0, d11 = u1 * u1, d22 = u2 * u2;
-    *f0 = cmk<T>(c * (d00 * F0.x + x01 * F1.x + x02 * F2.x), c * (d00 * F0.y + x01 * F1.y + x02 * F2.y));
-    *f1 = cmk<T>(c * (x01 * F0.x + d11 * F1.x + x12 * F2.x), c * (x01 * F0.y + d11 * F1.y + x12 * F2.y));
-    *f2 = cmk<T>(c * (x02 * F0.x + x12 * F1.x + d22 * F2.x), c * (x02 * F0.y + x12 * F1.y + d22 * F2.y));
+    if constexpr (NC == 3) {
+      double cf;
+      if (sp.jt) cf = sp.jt[mx * mx + my * my + mz * mz] * (sp.ex[kx] * sp.ey[y] * sp.ez[z]) * sp.inv_g;
+      else cf = kspace_coef(sp, k0, k1, k2) * sp.inv_g;
+      const T c = (T)cf;
+      // u_a u_b changes sign under the flip iff exactly one of a, b is a Nyquist axis
+      const T x01 = (nyq_x != nyq_y) ? (T)0 : u0 * u1;
+      const T x02 = (nyq_x != nyq_z) ? (T)0 : u0 * u2;
+      const T x12 = (nyq_y != nyq_z) ? (T)0 : u1 * u2;
+      const cplx<T> F0 = *f[0], F1 = *f[1], F2 = *f[2];
+      const T d00 = u0 * u0, d11 = u1 * u1, d22 = u2 * u2;
+      *f[0] = cmk<T>(c * (d00 * F0.x + x01 * F1.x + x02 * F2.x), c * (d00 * F0.y + x01 * F1.y + x02 * F2.y));
+      *f[1] = cmk<T>(c * (x01 * F0.x + d11 * F1.x + x12 * F2.x), c * (x01 * F0.y + d11 * F1.y + x12 * F2.y));
+      *f[2] = cmk<T>(c * (x02 * F0.x + x12 * F1.x + d22 * F2.x), c * (x02 * F0.y + x12 * F1.y + d22 * F2.y));
+    } else {
+      double a11, a12, a22, ex;
+      if (sp.jq) {
+        const double* q = sp.jq + 3 * (mx * mx + my * my + mz * mz);
+        a11 = q[0]; a12 = q[1]; a22 = q[2];
+        ex = sp.ex[kx] * sp.ey[y] * sp.ez[z];
+      } else {
+        double j1, j2;
+        sph_j12(kn, j1, j2);
+        a11 = j1 * j1 / kk; a12 = j1 * j2 / kk; a22 = j2 * j2 / kk;
+        const double ek =
+            sp.one_minus_eta[0] * k0 * k0 + sp.one_minus_eta[1] * k1 * k1 + sp.one_minus_eta[2] * k2 * k2;
+        ex = exp(-ek * sp.inv_4xi2);
+      }
+      const double w = ex * sp.inv_g;
+      const T ces = (T)(9.0 * a11 * w), ceq = (T)(-22.5 * a12 * w), cgs = (T)(45.0 * a12 * w),
+              cgq = (T)(112.5 * a22 * w);
+      cplx<T> F[NC], G[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) F[c] = *f[c];
+      quad_scale_point<T>(u0, u1, u2, ces, ceq, cgs, cgq, F, G);
+      if (nyq_x || nyq_y || nyq_z) {
+        cplx<T> G2[NC];
+        quad_scale_point<T>(nyq_x ? -u0 : u0, nyq_y ? -u1 : u1, nyq_z ? -u2 : u2, ces, ceq, cgs, cgq, F, G2);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) G[c] = cmk<T>((G[c].x + G2[c].x) * (T)0.5, (G[c].y + G2[c].y) * (T)0.5);
+      }
+#pragma unroll
+      for (int c = 0; c < NC; ++c) *f[c] = G[c];
+    }
   }
   __syncthreads();
-  fft_any<T, E, N>(buf, a.lstride, 3 * a.tx, a.plan, tw, true, scratch);
+  fft_any<T, E, N>(buf, a.lstride, NC * a.tx, a.plan, tw, true, scratch);
   for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
     const int c = idx / (a.tx * a.nz);
     const int rem = idx - c * a.tx * a.nz;
     const int z = rem / a.tx, cc = rem - z * a.tx;
     const int kx = kx0 + cc;
-    if (kx < a.kx) spec[((size_t)(rhs * 3 + c) * a.nz + z) * plane + (size_t)y * a.kx + kx] =
+    if (kx < a.kx) spec[((size_t)(rhs * NC + c) * a.nz + z) * plane + (size_t)y * a.kx + kx] =
         buf[(c * a.tx + cc) * a.lstride + pidx(z)];
   }
 }

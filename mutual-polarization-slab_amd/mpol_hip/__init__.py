@@ -5,6 +5,8 @@ Submodules:
   context  -- Context: one geometry on one GPU (tables, precompute, matvec, GMRES)
   build    -- hipcc build of libmpol.so for gfx950
 """
-from .context import Context, SolveResult, default_tables, real_space_table, require_device, FP32, FP64
+from .context import (Context, SolveResult, default_tables, real_space_table, real_space_table_quad,
+                      require_device, FP32, FP64)
 
-__all__ = ["Context", "SolveResult", "default_tables", "real_space_table", "require_device", "FP32", "FP64"]
+__all__ = ["Context", "SolveResult", "default_tables", "real_space_table", "real_space_table_quad", "require_device",
+           "FP32", "FP64"]

@@ -60,6 +60,13 @@ SIGNATURES = [
     ("mpol_real_space_table", ctypes.c_int, [ctypes.c_double, _DP, ctypes.c_int64, _DP]),
     ("mpol_compute_forces", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int64, _DP, ctypes.c_double, ctypes.c_double,
                                            ctypes.c_int32, ctypes.c_int32, _VP, _VP]),
+    ("mpol_set_permittivity_complex", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int32]),
+    ("mpol_set_quad_tables", ctypes.c_int, [_VP, _DP, ctypes.c_int64]),
+    ("mpol_quad_matvec", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int32, ctypes.c_int32]),
+    ("mpol_quad_matvec_components", ctypes.c_int, [_VP, _VP, _VP, _VP, ctypes.c_int32, ctypes.c_int32]),
+    ("mpol_quad_solve", ctypes.c_int, [_VP, _VP, _VP, _VP, ctypes.c_int32, ctypes.POINTER(SolveOpts),
+                                       ctypes.POINTER(SolveInfo)]),
+    ("mpol_real_space_table_quad", ctypes.c_int, [ctypes.c_double, _DP, ctypes.c_int64, _DP]),
 ]
 
 _lib = None

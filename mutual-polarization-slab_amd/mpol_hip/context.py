@@ -35,6 +35,17 @@ def real_space_table(r, xi):
     return tuple(out[k] for k in range(6))
 
 
+def real_space_table_quad(r, xi):
+    """The eight quadrupole couplings (self term first) in the order
+    pot_quad, field_quad_1..3, grad_quad_1..4, from the library's FP64 host
+    routine."""
+    r = np.ascontiguousarray(np.asarray(r, dtype=np.float64).reshape(-1))
+    out = np.zeros((8, r.size + 1), dtype=np.float64)
+    lib = _lib.load()
+    _lib.check(lib.mpol_real_space_table_quad(float(xi), _lib.dptr(r), r.size, _lib.dptr(out)))
+    return tuple(out[k] for k in range(8))
+
+
 _TABLES: dict = {}
 
 
@@ -76,6 +87,7 @@ class Context:
                                              ctypes.c_void_p(stream)))
         self.n = 0
         self.nsets = 0
+        self.complex = False
         self._keep = []
 
     # -- lifetime -------------------------------------------------------
@@ -125,6 +137,30 @@ class Context:
         self.nsets = 0
 
     def set_permittivity(self, eps):
+        """Permittivities (nsets, N) or (N,).  Complex values switch the
+        context to complex right-hand sides (every operator input and output
+        is then complex)."""
+        if isinstance(eps, torch.Tensor):
+            cplx = eps.is_complex()
+        else:
+            cplx = np.iscomplexobj(np.asarray(eps))
+        if cplx:
+            ec = eps.to(self.device) if isinstance(eps, torch.Tensor) else torch.as_tensor(
+                np.asarray(eps, dtype=np.complex128), device=self.device)
+            ec = ec.to(torch.complex128)
+            if ec.dim() == 1:
+                ec = ec.reshape(1, -1)
+            if ec.shape[1] != self.n:
+                raise ValueError("permittivity must have N entries per set")
+            er = ec.real.contiguous()
+            ei = ec.imag.contiguous()
+            self._sync_stream()
+            _lib.check(self._lib.mpol_set_permittivity_complex(self._ctx, ctypes.c_void_p(er.data_ptr()),
+                                                               ctypes.c_void_p(ei.data_ptr()), er.shape[0]))
+            self.nsets = er.shape[0]
+            self.complex = True
+            self._keep = [er, ei]
+            return
         et = self._dev(eps)
         if et.dim() == 1:
             et = et.reshape(1, -1)
@@ -133,7 +169,16 @@ class Context:
         self._sync_stream()
         _lib.check(self._lib.mpol_set_permittivity(self._ctx, ctypes.c_void_p(et.data_ptr()), et.shape[0]))
         self.nsets = et.shape[0]
+        self.complex = False
         self._keep = [et]
+
+    def set_quad_tables(self, qtabs):
+        """The seven quadrupole tables (7, ntab) on the rvals of set_tables:
+        field_quad_1..3, grad_quad_1..4, entry 0 the self term."""
+        q = np.ascontiguousarray(np.real(np.asarray(qtabs)), dtype=np.float64)
+        if q.ndim != 2 or q.shape[0] != 7:
+            raise ValueError("quadrupole tables must have shape (7, ntab)")
+        _lib.check(self._lib.mpol_set_quad_tables(self._ctx, _lib.dptr(q), q.shape[1]))
 
     TUNING_KNOBS = ("tile_ty", "tile_tz", "chunk", "y_cols", "z_cols", "z_threads")
 
@@ -152,22 +197,66 @@ class Context:
                 "tile_rows": s.tile_rows, "fft_e": tuple(s.fft_e)}
 
     # -- operator -------------------------------------------------------
-    def _as_rhs(self, m):
+    def _as_rhs(self, m, nc=3):
+        """Device FP64 right-hand sides (nrhs, N, nc); with complex
+        permittivities packed as (nrhs, 2, N, nc) = (real part, imaginary part)."""
+        if self.complex:
+            if isinstance(m, torch.Tensor):
+                mt = m.to(self.device)
+                mt = mt.to(torch.complex128) if mt.is_complex() else mt.to(torch.float64).to(torch.complex128)
+            else:
+                mt = torch.as_tensor(np.asarray(m, dtype=np.complex128), device=self.device)
+            if mt.dim() == 2:
+                mt = mt.reshape(1, -1, nc)
+            if mt.shape[1] != self.n or mt.shape[2] != nc:
+                raise ValueError(f"expected moments of shape (nrhs, {self.n}, {nc})")
+            return torch.stack((mt.real, mt.imag), dim=1).contiguous()
         mt = self._dev(m)
         if mt.dim() == 2:
-            mt = mt.reshape(1, -1, 3)
-        if mt.shape[1] != self.n or mt.shape[2] != 3:
-            raise ValueError(f"expected dipoles of shape (nrhs, {self.n}, 3)")
+            mt = mt.reshape(1, -1, nc)
+        if mt.shape[1] != self.n or mt.shape[2] != nc:
+            raise ValueError(f"expected moments of shape (nrhs, {self.n}, {nc})")
         return mt.contiguous()
+
+    def _from_rhs(self, t):
+        """Inverse of _as_rhs for an output buffer."""
+        if self.complex:
+            return torch.complex(t[:, 0], t[:, 1])
+        return t
 
     def matvec(self, m, precision=FP64, slab_factor=1.0, out=None):
         mt = self._as_rhs(m)
-        if out is None:
+        if out is None or self.complex:
             out = torch.empty_like(mt)
         self._sync_stream()
         _lib.check(self._lib.mpol_matvec(self._ctx, ctypes.c_void_p(mt.data_ptr()), ctypes.c_void_p(out.data_ptr()),
                                          mt.shape[0], int(precision), float(slab_factor)))
-        return out
+        return self._from_rhs(out)
+
+    # -- dipole + quadrupole operator -------------------------------------
+    def quad_matvec(self, m, precision=FP64):
+        """(nrhs, N, 8) moments (S, Q) -> (nrhs, N, 8) fields and gradients (E, G)."""
+        mt = self._as_rhs(m, 8)
+        out = torch.empty_like(mt)
+        self._sync_stream()
+        _lib.check(self._lib.mpol_quad_matvec(self._ctx, ctypes.c_void_p(mt.data_ptr()),
+                                              ctypes.c_void_p(out.data_ptr()), mt.shape[0], int(precision)))
+        return self._from_rhs(out)
+
+    def quad_components(self, m, precision=FP64):
+        """Wave-space and real-space parts of quad_matvec."""
+        mt = self._as_rhs(m, 8)
+        hk = torch.empty_like(mt)
+        hr = torch.empty_like(mt)
+        self._sync_stream()
+        _lib.check(self._lib.mpol_quad_matvec_components(self._ctx, ctypes.c_void_p(mt.data_ptr()),
+                                                         ctypes.c_void_p(hk.data_ptr()),
+                                                         ctypes.c_void_p(hr.data_ptr()), mt.shape[0],
+                                                         int(precision)))
+        return self._from_rhs(hk), self._from_rhs(hr)
+
+    def quad_solve(self, b, x0, tol, restart=10, maxit=100, mixed=False, max_refine=8):
+        return self._solve(self._lib.mpol_quad_solve, 8, b, x0, tol, restart, maxit, mixed, max_refine, 0.0)
 
     STAGES = ("slab_sum", "real_space", "spread_xfft", "fft_y_fwd", "fft_z_scale", "fft_y_inv", "xifft_contract")
 
@@ -180,7 +269,7 @@ class Context:
         _lib.check(self._lib.mpol_matvec_timed(self._ctx, ctypes.c_void_p(mt.data_ptr()),
                                                ctypes.c_void_p(out.data_ptr()), mt.shape[0], int(precision),
                                                float(slab_factor), int(repeats), _lib.dptr(ms)))
-        return dict(zip(self.STAGES, ms.tolist())), out
+        return dict(zip(self.STAGES, ms.tolist())), self._from_rhs(out)
 
     def debug_spectrum(self, m, stage, precision=FP64):
         """Grid spectrum after an operator stage, shape (nrhs, 3, nz, ny, nx//2+1)."""
@@ -203,11 +292,11 @@ class Context:
             self._ctx, ctypes.c_void_p(mt.data_ptr()), ctypes.c_void_p(hk.data_ptr()),
             ctypes.c_void_p(hr.data_ptr()), ctypes.c_void_p(hs.data_ptr()), mt.shape[0], int(precision),
             float(slab_factor)))
-        return hk, hr, hs
+        return self._from_rhs(hk), self._from_rhs(hr), self._from_rhs(hs)
 
-    def solve(self, b, x0, tol, restart=10, maxit=100, mixed=False, max_refine=8, slab_factor=1.0):
-        bt = self._as_rhs(b)
-        x0t = self._as_rhs(x0)
+    def _solve(self, fn, nc, b, x0, tol, restart, maxit, mixed, max_refine, slab_factor):
+        bt = self._as_rhs(b, nc)
+        x0t = self._as_rhs(x0, nc)
         if x0t.shape != bt.shape:
             raise ValueError("b and x0 must have the same shape")
         x = torch.empty_like(bt)
@@ -215,11 +304,13 @@ class Context:
                               float(slab_factor))
         info = _lib.SolveInfo()
         self._sync_stream()
-        _lib.check(self._lib.mpol_solve(self._ctx, ctypes.c_void_p(bt.data_ptr()), ctypes.c_void_p(x0t.data_ptr()),
-                                        ctypes.c_void_p(x.data_ptr()), bt.shape[0], ctypes.byref(opts),
-                                        ctypes.byref(info)))
-        return SolveResult(x, info.matvecs_fp64, info.matvecs_fp32, info.converged, info.refinements,
-                           info.max_relres)
+        _lib.check(fn(self._ctx, ctypes.c_void_p(bt.data_ptr()), ctypes.c_void_p(x0t.data_ptr()),
+                      ctypes.c_void_p(x.data_ptr()), bt.shape[0], ctypes.byref(opts), ctypes.byref(info)))
+        return SolveResult(self._from_rhs(x), info.matvecs_fp64, info.matvecs_fp32, info.converged,
+                           info.refinements, info.max_relres)
+
+    def solve(self, b, x0, tol, restart=10, maxit=100, mixed=False, max_refine=8, slab_factor=1.0):
+        return self._solve(self._lib.mpol_solve, 3, b, x0, tol, restart, maxit, mixed, max_refine, slab_factor)
 
     # -- fixed-dipole forces ----------------------------------------------
     def compute_forces(self, pos, dip, box, r_cut, prefactor=1.0, periodic_z=True, use_cell_list=True):

@@ -346,11 +346,13 @@ def prepare_real_space_pairs(x, box, p1, p2, rc, perp, para, rvals):
     return p1, p2, rhat, A, B
 
 
-def real_space_fast(m, lambdap, p1, p2, rhat, A, B, self_perp):
+def real_space_fast(m, lambdap, p1, p2, rhat, A, B, self_perp, complex_eps=False):
     """Hr = (-3/(4 pi (1-eps)) + perp[0]) m_i + sum_j [A m_j + (B-A)(m_j.r)r]
-    ([E]ewald.py:1019-1072)."""
+    ([E]ewald.py:1019-1072).  The Python reference casts eps to float64
+    ([E]ewald.py:1185); complex_eps keeps it complex as the MATLAB version
+    does (Het/RealSpace.m:22)."""
     m = np.asarray(m, dtype=np.complex128)
-    lam = np.asarray(lambdap, dtype=np.float64).reshape(-1)
+    lam = np.asarray(lambdap, dtype=np.complex128 if complex_eps else np.float64).reshape(-1)
     out = ((-3.0 / (4.0 * PI * (1.0 - lam)))[:, None]) * m + m * float(self_perp)
     if np.asarray(p1).size == 0:
         return out
@@ -386,7 +388,7 @@ class OperatorGeometry:
         self.kspace = precompute_kspace(self.Ngrid, self.box, self.xi, self.eta)
 
 
-def field_components(geo: OperatorGeometry, m, lambdap, slab_factor=1.0):
+def field_components(geo: OperatorGeometry, m, lambdap, slab_factor=1.0, complex_eps=False):
     """(Hk, Hr, H_slab) as in magnetic_field_components ([E]ewald.py:1239-1327)."""
     m = np.asarray(m, dtype=np.complex128)
     lin, ws, wc = geo.grid
@@ -396,13 +398,13 @@ def field_components(geo: OperatorGeometry, m, lambdap, slab_factor=1.0):
     Ht = np.fft.ifftn(np.fft.ifftshift(scale_fast(fH, khat, coef), axes=(0, 1, 2)), axes=(0, 1, 2))
     Hk = contract_fast(Ht, geo.Ngrid, lin, wc)
     p1f, p2f, rhat, A, B = geo.real
-    Hr = real_space_fast(m, lambdap, p1f, p2f, rhat, A, B, float(geo.perp[0]))
+    Hr = real_space_fast(m, lambdap, p1f, p2f, rhat, A, B, float(geo.perp[0]), complex_eps)
     slab = np.zeros((m.shape[0], 3), dtype=np.complex128)
     slab[:, 2] = -float(slab_factor) * np.sum(m[:, 2]) / np.prod(geo.box)
     return Hk, Hr, slab
 
 
-def field(geo: OperatorGeometry, m, lambdap, slab_factor=1.0):
+def field(geo: OperatorGeometry, m, lambdap, slab_factor=1.0, complex_eps=False):
     """A m = Hk + Hr - H_slab ([E]ewald.py:1236)."""
-    Hk, Hr, slab = field_components(geo, m, lambdap, slab_factor)
+    Hk, Hr, slab = field_components(geo, m, lambdap, slab_factor, complex_eps)
     return Hk + Hr - slab

@@ -132,8 +132,10 @@ def test_simple_cubic_lattice_is_maxwell_garnett():
 
 def test_real_part_equals_hermitian_scale():
     """The reference's result for real dipoles has an imaginary part only
-    from its Nyquist planes; zeroing the Nyquist cross terms of khat khat^T
-    reproduces its real part exactly (the GPU's formulation)."""
+    from its Nyquist planes (k = -N/2 on both sides of the spectrum); its
+    real part is the operator averaged over the sign of the Nyquist
+    components, i.e. khat khat^T without the cross terms that pair exactly one
+    Nyquist axis with a regular one (the GPU's formulation)."""
     d = np.load(__import__("conftest").GOLDEN + "/operator_n512_g42.npz")
     xi = float(d["xi"])
     perp, para, rv = tables(xi)
@@ -156,7 +158,7 @@ def test_real_part_equals_hermitian_scale():
     for a in range(3):
         for b in range(3):
             if a != b:
-                M[..., a, b] = np.where(NY[a] | NY[b], 0.0, M[..., a, b])
+                M[..., a, b] = np.where(NY[a] ^ NY[b], 0.0, M[..., a, b])
     G = coef[..., None] * np.einsum("...ab,...b->...a", M, F)
     Ht = np.fft.ifftn(G, axes=(0, 1, 2))
     assert np.max(np.abs(Ht.imag)) < 1e-14 * np.max(np.abs(Ht.real))
