@@ -188,6 +188,39 @@ int mpol_quad_matvec_components(mpol_ctx* ctx, const double* m_dev, double* hk_d
 int mpol_quad_solve(mpol_ctx* ctx, const double* b_dev, const double* x0_dev, double* x_dev, int32_t nrhs,
                     const mpol_solve_opts* opts, mpol_solve_info* info);
 
+/* Polydisperse operator (spheres of different radii, Capacitance_Sizes):
+ * radius (device FP64, N, caller's order) and either
+ *   ntypes > 0: radius types type (device int32, N, values in [0, ntypes))
+ *               and the pair tables perp / para (host FP64, one column of
+ *               ntab entries per type pair i <= j in the order
+ *               (0,0), (0,1), ..., (0,M-1), (1,1), ...; on the rvals of
+ *               mpol_set_tables, entry 0 the r -> 0 value), interpolated as
+ *               the reference does (Capacitance_Sizes/RealSpace.m:19-76), or
+ *   ntypes == 0: closed-form pair couplings of the particles' own radii
+ *               (any radius distribution; type, perp, para unused).
+ * The geometry must use the method's isotropic eta and P
+ * (Capacitance_Sizes/ComputeField.m:23-29) with P <= the grid size.
+ * Replaces the radius and table arguments of ComputeField
+ * (Capacitance_Sizes/ComputeField.m:1-2). */
+int mpol_set_sizes(mpol_ctx* ctx, const double* radius_dev, const int32_t* type_dev, int32_t ntypes,
+                   const double* perp_host, const double* para_host);
+
+/* out = A p for dipoles p (nrhs x N x 3): dielectric 3 p/(4 pi a^3 (eps-1)),
+ * real-space pair sum, and one scalar grid per right-hand side (radius-
+ * dependent spread, exp(-(1-eta) q^2/(4 xi^2))/q^2 scale, vector contraction).
+ * Replaces ComputeField (Capacitance_Sizes/ComputeField.m:22-57). */
+int mpol_sizes_matvec(mpol_ctx* ctx, const double* m_dev, double* out_dev, int32_t nrhs, int32_t precision);
+
+/* Wave-space part hk and real-space part hr separately (A p = hk + hr); no
+ * counterpart in the reference (used to check the Ewald splitting). */
+int mpol_sizes_matvec_components(mpol_ctx* ctx, const double* m_dev, double* hk_dev, double* hr_dev,
+                                 int32_t nrhs, int32_t precision);
+
+/* Restarted GMRES for the polydisperse dipoles.  opts.slab_factor is ignored.
+ * Replaces ComputeDipole (Capacitance_Sizes/ComputeDipole.m:25-38). */
+int mpol_sizes_solve(mpol_ctx* ctx, const double* b_dev, const double* x0_dev, double* x_dev, int32_t nrhs,
+                     const mpol_solve_opts* opts, mpol_solve_info* info);
+
 /* Launch-shape tuning (no counterpart in the reference; used by the
  * autotuner): knobs[0..5] = spread/contract tile rows along y and z (ty*tz a
  * power of two, each <= 4), particles staged per chunk (32..256), y-pass and
@@ -211,6 +244,14 @@ int mpol_real_space_table(double xi, const double* r_host, int64_t n, double* ou
  * grad_quad_1, grad_quad_2, grad_quad_3, grad_quad_4.  Replaces those outputs
  * of RealSpaceTable (Capacitance_Quadrupole/RealSpaceTable.m:1-380). */
 int mpol_real_space_table_quad(double xi, const double* r_host, int64_t n, double* out_host);
+
+/* Polydisperse pair tables: for the M radii (host) and separations r (host,
+ * n values, r[i] > 0 for i > 0), perp and para (host, M(M+1)/2 x n, pair
+ * order as in mpol_set_sizes) get the I - rr and rr couplings; entry 0 of
+ * every column is the r -> 0 value whatever r[0] is.
+ * Replaces RealSpaceTable (Capacitance_Sizes/RealSpaceTable.m:1-150). */
+int mpol_real_space_table_sizes(double xi, const double* r_host, int64_t n, const double* radii_host, int32_t M,
+                                double* perp_host, double* para_host);
 
 /* Truncated point-dipole forces and per-particle energies of fixed dipoles:
  * pos, dip (n x 3) and the outputs forces (n x 3), pe (n) are FP64 device

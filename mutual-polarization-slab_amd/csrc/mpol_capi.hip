@@ -15,6 +15,7 @@
 #include "mpol_forces.h"
 #include "mpol_gmres.h"
 #include "mpol_kernels.h"
+#include "mpol_sizes.h"
 
 namespace mpol {
 void real_space_tables(double xi, const double* r, int64_t n, double* out);
@@ -201,6 +202,15 @@ struct mpol_ctx {
   DevBuf<double> rs_Q;
   DevBuf<float> rs_Q32;
   int64_t jq_off = -1;      // offset of the quadrupole Bessel table in ktab (cubic boxes)
+  // polydisperse operator: radii (caller's order), radius types and pair tables
+  // (ntypes > 0) or closed-form couplings (ntypes == 0)
+  bool have_sizes = false;
+  int sz_ntypes = 0;
+  DevBuf<double> sz_radius, sz_rad_sorted, sz_perp, sz_para, sz_A, sz_BA, sz_diag, sz_diag_im, sz_slab;
+  DevBuf<float> sz_A32, sz_BA32;
+  DevBuf<int> sz_type, sz_col;
+  bool sz_pairs_valid = false, sz_diag_valid = false;
+  DevBuf<unsigned char> sz_grid;
   // workspaces
   DevBuf<unsigned char> spec;
   DevBuf<double> zsum;
@@ -281,7 +291,8 @@ const void* base_ptr(mpol_ctx* c) {
 // space).  Stage boundaries recorded when `ev` is given (kStages + 1 events).
 constexpr int kStages = 7;
 constexpr int kZsumBlocks = 64;
-enum { kOpDipole = 3, kOpQuad = 8 };
+enum { kOpDipole = 3, kOpQuad = 8, kOpSizes = 103 };
+inline int unknowns_per_particle(int kind) { return kind == kOpQuad ? 8 : 3; }
 
 // Largest tile (rows of nx nodes, NC components) and particle chunk that fit
 // the LDS budget.
@@ -505,9 +516,133 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
 // Right-hand sides per system: 2 (real, imaginary part) with complex permittivities.
 inline int rhs_per_system(const mpol_ctx* c) { return c->cplx_eps ? 2 : 1; }
 
+// Polydisperse operator: real space (pair couplings of the two radii,
+// dielectric 3/(4 pi a^3 (eps-1))) + one scalar grid per right-hand side.
+// hr != nullptr: hr <- real space, out <- wave space.
+template <typename T>
+int run_operator_sizes(mpol_ctx* c, const double* m, double* out, double* hr, int nrhs) {
+  const int n = c->n;
+  hipStream_t st = c->stream;
+  const size_t G = (size_t)c->ng[0] * c->ng[1] * c->ng[2];
+  const size_t cs = sizeof(cplx<T>);
+  HIPALLOC(c->spec, (size_t)nrhs * c->ng[2] * c->ng[1] * c->kx * cs);
+  HIPALLOC(c->sz_grid, (size_t)nrhs * G * sizeof(T));
+  HIPALLOC(c->zsum, (size_t)nrhs * (kZsumBlocks + 1));
+  HIPALLOC(c->ms, (size_t)nrhs * 3 * n);
+  cplx<T>* spec = reinterpret_cast<cplx<T>*>(c->spec.p);
+  T* grid = reinterpret_cast<T*>(c->sz_grid.p);
+  // real space (the slab term is absent: its sum only needs to be defined)
+  double* zpart = c->zsum.p + nrhs;
+  hipLaunchKernelGGL(k_sum_z, dim3(kZsumBlocks, nrhs), dim3(256), 0, st, m, n, zpart);
+  hipLaunchKernelGGL(k_sum_finish, dim3(cdiv(nrhs, 256)), dim3(256), 0, st, zpart, kZsumBlocks, nrhs, c->zsum.p);
+  const T* pA;
+  const T* pBA;
+  const T* prh;
+  if constexpr (sizeof(T) == 8) {
+    pA = c->sz_A.p; pBA = c->sz_BA.p; prh = c->rs_rh.p;
+  } else {
+    pA = c->sz_A32.p; pBA = c->sz_BA32.p; prh = c->rs_rh32.p;
+  }
+  if (hr) HIPALLOC(c->sz_slab, (size_t)nrhs * 3 * n);
+  hipLaunchKernelGGL((k_real_space<T>), dim3(cdiv(n, 256), nrhs), dim3(256), 0, st, n, nrhs, c->nsets, m, out,
+                     c->sz_diag.p, c->cplx_eps ? c->sz_diag_im.p : nullptr, c->rs_off.p, c->rs_j.p, pA, pBA, prh,
+                     c->zsum.p, 0.0, hr != nullptr ? 1 : 0, hr, hr ? c->sz_slab.p : nullptr);
+  hipLaunchKernelGGL(k_permute_m<3>, dim3(cdiv(n, 256), nrhs), dim3(256), 0, st, m, c->perm.p, n, c->ms.p);
+  // spread into real grids
+  HIPCHK(hipMemsetAsync(grid, 0, (size_t)nrhs * G * sizeof(T), st));
+  SizesGridArgs sa{};
+  sa.nx = c->ng[0]; sa.ny = c->ng[1]; sa.nz = c->ng[2];
+  sa.P = c->P; sa.half = c->half; sa.n = n; sa.nrhs = nrhs;
+  for (int d = 0; d < 3; ++d) sa.h[d] = c->h[d];
+  sa.xi = c->xi;
+  sa.eta = c->eta[0];
+  sa.hvol = c->h[0] * c->h[1] * c->h[2];
+  sa.rec = c->rec.p;
+  sa.base = c->base64.p;
+  sa.rad = c->sz_rad_sorted.p;
+  hipLaunchKernelGGL((k_sizes_spread<T>), dim3(n), dim3(256), 0, st, sa, c->ms.p, grid);
+  // x transform of the real rows
+  const bool gen_x = generic_fft(c->plan[0]);
+  TileCfg tr = tile_cfg_nc<1>(c->ng[0], gen_x, cs, 0, kLdsTile);
+  TileArgs ta{};
+  ta.nx = c->ng[0]; ta.ny = c->ng[1]; ta.nz = c->ng[2]; ta.kx = c->kx;
+  ta.ty = tr.ty; ta.tz = tr.tz; ta.rows = tr.rows; ta.ncl = tr.ncl; ta.lstride = tr.lstride;
+  ta.tw = tw_ptr<T>(c, 0); ta.plan = c->plan[0];
+  const size_t rlds = ((size_t)c->ng[0] + (size_t)tr.ncl * tr.lstride * (gen_x ? 2 : 1)) * cs;
+  if (rlds > kLdsMax) return fail(MPOL_EINVAL, "row transform tile does not fit in LDS");
+  const dim3 tgrid(cdiv(c->ng[1], tr.ty), cdiv(c->ng[2], tr.tz), nrhs);
+#define LAUNCH_R2C(E, N, ...)                                                                   \
+  do {                                                                                          \
+    set_lds(k_rows_r2c<T, E, N>, rlds);                                                         \
+    hipLaunchKernelGGL((k_rows_r2c<T, E, N>), tgrid, dim3(256), rlds, st, ta, grid, spec);      \
+  } while (0)
+  DISPATCH_FFT(c->ng[0], ta.plan.e, LAUNCH_R2C, 0);
+#undef LAUNCH_R2C
+  // y forward, z forward + scale + inverse, y inverse
+  const ColCfg cy = col_cfg(c->ng[1], generic_fft(c->plan[1]), c->kx, 1, cs, kLdsCol,
+                            c->tune[3] > 0 ? c->tune[3] : (sizeof(T) == 8 ? 8 : 16));
+  ColArgs ay{};
+  ay.nx = c->ng[0]; ay.ny = c->ng[1]; ay.nz = c->ng[2]; ay.kx = c->kx; ay.tx = cy.tx; ay.lstride = cy.lstride;
+  ay.plan = c->plan[1]; ay.tw = tw_ptr<T>(c, 1);
+  const dim3 ygrid(cdiv(c->kx, cy.tx), c->ng[2], nrhs);
+  if (cy.lds > kLdsMax) return fail(MPOL_EINVAL, "y-pass columns do not fit in LDS");
+#define LAUNCH_FFTY(E, N, INV)                                                                  \
+  do {                                                                                          \
+    set_lds(k_fft_y<T, E, N>, cy.lds);                                                          \
+    hipLaunchKernelGGL((k_fft_y<T, E, N>), ygrid, dim3(256), cy.lds, st, ay, spec, INV);       \
+  } while (0)
+  DISPATCH_FFT(c->ng[1], ay.plan.e, LAUNCH_FFTY, 0);
+  const ColCfg cz = col_cfg(c->ng[2], generic_fft(c->plan[2]), c->kx, 1, cs, kLdsCol,
+                            c->tune[4] > 0 ? c->tune[4] : (sizeof(T) == 8 ? 8 : 16));
+  ColArgs az{};
+  az.nx = c->ng[0]; az.ny = c->ng[1]; az.nz = c->ng[2]; az.kx = c->kx; az.tx = cz.tx; az.lstride = cz.lstride;
+  az.plan = c->plan[2]; az.tw = tw_ptr<T>(c, 2);
+  ScaleParams sp{};
+  for (int d = 0; d < 3; ++d) {
+    sp.twopi_over_l[d] = 2.0 * kPi / c->box[d];
+    sp.one_minus_eta[d] = 1.0 - c->eta[d];
+  }
+  sp.inv_4xi2 = 1.0 / (4.0 * c->xi * c->xi);
+  sp.inv_g = 1.0 / (double)G;
+  sp.ex = c->ktab.p;
+  sp.ey = sp.ex + c->ng[0];
+  sp.ez = sp.ey + c->ng[1];
+  const dim3 zgrid(cdiv(c->kx, cz.tx), c->ng[1], nrhs);
+  if (cz.lds > kLdsMax) return fail(MPOL_EINVAL, "z-pass columns do not fit in LDS");
+  int zthreads = 256;
+  {
+    const int e = is_ct_size(c->ng[2]) ? ct_plan(c->ng[2]).e : c->plan[2].e;
+    if (e > 0) zthreads = std::min(1024, std::max(256, cz.tx * (c->ng[2] / e)));
+    zthreads = (zthreads + 63) / 64 * 64;
+  }
+#define LAUNCH_FFTZ(E, N, ...)                                                                  \
+  do {                                                                                          \
+    set_lds(k_fft_z_scale<T, E, N, 1>, cz.lds);                                                 \
+    hipLaunchKernelGGL((k_fft_z_scale<T, E, N, 1>), zgrid, dim3(zthreads), cz.lds, st, az, sp, spec); \
+  } while (0)
+  DISPATCH_FFT(c->ng[2], az.plan.e, LAUNCH_FFTZ, 0);
+#undef LAUNCH_FFTZ
+  DISPATCH_FFT(c->ng[1], ay.plan.e, LAUNCH_FFTY, 1);
+#undef LAUNCH_FFTY
+#define LAUNCH_C2R(E, N, ...)                                                                   \
+  do {                                                                                          \
+    set_lds(k_rows_c2r<T, E, N>, rlds);                                                         \
+    hipLaunchKernelGGL((k_rows_c2r<T, E, N>), tgrid, dim3(256), rlds, st, ta, spec, grid);      \
+  } while (0)
+  DISPATCH_FFT(c->ng[0], ta.plan.e, LAUNCH_C2R, 0);
+#undef LAUNCH_C2R
+  hipLaunchKernelGGL((k_sizes_contract<T>), dim3(n), dim3(256), 0, st, sa, grid, out);
+  HIPCHK(hipGetLastError());
+  return MPOL_OK;
+}
+
 // Operator `kind` (kOpDipole / kOpQuad) on nsys systems.
 int apply(mpol_ctx* c, int kind, const double* m, double* out, int nsys, int precision, double slab) {
   const int nrhs = nsys * rhs_per_system(c);
+  if (kind == kOpSizes) {
+    if (precision == MPOL_FP32) return run_operator_sizes<float>(c, m, out, nullptr, nrhs);
+    return run_operator_sizes<double>(c, m, out, nullptr, nrhs);
+  }
   if (kind == kOpQuad) {
     if (precision == MPOL_FP32) return run_operator<float, kOpQuad>(c, m, out, nullptr, nullptr, nrhs, 0.0);
     return run_operator<double, kOpQuad>(c, m, out, nullptr, nullptr, nrhs, 0.0);
@@ -537,7 +672,7 @@ int dots(mpol_ctx* c, const double* A, size_t stride_i, int cnt, const double* w
 // (kind components per particle, times 2 with complex permittivities).
 int gmres_core(mpol_ctx* c, int kind, const double* b, double* x, int nrhs, double tol, int restart, int maxit,
                int precision, double slab, int64_t* nmv, double* relres_host, int* converged_host) {
-  const int64_t n3 = (int64_t)kind * c->n * rhs_per_system(c);
+  const int64_t n3 = (int64_t)unknowns_per_particle(kind) * c->n * rhs_per_system(c);
   const size_t tot = (size_t)nrhs * n3;
   const size_t sv = tot;  // stride between basis vectors
   HIPALLOC(c->gV, (size_t)(restart + 1) * tot);
@@ -695,6 +830,7 @@ int mpol_set_tables(mpol_ctx* c, const double* perp, const double* para, const d
   c->self_perp = perp[0];
   c->have_geo = false;   // pair data depends on the tables
   c->have_qtab = false;  // quadrupole tables live on the same rvals
+  c->have_sizes = false; // and so do the polydisperse tables
   return MPOL_OK;
 }
 
@@ -872,6 +1008,7 @@ int mpol_set_geometry(mpol_ctx* c, const double* x, int64_t n, const double box[
   HIPCHK(hipGetLastError());
   c->have_geo = true;
   c->quad_pairs_valid = false;
+  c->sz_pairs_valid = false;
   return MPOL_OK;
 }
 
@@ -946,12 +1083,56 @@ int prepare_quad(mpol_ctx* c) {
   return MPOL_OK;
 }
 
+// Polydisperse pair coefficients, sorted radii and diagonal, (re)built when the
+// geometry, the radii/tables or the permittivities changed.
+int prepare_sizes(mpol_ctx* c) {
+  if (!c->have_sizes) return fail(MPOL_ESTATE, "radii not set (call mpol_set_sizes first)");
+  hipStream_t st = c->stream;
+  for (int d = 0; d < 3; ++d)
+    if (c->P > c->ng[d]) return fail(MPOL_EINVAL, "polydisperse operator needs P <= grid size on every axis");
+  if (!c->sz_pairs_valid) {
+    const size_t np = (size_t)std::max<int64_t>(1, c->npairs);
+    HIPALLOC(c->sz_A, np);
+    HIPALLOC(c->sz_BA, np);
+    HIPALLOC(c->sz_A32, np);
+    HIPALLOC(c->sz_BA32, np);
+    HIPALLOC(c->sz_rad_sorted, (size_t)c->n);
+    hipLaunchKernelGGL(k_permute_scalar, dim3(cdiv(c->n, 256)), dim3(256), 0, st, c->sz_radius.p, c->perm.p, c->n,
+                       c->sz_rad_sorted.p);
+    PairParams pp{};
+    pp.n = c->n;
+    for (int d = 0; d < 3; ++d) {
+      pp.box[d] = c->box[d];
+      pp.nc[d] = c->nc[d];
+    }
+    pp.rc = c->rc;
+    pp.ntab = c->ntab;
+    hipLaunchKernelGGL(k_pairs_sizes, dim3(cdiv(c->n, 256)), dim3(256), 0, st, pp, c->x.p, c->rs_off.p, c->rs_j.p,
+                       c->rvals.p, c->sz_radius.p, c->sz_type.p, c->sz_ntypes, c->sz_col.p, c->sz_perp.p,
+                       c->sz_para.p, c->xi, c->sz_A.p, c->sz_BA.p, c->sz_A32.p, c->sz_BA32.p);
+    HIPCHK(hipGetLastError());
+    c->sz_pairs_valid = true;
+  }
+  if (!c->sz_diag_valid) {
+    const int64_t total = (int64_t)c->nsets * c->n;
+    HIPALLOC(c->sz_diag, (size_t)total);
+    if (c->cplx_eps) HIPALLOC(c->sz_diag_im, (size_t)total);
+    hipLaunchKernelGGL(k_diag_sizes, dim3(cdiv(total, 256)), dim3(256), 0, st, c->eps_re.p,
+                       c->cplx_eps ? c->eps_im.p : nullptr, c->n, total, c->sz_radius.p, c->sz_type.p, c->sz_ntypes,
+                       c->sz_col.p, c->sz_perp.p, c->ntab, c->xi, c->sz_diag.p,
+                       c->cplx_eps ? c->sz_diag_im.p : nullptr);
+    HIPCHK(hipGetLastError());
+    c->sz_diag_valid = true;
+  }
+  return MPOL_OK;
+}
+
 // Restarted GMRES, plain FP64 or FP64 iterative refinement around an
 // FP32-operator GMRES, for operator `kind` on nsys systems.
 int solve_impl(mpol_ctx* c, int kind, const double* b, const double* x0, double* x, int nsys,
                const mpol_solve_opts* o, mpol_solve_info* info) {
   hipStream_t st = c->stream;
-  const int64_t n3 = (int64_t)kind * c->n * rhs_per_system(c);
+  const int64_t n3 = (int64_t)unknowns_per_particle(kind) * c->n * rhs_per_system(c);
   const int64_t tot = n3 * nsys;
   const int nrhs = nsys;
   const double slab = kind == kOpDipole ? o->slab_factor : 0.0;
@@ -1042,6 +1223,7 @@ int mpol_set_permittivity(mpol_ctx* c, const double* eps, int32_t nsets) {
   c->nsets = nsets;
   c->cplx_eps = false;
   c->diag_q_valid = false;
+  c->sz_diag_valid = false;
   return MPOL_OK;
 }
 
@@ -1063,6 +1245,7 @@ int mpol_set_permittivity_complex(mpol_ctx* c, const double* eps_re, const doubl
   c->nsets = nsets;
   c->cplx_eps = true;
   c->diag_q_valid = false;
+  c->sz_diag_valid = false;
   return MPOL_OK;
 }
 
@@ -1140,6 +1323,75 @@ int mpol_solve(mpol_ctx* c, const double* b, const double* x0, double* x, int32_
   CHECK_STATUS(check_solve_args(c, b, x0, x, nrhs, o));
   HIPCHK(hipSetDevice(c->device));
   return solve_impl(c, kOpDipole, b, x0, x, nrhs, o, info);
+}
+
+int mpol_set_sizes(mpol_ctx* c, const double* radius, const int32_t* type, int32_t ntypes, const double* perp,
+                   const double* para) {
+  if (!c || !radius) return fail(MPOL_EINVAL, "null argument");
+  if (!c->have_geo) return fail(MPOL_ESTATE, "geometry not set (call mpol_set_geometry first)");
+  if (ntypes < 0 || ntypes > 4096) return fail(MPOL_EINVAL, "ntypes must be in [0, 4096]");
+  if (ntypes > 0 && (!type || !perp || !para)) return fail(MPOL_EINVAL, "tables and types needed when ntypes > 0");
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = c->stream;
+  const int n = c->n;
+  HIPALLOC(c->sz_radius, (size_t)n);
+  HIPCHK(hipMemcpyAsync(c->sz_radius.p, radius, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  if (ntypes > 0) {
+    const int64_t m = (int64_t)ntypes * (ntypes + 1) / 2;
+    std::vector<int> col((size_t)ntypes * ntypes);
+    int lin = 0;
+    for (int i = 0; i < ntypes; ++i)
+      for (int j = i; j < ntypes; ++j) {
+        col[(size_t)i * ntypes + j] = lin;
+        col[(size_t)j * ntypes + i] = lin;
+        ++lin;
+      }
+    HIPALLOC(c->sz_type, (size_t)n);
+    HIPALLOC(c->sz_col, col.size());
+    HIPALLOC(c->sz_perp, (size_t)m * c->ntab);
+    HIPALLOC(c->sz_para, (size_t)m * c->ntab);
+    HIPCHK(hipMemcpyAsync(c->sz_type.p, type, n * sizeof(int), hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(c->sz_col.p, col.data(), col.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(c->sz_perp.p, perp, (size_t)m * c->ntab * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(c->sz_para.p, para, (size_t)m * c->ntab * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  c->sz_ntypes = ntypes;
+  c->have_sizes = true;
+  c->sz_pairs_valid = false;
+  c->sz_diag_valid = false;
+  return MPOL_OK;
+}
+
+int mpol_sizes_matvec(mpol_ctx* c, const double* m, double* out, int32_t nrhs, int32_t precision) {
+  CHECK_STATUS(check_ready(c));
+  if (!m || !out || nrhs < 1) return fail(MPOL_EINVAL, "bad matvec arguments");
+  CHECK_STATUS(check_sets(c, nrhs));
+  if (precision != MPOL_FP64 && precision != MPOL_FP32) return fail(MPOL_EINVAL, "unknown precision");
+  HIPCHK(hipSetDevice(c->device));
+  CHECK_STATUS(prepare_sizes(c));
+  return apply(c, kOpSizes, m, out, nrhs, precision, 0.0);
+}
+
+int mpol_sizes_matvec_components(mpol_ctx* c, const double* m, double* hk, double* hr, int32_t nrhs,
+                                 int32_t precision) {
+  CHECK_STATUS(check_ready(c));
+  if (!m || !hk || !hr || nrhs < 1) return fail(MPOL_EINVAL, "bad matvec arguments");
+  CHECK_STATUS(check_sets(c, nrhs));
+  HIPCHK(hipSetDevice(c->device));
+  CHECK_STATUS(prepare_sizes(c));
+  const int nr = nrhs * rhs_per_system(c);
+  if (precision == MPOL_FP32) return run_operator_sizes<float>(c, m, hk, hr, nr);
+  return run_operator_sizes<double>(c, m, hk, hr, nr);
+}
+
+int mpol_sizes_solve(mpol_ctx* c, const double* b, const double* x0, double* x, int32_t nrhs,
+                     const mpol_solve_opts* o, mpol_solve_info* info) {
+  CHECK_STATUS(check_ready(c));
+  CHECK_STATUS(check_solve_args(c, b, x0, x, nrhs, o));
+  HIPCHK(hipSetDevice(c->device));
+  CHECK_STATUS(prepare_sizes(c));
+  return solve_impl(c, kOpSizes, b, x0, x, nrhs, o, info);
 }
 
 int mpol_set_quad_tables(mpol_ctx* c, const double* qtab, int64_t ntab) {
@@ -1225,6 +1477,25 @@ int mpol_real_space_table(double xi, const double* r, int64_t n, double* out) {
   for (int64_t i = 0; i < n; ++i)
     if (!(r[i] > 0.0)) return fail(MPOL_EINVAL, "table separations must be positive");
   real_space_tables(xi, r, n, out);
+  return MPOL_OK;
+}
+
+int mpol_real_space_table_sizes(double xi, const double* r, int64_t n, const double* radii, int32_t M,
+                                double* perp, double* para) {
+  if (!r || !radii || !perp || !para || n < 1 || M < 1) return fail(MPOL_EINVAL, "bad table arguments");
+  if (!(xi > 0.0)) return fail(MPOL_EINVAL, "xi must be positive");
+  for (int32_t k = 0; k < M; ++k)
+    if (!(radii[k] > 0.0)) return fail(MPOL_EINVAL, "radii must be positive");
+  for (int64_t i = 1; i < n; ++i)
+    if (!(r[i] > 0.0)) return fail(MPOL_EINVAL, "table separations after the first must be positive");
+  int64_t lin = 0;
+  for (int32_t i = 0; i < M; ++i)
+    for (int32_t j = i; j < M; ++j, ++lin) {
+      double* tp = perp + lin * n;
+      double* tq = para + lin * n;
+      tp[0] = tq[0] = sizes_self(radii[i], radii[j], xi);  // row 0 is the r -> 0 value
+      for (int64_t k = 1; k < n; ++k) sizes_coupling(r[k], radii[i], radii[j], xi, tp[k], tq[k]);
+    }
   return MPOL_OK;
 }
 

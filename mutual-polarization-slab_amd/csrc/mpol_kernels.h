@@ -1096,7 +1096,12 @@ __global__ void __launch_bounds__(1024) k_fft_z_scale(ColArgs a, ScaleParams sp,
     }
     const double kn = sqrt(kk);
     const T u0 = (T)(k0 / kn), u1 = (T)(k1 / kn), u2 = (T)(k2 / kn);
-    if constexpr (NC == 3) {
+    if constexpr (NC == 1) {
+      // polydisperse scalar grid: exp(-(1-eta) q^2/(4 xi^2))/q^2
+      const double cf = (sp.ex[kx] * sp.ey[y] * sp.ez[z]) / kk * sp.inv_g;
+      const cplx<T> F0 = *f[0];
+      *f[0] = cmk<T>((T)cf * F0.x, (T)cf * F0.y);
+    } else if constexpr (NC == 3) {
       double cf;
       if (sp.jt) cf = sp.jt[mx * mx + my * my + mz * mz] * (sp.ex[kx] * sp.ey[y] * sp.ez[z]) * sp.inv_g;
       else cf = kspace_coef(sp, k0, k1, k2) * sp.inv_g;

@@ -6,7 +6,9 @@ Submodules:
   build    -- hipcc build of libmpol.so for gfx950
 """
 from .context import (Context, SolveResult, default_tables, real_space_table, real_space_table_quad,
+                      real_space_table_sizes,
                       require_device, FP32, FP64)
 
-__all__ = ["Context", "SolveResult", "default_tables", "real_space_table", "real_space_table_quad", "require_device",
+__all__ = ["Context", "SolveResult", "default_tables", "real_space_table", "real_space_table_quad",
+           "real_space_table_sizes", "require_device",
            "FP32", "FP64"]

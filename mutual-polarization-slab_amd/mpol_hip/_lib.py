@@ -67,6 +67,13 @@ SIGNATURES = [
     ("mpol_quad_solve", ctypes.c_int, [_VP, _VP, _VP, _VP, ctypes.c_int32, ctypes.POINTER(SolveOpts),
                                        ctypes.POINTER(SolveInfo)]),
     ("mpol_real_space_table_quad", ctypes.c_int, [ctypes.c_double, _DP, ctypes.c_int64, _DP]),
+    ("mpol_set_sizes", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int32, _DP, _DP]),
+    ("mpol_sizes_matvec", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int32, ctypes.c_int32]),
+    ("mpol_sizes_matvec_components", ctypes.c_int, [_VP, _VP, _VP, _VP, ctypes.c_int32, ctypes.c_int32]),
+    ("mpol_sizes_solve", ctypes.c_int, [_VP, _VP, _VP, _VP, ctypes.c_int32, ctypes.POINTER(SolveOpts),
+                                        ctypes.POINTER(SolveInfo)]),
+    ("mpol_real_space_table_sizes", ctypes.c_int, [ctypes.c_double, _DP, ctypes.c_int64, _DP, ctypes.c_int32, _DP,
+                                                   _DP]),
 ]
 
 _lib = None

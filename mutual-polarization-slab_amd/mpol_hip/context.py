@@ -46,6 +46,20 @@ def real_space_table_quad(r, xi):
     return tuple(out[k] for k in range(8))
 
 
+def real_space_table_sizes(r, radii, xi):
+    """(perp, para), each (len(r), M(M+1)/2) for the M radii: the I - rr and
+    rr pair couplings of every radius pair (i <= j), row 0 the r -> 0 value."""
+    r = np.ascontiguousarray(np.asarray(r, dtype=np.float64).reshape(-1))
+    a = np.ascontiguousarray(np.asarray(radii, dtype=np.float64).reshape(-1))
+    m = a.size * (a.size + 1) // 2
+    perp = np.zeros((m, r.size), dtype=np.float64)
+    para = np.zeros((m, r.size), dtype=np.float64)
+    lib = _lib.load()
+    _lib.check(lib.mpol_real_space_table_sizes(float(xi), _lib.dptr(r), r.size, _lib.dptr(a), a.size,
+                                               _lib.dptr(perp), _lib.dptr(para)))
+    return perp.T.copy(), para.T.copy()
+
+
 _TABLES: dict = {}
 
 
@@ -257,6 +271,51 @@ class Context:
 
     def quad_solve(self, b, x0, tol, restart=10, maxit=100, mixed=False, max_refine=8):
         return self._solve(self._lib.mpol_quad_solve, 8, b, x0, tol, restart, maxit, mixed, max_refine, 0.0)
+
+    # -- polydisperse operator ---------------------------------------------
+    def set_sizes(self, radius, types=None, perp=None, para=None):
+        """Radii (N,).  With types (N,) in [0, M) and pair tables perp, para
+        (ntab, M(M+1)/2) on the rvals of set_tables the pair couplings are
+        interpolated; without them they are evaluated in closed form."""
+        rt = self._dev(radius).reshape(-1)
+        if rt.shape[0] != self.n:
+            raise ValueError("radius must have N entries")
+        if types is None:
+            _lib.check(self._lib.mpol_set_sizes(self._ctx, ctypes.c_void_p(rt.data_ptr()), None, 0, None, None))
+            self._keep_sizes = [rt]
+            return
+        tt = torch.as_tensor(np.asarray(types, dtype=np.int32).reshape(-1), device=self.device).contiguous()
+        M = int(tt.max().item()) + 1
+        pq = np.ascontiguousarray(np.asarray(perp, dtype=np.float64).T)
+        pa = np.ascontiguousarray(np.asarray(para, dtype=np.float64).T)
+        if pq.shape[0] != M * (M + 1) // 2 or pa.shape != pq.shape:
+            raise ValueError("tables must have M(M+1)/2 columns for M radius types")
+        self._sync_stream()
+        _lib.check(self._lib.mpol_set_sizes(self._ctx, ctypes.c_void_p(rt.data_ptr()), ctypes.c_void_p(tt.data_ptr()),
+                                            M, _lib.dptr(pq), _lib.dptr(pa)))
+        self._keep_sizes = [rt, tt]
+
+    def sizes_matvec(self, p, precision=FP64):
+        pt = self._as_rhs(p)
+        out = torch.empty_like(pt)
+        self._sync_stream()
+        _lib.check(self._lib.mpol_sizes_matvec(self._ctx, ctypes.c_void_p(pt.data_ptr()),
+                                               ctypes.c_void_p(out.data_ptr()), pt.shape[0], int(precision)))
+        return self._from_rhs(out)
+
+    def sizes_components(self, p, precision=FP64):
+        pt = self._as_rhs(p)
+        hk = torch.empty_like(pt)
+        hr = torch.empty_like(pt)
+        self._sync_stream()
+        _lib.check(self._lib.mpol_sizes_matvec_components(self._ctx, ctypes.c_void_p(pt.data_ptr()),
+                                                          ctypes.c_void_p(hk.data_ptr()),
+                                                          ctypes.c_void_p(hr.data_ptr()), pt.shape[0],
+                                                          int(precision)))
+        return self._from_rhs(hk), self._from_rhs(hr)
+
+    def sizes_solve(self, b, x0, tol, restart=10, maxit=100, mixed=False, max_refine=8):
+        return self._solve(self._lib.mpol_sizes_solve, 3, b, x0, tol, restart, maxit, mixed, max_refine, 0.0)
 
     STAGES = ("slab_sum", "real_space", "spread_xfft", "fft_y_fwd", "fft_z_scale", "fft_y_inv", "xifft_contract")
 

@@ -15,7 +15,7 @@ grid spacing, P = ceil(2/mean(h) - 2 ln tol / pi)).
 Pinned by (tests/test_sizes_oracle.py): for unit radii the tables equal the
 pinned dipole tables of ewald_oracle and the whole operator equals the
 dipole operator to the spectral tolerance; for mixed radii the total operator
-is independent of xi and symmetric in the metric diag(a^3); the tables are
+is independent of xi and symmetric; the tables are
 continuous at contact and at containment; the small-d branch of the kernel
 is its d -> 0 limit.  The MATLAB cannot run here, so there are no golden
 vectors.
@@ -218,9 +218,10 @@ def kernel(rvec, a, xi, eta):
 
 class SizesGeometry:
     """Everything of ComputeField.m that depends only on positions and radii;
-    tables (Ep_perp, Ep_para) on r_table with radius types from uniquetol."""
+    tables (Ep_perp, Ep_para) on r_table with radius types from uniquetol,
+    or (exact=True) the couplings of each pair's own radii at its separation."""
 
-    def __init__(self, x, a, box, xi, tol, r_table, Ep_perp, Ep_para, Ngrid=None, P=None):
+    def __init__(self, x, a, box, xi, tol, r_table, Ep_perp, Ep_para, Ngrid=None, P=None, exact=False):
         self.x = np.asarray(x, dtype=np.float64)
         self.a = np.asarray(a, dtype=np.float64).reshape(-1)
         self.box = np.asarray(box, dtype=np.float64).reshape(3)
@@ -263,6 +264,16 @@ class SizesGeometry:
         keep = d < self.rc
         self.p1, self.p2, d, r = p1[keep], p2[keep], d[keep], r[keep]
         self.rhat = r / d[:, None]
+        self.exact = exact
+        if exact:
+            # couplings of each pair's own radii evaluated at d (no table)
+            self.A = np.zeros(d.size)
+            self.B = np.zeros(d.size)
+            for k in range(d.size):
+                pa, pb = _couplings(np.array([d[k]]), self.a[self.p1[k]], self.a[self.p2[k]], self.xi)
+                self.A[k], self.B[k] = pa[0], pb[0]
+            self.self_val = np.array([self_coupling(ai, ai, self.xi) for ai in self.a])
+            return
         idx = np.searchsorted(self.r_table, d, side="left")
         if np.any(idx <= 0) or np.any(idx >= self.r_table.size):
             raise ValueError("pair separation outside the real-space table range")
@@ -272,6 +283,7 @@ class SizesGeometry:
         self.A = self.Ep_perp[idx - 1, cols] * wl + self.Ep_perp[idx, cols] * wh
         self.B = self.Ep_para[idx - 1, cols] * wl + self.Ep_para[idx, cols] * wh
         self.self_col = self.col[self.a_ind, self.a_ind]
+        self.self_val = self.Ep_perp[0, self.self_col]
 
 
 def field(geo: SizesGeometry, p, eps_p, parts=False):
@@ -287,7 +299,7 @@ def field(geo: SizesGeometry, p, eps_p, parts=False):
     Et = np.fft.ifftn(np.fft.fftn(grid) * geo.scale)
     flat = Et.transpose(2, 1, 0).reshape(-1)
     Eq = np.prod(geo.h) * np.einsum("nkc,nk->nc", geo.w, flat[geo.linnode])
-    Er = 3 * p / (4 * PI * geo.a[:, None] ** 3 * (eps[:, None] - 1)) + p * geo.Ep_perp[0, geo.self_col][:, None]
+    Er = 3 * p / (4 * PI * geo.a[:, None] ** 3 * (eps[:, None] - 1)) + p * geo.self_val[:, None]
     pj = p[geo.p2]
     proj = np.sum(pj * geo.rhat, axis=1)
     contrib = geo.A[:, None] * (pj - proj[:, None] * geo.rhat) + (geo.B * proj)[:, None] * geo.rhat
