@@ -36,7 +36,7 @@ __device__ inline int force_cell_1d(double xc, double L, int nc) {
   return ix;
 }
 
-__global__ void k_force_cells(ForceParams p, const double* __restrict__ pos, int* __restrict__ cell,
+static __global__ void k_force_cells(ForceParams p, const double* __restrict__ pos, int* __restrict__ cell,
                               int* __restrict__ count) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= p.n) return;
@@ -48,7 +48,7 @@ __global__ void k_force_cells(ForceParams p, const double* __restrict__ pos, int
   atomicAdd(&count[cid], 1);
 }
 
-__global__ void k_force_scatter(int n, const int* __restrict__ cell, const int* __restrict__ start,
+static __global__ void k_force_scatter(int n, const int* __restrict__ cell, const int* __restrict__ start,
                                 int* __restrict__ cursor, int* __restrict__ members) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -101,7 +101,7 @@ __device__ inline void pair_force(const ForceParams& p, const double* __restrict
   fz += coef * (mimj * dz + mjr * miz + mir * mjz - five * dz);
 }
 
-__global__ void k_forces(ForceParams p, const double* __restrict__ pos, const double* __restrict__ dip,
+static __global__ void k_forces(ForceParams p, const double* __restrict__ pos, const double* __restrict__ dip,
                          const int* __restrict__ cell, const int* __restrict__ start, const int* __restrict__ members,
                          double* __restrict__ f, double* __restrict__ pe) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;

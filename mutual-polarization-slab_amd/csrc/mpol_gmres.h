@@ -15,7 +15,7 @@ constexpr int kDotBlocks = 120;  // partial-sum blocks per right-hand side
 
 // partial[(rhs*cnt + i)*kDotBlocks + blk] = sum over the block's chunk of a_i . w
 // with a_i = A + i*stride_i (rhs offset rhs*n).  Deterministic.
-__global__ void __launch_bounds__(256) k_dots(const double* __restrict__ A, size_t stride_i, int cnt,
+static __global__ void __launch_bounds__(256) k_dots(const double* __restrict__ A, size_t stride_i, int cnt,
                                               const double* __restrict__ w, int64_t n, int nrhs,
                                               double* __restrict__ partial) {
   __shared__ double red[256];
@@ -40,7 +40,7 @@ __global__ void __launch_bounds__(256) k_dots(const double* __restrict__ A, size
 }
 
 // out[rhs*cnt + i] = sum of the partials (fixed order)
-__global__ void k_dots_finish(const double* __restrict__ partial, int nblk, int total, double* __restrict__ out) {
+static __global__ void k_dots_finish(const double* __restrict__ partial, int nblk, int total, double* __restrict__ out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
   double s = 0.0;
@@ -49,7 +49,7 @@ __global__ void k_dots_finish(const double* __restrict__ partial, int nblk, int 
 }
 
 // w[rhs] -= sum_i h[rhs*cnt + i] V_i[rhs]   (only rhs in state kIter)
-__global__ void k_sub_basis(const double* __restrict__ V, size_t stride_i, int cnt, const double* __restrict__ h,
+static __global__ void k_sub_basis(const double* __restrict__ V, size_t stride_i, int cnt, const double* __restrict__ h,
                             double* __restrict__ w, int64_t n, const int* __restrict__ state) {
   const int rhs = blockIdx.y;
   if (state[rhs] != kIter) return;
@@ -61,25 +61,25 @@ __global__ void k_sub_basis(const double* __restrict__ V, size_t stride_i, int c
 }
 
 // r = b - Ax
-__global__ void k_residual(const double* __restrict__ b, const double* __restrict__ ax, double* __restrict__ r,
+static __global__ void k_residual(const double* __restrict__ b, const double* __restrict__ ax, double* __restrict__ r,
                            int64_t total) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < total) r[k] = b[k] - ax[k];
 }
 
-__global__ void k_copy(const double* __restrict__ src, double* __restrict__ dst, int64_t total) {
+static __global__ void k_copy(const double* __restrict__ src, double* __restrict__ dst, int64_t total) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < total) dst[k] = src[k];
 }
 
-__global__ void k_axpy_all(double a, const double* __restrict__ x, double* __restrict__ y, int64_t total) {
+static __global__ void k_axpy_all(double a, const double* __restrict__ x, double* __restrict__ y, int64_t total) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < total) y[k] += a * x[k];
 }
 
 // Start of a restart cycle: rnorm2[rhs] = ||b - A x||^2.
 // kWait/kIter -> kDone if converged, else kIter with g = [beta, 0, ...].
-__global__ void k_cycle_start(int nrhs, int restart, const double* __restrict__ rnorm2,
+static __global__ void k_cycle_start(int nrhs, int restart, const double* __restrict__ rnorm2,
                               const double* __restrict__ bnorm, double tol, int* __restrict__ state,
                               double* __restrict__ g, double* __restrict__ beta_out, double* __restrict__ relres) {
   const int rhs = blockIdx.x * blockDim.x + threadIdx.x;
@@ -100,7 +100,7 @@ __global__ void k_cycle_start(int nrhs, int restart, const double* __restrict__ 
 }
 
 // V_0[rhs] = r[rhs] / beta[rhs] for rhs in kIter.
-__global__ void k_scale_into(const double* __restrict__ r, const double* __restrict__ beta,
+static __global__ void k_scale_into(const double* __restrict__ r, const double* __restrict__ beta,
                              double* __restrict__ v, int64_t n, const int* __restrict__ state) {
   const int rhs = blockIdx.y;
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -110,7 +110,7 @@ __global__ void k_scale_into(const double* __restrict__ r, const double* __restr
 }
 
 // Store h (+ h2) into column j of H and normalise w into V_{j+1}; hnorm2 = ||w||^2.
-__global__ void k_arnoldi_finish(int nrhs, int restart, int j, const double* __restrict__ h1,
+static __global__ void k_arnoldi_finish(int nrhs, int restart, int j, const double* __restrict__ h1,
                                  const double* __restrict__ h2, const double* __restrict__ hnorm2,
                                  double* __restrict__ H, const int* __restrict__ state) {
   const int rhs = blockIdx.x * blockDim.x + threadIdx.x;
@@ -120,7 +120,7 @@ __global__ void k_arnoldi_finish(int nrhs, int restart, int j, const double* __r
   Hr[(j + 1) * restart + j] = sqrt(hnorm2[rhs]);
 }
 
-__global__ void k_normalize_next(const double* __restrict__ w, double* __restrict__ vnext, const double* __restrict__ H,
+static __global__ void k_normalize_next(const double* __restrict__ w, double* __restrict__ vnext, const double* __restrict__ H,
                                  int restart, int j, int64_t n, const int* __restrict__ state) {
   const int rhs = blockIdx.y;
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -130,7 +130,7 @@ __global__ void k_normalize_next(const double* __restrict__ w, double* __restric
 }
 
 // Givens rotations on column j; decide whether the cycle ends for each rhs.
-__global__ void k_givens(int nrhs, int restart, int j, double* __restrict__ H, double* __restrict__ cs,
+static __global__ void k_givens(int nrhs, int restart, int j, double* __restrict__ H, double* __restrict__ cs,
                          double* __restrict__ sn, double* __restrict__ g, const double* __restrict__ bnorm,
                          double tol, int* __restrict__ state, int* __restrict__ kdim) {
   const int rhs = blockIdx.x * blockDim.x + threadIdx.x;
@@ -167,7 +167,7 @@ __global__ void k_givens(int nrhs, int restart, int j, double* __restrict__ H, d
 }
 
 // Back substitution of the rotated (upper triangular) H y = g for rhs in kSolve.
-__global__ void k_backsolve(int nrhs, int restart, const double* __restrict__ H, const double* __restrict__ g,
+static __global__ void k_backsolve(int nrhs, int restart, const double* __restrict__ H, const double* __restrict__ g,
                             const int* __restrict__ kdim, const int* __restrict__ state, double* __restrict__ y) {
   const int rhs = blockIdx.x * blockDim.x + threadIdx.x;
   if (rhs >= nrhs || state[rhs] != kSolve) return;
@@ -184,7 +184,7 @@ __global__ void k_backsolve(int nrhs, int restart, const double* __restrict__ H,
 }
 
 // x[rhs] += sum_i y_i V_i[rhs] for rhs in kSolve.
-__global__ void k_update_x(const double* __restrict__ V, size_t stride_i, const double* __restrict__ y,
+static __global__ void k_update_x(const double* __restrict__ V, size_t stride_i, const double* __restrict__ y,
                            const int* __restrict__ kdim, const int* __restrict__ state, int restart,
                            double* __restrict__ x, int64_t n) {
   const int rhs = blockIdx.y;
@@ -197,7 +197,7 @@ __global__ void k_update_x(const double* __restrict__ V, size_t stride_i, const 
   x[(size_t)rhs * n + k] = acc;
 }
 
-__global__ void k_solve_to_wait(int nrhs, int* __restrict__ state) {
+static __global__ void k_solve_to_wait(int nrhs, int* __restrict__ state) {
   const int rhs = blockIdx.x * blockDim.x + threadIdx.x;
   if (rhs < nrhs && state[rhs] == kSolve) state[rhs] = kWait;
 }

@@ -1,0 +1,647 @@
+// Internal host-side definitions of libmpol.so shared by its translation
+// units: error handling, device buffers, FFT plans, launch-shape choices, the
+// context structure and the operator launch sequences (templates, instantiated
+// once each in mpol_ops_*.hip).
+#pragma once
+
+#include "mpol.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mpol_forces.h"
+#include "mpol_gmres.h"
+#include "mpol_kernels.h"
+#include "mpol_sizes.h"
+
+namespace mpol {
+void real_space_tables(double xi, const double* r, int64_t n, double* out);
+void real_space_tables_quad(double xi, const double* r, int64_t n, double* out);
+}
+
+using namespace mpol;
+
+namespace mpol {
+
+// Records msg as the calling thread's last error and returns code.
+int fail(int code, const std::string& msg);
+
+#define HIPCHK(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) return fail(MPOL_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define HIPALLOC(buf, cnt)                                                                  \
+  do {                                                                                      \
+    hipError_t e_ = (buf).reserve(cnt);                                                     \
+    if (e_ != hipSuccess) return fail(MPOL_ENOMEM, std::string("device allocation failed: ") + #buf); \
+  } while (0)
+
+#define CHECK_STATUS(expr)        \
+  do {                            \
+    int s_ = (expr);              \
+    if (s_ != MPOL_OK) return s_; \
+  } while (0)
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  hipError_t reserve(size_t cnt) {
+    if (cnt <= n && p) return hipSuccess;
+    release();
+    if (cnt == 0) return hipSuccess;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), cnt * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      n = 0;
+      return e;
+    }
+    n = cnt;
+    return hipSuccess;
+  }
+  void release() {
+    // a buffer is only replaced between calls; wait for kernels that may still read it
+    if (p) (void)hipDeviceSynchronize();
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { release(); }
+};
+
+inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+inline FftPlan make_plan(int n) {
+  FftPlan p{};
+  p.n = n;
+  p.e = 0;
+  p.nstages = 0;
+  int m = n;
+  for (int q : {2, 3, 5})
+    while (m % q == 0) m /= q;
+  const bool smooth = (m == 1);
+  if (!smooth || n < 12) return p;
+  for (int e : {16, 24, 20, 12}) {
+    if (n % e != 0 || n / e > 256) continue;
+    bool ok = true;
+    for (int q : {2, 3, 5})
+      if (n % q == 0 && e % q != 0) ok = false;
+    if (!ok) continue;
+    FftPlan t{};
+    t.n = n;
+    t.e = e;
+    int rem = n, ns = 1, s = 0;
+    while (rem > 1 && s < kMaxStages) {
+      int r = 0;
+      for (int cand : {16, 8, 4, 2, 3, 5})
+        if (rem % cand == 0 && e % cand == 0) {
+          r = cand;
+          break;
+        }
+      if (r == 0) break;
+      t.radix[s] = r;
+      t.ns[s] = ns;
+      ns *= r;
+      rem /= r;
+      ++s;
+    }
+    if (rem != 1) continue;
+    t.nstages = s;
+    return t;
+  }
+  return p;
+}
+
+inline void make_twiddles(int n, std::vector<double>& tw) {
+  tw.resize(2 * (size_t)n);
+  for (int k = 0; k < n; ++k) {
+    const long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)n;
+    tw[2 * k] = (double)std::cos(ang);
+    tw[2 * k + 1] = -(double)std::sin(ang);
+  }
+}
+
+struct TileCfg {
+  int ty = 1, tz = 1, rows = 1, ncl = 2, lstride = 1, ch = 64;
+  size_t lds = 0;
+};
+
+struct ColCfg {
+  int tx = 1, lstride = 1;
+  size_t lds = 0;
+};
+
+inline ColCfg col_cfg(int n, bool generic, int kx, int lines_per_col, size_t csize, size_t budget, int tx_pref) {
+  ColCfg c;
+  int tx = std::min(tx_pref, kx);
+  for (;; tx = std::max(1, tx / 2)) {
+    c.tx = tx;
+    c.lstride = padded_len(n);
+    c.lds = ((size_t)n + (size_t)lines_per_col * tx * c.lstride * (generic ? 2 : 1)) * csize;
+    if (c.lds <= budget || tx == 1) break;
+  }
+  return c;
+}
+
+constexpr size_t kLdsTile = 48 * 1024;
+constexpr size_t kLdsMax = 150 * 1024;  // dynamic LDS per workgroup (gfx950 has 160 KB per CU)
+constexpr size_t kLdsCol = 112 * 1024;
+
+}  // namespace mpol
+
+struct mpol_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;  // launch stream; nullptr is the default stream
+  // tables
+  int ntab = 0;
+  double self_perp = 0.0;
+  DevBuf<double> perp, para, rvals;
+  // geometry
+  bool have_geo = false;
+  int n = 0;
+  double box[3] = {0, 0, 0}, h[3] = {0, 0, 0}, eta[3] = {0, 0, 0};
+  double xi = 0, rc = 0;
+  int ng[3] = {0, 0, 0}, kx = 0, P = 0, half = 0, nc[3] = {0, 0, 0};
+  int64_t npairs = 0;
+  DevBuf<double> x, base;
+  DevBuf<int> bin_key, bin_count, bin_start, bin_cursor, cell_key, cell_count, cell_start, cell_cursor,
+      cell_members, g0u, perm, rs_count, rs_off, rs_j, err;
+  DevBuf<int4> rec;
+  DevBuf<double4> base64;
+  DevBuf<float4> base32;
+  DevBuf<double> ms;
+  double norm = 0.0;
+  DevBuf<double> rs_A, rs_BA, rs_rh;
+  DevBuf<float> rs_A32, rs_BA32, rs_rh32;
+  FftPlan plan[3];
+  DevBuf<double> ktab;      // e_x, e_y, e_z, then (cubic boxes) the Bessel factor table
+  bool cubic = false;
+  DevBuf<double> tw64[3];
+  DevBuf<float> tw32[3];
+  // permittivity: diagonal of the dipole rows (and, when quadrupole tables are
+  // set, of the quadrupole rows); imaginary parts when complex
+  int nsets = 0;
+  bool cplx_eps = false;
+  DevBuf<double> eps_re, eps_im;
+  DevBuf<double> diag, diag_im, diag_q, diag_qim;
+  bool diag_q_valid = false;
+  // dipole + quadrupole operator: 7 tables on the rvals grid, per-pair coefficients
+  bool have_qtab = false;
+  double self_gq2 = 0.0;
+  DevBuf<double> qtab;
+  bool quad_pairs_valid = false;
+  DevBuf<double> rs_Q;
+  DevBuf<float> rs_Q32;
+  int64_t jq_off = -1;      // offset of the quadrupole Bessel table in ktab (cubic boxes)
+  // polydisperse operator: radii (caller's order), radius types and pair tables
+  // (ntypes > 0) or closed-form couplings (ntypes == 0)
+  bool have_sizes = false;
+  int sz_ntypes = 0;
+  DevBuf<double> sz_radius, sz_rad_sorted, sz_perp, sz_para, sz_A, sz_BA, sz_diag, sz_diag_im, sz_slab;
+  DevBuf<float> sz_A32, sz_BA32;
+  DevBuf<int> sz_type, sz_col;
+  bool sz_pairs_valid = false, sz_diag_valid = false;
+  DevBuf<unsigned char> sz_grid;
+  // workspaces
+  DevBuf<unsigned char> spec;
+  DevBuf<double> zsum;
+  // launch-shape overrides (0 = automatic): tile ty, tile tz, particle chunk,
+  // y-pass columns, z-pass columns, z-pass threads
+  int tune[6] = {0, 0, 0, 0, 0, 0};
+  // gmres
+  DevBuf<double> gV, gW, gR, gAx, gH, gcs, gsn, gg, gy, gh1, gh2, gpart, gbn, gbeta, grel, grn2, gx, gd;
+  DevBuf<int> gstate, gkdim;
+  // truncated fixed-dipole forces: cell list
+  DevBuf<int> f_cell, f_count, f_start, f_cursor, f_members;
+};
+
+namespace mpol {
+
+template <typename K>
+void set_lds(K kernel, size_t bytes) {
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)bytes);
+}
+
+// Launch KERNEL(E, N, ...) with the compile-time plan for length n when there
+// is one, else with the run-time plan of E elements per thread.
+#define DISPATCH_FFT(n, e, KERNEL, ...)                          \
+  switch (n) {                                                   \
+    case 32: KERNEL(0, 32, __VA_ARGS__); break;                  \
+    case 64: KERNEL(0, 64, __VA_ARGS__); break;                  \
+    case 96: KERNEL(0, 96, __VA_ARGS__); break;                  \
+    case 128: KERNEL(0, 128, __VA_ARGS__); break;                \
+    case 160: KERNEL(0, 160, __VA_ARGS__); break;                \
+    case 192: KERNEL(0, 192, __VA_ARGS__); break;                \
+    case 256: KERNEL(0, 256, __VA_ARGS__); break;                \
+    case 320: KERNEL(0, 320, __VA_ARGS__); break;                \
+    case 512: KERNEL(0, 512, __VA_ARGS__); break;                \
+    default:                                                     \
+      switch (e) {                                               \
+        case 0: KERNEL(0, 0, __VA_ARGS__); break;                \
+        case 12: KERNEL(12, 0, __VA_ARGS__); break;              \
+        case 16: KERNEL(16, 0, __VA_ARGS__); break;              \
+        case 20: KERNEL(20, 0, __VA_ARGS__); break;              \
+        case 24: KERNEL(24, 0, __VA_ARGS__); break;              \
+        default: return fail(MPOL_EINVAL, "unsupported FFT plan"); \
+      }                                                          \
+  }
+
+inline bool generic_fft(const FftPlan& p) { return !is_ct_size(p.n) && p.e == 0; }
+
+// Launch KERNEL(N, ...) for a length with a compile-time FFT plan.
+#define DISPATCH_CT(n, KERNEL, ...)                                  \
+  switch (n) {                                                       \
+    case 32: KERNEL(32, __VA_ARGS__); break;                         \
+    case 64: KERNEL(64, __VA_ARGS__); break;                         \
+    case 96: KERNEL(96, __VA_ARGS__); break;                         \
+    case 128: KERNEL(128, __VA_ARGS__); break;                       \
+    case 160: KERNEL(160, __VA_ARGS__); break;                       \
+    case 192: KERNEL(192, __VA_ARGS__); break;                       \
+    case 256: KERNEL(256, __VA_ARGS__); break;                       \
+    case 320: KERNEL(320, __VA_ARGS__); break;                       \
+    case 512: KERNEL(512, __VA_ARGS__); break;                       \
+    default: return fail(MPOL_EINVAL, "no compile-time FFT plan");   \
+  }
+
+template <typename T>
+const void* tw_ptr(mpol_ctx* c, int axis) {
+  if constexpr (sizeof(T) == 8) return c->tw64[axis].p;
+  else return c->tw32[axis].p;
+}
+
+template <typename T>
+const void* base_ptr(mpol_ctx* c) {
+  if constexpr (sizeof(T) == 8) return c->base64.p;
+  else return c->base32.p;
+}
+
+// out = A m (or its parts when hr/hslab are given) in precision T, for NC = 3
+// (dipole operator: hr <- real space, hslab <- slab term, out <- wave space)
+// or NC = 8 (dipole + quadrupole operator: hr <- real space, out <- wave
+// space).  Stage boundaries recorded when `ev` is given (kStages + 1 events).
+constexpr int kStages = 7;
+constexpr int kZsumBlocks = 64;
+enum { kOpDipole = 3, kOpQuad = 8, kOpSizes = 103 };
+inline int unknowns_per_particle(int kind) { return kind == kOpQuad ? 8 : 3; }
+
+// Largest tile (rows of nx nodes, NC components) and particle chunk that fit
+// the LDS budget.
+template <int NC>
+TileCfg tile_cfg_nc(int nx, bool generic, size_t csize, int P, size_t budget, bool sort_arrays = false) {
+  const int opts[][2] = {{4, 4}, {4, 2}, {2, 2}, {2, 1}, {1, 1}};
+  const size_t rsize = csize / 2;
+  TileCfg c;
+  for (const auto& o : opts) {
+    c.ty = o[0];
+    c.tz = o[1];
+    c.rows = c.ty * c.tz;
+    c.ncl = (NC * c.rows + 1) / 2;
+    c.lstride = padded_len(nx);
+    const size_t grid = ((size_t)nx + (size_t)c.ncl * c.lstride * (generic ? 2 : 1)) * csize;
+    const size_t per = (NC + (size_t)P + c.rows) * rsize + 2 * sizeof(int);
+    const size_t fixed = sort_arrays ? ((size_t)nx + 1) * sizeof(int) : 0;
+    for (c.ch = 256; c.ch >= 64; c.ch /= 2) {
+      c.lds = grid + fixed + c.ch * per;
+      if (c.lds <= budget) return c;
+    }
+  }
+  return c;
+}
+
+// LDS bytes of a tile kernel with the given shape.
+template <int NC>
+size_t tile_lds(int nx, bool generic, size_t csize, int P, const TileCfg& t, bool sort_arrays) {
+  return ((size_t)nx + (size_t)t.ncl * t.lstride * (generic ? 2 : 1)) * csize +
+         (sort_arrays ? ((size_t)nx + 1) * sizeof(int) : 0) +
+         (size_t)t.ch * ((NC + (size_t)P + t.rows) * (csize / 2) + 2 * sizeof(int));
+}
+
+template <int NC>
+void apply_tile_tuning(const mpol_ctx* c, TileCfg& t) {
+  if (c->tune[0] > 0 && c->tune[1] > 0) {
+    t.ty = c->tune[0];
+    t.tz = c->tune[1];
+    t.rows = t.ty * t.tz;
+    t.ncl = (NC * t.rows + 1) / 2;
+  }
+  if (c->tune[2] > 0) t.ch = c->tune[2];
+}
+
+template <typename T, int NC>
+int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* hslab, int nrhs, double slab,
+                 hipEvent_t* ev = nullptr, int stop_after = kStages) {
+  const size_t cs = sizeof(cplx<T>);
+  const size_t spec_bytes = (size_t)NC * nrhs * c->ng[2] * c->ng[1] * c->kx * cs;
+  HIPALLOC(c->spec, spec_bytes);
+  HIPALLOC(c->zsum, (size_t)nrhs * (kZsumBlocks + 1));
+  cplx<T>* spec = reinterpret_cast<cplx<T>*>(c->spec.p);
+  const int n = c->n;
+  hipStream_t st = c->stream;
+
+  if (ev) HIPCHK(hipEventRecord(ev[0], st));
+  const T* pA;
+  const T* pBA;
+  const T* prh;
+  if constexpr (sizeof(T) == 8) {
+    pA = c->rs_A.p; pBA = c->rs_BA.p; prh = c->rs_rh.p;
+  } else {
+    pA = c->rs_A32.p; pBA = c->rs_BA32.p; prh = c->rs_rh32.p;
+  }
+  if constexpr (NC == kOpDipole) {
+    double* zpart = c->zsum.p + nrhs;
+    hipLaunchKernelGGL(k_sum_z, dim3(kZsumBlocks, nrhs), dim3(256), 0, st, m, n, zpart);
+    hipLaunchKernelGGL(k_sum_finish, dim3(cdiv(nrhs, 256)), dim3(256), 0, st, zpart, kZsumBlocks, nrhs, c->zsum.p);
+    if (ev) HIPCHK(hipEventRecord(ev[1], st));
+    const double V = c->box[0] * c->box[1] * c->box[2];
+    hipLaunchKernelGGL((k_real_space<T>), dim3(cdiv(n, 256), nrhs), dim3(256), 0, st, n, nrhs, c->nsets, m, out,
+                       c->diag.p, c->cplx_eps ? c->diag_im.p : nullptr, c->rs_off.p, c->rs_j.p, pA, pBA, prh,
+                       c->zsum.p, slab / V, hr != nullptr ? 1 : 0, hr, hslab);
+  } else {
+    if (ev) HIPCHK(hipEventRecord(ev[1], st));
+    const T* pq;
+    if constexpr (sizeof(T) == 8) pq = c->rs_Q.p;
+    else pq = c->rs_Q32.p;
+    hipLaunchKernelGGL((k_real_space_quad<T>), dim3(cdiv(n, 256), nrhs), dim3(256), 0, st, n, nrhs, c->nsets, m,
+                       out, c->diag.p, c->diag_q.p, c->cplx_eps ? c->diag_im.p : nullptr,
+                       c->cplx_eps ? c->diag_qim.p : nullptr, c->rs_off.p, c->rs_j.p, pA, pBA, prh, pq,
+                       hr != nullptr ? 1 : 0, hr);
+  }
+  HIPALLOC(c->ms, (size_t)nrhs * NC * n);
+  hipLaunchKernelGGL(k_permute_m<NC>, dim3(cdiv(n, 256), nrhs), dim3(256), 0, st, m, c->perm.p, n, c->ms.p);
+  if (ev) HIPCHK(hipEventRecord(ev[2], st));
+
+  // tile (x axis) kernels
+  const bool gen_x = generic_fft(c->plan[0]);
+  TileCfg tc = tile_cfg_nc<NC>(c->ng[0], gen_x, cs, c->P, kLdsTile);
+  apply_tile_tuning<NC>(c, tc);
+  tc.lds = tile_lds<NC>(c->ng[0], gen_x, cs, c->P, tc, false);
+  TileArgs ta{};
+  ta.nx = c->ng[0]; ta.ny = c->ng[1]; ta.nz = c->ng[2]; ta.kx = c->kx;
+  ta.ty = tc.ty; ta.tz = tc.tz; ta.rows = tc.rows; ta.ncl = tc.ncl; ta.lstride = tc.lstride; ta.ch = tc.ch;
+  ta.P = c->P; ta.half = c->half; ta.n = n; ta.nrhs = nrhs;
+  ta.rec = c->rec.p; ta.base = base_ptr<T>(c); ta.ms = c->ms.p; ta.bin_start = c->bin_start.p;
+  for (int d = 0; d < 3; ++d) {
+    ta.h[d] = c->h[d];
+    ta.cexp[d] = 2.0 * c->xi * c->xi / c->eta[d];
+  }
+  ta.norm = c->norm;
+  ta.tw = tw_ptr<T>(c, 0); ta.plan = c->plan[0];
+  ta.hvol = c->h[0] * c->h[1] * c->h[2];
+  const dim3 tgrid(cdiv(c->ng[1], tc.ty), cdiv(c->ng[2], tc.tz), nrhs);
+  if (tc.lds > kLdsMax) return fail(MPOL_EINVAL, "spread/contract tile does not fit in LDS (reduce tile or chunk)");
+
+  // FP32: atomic-free gather spread; FP64: LDS FP64 atomics (faster there)
+  const bool gather = sizeof(T) == 4 && is_ct_size(c->ng[0]) && c->P + 3 <= c->ng[0];
+  if (gather) {
+    TileCfg tg = tile_cfg_nc<NC>(c->ng[0], false, cs, c->P, kLdsTile, true);
+    apply_tile_tuning<NC>(c, tg);
+    tg.lds = tile_lds<NC>(c->ng[0], false, cs, c->P, tg, true);
+    if (tg.lds > kLdsMax) return fail(MPOL_EINVAL, "spread tile does not fit in LDS (reduce tile or chunk)");
+    TileArgs tga = ta;
+    tga.ty = tg.ty; tga.tz = tg.tz; tga.rows = tg.rows; tga.ncl = tg.ncl; tga.lstride = tg.lstride; tga.ch = tg.ch;
+    const dim3 ggrid(cdiv(c->ng[1], tg.ty), cdiv(c->ng[2], tg.tz), nrhs);
+#define LAUNCH_GATHER(N, ...)                                                                   \
+  do {                                                                                          \
+    set_lds(k_spread_gather_xfft<T, 0, N, 4, NC>, tg.lds);                                      \
+    hipLaunchKernelGGL((k_spread_gather_xfft<T, 0, N, 4, NC>), ggrid, dim3(256), tg.lds, st, tga, m, spec); \
+  } while (0)
+    DISPATCH_CT(c->ng[0], LAUNCH_GATHER, 0);
+#undef LAUNCH_GATHER
+  } else {
+    // the spread tile accumulates in FP64 in both precisions
+    using TA = double;
+    TileCfg tsc = tile_cfg_nc<NC>(c->ng[0], gen_x, sizeof(cplx<TA>), c->P, kLdsTile);
+    apply_tile_tuning<NC>(c, tsc);
+    const size_t slds = tile_lds<NC>(c->ng[0], gen_x, sizeof(cplx<TA>), c->P, tsc, false);
+    if (slds > kLdsMax) return fail(MPOL_EINVAL, "spread tile does not fit in LDS (reduce tile or chunk)");
+    TileArgs tsa = ta;
+    tsa.ty = tsc.ty; tsa.tz = tsc.tz; tsa.rows = tsc.rows; tsa.ncl = tsc.ncl; tsa.lstride = tsc.lstride;
+    tsa.ch = tsc.ch;
+    tsa.base = c->base64.p;
+    tsa.tw = c->tw64[0].p;
+    const dim3 sgrid(cdiv(c->ng[1], tsa.ty), cdiv(c->ng[2], tsa.tz), nrhs);
+#define LAUNCH_SPREAD(E, N, ...)                                                                \
+  do {                                                                                          \
+    set_lds(k_spread_xfft<TA, E, N, T, NC>, slds);                                              \
+    hipLaunchKernelGGL((k_spread_xfft<TA, E, N, T, NC>), sgrid, dim3(256), slds, st, tsa, m, spec); \
+  } while (0)
+    DISPATCH_FFT(c->ng[0], ta.plan.e, LAUNCH_SPREAD, 0);
+#undef LAUNCH_SPREAD
+  }
+  if (ev) HIPCHK(hipEventRecord(ev[3], st));
+  if (stop_after <= 2) return MPOL_OK;
+
+  // y axis forward
+  const ColCfg cy = col_cfg(c->ng[1], generic_fft(c->plan[1]), c->kx, 1, cs, kLdsCol,
+                            c->tune[3] > 0 ? c->tune[3] : (sizeof(T) == 8 ? 8 : 16));  // 128-byte rows
+  ColArgs ay{};
+  ay.nx = c->ng[0]; ay.ny = c->ng[1]; ay.nz = c->ng[2]; ay.kx = c->kx; ay.tx = cy.tx; ay.lstride = cy.lstride;
+  ay.plan = c->plan[1]; ay.tw = tw_ptr<T>(c, 1);
+  const dim3 ygrid(cdiv(c->kx, cy.tx), c->ng[2], NC * nrhs);
+  if (cy.lds > kLdsMax) return fail(MPOL_EINVAL, "y-pass columns do not fit in LDS");
+#define LAUNCH_FFTY(E, N, INV)                                                                  \
+  do {                                                                                          \
+    set_lds(k_fft_y<T, E, N>, cy.lds);                                                          \
+    hipLaunchKernelGGL((k_fft_y<T, E, N>), ygrid, dim3(256), cy.lds, st, ay, spec, INV);       \
+  } while (0)
+  DISPATCH_FFT(c->ng[1], ay.plan.e, LAUNCH_FFTY, 0);
+  if (ev) HIPCHK(hipEventRecord(ev[4], st));
+  if (stop_after <= 3) return MPOL_OK;
+
+  // z axis forward + scale + inverse
+  const ColCfg cz = col_cfg(c->ng[2], generic_fft(c->plan[2]), c->kx, NC, cs, kLdsCol,
+                            c->tune[4] > 0 ? c->tune[4] : (sizeof(T) == 8 ? 4 : 8));
+  ColArgs az{};
+  az.nx = c->ng[0]; az.ny = c->ng[1]; az.nz = c->ng[2]; az.kx = c->kx; az.tx = cz.tx; az.lstride = cz.lstride;
+  az.plan = c->plan[2]; az.tw = tw_ptr<T>(c, 2);
+  ScaleParams sp{};
+  for (int d = 0; d < 3; ++d) {
+    sp.twopi_over_l[d] = 2.0 * kPi / c->box[d];
+    sp.one_minus_eta[d] = 1.0 - c->eta[d];
+  }
+  sp.inv_4xi2 = 1.0 / (4.0 * c->xi * c->xi);
+  sp.inv_g = 1.0 / ((double)c->ng[0] * (double)c->ng[1] * (double)c->ng[2]);
+  sp.ex = c->ktab.p;
+  sp.ey = sp.ex + c->ng[0];
+  sp.ez = sp.ey + c->ng[1];
+  sp.jt = c->cubic ? sp.ez + c->ng[2] : nullptr;
+  sp.jq = (c->cubic && c->jq_off >= 0) ? c->ktab.p + c->jq_off : nullptr;
+  const dim3 zgrid(cdiv(c->kx, cz.tx), c->ng[1], nrhs);
+  if (cz.lds > kLdsMax) return fail(MPOL_EINVAL, "z-pass columns do not fit in LDS");
+  // one thread group per line of the z transform, all NC*tx lines at once
+  int zthreads = 256;
+  {
+    const int e = is_ct_size(c->ng[2]) ? ct_plan(c->ng[2]).e : c->plan[2].e;
+    if (e > 0) zthreads = std::min(1024, std::max(512, NC * cz.tx * (c->ng[2] / e)));
+    zthreads = (zthreads + 63) / 64 * 64;
+    if (c->tune[5] > 0) zthreads = c->tune[5];
+  }
+#define LAUNCH_FFTZ(E, N, ...)                                                                  \
+  do {                                                                                          \
+    set_lds(k_fft_z_scale<T, E, N, NC>, cz.lds);                                                \
+    hipLaunchKernelGGL((k_fft_z_scale<T, E, N, NC>), zgrid, dim3(zthreads), cz.lds, st, az, sp, spec); \
+  } while (0)
+  DISPATCH_FFT(c->ng[2], az.plan.e, LAUNCH_FFTZ, 0);
+  if (ev) HIPCHK(hipEventRecord(ev[5], st));
+  if (stop_after <= 4) return MPOL_OK;
+
+  DISPATCH_FFT(c->ng[1], ay.plan.e, LAUNCH_FFTY, 1);
+  if (ev) HIPCHK(hipEventRecord(ev[6], st));
+  if (stop_after <= 5) return MPOL_OK;
+
+#define LAUNCH_CONTRACT(E, N, ...)                                                              \
+  do {                                                                                          \
+    set_lds(k_xifft_contract<T, E, N, NC>, tc.lds);                                             \
+    hipLaunchKernelGGL((k_xifft_contract<T, E, N, NC>), tgrid, dim3(256), tc.lds, st, ta, spec, out); \
+  } while (0)
+  DISPATCH_FFT(c->ng[0], ta.plan.e, LAUNCH_CONTRACT, 0);
+  if (ev) HIPCHK(hipEventRecord(ev[7], st));
+#undef LAUNCH_FFTY
+#undef LAUNCH_FFTZ
+#undef LAUNCH_CONTRACT
+  HIPCHK(hipGetLastError());
+  return MPOL_OK;
+}
+
+// Right-hand sides per system: 2 (real, imaginary part) with complex permittivities.
+inline int rhs_per_system(const mpol_ctx* c) { return c->cplx_eps ? 2 : 1; }
+
+// Polydisperse operator: real space (pair couplings of the two radii,
+// dielectric 3/(4 pi a^3 (eps-1))) + one scalar grid per right-hand side.
+// hr != nullptr: hr <- real space, out <- wave space.
+template <typename T>
+int run_operator_sizes(mpol_ctx* c, const double* m, double* out, double* hr, int nrhs) {
+  const int n = c->n;
+  hipStream_t st = c->stream;
+  const size_t G = (size_t)c->ng[0] * c->ng[1] * c->ng[2];
+  const size_t cs = sizeof(cplx<T>);
+  HIPALLOC(c->spec, (size_t)nrhs * c->ng[2] * c->ng[1] * c->kx * cs);
+  HIPALLOC(c->sz_grid, (size_t)nrhs * G * sizeof(T));
+  HIPALLOC(c->zsum, (size_t)nrhs * (kZsumBlocks + 1));
+  HIPALLOC(c->ms, (size_t)nrhs * 3 * n);
+  cplx<T>* spec = reinterpret_cast<cplx<T>*>(c->spec.p);
+  T* grid = reinterpret_cast<T*>(c->sz_grid.p);
+  // real space (the slab term is absent: its sum only needs to be defined)
+  double* zpart = c->zsum.p + nrhs;
+  hipLaunchKernelGGL(k_sum_z, dim3(kZsumBlocks, nrhs), dim3(256), 0, st, m, n, zpart);
+  hipLaunchKernelGGL(k_sum_finish, dim3(cdiv(nrhs, 256)), dim3(256), 0, st, zpart, kZsumBlocks, nrhs, c->zsum.p);
+  const T* pA;
+  const T* pBA;
+  const T* prh;
+  if constexpr (sizeof(T) == 8) {
+    pA = c->sz_A.p; pBA = c->sz_BA.p; prh = c->rs_rh.p;
+  } else {
+    pA = c->sz_A32.p; pBA = c->sz_BA32.p; prh = c->rs_rh32.p;
+  }
+  if (hr) HIPALLOC(c->sz_slab, (size_t)nrhs * 3 * n);
+  hipLaunchKernelGGL((k_real_space<T>), dim3(cdiv(n, 256), nrhs), dim3(256), 0, st, n, nrhs, c->nsets, m, out,
+                     c->sz_diag.p, c->cplx_eps ? c->sz_diag_im.p : nullptr, c->rs_off.p, c->rs_j.p, pA, pBA, prh,
+                     c->zsum.p, 0.0, hr != nullptr ? 1 : 0, hr, hr ? c->sz_slab.p : nullptr);
+  hipLaunchKernelGGL(k_permute_m<3>, dim3(cdiv(n, 256), nrhs), dim3(256), 0, st, m, c->perm.p, n, c->ms.p);
+  // spread into real grids
+  HIPCHK(hipMemsetAsync(grid, 0, (size_t)nrhs * G * sizeof(T), st));
+  SizesGridArgs sa{};
+  sa.nx = c->ng[0]; sa.ny = c->ng[1]; sa.nz = c->ng[2];
+  sa.P = c->P; sa.half = c->half; sa.n = n; sa.nrhs = nrhs;
+  for (int d = 0; d < 3; ++d) sa.h[d] = c->h[d];
+  sa.xi = c->xi;
+  sa.eta = c->eta[0];
+  sa.hvol = c->h[0] * c->h[1] * c->h[2];
+  sa.rec = c->rec.p;
+  sa.base = c->base64.p;
+  sa.rad = c->sz_rad_sorted.p;
+  hipLaunchKernelGGL((k_sizes_spread<T>), dim3(n), dim3(256), 0, st, sa, c->ms.p, grid);
+  // x transform of the real rows
+  const bool gen_x = generic_fft(c->plan[0]);
+  TileCfg tr = tile_cfg_nc<1>(c->ng[0], gen_x, cs, 0, kLdsTile);
+  TileArgs ta{};
+  ta.nx = c->ng[0]; ta.ny = c->ng[1]; ta.nz = c->ng[2]; ta.kx = c->kx;
+  ta.ty = tr.ty; ta.tz = tr.tz; ta.rows = tr.rows; ta.ncl = tr.ncl; ta.lstride = tr.lstride;
+  ta.tw = tw_ptr<T>(c, 0); ta.plan = c->plan[0];
+  const size_t rlds = ((size_t)c->ng[0] + (size_t)tr.ncl * tr.lstride * (gen_x ? 2 : 1)) * cs;
+  if (rlds > kLdsMax) return fail(MPOL_EINVAL, "row transform tile does not fit in LDS");
+  const dim3 tgrid(cdiv(c->ng[1], tr.ty), cdiv(c->ng[2], tr.tz), nrhs);
+#define LAUNCH_R2C(E, N, ...)                                                                   \
+  do {                                                                                          \
+    set_lds(k_rows_r2c<T, E, N>, rlds);                                                         \
+    hipLaunchKernelGGL((k_rows_r2c<T, E, N>), tgrid, dim3(256), rlds, st, ta, grid, spec);      \
+  } while (0)
+  DISPATCH_FFT(c->ng[0], ta.plan.e, LAUNCH_R2C, 0);
+#undef LAUNCH_R2C
+  // y forward, z forward + scale + inverse, y inverse
+  const ColCfg cy = col_cfg(c->ng[1], generic_fft(c->plan[1]), c->kx, 1, cs, kLdsCol,
+                            c->tune[3] > 0 ? c->tune[3] : (sizeof(T) == 8 ? 8 : 16));
+  ColArgs ay{};
+  ay.nx = c->ng[0]; ay.ny = c->ng[1]; ay.nz = c->ng[2]; ay.kx = c->kx; ay.tx = cy.tx; ay.lstride = cy.lstride;
+  ay.plan = c->plan[1]; ay.tw = tw_ptr<T>(c, 1);
+  const dim3 ygrid(cdiv(c->kx, cy.tx), c->ng[2], nrhs);
+  if (cy.lds > kLdsMax) return fail(MPOL_EINVAL, "y-pass columns do not fit in LDS");
+#define LAUNCH_FFTY(E, N, INV)                                                                  \
+  do {                                                                                          \
+    set_lds(k_fft_y<T, E, N>, cy.lds);                                                          \
+    hipLaunchKernelGGL((k_fft_y<T, E, N>), ygrid, dim3(256), cy.lds, st, ay, spec, INV);       \
+  } while (0)
+  DISPATCH_FFT(c->ng[1], ay.plan.e, LAUNCH_FFTY, 0);
+  const ColCfg cz = col_cfg(c->ng[2], generic_fft(c->plan[2]), c->kx, 1, cs, kLdsCol,
+                            c->tune[4] > 0 ? c->tune[4] : (sizeof(T) == 8 ? 8 : 16));
+  ColArgs az{};
+  az.nx = c->ng[0]; az.ny = c->ng[1]; az.nz = c->ng[2]; az.kx = c->kx; az.tx = cz.tx; az.lstride = cz.lstride;
+  az.plan = c->plan[2]; az.tw = tw_ptr<T>(c, 2);
+  ScaleParams sp{};
+  for (int d = 0; d < 3; ++d) {
+    sp.twopi_over_l[d] = 2.0 * kPi / c->box[d];
+    sp.one_minus_eta[d] = 1.0 - c->eta[d];
+  }
+  sp.inv_4xi2 = 1.0 / (4.0 * c->xi * c->xi);
+  sp.inv_g = 1.0 / (double)G;
+  sp.ex = c->ktab.p;
+  sp.ey = sp.ex + c->ng[0];
+  sp.ez = sp.ey + c->ng[1];
+  const dim3 zgrid(cdiv(c->kx, cz.tx), c->ng[1], nrhs);
+  if (cz.lds > kLdsMax) return fail(MPOL_EINVAL, "z-pass columns do not fit in LDS");
+  int zthreads = 256;
+  {
+    const int e = is_ct_size(c->ng[2]) ? ct_plan(c->ng[2]).e : c->plan[2].e;
+    if (e > 0) zthreads = std::min(1024, std::max(256, cz.tx * (c->ng[2] / e)));
+    zthreads = (zthreads + 63) / 64 * 64;
+  }
+#define LAUNCH_FFTZ(E, N, ...)                                                                  \
+  do {                                                                                          \
+    set_lds(k_fft_z_scale<T, E, N, 1>, cz.lds);                                                 \
+    hipLaunchKernelGGL((k_fft_z_scale<T, E, N, 1>), zgrid, dim3(zthreads), cz.lds, st, az, sp, spec); \
+  } while (0)
+  DISPATCH_FFT(c->ng[2], az.plan.e, LAUNCH_FFTZ, 0);
+#undef LAUNCH_FFTZ
+  DISPATCH_FFT(c->ng[1], ay.plan.e, LAUNCH_FFTY, 1);
+#undef LAUNCH_FFTY
+#define LAUNCH_C2R(E, N, ...)                                                                   \
+  do {                                                                                          \
+    set_lds(k_rows_c2r<T, E, N>, rlds);                                                         \
+    hipLaunchKernelGGL((k_rows_c2r<T, E, N>), tgrid, dim3(256), rlds, st, ta, spec, grid);      \
+  } while (0)
+  DISPATCH_FFT(c->ng[0], ta.plan.e, LAUNCH_C2R, 0);
+#undef LAUNCH_C2R
+  hipLaunchKernelGGL((k_sizes_contract<T>), dim3(n), dim3(256), 0, st, sa, grid, out);
+  HIPCHK(hipGetLastError());
+  return MPOL_OK;
+}
+
+extern template int run_operator<float, kOpDipole>(mpol_ctx*, const double*, double*, double*, double*, int, double,
+                                                   hipEvent_t*, int);
+extern template int run_operator<double, kOpDipole>(mpol_ctx*, const double*, double*, double*, double*, int, double,
+                                                    hipEvent_t*, int);
+extern template int run_operator<float, kOpQuad>(mpol_ctx*, const double*, double*, double*, double*, int, double,
+                                                 hipEvent_t*, int);
+extern template int run_operator<double, kOpQuad>(mpol_ctx*, const double*, double*, double*, double*, int, double,
+                                                  hipEvent_t*, int);
+extern template int run_operator_sizes<float>(mpol_ctx*, const double*, double*, double*, int);
+extern template int run_operator_sizes<double>(mpol_ctx*, const double*, double*, double*, int);
+
+}  // namespace mpol
+

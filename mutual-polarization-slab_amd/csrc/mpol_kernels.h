@@ -55,7 +55,7 @@ struct GeoParams {
 };
 
 // Pass 1: bin keys and histograms.
-__global__ void k_geo_keys(GeoParams g, const double* __restrict__ x, int* __restrict__ bin_key,
+static __global__ void k_geo_keys(GeoParams g, const double* __restrict__ x, int* __restrict__ bin_key,
                            int* __restrict__ bin_count, int* __restrict__ cell_key, int* __restrict__ cell_count,
                            double* __restrict__ base, int* __restrict__ g0_unsorted) {
 #pragma clang fp contract(off)
@@ -85,7 +85,7 @@ __global__ void k_geo_keys(GeoParams g, const double* __restrict__ x, int* __res
 }
 
 // Single-workgroup exclusive scan of n ints (out has n+1 entries; out[n] = total).
-__global__ void k_scan(const int* __restrict__ in, int* __restrict__ out, int n) {
+static __global__ void k_scan(const int* __restrict__ in, int* __restrict__ out, int n) {
   __shared__ int part[1024];
   const int t = threadIdx.x, nt = blockDim.x;
   const int chunk = (n + nt - 1) / nt;
@@ -112,7 +112,7 @@ __global__ void k_scan(const int* __restrict__ in, int* __restrict__ out, int n)
 // stencil node on x, y, z, caller's index) and base[s] = node0*h - x on each
 // axis (FP64 and FP32 copies), from which the kernels recompute the Gaussian
 // factors exp(-2 xi^2 (base_d + (o - half) h_d)^2 / eta_d).
-__global__ void k_geo_scatter(GeoParams g, const int* __restrict__ bin_key, const int* __restrict__ bin_start,
+static __global__ void k_geo_scatter(GeoParams g, const int* __restrict__ bin_key, const int* __restrict__ bin_start,
                               int* __restrict__ bin_cursor, const int* __restrict__ cell_key,
                               const int* __restrict__ cell_start, int* __restrict__ cell_cursor,
                               const double* __restrict__ base, const int* __restrict__ g0_unsorted,
@@ -236,7 +236,7 @@ __global__ void k_pairs(PairParams p, const double* __restrict__ x, const int* _
 // ---------------------------------------------------------------------------
 // Partial sums of the z components per right-hand side (slab term);
 // partial[rhs*gridDim.x + block], deterministic.
-__global__ void k_sum_z(const double* __restrict__ m, int n, double* __restrict__ out) {
+static __global__ void k_sum_z(const double* __restrict__ m, int n, double* __restrict__ out) {
   __shared__ double red[256];
   const int rhs = blockIdx.y;
   const double* mr = m + (size_t)rhs * n * 3;
@@ -251,7 +251,7 @@ __global__ void k_sum_z(const double* __restrict__ m, int n, double* __restrict_
   if (threadIdx.x == 0) out[(size_t)rhs * gridDim.x + blockIdx.x] = red[0];
 }
 
-__global__ void k_sum_finish(const double* __restrict__ partial, int nblk, int nrhs, double* __restrict__ out) {
+static __global__ void k_sum_finish(const double* __restrict__ partial, int nblk, int nrhs, double* __restrict__ out) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrhs) return;
   double s = 0.0;
@@ -323,7 +323,7 @@ constexpr int kQuadTabs = 7;  // field_quad_1..3, grad_quad_1..4 (the field/dipo
 
 // Coefficients C..I of every CSR pair from the seven quadrupole tables, with
 // the distance and interval decisions of k_pairs (same FP64 operations).
-__global__ void k_pairs_quad(PairParams p, const double* __restrict__ x, const int* __restrict__ off,
+static __global__ void k_pairs_quad(PairParams p, const double* __restrict__ x, const int* __restrict__ off,
                              const int* __restrict__ pj, const double* __restrict__ rv,
                              const double* __restrict__ qtab, double* __restrict__ pq,
                              float* __restrict__ pq32) {

@@ -135,7 +135,7 @@ __host__ __device__ inline double sizes_self(double ai, double aj, double x) {
 // types (ntypes > 0; column col[ti*M+tj] of perp/para, each ntab long on rv)
 // or from the closed form with the particles' own radii (ntypes == 0).
 // Same distance decisions as k_pairs (FP64, no contraction).
-__global__ void k_pairs_sizes(PairParams p, const double* __restrict__ x, const int* __restrict__ off,
+static __global__ void k_pairs_sizes(PairParams p, const double* __restrict__ x, const int* __restrict__ off,
                               const int* __restrict__ pj, const double* __restrict__ rv,
                               const double* __restrict__ radius, const int* __restrict__ type, int ntypes,
                               const int* __restrict__ col, const double* __restrict__ tperp,
@@ -176,7 +176,7 @@ __global__ void k_pairs_sizes(PairParams p, const double* __restrict__ x, const 
 
 // diag = 3/(4 pi a^3 (eps - 1)) + self coupling (RealSpace.m:36-40); complex
 // eps gives the imaginary part too.
-__global__ void k_diag_sizes(const double* __restrict__ er, const double* __restrict__ ei, int n, int64_t total,
+static __global__ void k_diag_sizes(const double* __restrict__ er, const double* __restrict__ ei, int n, int64_t total,
                              const double* __restrict__ radius, const int* __restrict__ type, int ntypes,
                              const int* __restrict__ col, const double* __restrict__ tperp, int ntab, double xi,
                              double* __restrict__ dre, double* __restrict__ dim) {
@@ -196,7 +196,7 @@ __global__ void k_diag_sizes(const double* __restrict__ er, const double* __rest
   }
 }
 
-__global__ void k_permute_scalar(const double* __restrict__ v, const int* __restrict__ perm, int n,
+static __global__ void k_permute_scalar(const double* __restrict__ v, const int* __restrict__ perm, int n,
                                  double* __restrict__ vs) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s < n) vs[s] = v[perm[s]];

@@ -3,7 +3,8 @@
     python -m mpol_hip.build          # from mutual-polarization-slab_amd/
 
 The library lands next to this file, in-tree, so it travels with the source
-tree.  Compiles every time it is called.
+tree.  Compiles every time it is called: one hipcc process per translation
+unit, in parallel, then one link.
 """
 from __future__ import annotations
 
@@ -11,6 +12,8 @@ import os
 import shutil
 import subprocess
 import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 PKG_ROOT = os.path.dirname(HERE)
@@ -21,13 +24,13 @@ LIB_NAME = "libmpol.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
 
 HIP_FLAGS = [
-    "-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
+    "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
     # keep the compiler from emitting packed FP32 ops
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
 ]
 
-SOURCES = ["mpol_capi.hip", "mpol_tables.cpp"]
+SOURCES = ["mpol_capi.hip", "mpol_ops_dipole.hip", "mpol_ops_quad.hip", "mpol_ops_sizes.hip", "mpol_tables.cpp"]
 
 
 def hipcc() -> str:
@@ -37,19 +40,29 @@ def hipcc() -> str:
     return path
 
 
-def build(verbose: bool = False) -> str:
-    tmp = LIB_PATH + ".tmp"
-    cmd = [hipcc(), *HIP_FLAGS, "-I", INCLUDE, "-I", CSRC,
-           *[os.path.join(CSRC, s) for s in SOURCES], "-o", tmp]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"hipcc failed building {LIB_NAME}")
+        raise RuntimeError(f"hipcc failed: {' '.join(cmd[-3:])}")
     if verbose and res.stderr.strip():
         sys.stderr.write(res.stderr)
-    os.replace(tmp, LIB_PATH)
+
+
+def build(verbose: bool = False) -> str:
+    cc = hipcc()
+    jobs = max(1, min(len(SOURCES), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with tempfile.TemporaryDirectory(prefix="mpol_build_") as tmpdir:
+        objs = [os.path.join(tmpdir, os.path.splitext(s)[0] + ".o") for s in SOURCES]
+        cmds = [[cc, *HIP_FLAGS, "-I", INCLUDE, "-I", CSRC, "-c", os.path.join(CSRC, s), "-o", o]
+                for s, o in zip(SOURCES, objs)]
+        with ThreadPoolExecutor(max_workers=jobs) as ex:
+            list(ex.map(lambda c: _run(c, verbose), cmds))
+        tmp = LIB_PATH + ".tmp"
+        _run([cc, "-shared", "--offload-arch=gfx950", "-fPIC", *objs, "-o", tmp], verbose)
+        os.replace(tmp, LIB_PATH)
     return LIB_PATH
 
 
