@@ -221,6 +221,20 @@ int mpol_sizes_matvec_components(mpol_ctx* ctx, const double* m_dev, double* hk_
 int mpol_sizes_solve(mpol_ctx* ctx, const double* b_dev, const double* x0_dev, double* x_dev, int32_t nrhs,
                      const mpol_solve_opts* opts, mpol_solve_info* info);
 
+/* Vector-Jacobian product of the dipole operator for nrhs pairs (p, lam)
+ * (device, nrhs x N x 3 each): grad_x (device, N x 3) = d/dx of
+ * sum_rhs lam^T A(x) p at the current positions -- the real-space pair sum
+ * through the derivative of the interpolated tables and the wave-space part
+ * through the derivative of the Gaussian weights (slab and diagonal terms do
+ * not depend on x) -- and, when grad_eps (device, nsets x N) is not NULL,
+ * d/d eps of the same sum.  With the adjoint lam of a solve A p = b for an
+ * objective L(p) (A lam = dL/dp, A is symmetric), dL/dx = -grad_x.  Real
+ * permittivities only.  No counterpart in the reference (its HOOMD forces
+ * come from truncated point dipoles, dipole_forces_ext/hoomd_force.py:192-226);
+ * it gives the forces of the Ewald-summed polarization energy. */
+int mpol_matvec_vjp(mpol_ctx* ctx, const double* p_dev, const double* lam_dev, int32_t nrhs, double* grad_x_dev,
+                    double* grad_eps_dev);
+
 /* Launch-shape tuning (no counterpart in the reference; used by the
  * autotuner): knobs[0..5] = spread/contract tile rows along y and z (ty*tz a
  * power of two, each <= 4), particles staged per chunk (32..256), y-pass and

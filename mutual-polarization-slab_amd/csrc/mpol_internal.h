@@ -207,6 +207,8 @@ struct mpol_ctx {
   DevBuf<int> sz_type, sz_col;
   bool sz_pairs_valid = false, sz_diag_valid = false;
   DevBuf<unsigned char> sz_grid;
+  // vector-Jacobian product workspaces
+  DevBuf<double> vjp_in, vjp_out, vjp_grid;
   // workspaces
   DevBuf<unsigned char> spec;
   DevBuf<double> zsum;

@@ -30,7 +30,8 @@ HIP_FLAGS = [
     "-Wno-unused-result",
 ]
 
-SOURCES = ["mpol_capi.hip", "mpol_ops_dipole.hip", "mpol_ops_quad.hip", "mpol_ops_sizes.hip", "mpol_tables.cpp"]
+SOURCES = ["mpol_capi.hip", "mpol_ops_dipole.hip", "mpol_ops_quad.hip", "mpol_ops_sizes.hip", "mpol_adjoint.hip",
+           "mpol_tables.cpp"]
 
 
 def hipcc() -> str:
