@@ -236,10 +236,12 @@ int mpol_matvec_vjp(mpol_ctx* ctx, const double* p_dev, const double* lam_dev, i
                     double* grad_eps_dev);
 
 /* Launch-shape tuning (no counterpart in the reference; used by the
- * autotuner): knobs[0..5] = spread/contract tile rows along y and z (ty*tz a
+ * autotuner): knobs[0..6] = spread/contract tile rows along y and z (ty*tz a
  * power of two, each <= 4), particles staged per chunk (32..256), y-pass and
- * z-pass columns per workgroup (<= 64), z-pass threads (multiple of 64).
- * 0 keeps the automatic choice; nknobs may be shorter than 6. */
+ * z-pass columns per workgroup (<= 64), z-pass threads (multiple of 64),
+ * spread/contract engine (1 matrix cores over per-tile candidate images,
+ * 2 vector loops).  0 keeps the automatic choice; nknobs may be shorter
+ * than 7. */
 int mpol_set_tuning(mpol_ctx* ctx, const int32_t* knobs, int32_t nknobs);
 
 /* Geometry summary (host). */

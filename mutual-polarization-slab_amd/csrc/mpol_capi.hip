@@ -230,6 +230,8 @@ int mpol_set_geometry(mpol_ctx* c, const double* x, int64_t n, const double box[
   HIPCHK(hipSetDevice(c->device));
   hipStream_t st = c->stream;
   c->have_geo = false;
+  c->img[0].valid = false;
+  c->img[1].valid = false;
   c->n = (int)n;
   for (int d = 0; d < 3; ++d) {
     c->box[d] = box[d];
@@ -823,8 +825,8 @@ int mpol_quad_solve(mpol_ctx* c, const double* b, const double* x0, double* x, i
 }
 
 int mpol_set_tuning(mpol_ctx* c, const int32_t* knobs, int32_t nknobs) {
-  if (!c || (nknobs > 0 && !knobs) || nknobs < 0 || nknobs > 6) return fail(MPOL_EINVAL, "bad tuning arguments");
-  int t[6] = {0, 0, 0, 0, 0, 0};
+  if (!c || (nknobs > 0 && !knobs) || nknobs < 0 || nknobs > kTuneKnobs) return fail(MPOL_EINVAL, "bad tuning arguments");
+  int t[kTuneKnobs] = {0, 0, 0, 0, 0, 0, 0};
   for (int i = 0; i < nknobs; ++i) t[i] = knobs[i];
   const bool pow2_rows = (t[0] == 0 && t[1] == 0) ||
                          (t[0] >= 1 && t[0] <= 4 && t[1] >= 1 && t[1] <= 4 && ((t[0] * t[1]) & (t[0] * t[1] - 1)) == 0);
@@ -833,7 +835,8 @@ int mpol_set_tuning(mpol_ctx* c, const int32_t* knobs, int32_t nknobs) {
   if (t[3] < 0 || t[3] > 64 || t[4] < 0 || t[4] > 64) return fail(MPOL_EINVAL, "column widths must be in [1, 64]");
   if (t[5] != 0 && (t[5] < 64 || t[5] > 1024 || t[5] % 64 != 0))
     return fail(MPOL_EINVAL, "z-pass threads must be a multiple of 64 in [64, 1024]");
-  for (int i = 0; i < 6; ++i) c->tune[i] = t[i];
+  if (t[6] < 0 || t[6] > 2) return fail(MPOL_EINVAL, "engine must be 0 (automatic), 1 (matrix cores) or 2 (vector loops)");
+  for (int i = 0; i < kTuneKnobs; ++i) c->tune[i] = t[i];
   return MPOL_OK;
 }
 

@@ -15,6 +15,7 @@
 #include "mpol_forces.h"
 #include "mpol_gmres.h"
 #include "mpol_kernels.h"
+#include "mpol_mfma.h"
 #include "mpol_sizes.h"
 
 namespace mpol {
@@ -149,6 +150,20 @@ inline ColCfg col_cfg(int n, bool generic, int kx, int lines_per_col, size_t csi
 }
 
 constexpr size_t kLdsTile = 48 * 1024;
+constexpr size_t kLdsMfma = 80 * 1024;  // two matrix-core tile workgroups per CU
+constexpr int kTuneKnobs = 7;
+
+// Candidates of every tile, sorted by stencil start along x within chunks of
+// ch, for one tile shape and precision (built on first use after
+// mpol_set_geometry; see k_img_fill).
+struct TileImage {
+  bool valid = false;
+  int ty = 0, tz = 0, ch = 0;
+  int64_t total = 0;
+  DevBuf<int> cnt, off;
+  DevBuf<int2> gs;
+  DevBuf<unsigned char> fw;
+};
 constexpr size_t kLdsMax = 150 * 1024;  // dynamic LDS per workgroup (gfx950 has 160 KB per CU)
 constexpr size_t kLdsCol = 112 * 1024;
 
@@ -213,8 +228,11 @@ struct mpol_ctx {
   DevBuf<unsigned char> spec;
   DevBuf<double> zsum;
   // launch-shape overrides (0 = automatic): tile ty, tile tz, particle chunk,
-  // y-pass columns, z-pass columns, z-pass threads
-  int tune[6] = {0, 0, 0, 0, 0, 0};
+  // y-pass columns, z-pass columns, z-pass threads, spread/contract engine
+  // (1 matrix cores, 2 vector loops)
+  int tune[kTuneKnobs] = {0, 0, 0, 0, 0, 0, 0};
+  // per-tile candidate images of the matrix-core spread/contract ([0] FP32, [1] FP64)
+  TileImage img[2];
   // gmres
   DevBuf<double> gV, gW, gR, gAx, gH, gcs, gsn, gg, gy, gh1, gh2, gpart, gbn, gbeta, grel, grn2, gx, gd;
   DevBuf<int> gstate, gkdim;
@@ -335,6 +353,79 @@ void apply_tile_tuning(const mpol_ctx* c, TileCfg& t) {
   if (c->tune[2] > 0) t.ch = c->tune[2];
 }
 
+// Tile shape and chunk of the matrix-core spread/contract: the largest tile
+// (rows first) whose chunk of at least 128 candidates fits kLdsMfma, else the
+// largest chunk that fits.  ok = false when the grid or stencil does not suit
+// that path (compile-time x plan, nx % 16 == 0, P + 16 <= nx) or the engine
+// knob asks for the vector loops.
+template <typename T, int NC>
+TileCfg mfma_tile_cfg(const mpol_ctx* c, bool& ok) {
+  const int nx = c->ng[0];
+  TileCfg t;
+  ok = c->tune[6] != 2 && is_ct_size(nx) && nx % 16 == 0 && c->P + 16 <= nx;
+  if (!ok) return t;
+  const size_t cs = sizeof(cplx<T>);
+  // 4 or 8 rows (the kernels are instantiated for those)
+  const int opts[][2] = {{4, 2}, {2, 2}};
+  bool found = false;
+  for (int pass = 0; pass < 3 && !found; ++pass) {
+    const size_t budget = pass < 2 ? kLdsMfma : kLdsMax;
+    for (const auto& o : opts) {
+      t.ty = o[0];
+      t.tz = o[1];
+      t.rows = t.ty * t.tz;
+      t.ncl = (NC * t.rows + 1) / 2;
+      t.lstride = padded_len(nx);
+      for (t.ch = 256; t.ch >= (pass == 0 ? 128 : 32); t.ch /= 2) {
+        t.lds = mfma_tile_lds<NC>(nx, cs, c->P, t.ncl, t.lstride, t.ch, t.rows);
+        if (t.lds <= budget) {
+          found = true;
+          break;
+        }
+      }
+      if (found) break;
+    }
+  }
+  apply_tile_tuning<NC>(c, t);
+  t.ncl = (NC * t.rows + 1) / 2;
+  t.lds = mfma_tile_lds<NC>(nx, cs, c->P, t.ncl, t.lstride, t.ch, t.rows);
+  if (t.lds > kLdsMax || (t.rows != 4 && t.rows != 8)) ok = false;
+  // candidate slots of the image are int32
+  const double redund = (double)(c->P + t.ty) / t.ty * (double)(c->P + t.tz) / t.tz;
+  if ((double)c->n * redund >= 2.0e9) ok = false;
+  return t;
+}
+
+// Build the tile image of precision T for tile shape ta (no-op when the
+// current one matches).
+template <typename T>
+int ensure_tile_image(mpol_ctx* c, const TileArgs& ta, TileImage& im) {
+  if (im.valid && im.ty == ta.ty && im.tz == ta.tz && im.ch == ta.ch) return MPOL_OK;
+  im.valid = false;
+  hipStream_t st = c->stream;
+  const int ntx = cdiv(c->ng[1], ta.ty), ntz = cdiv(c->ng[2], ta.tz), ntiles = ntx * ntz;
+  HIPALLOC(im.cnt, (size_t)ntiles);
+  HIPALLOC(im.off, (size_t)ntiles + 1);
+  hipLaunchKernelGGL(k_img_count, dim3(cdiv(ntiles, 256)), dim3(256), 0, st, ta, ntx, ntiles, im.cnt.p);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, im.cnt.p, im.off.p, ntiles);
+  int total = 0;
+  HIPCHK(hipMemcpyAsync(&total, im.off.p + ntiles, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (total < 0) return fail(MPOL_EINVAL, "tile image too large");
+  const size_t slots = total > 0 ? (size_t)total : 1;
+  HIPALLOC(im.gs, slots);
+  HIPALLOC(im.fw, slots * (size_t)(ta.P + ta.rows) * sizeof(T));
+  hipLaunchKernelGGL((k_img_fill<T>), dim3(ntx, ntz), dim3(256), 0, st, ta, im.off.p, im.gs.p,
+                     reinterpret_cast<T*>(im.fw.p));
+  HIPCHK(hipGetLastError());
+  im.valid = true;
+  im.ty = ta.ty;
+  im.tz = ta.tz;
+  im.ch = ta.ch;
+  im.total = total;
+  return MPOL_OK;
+}
+
 template <typename T, int NC>
 int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* hslab, int nrhs, double slab,
                  hipEvent_t* ev = nullptr, int stop_after = kStages) {
@@ -398,9 +489,40 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   const dim3 tgrid(cdiv(c->ng[1], tc.ty), cdiv(c->ng[2], tc.tz), nrhs);
   if (tc.lds > kLdsMax) return fail(MPOL_EINVAL, "spread/contract tile does not fit in LDS (reduce tile or chunk)");
 
+  // matrix-core spread/contract over the tile image when the grid suits it
+  bool use_mfma = false;
+  const TileCfg tm = mfma_tile_cfg<T, NC>(c, use_mfma);
+  if (c->tune[6] == 1 && !use_mfma) return fail(MPOL_EINVAL, "matrix-core spread/contract does not apply to this grid");
+  TileArgs tma = ta;
+  ImgArgs ima{};
+  dim3 mgrid(1, 1, 1);
+  if (use_mfma) {
+    tma.ty = tm.ty; tma.tz = tm.tz; tma.rows = tm.rows; tma.ncl = tm.ncl; tma.lstride = tm.lstride; tma.ch = tm.ch;
+    TileImage& im = c->img[sizeof(T) == 8 ? 1 : 0];
+    CHECK_STATUS(ensure_tile_image<T>(c, tma, im));
+    ima.off = im.off.p;
+    ima.gs = im.gs.p;
+    ima.fw = im.fw.p;
+    ima.perm = c->perm.p;
+    mgrid = dim3(cdiv(c->ng[1], tm.ty), cdiv(c->ng[2], tm.tz), nrhs);
+  }
+
   // FP32: atomic-free gather spread; FP64: LDS FP64 atomics (faster there)
   const bool gather = sizeof(T) == 4 && is_ct_size(c->ng[0]) && c->P + 3 <= c->ng[0];
-  if (gather) {
+  if (use_mfma) {
+#define LAUNCH_SPREAD_MFMA(N, ...)                                                              \
+  do {                                                                                          \
+    if (tm.rows == 8) {                                                                         \
+      set_lds(k_spread_mfma_xfft<T, N, NC, 8>, tm.lds);                                         \
+      hipLaunchKernelGGL((k_spread_mfma_xfft<T, N, NC, 8>), mgrid, dim3(256), tm.lds, st, tma, ima, spec); \
+    } else {                                                                                    \
+      set_lds(k_spread_mfma_xfft<T, N, NC, 4>, tm.lds);                                         \
+      hipLaunchKernelGGL((k_spread_mfma_xfft<T, N, NC, 4>), mgrid, dim3(256), tm.lds, st, tma, ima, spec); \
+    }                                                                                           \
+  } while (0)
+    DISPATCH_CT(c->ng[0], LAUNCH_SPREAD_MFMA, 0);
+#undef LAUNCH_SPREAD_MFMA
+  } else if (gather) {
     TileCfg tg = tile_cfg_nc<NC>(c->ng[0], false, cs, c->P, kLdsTile, true);
     apply_tile_tuning<NC>(c, tg);
     tg.lds = tile_lds<NC>(c->ng[0], false, cs, c->P, tg, true);
@@ -502,7 +624,22 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
     set_lds(k_xifft_contract<T, E, N, NC>, tc.lds);                                             \
     hipLaunchKernelGGL((k_xifft_contract<T, E, N, NC>), tgrid, dim3(256), tc.lds, st, ta, spec, out); \
   } while (0)
-  DISPATCH_FFT(c->ng[0], ta.plan.e, LAUNCH_CONTRACT, 0);
+#define LAUNCH_CONTRACT_MFMA(N, ...)                                                            \
+  do {                                                                                          \
+    if (tm.rows == 8) {                                                                         \
+      set_lds(k_xifft_contract_mfma<T, N, NC, 8>, tm.lds);                                      \
+      hipLaunchKernelGGL((k_xifft_contract_mfma<T, N, NC, 8>), mgrid, dim3(256), tm.lds, st, tma, ima, spec, out); \
+    } else {                                                                                    \
+      set_lds(k_xifft_contract_mfma<T, N, NC, 4>, tm.lds);                                      \
+      hipLaunchKernelGGL((k_xifft_contract_mfma<T, N, NC, 4>), mgrid, dim3(256), tm.lds, st, tma, ima, spec, out); \
+    }                                                                                           \
+  } while (0)
+  if (use_mfma) {
+    DISPATCH_CT(c->ng[0], LAUNCH_CONTRACT_MFMA, 0);
+  } else {
+    DISPATCH_FFT(c->ng[0], ta.plan.e, LAUNCH_CONTRACT, 0);
+  }
+#undef LAUNCH_CONTRACT_MFMA
   if (ev) HIPCHK(hipEventRecord(ev[7], st));
 #undef LAUNCH_FFTY
 #undef LAUNCH_FFTZ

@@ -1,0 +1,444 @@
+// Spread and contract on the matrix cores of gfx950 (v_mfma_*_16x16x4).
+//
+// Along x the Gaussian stencil of a particle is a band: particle q adds
+// fx_q[x - gx_q] (x - gx_q in [0, P)) to node x of every tile row, weighted
+// by the row weight w_q[row] (product of its y and z factors) and its moment
+// m_q[c].  For one tile (R rows, NC components, all nx nodes of each row) the
+// spread is therefore a matrix product
+//
+//     G[x][(c, row)] = sum_q  A[x][q] * B[q][(c, row)],
+//     A[x][q] = fx_q[x - gx_q],   B[q][(c, row)] = w_q[row] * m_q[c],
+//
+// and the contraction its transpose followed by a row-weighted sum:
+//
+//     out_q[c] = sum_row w_q[row] * sum_x A[x][q] * G[x][(c, row)].
+//
+// With the candidates of a tile sorted by gx, the candidates that reach a
+// 16-node x block are a contiguous range (about P + 15 of them), and the 16
+// consecutive candidates of a block span about 16 + P nodes, so both products
+// are short bands of 16 x 16 x 4 MFMA steps instead of per-candidate VALU
+// loops.
+//
+// The per-tile candidate data (first x node, sorted particle index, P x
+// factors times the Gaussian norm, R row weights) depends only on the
+// geometry, so it is built once per geometry into a "tile image" in HBM,
+// sorted by gx within chunks of ch candidates; the per-matvec kernels copy a
+// chunk into LDS with straight coalesced loads.
+#pragma once
+
+#include "mpol_kernels.h"
+
+namespace mpol {
+
+constexpr int kMaxMfmaNx = 512;  // largest x grid of the compile-time FFT plans
+
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef double f64x4_t __attribute__((ext_vector_type(4)));
+
+// One 16x16x4 step D += A B.  Lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16]; its four results are D[row(l, r)][l % 16], r = 0..3
+// (the two precisions order the rows differently).
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<float> {
+  using acc_t = f32x4_t;
+  static __device__ __forceinline__ acc_t mac(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+template <>
+struct Mfma<double> {
+  using acc_t = f64x4_t;
+  static __device__ __forceinline__ acc_t mac(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+
+// Sum over groups of R consecutive lanes (R a power of two <= 16, groups
+// inside one 16-lane DPP row); every lane of a group gets the group's sum.
+// Uses row mirrors and quad permutations (no LDS traffic).
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+constexpr int kDppRowMirror = 0x140, kDppRowHalfMirror = 0x141, kDppSwap1 = 0xB1, kDppSwap2 = 0x4E;
+
+template <typename T>
+__device__ __forceinline__ T group_sum(T v, int R) {
+  if (R >= 16) v += dpp_mov<kDppRowMirror>(v);
+  if (R >= 8) v += dpp_mov<kDppRowHalfMirror>(v);
+  if (R >= 4) v += dpp_mov<kDppSwap2>(v);
+  if (R >= 2) v += dpp_mov<kDppSwap1>(v);
+  return v;
+}
+
+struct ImgArgs {
+  const int* off;    // [ntiles + 1] first entry of each tile
+  const int2* gs;    // [total] {first stencil node along x, sorted particle index}
+  const void* fw;    // [total][P + rows] T: norm * x factors, then the row weights
+  const int* perm;   // sorted particle index -> caller's index
+};
+
+// Candidates of tile (ty0, tz0): the total of the column-bin ranges that
+// tile_ranges enumerates (one thread per tile).
+static __global__ void k_img_count(TileArgs a, int ntx, int ntiles, int* __restrict__ cnt) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  const int ty0 = (t % ntx) * a.ty, tz0 = (t / ntx) * a.tz;
+  int ylo = ty0 - a.P + 1, ycount = a.ty + a.P - 1;
+  if (ycount >= a.ny) { ylo = 0; ycount = a.ny; }
+  int zlo = tz0 - a.P + 1, zcount = a.tz + a.P - 1;
+  if (zcount >= a.nz) { zlo = 0; zcount = a.nz; }
+  const int y0 = (int)imod(ylo, a.ny);
+  const int len1 = min(ycount, a.ny - y0);
+  const int len2 = ycount - len1;
+  int total = 0;
+  for (int zi = 0; zi < zcount; ++zi) {
+    const int gz = (int)imod(zlo + zi, a.nz);
+    const int k0 = gz * a.ny + y0;
+    total += a.bin_start[k0 + len1] - a.bin_start[k0];
+    if (len2 > 0) total += a.bin_start[gz * a.ny + len2] - a.bin_start[gz * a.ny];
+  }
+  cnt[t] = total;
+}
+
+// Tile image: one workgroup per tile.  Candidates come in chunks of a.ch in
+// the enumeration order of tile_ranges; within a chunk each candidate's slot
+// is its rank by (gx, enumeration index), a deterministic stable sort.
+template <typename T>
+__global__ void __launch_bounds__(256) k_img_fill(TileArgs a, const int* __restrict__ off, int2* __restrict__ gs,
+                                                  T* __restrict__ fw) {
+  __shared__ int s_start[kMaxRanges], s_cum[kMaxRanges + 1], s_nr;
+  __shared__ int s_gx[256];
+  const int tile = blockIdx.x + blockIdx.y * gridDim.x;
+  const int ty0 = blockIdx.x * a.ty, tz0 = blockIdx.y * a.tz;
+  tile_ranges(a, ty0, tz0, s_start, s_cum, &s_nr);
+  const int total = s_cum[s_nr];
+  const int q0 = off[tile];
+  const int PW = a.P + a.rows;
+  using V4 = typename Vec4Of<T>::type;
+  for (int base = 0; base < total; base += a.ch) {
+    const int nc = min(a.ch, total - base);
+    const int t = threadIdx.x;
+    int s = 0;
+    int4 rc = make_int4(0, 0, 0, 0);
+    if (t < nc) {
+      s = candidate_particle(s_start, s_cum, s_nr, base + t);
+      rc = a.rec[s];
+      s_gx[t] = rc.x;
+    }
+    __syncthreads();
+    if (t < nc) {
+      int rank = 0;
+      for (int u = 0; u < nc; ++u) {
+        const int g = s_gx[u];
+        rank += (g < rc.x) || (g == rc.x && u < t);
+      }
+      const size_t slot = (size_t)q0 + base + rank;
+      gs[slot] = make_int2(rc.x, s);
+      const V4 b = reinterpret_cast<const V4*>(a.base)[s];
+      T* f = fw + slot * PW;
+      for (int o = 0; o < a.P; ++o) f[o] = (T)a.norm * stencil_factor<T>(a, 0, b.x, o);
+      T wy[4], wz[4];
+      for (int l = 0; l < a.ty; ++l) {
+        const int y = ty0 + l;
+        wy[l] = (y < a.ny) ? axis_weight_fn<T>(a, 1, b.y, y, rc.y, a.ny) : (T)0;
+      }
+      for (int l = 0; l < a.tz; ++l) {
+        const int z = tz0 + l;
+        wz[l] = (z < a.nz) ? axis_weight_fn<T>(a, 2, b.z, z, rc.z, a.nz) : (T)0;
+      }
+      for (int r = 0; r < a.rows; ++r) f[a.P + r] = wy[r % a.ty] * wz[r / a.ty];
+    }
+    __syncthreads();
+  }
+}
+
+// First index in the sorted s_gx[0, n) whose value is >= v.
+__device__ inline int gx_lower_bound(const int* s_gx, int n, int v) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (s_gx[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Spectrum row of real line l = c*rows + row of tile (ty0, tz0) for
+// right-hand side rhs, or -1 when the row lies outside the grid (tiles at the
+// upper y/z edges).  rows, ty and tz are powers of two.
+__device__ __forceinline__ long long tile_line_row(const TileArgs& a, int l, int ty0, int tz0, int rhs, int nc_comp) {
+  const int lr = __builtin_ctz(a.rows), lty = __builtin_ctz(a.ty);
+  const int c = l >> lr, row = l & (a.rows - 1);
+  const int y = ty0 + (row & (a.ty - 1)), z = tz0 + (row >> lty);
+  if (y >= a.ny || z >= a.nz) return -1;
+  return ((long long)(rhs * nc_comp + c) * a.nz + z) * a.ny + y;
+}
+
+// LDS layout of one staged chunk (after the twiddles and the tile lines).
+template <typename T>
+struct MfmaStage {
+  T* fw;     // [ch][P + rows]
+  T* m;      // [ch][NC] (spread)
+  int* gx;   // [ch]
+  int* idx;  // [ch] sorted particle index (spread) or caller's index (contract)
+};
+
+template <typename T, int NC>
+__device__ inline MfmaStage<T> mfma_stage_layout(unsigned char* base, int ch, int PW) {
+  MfmaStage<T> st;
+  st.fw = reinterpret_cast<T*>(base);
+  st.m = st.fw + (size_t)ch * PW;
+  st.gx = reinterpret_cast<int*>(st.m + (size_t)ch * NC);
+  st.idx = st.gx + ch;
+  return st;
+}
+
+// LDS bytes of the MFMA tile kernels: twiddles, tile lines, staged chunk.
+template <int NC>
+inline size_t mfma_tile_lds(int nx, size_t csize, int P, int ncl, int lstride, int ch, int rows) {
+  return ((size_t)nx + (size_t)ncl * lstride) * csize + (size_t)ch * ((P + rows + NC) * (csize / 2) + 2 * sizeof(int));
+}
+
+// Copy chunk [q, q + nc) of the tile image into LDS; the spread also gathers
+// the moments of right-hand side rhs (WITH_M), the contraction the caller's
+// particle indices.
+template <typename T, bool WITH_M, int NC>
+__device__ inline void mfma_stage_chunk(const TileArgs& a, const ImgArgs& im, const MfmaStage<T>& st, size_t q,
+                                        int nc, int rhs) {
+  const int PW = a.P + a.rows;
+  const T* src = reinterpret_cast<const T*>(im.fw) + q * PW;
+  for (int k = threadIdx.x; k < nc * PW; k += blockDim.x) st.fw[k] = src[k];
+  for (int u = threadIdx.x; u < nc; u += blockDim.x) {
+    const int2 g = im.gs[q + u];
+    st.gx[u] = g.x;
+    if (WITH_M) {
+      const double* mp = a.ms + NC * ((size_t)rhs * a.n + g.y);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) st.m[NC * u + c] = (T)mp[c];
+    } else {
+      st.idx[u] = im.perm[g.y];
+    }
+  }
+}
+
+// Spread + forward x-FFT.  Wave w handles x blocks w, w + 4, ...; lane l of a
+// block holds node x0 + l % 16 (A rows) and columns (c, row) = nb*16 + l % 16
+// of the tile lines (column index = line index c*R + row).  R (tile rows) is
+// a template parameter so that the column blocks, the row of a column and
+// the lane reductions are compile-time.  Needs nx % 16 == 0, P + 16 <= nx.
+template <typename T, int N, int NC, int R>
+__global__ void __launch_bounds__(256) k_spread_mfma_xfft(TileArgs a, ImgArgs im, cplx<T>* __restrict__ spec) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  using MF = Mfma<T>;
+  using acc_t = typename MF::acc_t;
+  static_assert(R == 4 || R == 8, "tile rows");
+  constexpr int NCOL = NC * R, NB = (NCOL + 15) / 16, NXB = N / 16, KX = N / 2 + 1;
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* buf = tw + N;
+  const int P = a.P, PW = P + R;
+  const MfmaStage<T> st = mfma_stage_layout<T, NC>(reinterpret_cast<unsigned char*>(buf + a.ncl * a.lstride), a.ch, PW);
+  __shared__ int s_xr[4 * NXB];
+  const int tile = blockIdx.x + blockIdx.y * gridDim.x;
+  const int ty0 = blockIdx.x * a.ty, tz0 = blockIdx.y * a.tz, rhs = blockIdx.z;
+  const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
+  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];
+  for (int k = threadIdx.x; k < a.ncl * a.lstride; k += blockDim.x) buf[k] = cmk<T>(0, 0);
+  const size_t qt = (size_t)im.off[tile];
+  const int total = im.off[tile + 1] - (int)qt;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  const int i16 = lane & 15, k4 = lane >> 4;
+  const int colrow = i16 % R;  // row of column nb*16 + i16 for every nb (R | 16)
+  int colc[NB];
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) colc[nb] = (nb * 16 + i16) / R;  // >= NC: unused column
+  for (int base = 0; base < total; base += a.ch) {
+    const int nc = min(a.ch, total - base);
+    mfma_stage_chunk<T, true, NC>(a, im, st, qt + base, nc, rhs);
+    __syncthreads();
+    // candidate ranges of every x block: gx in [x0 - P + 1, x0 + 15] (mod nx),
+    // one range, or two when the window wraps below 0
+    for (int xb = threadIdx.x; xb < NXB; xb += blockDim.x) {
+      const int x0 = xb * 16, lo = x0 - P + 1;
+      const int hi_idx = gx_lower_bound(st.gx, nc, x0 + 16);
+      int* r = s_xr + 4 * xb;
+      if (lo >= 0) {
+        r[0] = gx_lower_bound(st.gx, nc, lo); r[1] = hi_idx; r[2] = 0; r[3] = 0;
+      } else {
+        r[0] = gx_lower_bound(st.gx, nc, lo + N); r[1] = nc; r[2] = 0; r[3] = hi_idx;
+      }
+    }
+    __syncthreads();
+    for (int xb = wave; xb < NXB; xb += nwave) {
+      const int x0 = xb * 16;
+      acc_t acc[NB];
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) acc[nb] = acc_t{0, 0, 0, 0};
+#pragma unroll
+      for (int sg = 0; sg < 2; ++sg) {
+        const int qa = __builtin_amdgcn_readfirstlane(s_xr[4 * xb + 2 * sg]);
+        const int qb = __builtin_amdgcn_readfirstlane(s_xr[4 * xb + 2 * sg + 1]);
+        for (int q = qa; q < qb; q += 4) {
+          // lanes past the range read candidate qa and contribute 0
+          const int cand = q + k4;
+          const bool ok = cand < qb;
+          const int cq = ok ? cand : qa;
+          int o = x0 + i16 - st.gx[cq];
+          if (o < 0) o += N;
+          const T f = st.fw[cq * PW + min(o, P - 1)];
+          const T av = (ok && o < P) ? f : (T)0;
+          const T wv = st.fw[cq * PW + P + colrow];
+#pragma unroll
+          for (int nb = 0; nb < NB; ++nb) {
+            const T mv = st.m[cq * NC + min(colc[nb], NC - 1)];
+            const T bv = colc[nb] < NC ? wv * mv : (T)0;
+            acc[nb] = MF::mac(av, bv, acc[nb]);
+          }
+        }
+      }
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        if (colc[nb] < NC) {
+          const int l = nb * 16 + i16;
+          T* pl = reinterpret_cast<T*>(buf + (l >> 1) * a.lstride) + (l & 1);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) pl[2 * pidx(x0 + MF::row(lane, r))] += acc[nb][r];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  fft_any<T, 0, N>(buf, a.lstride, a.ncl, a.plan, tw, false, nullptr);
+  // half spectrum of the NC*R real lines (packed two per complex line): one
+  // wave per line, lanes along kx
+  for (int l = wave; l < NCOL; l += nwave) {
+    const long long row = tile_line_row(a, l, ty0, tz0, rhs, NC);
+    if (row < 0) continue;
+    const cplx<T>* line = buf + (l >> 1) * a.lstride;
+    cplx<T>* dst = spec + row * KX;
+    for (int k = lane; k < KX; k += 64) {
+      const cplx<T> Z = line[pidx(k)];
+      const cplx<T> Zc = cconj(line[pidx(k == 0 ? 0 : N - k)]);
+      cplx<T> v;
+      if ((l & 1) == 0) v = cmk<T>((Z.x + Zc.x) * (T)0.5, (Z.y + Zc.y) * (T)0.5);
+      else v = cmk<T>((Z.y - Zc.y) * (T)0.5, (Zc.x - Z.x) * (T)0.5);
+      dst[k] = v;
+    }
+  }
+}
+
+// Inverse x-FFT + contraction.  Wave w handles blocks of 16 consecutive
+// sorted candidates w, w + 4, ...: A[q][x] over the x span of the block (in
+// unwrapped coordinates, the grid read at x mod nx), then the row weights and
+// a reduction over the R lanes of each component; FP64 atomics into out.
+template <typename T, int N, int NC, int R>
+__global__ void __launch_bounds__(256) k_xifft_contract_mfma(TileArgs a, ImgArgs im, const cplx<T>* __restrict__ spec,
+                                                             double* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  using MF = Mfma<T>;
+  using acc_t = typename MF::acc_t;
+  static_assert(R == 4 || R == 8, "tile rows");
+  constexpr int NCOL = NC * R, NB = (NCOL + 15) / 16, KX = N / 2 + 1;
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* buf = tw + N;
+  const int P = a.P, PW = P + R;
+  const MfmaStage<T> st = mfma_stage_layout<T, NC>(reinterpret_cast<unsigned char*>(buf + a.ncl * a.lstride), a.ch, PW);
+  const int tile = blockIdx.x + blockIdx.y * gridDim.x;
+  const int ty0 = blockIdx.x * a.ty, tz0 = blockIdx.y * a.tz, rhs = blockIdx.z;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
+  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];
+  // pack two half spectra per complex line: Z = A + iB (Hermitian
+  // extension); one wave per complex line, lanes along kx
+  for (int q = wave; q < a.ncl; q += nwave) {
+    const long long ra = tile_line_row(a, 2 * q, ty0, tz0, rhs, NC);
+    const long long rb = 2 * q + 1 < NCOL ? tile_line_row(a, 2 * q + 1, ty0, tz0, rhs, NC) : -1;
+    cplx<T>* line = buf + q * a.lstride;
+    for (int k = lane; k < KX; k += 64) {
+      const cplx<T> A = ra >= 0 ? spec[ra * KX + k] : cmk<T>(0, 0);
+      const cplx<T> B = rb >= 0 ? spec[rb * KX + k] : cmk<T>(0, 0);
+      if (k == 0 || 2 * k == N) {
+        line[pidx(k)] = cmk<T>(A.x, B.x);
+      } else {
+        line[pidx(k)] = cmk<T>(A.x - B.y, A.y + B.x);
+        line[pidx(N - k)] = cmk<T>(A.x + B.y, B.x - A.y);
+      }
+    }
+  }
+  __syncthreads();
+  fft_any<T, 0, N>(buf, a.lstride, a.ncl, a.plan, tw, true, nullptr);
+  const size_t qt = (size_t)im.off[tile];
+  const int total = im.off[tile + 1] - (int)qt;
+  const int i16 = lane & 15, k4 = lane >> 4;
+  const int colrow = i16 % R;
+  int colc[NB];
+  const T* colp[NB];
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) {
+    const int j = nb * 16 + i16;
+    colc[nb] = j / R;
+    const int jl = min(j, NCOL - 1);
+    colp[nb] = reinterpret_cast<const T*>(buf + (jl >> 1) * a.lstride) + (jl & 1);
+  }
+  double* outr = out + (size_t)rhs * NC * (size_t)a.n;
+  for (int base = 0; base < total; base += a.ch) {
+    const int nc = min(a.ch, total - base);
+    mfma_stage_chunk<T, false, NC>(a, im, st, qt + base, nc, rhs);
+    __syncthreads();
+    const int nblk = (nc + 15) >> 4;
+    for (int cb = wave; cb < nblk; cb += nwave) {
+      const int c0 = cb * 16;
+      const int cand = c0 + i16;
+      const bool ok = cand < nc;
+      const int cq = ok ? cand : c0;
+      const int gxc = st.gx[cq];
+      const int xs = __builtin_amdgcn_readfirstlane(st.gx[c0]);
+      const int xe = __builtin_amdgcn_readfirstlane(st.gx[min(c0 + 15, nc - 1)]) + P;
+      acc_t acc[NB];
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) acc[nb] = acc_t{0, 0, 0, 0};
+      for (int xx = xs; xx < xe; xx += 4) {
+        const int xk = xx + k4;
+        const int o = xk - gxc;
+        const T f = st.fw[cq * PW + min(max(o, 0), P - 1)];
+        const T av = (ok && o >= 0 && o < P) ? f : (T)0;
+        const int xw = xk >= N ? xk - N : xk;
+        const int px = 2 * pidx(xw);
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+          const T g = colp[nb][px];
+          acc[nb] = MF::mac(av, colc[nb] < NC ? g : (T)0, acc[nb]);
+        }
+      }
+      // out[q][c] = sum_row w_q[row] D[q][(c, row)]: weight, then sum the R
+      // lanes of each component
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int cr = c0 + MF::row(lane, r);
+        const bool okr = cr < nc;
+        const int crq = okr ? cr : c0;
+        const T wq = st.fw[crq * PW + P + colrow];
+        const int orig = st.idx[crq];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+          const T v = group_sum<T>(acc[nb][r] * wq, R);
+          if (colc[nb] < NC && colrow == 0 && okr) atomicAdd(outr + NC * (size_t)orig + colc[nb], (double)v * a.hvol);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace mpol

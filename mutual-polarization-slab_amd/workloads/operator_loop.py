@@ -1,0 +1,56 @@
+"""Operator driver for kernel profiles: builds a named workload, sets up one
+context and applies the dipole operator a few times.
+
+    python -m workloads.operator_loop [--workload n1e5_phi0.4] [--precision fp32|fp64]
+                                      [--reps 3] [--nrhs 1] [--engine 0|1|2] [--sweeps 20]
+
+(run from mutual-polarization-slab_amd/, or with it on sys.path).  Prints
+one line per step so a profiler run shows progress.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if _PKG not in sys.path:
+    sys.path.insert(0, _PKG)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="n1e5_phi0.4")
+    ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--nrhs", type=int, default=1)
+    ap.add_argument("--engine", type=int, default=0)
+    ap.add_argument("--sweeps", type=int, default=20)
+    args = ap.parse_args(argv)
+
+    import torch
+
+    import workloads
+    from mpol_hip import FP32, FP64, Context, default_tables
+
+    cfg = workloads.configuration(args.workload, sweeps=args.sweeps)
+    print(f"workload {args.workload}: n={cfg['x'].shape[0]} grid={list(cfg['ngrid'])} P={cfg['P']}", flush=True)
+    perp, para, rv = default_tables(cfg["xi"])
+    ctx = Context()
+    ctx.set_tables(perp, para, rv)
+    ctx.set_geometry(cfg["x"], cfg["box"], cfg["ngrid"], cfg["P"], cfg["xi"], cfg["eta"], cfg["rc"])
+    ctx.set_permittivity(cfg["eps"])
+    ctx.set_tuning(engine=args.engine)
+    prec = FP32 if args.precision == "fp32" else FP64
+    g = torch.Generator().manual_seed(7)
+    m = torch.randn((args.nrhs, cfg["x"].shape[0], 3), generator=g, dtype=torch.float64).to("cuda")
+    for r in range(args.reps):
+        out = ctx.matvec(m, precision=prec, slab_factor=0.0)
+        torch.cuda.synchronize()
+        print(f"matvec {r + 1}/{args.reps} ({args.precision}, {args.nrhs} rhs): |out| = {out.norm().item():.6e}",
+              flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

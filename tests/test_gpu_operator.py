@@ -162,3 +162,38 @@ def test_operator_noncubic_box(gpu):
     geo = eo.OperatorGeometry(x, box, ng, P, xi, rc, perp, para, rv)
     ref = np.real(eo.field(geo, m, eps, 1.0))
     assert relerr(out, ref) < 1e-12
+
+
+@pytest.mark.parametrize("ng,P,tile,chunk", [((32, 32, 32), 8, (0, 0), 0), ((64, 30, 36), 10, (4, 2), 32),
+                                              ((96, 40, 24), 9, (2, 2), 64), ((64, 64, 64), 12, (2, 2), 256)])
+def test_spread_contract_engines_agree(gpu, ng, P, tile, chunk):
+    """The matrix-core spread/contract (per-tile candidate images, several
+    chunks per tile, partial edge tiles) against the vector-loop kernels and
+    the FP64 oracle, both precisions."""
+    from mpol_hip import FP32, Context
+    print("case", ng, P, tile, chunk, flush=True)
+    rng = np.random.default_rng(sum(ng) + P)
+    box = np.array(ng, dtype=float) * 0.41
+    n = 700
+    x = rng.random((n, 3)) * box
+    xi = 1.0
+    rc = 3.2
+    perp, para, rv = tables(xi)
+    eps = 2.0 + 10 * rng.random(n)
+    c = Context()
+    c.set_tables(perp, para, rv)
+    h = box / np.array(ng)
+    eta = P * (h * xi) ** 2 / np.pi
+    c.set_geometry(x, box, np.array(ng), P, xi, eta, rc)
+    c.set_permittivity(eps)
+    m = rng.standard_normal((2, n, 3))
+    out = {}
+    for eng in (1, 2):
+        c.set_tuning(tile_ty=tile[0], tile_tz=tile[1], chunk=chunk if eng == 1 else 0, engine=eng)
+        out[eng, 64] = c.matvec(m, slab_factor=1.0).cpu().numpy()
+        out[eng, 32] = c.matvec(m, precision=FP32, slab_factor=1.0).cpu().numpy()
+    assert relerr(out[1, 64], out[2, 64]) < 1e-13
+    assert relerr(out[1, 32], out[2, 64]) < 2e-6
+    geo = eo.OperatorGeometry(x, box, np.array(ng), P, xi, rc, perp, para, rv)
+    ref = np.real(eo.field(geo, m[1], eps, 1.0))
+    assert relerr(out[1, 64][1], ref) < 1e-12

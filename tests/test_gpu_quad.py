@@ -260,3 +260,20 @@ def test_dipole_tensor_spectrum_complex_eps(gpu):
         for d in range(3):
             sol = np.linalg.solve(A, np.tile(np.eye(3)[d], 10)).reshape(10, 3)
             assert _rel(p[:, :, d, k], sol) < 1e-8
+
+
+@pytest.mark.parametrize("tile,chunk", [((2, 2), 0), ((4, 2), 32)])
+def test_quad_spread_contract_engines_agree(gpu, tile, chunk):
+    """Eight-component matrix-core spread/contract against the vector loops
+    and the oracle (several chunks per tile with chunk 32)."""
+    from mpol_hip import FP32
+    c = Case(gpu, n=60, phi=0.05, ngrid=[64, 40, 36], P=9)
+    m = _moments(c.n, 9, k=2)
+    out = {}
+    for eng in (1, 2):
+        c.ctx.set_tuning(tile_ty=tile[0], tile_tz=tile[1], chunk=chunk if eng == 1 else 0, engine=eng)
+        out[eng, 64] = c.ctx.quad_matvec(m).cpu().numpy()
+        out[eng, 32] = c.ctx.quad_matvec(m, precision=FP32).cpu().numpy()
+    assert _rel(out[1, 64], out[2, 64]) < 1e-13
+    assert _rel(out[1, 32], out[2, 64]) < 2e-6
+    assert _rel(out[1, 64][0], _oracle(c, m[0]).real) < 1e-12
