@@ -51,7 +51,11 @@ typedef struct {
   int restart;         /* Krylov dimension per cycle                            */
   int maxit;           /* maximum number of restart cycles                      */
   int mixed;           /* 0: FP64 operator; 1: FP32 operator inside FP64
-                          iterative refinement (FP64 Arnoldi/Givens always)     */
+                          iterative refinement (FP64 Arnoldi/Givens always):
+                          each sweep takes an FP64 residual, then an FP32
+                          correction solve from zero with Krylov dimension
+                          max(restart, 32) that stops on its GMRES residual
+                          estimate                                             */
   int max_refine;      /* refinement sweeps in mixed mode                       */
   double slab_factor;  /* coefficient of the slab correction                    */
 } mpol_solve_opts;

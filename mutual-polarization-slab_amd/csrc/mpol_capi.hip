@@ -51,8 +51,17 @@ int dots(mpol_ctx* c, const double* A, size_t stride_i, int cnt, const double* w
 // Restarted GMRES on x (in: initial guess, out: solution) for A x = b.
 // nrhs here counts systems; a system's vector holds all its right-hand sides
 // (kind components per particle, times 2 with complex permittivities).
+// Krylov dimension of the FP32 correction solves of the mixed scheme (at
+// least the caller's restart): one cycle usually reaches the inner tolerance.
+constexpr int kInnerRestart = 32;
+
+// Restarted GMRES.  inner = true is the correction solve of the mixed scheme:
+// x starts at zero (the first residual is b, no operator application) and
+// the solve ends on the Givens estimate of the residual without a final
+// true-residual application (the caller's FP64 residual certifies it).
 int gmres_core(mpol_ctx* c, int kind, const double* b, double* x, int nrhs, double tol, int restart, int maxit,
-               int precision, double slab, int64_t* nmv, double* relres_host, int* converged_host) {
+               int precision, double slab, int64_t* nmv, double* relres_host, int* converged_host,
+               bool inner = false) {
   const int64_t n3 = (int64_t)unknowns_per_particle(kind) * c->n * rhs_per_system(c);
   const size_t tot = (size_t)nrhs * n3;
   const size_t sv = tot;  // stride between basis vectors
@@ -96,10 +105,14 @@ int gmres_core(mpol_ctx* c, int kind, const double* b, double* x, int nrhs, doub
   bool all_done = false;
   for (int outer = 0; outer <= maxit; ++outer) {
     // true residual at the start of each cycle (and once after the last)
-    CHECK_STATUS(apply(c, kind, x, c->gAx.p, nrhs, precision, slab));
-    ++*nmv;
-    hipLaunchKernelGGL(k_residual, dim3(cdiv((int64_t)tot, 256)), dim3(256), 0, st, b, c->gAx.p, c->gR.p,
-                       (int64_t)tot);
+    if (inner && outer == 0) {
+      hipLaunchKernelGGL(k_copy, dim3(cdiv((int64_t)tot, 256)), dim3(256), 0, st, b, c->gR.p, (int64_t)tot);
+    } else {
+      CHECK_STATUS(apply(c, kind, x, c->gAx.p, nrhs, precision, slab));
+      ++*nmv;
+      hipLaunchKernelGGL(k_residual, dim3(cdiv((int64_t)tot, 256)), dim3(256), 0, st, b, c->gAx.p, c->gR.p,
+                         (int64_t)tot);
+    }
     dots(c, c->gR.p, 0, 1, c->gR.p, n3, nrhs, c->grn2.p);
     hipLaunchKernelGGL(k_cycle_start, dim3(sgrid), dim3(256), 0, st, nrhs, restart, c->grn2.p, c->gbn.p, tol,
                        c->gstate.p, c->gg.p, c->gbeta.p, c->grel.p);
@@ -126,7 +139,7 @@ int gmres_core(mpol_ctx* c, int kind, const double* b, double* x, int nrhs, doub
       hipLaunchKernelGGL(k_normalize_next, vgrid, dim3(256), 0, st, c->gW.p, c->gV.p + (size_t)(j + 1) * sv,
                          c->gH.p, restart, j, n3, c->gstate.p);
       hipLaunchKernelGGL(k_givens, dim3(sgrid), dim3(256), 0, st, nrhs, restart, j, c->gH.p, c->gcs.p, c->gsn.p,
-                         c->gg.p, c->gbn.p, tol, c->gstate.p, c->gkdim.p);
+                         c->gg.p, c->gbn.p, tol, c->gstate.p, c->gkdim.p, c->grel.p);
       hipLaunchKernelGGL(k_backsolve, dim3(sgrid), dim3(256), 0, st, nrhs, restart, c->gH.p, c->gg.p,
                          c->gkdim.p, c->gstate.p, c->gy.p);
       hipLaunchKernelGGL(k_update_x, vgrid, dim3(256), 0, st, c->gV.p, sv, c->gy.p, c->gkdim.p, c->gstate.p,
@@ -135,6 +148,15 @@ int gmres_core(mpol_ctx* c, int kind, const double* b, double* x, int nrhs, doub
       HIPCHK(hipMemcpyAsync(state.data(), c->gstate.p, nrhs * sizeof(int), hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
       if (std::none_of(state.begin(), state.end(), [](int s) { return s == kIter; })) break;
+    }
+    if (inner) {
+      // every right-hand side whose estimated residual met tol is done
+      HIPCHK(hipMemcpyAsync(rel.data(), c->grel.p, nrhs * sizeof(double), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      bool est_done = true;
+      for (int r = 0; r < nrhs; ++r)
+        if (state[r] != kDone && !(rel[r] <= tol)) est_done = false;
+      if (est_done) break;
     }
   }
   HIPCHK(hipGetLastError());
@@ -563,8 +585,8 @@ int solve_impl(mpol_ctx* c, int kind, const double* b, const double* x0, double*
       hipLaunchKernelGGL(k_copy, dim3(cdiv(tot, 256)), dim3(256), 0, st, c->gd.p, c->gx.p, tot);
       HIPCHK(hipMemsetAsync(c->gd.p, 0, tot * sizeof(double), st));
       int64_t nmv = 0;
-      CHECK_STATUS(gmres_core(c, kind, c->gx.p, c->gd.p, nrhs, tin, o->restart, o->maxit, MPOL_FP32, slab, &nmv,
-                              nullptr, nullptr));
+      CHECK_STATUS(gmres_core(c, kind, c->gx.p, c->gd.p, nrhs, tin, std::max(o->restart, kInnerRestart), o->maxit,
+                              MPOL_FP32, slab, &nmv, nullptr, nullptr, true));
       nmv32 += nmv;
       hipLaunchKernelGGL(k_axpy_all, dim3(cdiv(tot, 256)), dim3(256), 0, st, 1.0, c->gd.p, x, tot);
     }

@@ -129,10 +129,11 @@ static __global__ void k_normalize_next(const double* __restrict__ w, double* __
   vnext[(size_t)rhs * n + k] = hn != 0.0 ? w[(size_t)rhs * n + k] / hn : 0.0;
 }
 
-// Givens rotations on column j; decide whether the cycle ends for each rhs.
+// Givens rotations on column j; decide whether the cycle ends for each rhs;
+// relest[rhs] = the estimated relative residual |g_{j+1}| / ||b||.
 static __global__ void k_givens(int nrhs, int restart, int j, double* __restrict__ H, double* __restrict__ cs,
                          double* __restrict__ sn, double* __restrict__ g, const double* __restrict__ bnorm,
-                         double tol, int* __restrict__ state, int* __restrict__ kdim) {
+                         double tol, int* __restrict__ state, int* __restrict__ kdim, double* __restrict__ relest) {
   const int rhs = blockIdx.x * blockDim.x + threadIdx.x;
   if (rhs >= nrhs || state[rhs] != kIter) return;
   double* Hr = H + (size_t)rhs * (restart + 1) * restart;
@@ -160,6 +161,7 @@ static __global__ void k_givens(int nrhs, int restart, int j, double* __restrict
   gr[j] = c[j] * gj;
   gr[j + 1] = -s[j] * gj;
   const double rel = fabs(gr[j + 1]) / bnorm[rhs];
+  relest[rhs] = rel;
   if (rel <= tol || j == restart - 1 || breakdown) {
     state[rhs] = kSolve;
     kdim[rhs] = j + 1;

@@ -967,18 +967,18 @@ __global__ void __launch_bounds__(256) k_fft_y(ColArgs a, cplx<T>* __restrict__ 
   const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
   for (int k = threadIdx.x; k < a.ny; k += blockDim.x) tw[k] = twg[k];
   cplx<T>* base = spec + ((size_t)b * a.nz + z) * a.ny * a.kx;
-  const int tot = a.tx * a.ny;
-  for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
-    const int c = idx % a.tx, y = idx / a.tx;
-    const int kx = kx0 + c;
-    buf[c * a.lstride + pidx(y)] = (kx < a.kx) ? base[(size_t)y * a.kx + kx] : cmk<T>(0, 0);
+  // thread -> (column c, first row); rows step by the threads per column set
+  const int c = threadIdx.x % a.tx, ystep = blockDim.x / a.tx;
+  const int kx = kx0 + c;
+  const bool act = (int)threadIdx.x < ystep * a.tx;
+  if (act) {
+    for (int y = threadIdx.x / a.tx; y < a.ny; y += ystep)
+      buf[c * a.lstride + pidx(y)] = (kx < a.kx) ? base[(size_t)y * a.kx + kx] : cmk<T>(0, 0);
   }
   __syncthreads();
   fft_any<T, E, N>(buf, a.lstride, a.tx, a.plan, tw, inverse != 0, scratch);
-  for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
-    const int c = idx % a.tx, y = idx / a.tx;
-    const int kx = kx0 + c;
-    if (kx < a.kx) base[(size_t)y * a.kx + kx] = buf[c * a.lstride + pidx(y)];
+  if (act && kx < a.kx) {
+    for (int y = threadIdx.x / a.tx; y < a.ny; y += ystep) base[(size_t)y * a.kx + kx] = buf[c * a.lstride + pidx(y)];
   }
 }
 
@@ -1058,15 +1058,17 @@ __global__ void __launch_bounds__(1024) k_fft_z_scale(ColArgs a, ScaleParams sp,
   const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
   for (int k = threadIdx.x; k < a.nz; k += blockDim.x) tw[k] = twg[k];
   const size_t plane = (size_t)a.ny * a.kx;
-  const int tot = NC * a.tx * a.nz;
-  for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
-    const int c = idx / (a.tx * a.nz);
-    const int rem = idx - c * a.tx * a.nz;
-    const int z = rem / a.tx, cc = rem - z * a.tx;
-    const int kx = kx0 + cc;
-    cplx<T> v = cmk<T>(0, 0);
-    if (kx < a.kx) v = spec[((size_t)(rhs * NC + c) * a.nz + z) * plane + (size_t)y * a.kx + kx];
-    buf[(c * a.tx + cc) * a.lstride + pidx(z)] = v;
+  // thread -> (column cc, first z); z steps by the threads per column set
+  const int cc_t = threadIdx.x % a.tx, zstep = blockDim.x / a.tx, z_t = threadIdx.x / a.tx;
+  const int kx_t = kx0 + cc_t;
+  const bool act = (int)threadIdx.x < zstep * a.tx;
+  if (act) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const cplx<T>* src = spec + (size_t)(rhs * NC + c) * a.nz * plane + (size_t)y * a.kx + kx_t;
+      cplx<T>* dst = buf + (c * a.tx + cc_t) * a.lstride;
+      for (int z = z_t; z < a.nz; z += zstep) dst[pidx(z)] = (kx_t < a.kx) ? src[(size_t)z * plane] : cmk<T>(0, 0);
+    }
   }
   __syncthreads();
   fft_any<T, E, N>(buf, a.lstride, NC * a.tx, a.plan, tw, false, scratch);
@@ -1077,10 +1079,8 @@ __global__ void __launch_bounds__(1024) k_fft_z_scale(ColArgs a, ScaleParams sp,
   // so real grids stay real.
   const int my = fft_freq(y, a.ny);
   const bool nyq_y = (a.ny % 2 == 0) && (2 * my == -a.ny);
-  for (int idx = threadIdx.x; idx < a.tx * a.nz; idx += blockDim.x) {
-    const int cc = idx % a.tx, z = idx / a.tx;
-    const int kx = kx0 + cc;
-    if (kx >= a.kx) continue;
+  for (int z = z_t; act && kx_t < a.kx && z < a.nz; z += zstep) {
+    const int cc = cc_t, kx = kx_t;
     const int mx = fft_freq(kx, a.nx), mz = fft_freq(z, a.nz);
     const bool nyq_x = (a.nx % 2 == 0) && (2 * mx == -a.nx);
     const bool nyq_z = (a.nz % 2 == 0) && (2 * mz == -a.nz);
@@ -1094,8 +1094,8 @@ __global__ void __launch_bounds__(1024) k_fft_z_scale(ColArgs a, ScaleParams sp,
       for (int c = 0; c < NC; ++c) *f[c] = cmk<T>(0, 0);
       continue;
     }
-    const double kn = sqrt(kk);
-    const T u0 = (T)(k0 / kn), u1 = (T)(k1 / kn), u2 = (T)(k2 / kn);
+    const double ikn = 1.0 / sqrt(kk);
+    const T u0 = (T)(k0 * ikn), u1 = (T)(k1 * ikn), u2 = (T)(k2 * ikn);
     if constexpr (NC == 1) {
       // polydisperse scalar grid: exp(-(1-eta) q^2/(4 xi^2))/q^2
       const double cf = (sp.ex[kx] * sp.ey[y] * sp.ez[z]) / kk * sp.inv_g;
@@ -1123,7 +1123,7 @@ __global__ void __launch_bounds__(1024) k_fft_z_scale(ColArgs a, ScaleParams sp,
         ex = sp.ex[kx] * sp.ey[y] * sp.ez[z];
       } else {
         double j1, j2;
-        sph_j12(kn, j1, j2);
+        sph_j12(kk * ikn, j1, j2);
         a11 = j1 * j1 / kk; a12 = j1 * j2 / kk; a22 = j2 * j2 / kk;
         const double ek =
             sp.one_minus_eta[0] * k0 * k0 + sp.one_minus_eta[1] * k1 * k1 + sp.one_minus_eta[2] * k2 * k2;
@@ -1148,13 +1148,13 @@ __global__ void __launch_bounds__(1024) k_fft_z_scale(ColArgs a, ScaleParams sp,
   }
   __syncthreads();
   fft_any<T, E, N>(buf, a.lstride, NC * a.tx, a.plan, tw, true, scratch);
-  for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
-    const int c = idx / (a.tx * a.nz);
-    const int rem = idx - c * a.tx * a.nz;
-    const int z = rem / a.tx, cc = rem - z * a.tx;
-    const int kx = kx0 + cc;
-    if (kx < a.kx) spec[((size_t)(rhs * NC + c) * a.nz + z) * plane + (size_t)y * a.kx + kx] =
-        buf[(c * a.tx + cc) * a.lstride + pidx(z)];
+  if (act && kx_t < a.kx) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      cplx<T>* dst = spec + (size_t)(rhs * NC + c) * a.nz * plane + (size_t)y * a.kx + kx_t;
+      const cplx<T>* src = buf + (c * a.tx + cc_t) * a.lstride;
+      for (int z = z_t; z < a.nz; z += zstep) dst[(size_t)z * plane] = src[pidx(z)];
+    }
   }
 }
 

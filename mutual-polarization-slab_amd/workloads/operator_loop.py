@@ -3,6 +3,7 @@ context and applies the dipole operator a few times.
 
     python -m workloads.operator_loop [--workload n1e5_phi0.4] [--precision fp32|fp64]
                                       [--reps 3] [--nrhs 1] [--engine 0|1|2] [--sweeps 20]
+                                      [--copy-mb 256]
 
 (run from mutual-polarization-slab_amd/, or with it on sys.path).  Prints
 one line per step so a profiler run shows progress.
@@ -26,6 +27,8 @@ def main(argv=None):
     ap.add_argument("--nrhs", type=int, default=1)
     ap.add_argument("--engine", type=int, default=0)
     ap.add_argument("--sweeps", type=int, default=20)
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="first copy a float32 device buffer of this many MiB (counter calibration)")
     args = ap.parse_args(argv)
 
     import torch
@@ -42,6 +45,13 @@ def main(argv=None):
     ctx.set_permittivity(cfg["eps"])
     ctx.set_tuning(engine=args.engine)
     prec = FP32 if args.precision == "fp32" else FP64
+    if args.copy_mb > 0:
+        src = torch.ones(args.copy_mb << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"copied {args.copy_mb} MiB device to device", flush=True)
+        del src, dst
     g = torch.Generator().manual_seed(7)
     m = torch.randn((args.nrhs, cfg["x"].shape[0], 3), generator=g, dtype=torch.float64).to("cuda")
     for r in range(args.reps):
