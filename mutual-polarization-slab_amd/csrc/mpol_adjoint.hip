@@ -31,7 +31,7 @@ extern "C" int mpol_matvec_vjp(mpol_ctx* c, const double* m, const double* lam, 
   const bool gen_x = generic_fft(c->plan[0]);
   TileCfg tr = tile_cfg_nc<1>(c->ng[0], gen_x, cs, 0, kLdsTile);
   TileArgs ta{};
-  ta.nx = c->ng[0]; ta.ny = c->ng[1]; ta.nz = c->ng[2]; ta.kx = c->kx;
+  ta.nx = c->ng[0]; ta.ny = c->ng[1]; ta.nz = c->ng[2]; ta.kx = c->kx; ta.kxs = c->kxs;
   ta.ty = tr.ty; ta.tz = tr.tz; ta.rows = tr.rows; ta.ncl = tr.ncl; ta.lstride = tr.lstride;
   ta.tw = c->tw64[0].p; ta.plan = c->plan[0];
   const size_t rlds = ((size_t)c->ng[0] + (size_t)tr.ncl * tr.lstride * (gen_x ? 2 : 1)) * cs;

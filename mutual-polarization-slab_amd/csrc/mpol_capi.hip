@@ -264,6 +264,7 @@ int mpol_set_geometry(mpol_ctx* c, const double* x, int64_t n, const double box[
     c->nc[d] = ncell < 3 ? 3 : ncell;
   }
   c->kx = c->ng[0] / 2 + 1;
+  c->kxs = (c->kx + 15) / 16 * 16;
   c->P = P;
   c->half = (P - 1) / 2;
   c->xi = xi;
@@ -704,9 +705,11 @@ int mpol_debug_spectrum(mpol_ctx* c, const double* m, int32_t nrhs, int32_t prec
                      ? run_operator<float, kOpDipole>(c, m, tmp.p, nullptr, nullptr, nrhs, 0.0, nullptr, stage)
                      : run_operator<double, kOpDipole>(c, m, tmp.p, nullptr, nullptr, nrhs, 0.0, nullptr, stage);
   CHECK_STATUS(st);
-  const size_t bytes = (size_t)3 * nrhs * c->ng[2] * c->ng[1] * c->kx *
-                       (precision == MPOL_FP32 ? sizeof(cplx<float>) : sizeof(cplx<double>));
-  HIPCHK(hipMemcpyAsync(spec_out, c->spec.p, bytes, hipMemcpyDeviceToDevice, c->stream));
+  const size_t cs = precision == MPOL_FP32 ? sizeof(cplx<float>) : sizeof(cplx<double>);
+  const size_t rows = (size_t)3 * nrhs * c->ng[2] * c->ng[1];
+  // drop the row padding of the internal layout
+  HIPCHK(hipMemcpy2DAsync(spec_out, (size_t)c->kx * cs, c->spec.p, (size_t)c->kxs * cs, (size_t)c->kx * cs, rows,
+                          hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return MPOL_OK;
 }

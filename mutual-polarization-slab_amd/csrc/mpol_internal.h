@@ -163,6 +163,8 @@ struct TileImage {
   DevBuf<int> cnt, off;
   DevBuf<int2> gs;
   DevBuf<unsigned char> fw;
+  DevBuf<int> pcnt, poff, pslot;  // slots of each sorted particle
+  DevBuf<unsigned char> part;     // contraction partials
 };
 constexpr size_t kLdsMax = 150 * 1024;  // dynamic LDS per workgroup (gfx950 has 160 KB per CU)
 constexpr size_t kLdsCol = 112 * 1024;
@@ -181,7 +183,7 @@ struct mpol_ctx {
   int n = 0;
   double box[3] = {0, 0, 0}, h[3] = {0, 0, 0}, eta[3] = {0, 0, 0};
   double xi = 0, rc = 0;
-  int ng[3] = {0, 0, 0}, kx = 0, P = 0, half = 0, nc[3] = {0, 0, 0};
+  int ng[3] = {0, 0, 0}, kx = 0, kxs = 0, P = 0, half = 0, nc[3] = {0, 0, 0};
   int64_t npairs = 0;
   DevBuf<double> x, base;
   DevBuf<int> bin_key, bin_count, bin_start, bin_cursor, cell_key, cell_count, cell_start, cell_cursor,
@@ -354,8 +356,8 @@ void apply_tile_tuning(const mpol_ctx* c, TileCfg& t) {
 }
 
 // Tile shape and chunk of the matrix-core spread/contract: the largest tile
-// (rows first) whose chunk of at least 128 candidates fits kLdsMfma, else the
-// largest chunk that fits.  ok = false when the grid or stencil does not suit
+// (rows first) whose chunk of 128 candidates fits kLdsMfma, else the largest
+// chunk that fits.  ok = false when the grid or stencil does not suit
 // that path (compile-time x plan, nx % 16 == 0, P + 16 <= nx) or the engine
 // knob asks for the vector loops.
 template <typename T, int NC>
@@ -376,7 +378,8 @@ TileCfg mfma_tile_cfg(const mpol_ctx* c, bool& ok) {
       t.rows = t.ty * t.tz;
       t.ncl = (NC * t.rows + 1) / 2;
       t.lstride = padded_len(nx);
-      for (t.ch = 256; t.ch >= (pass == 0 ? 128 : 32); t.ch /= 2) {
+      // 128 candidates per chunk keep four FP32 workgroups per CU
+      for (t.ch = 128; t.ch >= (pass == 0 ? 128 : 32); t.ch /= 2) {
         t.lds = mfma_tile_lds<NC>(nx, cs, c->P, t.ncl, t.lstride, t.ch, t.rows);
         if (t.lds <= budget) {
           found = true;
@@ -417,6 +420,18 @@ int ensure_tile_image(mpol_ctx* c, const TileArgs& ta, TileImage& im) {
   HIPALLOC(im.fw, slots * (size_t)(ta.P + ta.rows) * sizeof(T));
   hipLaunchKernelGGL((k_img_fill<T>), dim3(ntx, ntz), dim3(256), 0, st, ta, im.off.p, im.gs.p,
                      reinterpret_cast<T*>(im.fw.p));
+  // per-particle slot lists (the contraction's reduction order)
+  const int n = c->n;
+  HIPALLOC(im.pcnt, (size_t)n);
+  HIPALLOC(im.poff, (size_t)n + 1);
+  HIPALLOC(im.pslot, slots);
+  HIPCHK(hipMemsetAsync(im.pcnt.p, 0, (size_t)n * sizeof(int), st));
+  hipLaunchKernelGGL(k_img_pcount, dim3(cdiv(total, 256)), dim3(256), 0, st, im.gs.p, (int64_t)total, im.pcnt.p);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, im.pcnt.p, im.poff.p, n);
+  HIPCHK(hipMemsetAsync(im.pcnt.p, 0, (size_t)n * sizeof(int), st));
+  hipLaunchKernelGGL(k_img_pscatter, dim3(cdiv(total, 256)), dim3(256), 0, st, im.gs.p, (int64_t)total, im.poff.p,
+                     im.pcnt.p, im.pslot.p);
+  hipLaunchKernelGGL(k_img_psort, dim3(cdiv(n, 256)), dim3(256), 0, st, n, im.poff.p, im.pslot.p);
   HIPCHK(hipGetLastError());
   im.valid = true;
   im.ty = ta.ty;
@@ -430,7 +445,7 @@ template <typename T, int NC>
 int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* hslab, int nrhs, double slab,
                  hipEvent_t* ev = nullptr, int stop_after = kStages) {
   const size_t cs = sizeof(cplx<T>);
-  const size_t spec_bytes = (size_t)NC * nrhs * c->ng[2] * c->ng[1] * c->kx * cs;
+  const size_t spec_bytes = (size_t)NC * nrhs * c->ng[2] * c->ng[1] * c->kxs * cs;
   HIPALLOC(c->spec, spec_bytes);
   HIPALLOC(c->zsum, (size_t)nrhs * (kZsumBlocks + 1));
   cplx<T>* spec = reinterpret_cast<cplx<T>*>(c->spec.p);
@@ -475,7 +490,7 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   apply_tile_tuning<NC>(c, tc);
   tc.lds = tile_lds<NC>(c->ng[0], gen_x, cs, c->P, tc, false);
   TileArgs ta{};
-  ta.nx = c->ng[0]; ta.ny = c->ng[1]; ta.nz = c->ng[2]; ta.kx = c->kx;
+  ta.nx = c->ng[0]; ta.ny = c->ng[1]; ta.nz = c->ng[2]; ta.kx = c->kx; ta.kxs = c->kxs;
   ta.ty = tc.ty; ta.tz = tc.tz; ta.rows = tc.rows; ta.ncl = tc.ncl; ta.lstride = tc.lstride; ta.ch = tc.ch;
   ta.P = c->P; ta.half = c->half; ta.n = n; ta.nrhs = nrhs;
   ta.rec = c->rec.p; ta.base = base_ptr<T>(c); ta.ms = c->ms.p; ta.bin_start = c->bin_start.p;
@@ -504,6 +519,9 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
     ima.gs = im.gs.p;
     ima.fw = im.fw.p;
     ima.perm = c->perm.p;
+    ima.total = im.total;
+    HIPALLOC(im.part, (size_t)nrhs * (size_t)std::max<int64_t>(im.total, 1) * NC * sizeof(T));
+    ima.part = im.part.p;
     mgrid = dim3(cdiv(c->ng[1], tm.ty), cdiv(c->ng[2], tm.tz), nrhs);
   }
 
@@ -565,7 +583,7 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   const ColCfg cy = col_cfg(c->ng[1], generic_fft(c->plan[1]), c->kx, 1, cs, kLdsCol,
                             c->tune[3] > 0 ? c->tune[3] : (sizeof(T) == 8 ? 8 : 16));  // 128-byte rows
   ColArgs ay{};
-  ay.nx = c->ng[0]; ay.ny = c->ng[1]; ay.nz = c->ng[2]; ay.kx = c->kx; ay.tx = cy.tx; ay.lstride = cy.lstride;
+  ay.nx = c->ng[0]; ay.ny = c->ng[1]; ay.nz = c->ng[2]; ay.kx = c->kx; ay.kxs = c->kxs; ay.tx = cy.tx; ay.lstride = cy.lstride;
   ay.plan = c->plan[1]; ay.tw = tw_ptr<T>(c, 1);
   const dim3 ygrid(cdiv(c->kx, cy.tx), c->ng[2], NC * nrhs);
   if (cy.lds > kLdsMax) return fail(MPOL_EINVAL, "y-pass columns do not fit in LDS");
@@ -582,7 +600,7 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   const ColCfg cz = col_cfg(c->ng[2], generic_fft(c->plan[2]), c->kx, NC, cs, kLdsCol,
                             c->tune[4] > 0 ? c->tune[4] : (sizeof(T) == 8 ? 4 : 8));
   ColArgs az{};
-  az.nx = c->ng[0]; az.ny = c->ng[1]; az.nz = c->ng[2]; az.kx = c->kx; az.tx = cz.tx; az.lstride = cz.lstride;
+  az.nx = c->ng[0]; az.ny = c->ng[1]; az.nz = c->ng[2]; az.kx = c->kx; az.kxs = c->kxs; az.tx = cz.tx; az.lstride = cz.lstride;
   az.plan = c->plan[2]; az.tw = tw_ptr<T>(c, 2);
   ScaleParams sp{};
   for (int d = 0; d < 3; ++d) {
@@ -628,14 +646,17 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   do {                                                                                          \
     if (tm.rows == 8) {                                                                         \
       set_lds(k_xifft_contract_mfma<T, N, NC, 8>, tm.lds);                                      \
-      hipLaunchKernelGGL((k_xifft_contract_mfma<T, N, NC, 8>), mgrid, dim3(256), tm.lds, st, tma, ima, spec, out); \
+      hipLaunchKernelGGL((k_xifft_contract_mfma<T, N, NC, 8>), mgrid, dim3(256), tm.lds, st, tma, ima, spec); \
     } else {                                                                                    \
       set_lds(k_xifft_contract_mfma<T, N, NC, 4>, tm.lds);                                      \
-      hipLaunchKernelGGL((k_xifft_contract_mfma<T, N, NC, 4>), mgrid, dim3(256), tm.lds, st, tma, ima, spec, out); \
+      hipLaunchKernelGGL((k_xifft_contract_mfma<T, N, NC, 4>), mgrid, dim3(256), tm.lds, st, tma, ima, spec); \
     }                                                                                           \
   } while (0)
   if (use_mfma) {
     DISPATCH_CT(c->ng[0], LAUNCH_CONTRACT_MFMA, 0);
+    const TileImage& im = c->img[sizeof(T) == 8 ? 1 : 0];
+    hipLaunchKernelGGL((k_contract_reduce<T, NC>), dim3(cdiv(n, 256), nrhs), dim3(256), 0, st, n, ima.total,
+                       im.poff.p, im.pslot.p, c->perm.p, reinterpret_cast<const T*>(ima.part), ta.hvol, out);
   } else {
     DISPATCH_FFT(c->ng[0], ta.plan.e, LAUNCH_CONTRACT, 0);
   }
@@ -660,7 +681,7 @@ int run_operator_sizes(mpol_ctx* c, const double* m, double* out, double* hr, in
   hipStream_t st = c->stream;
   const size_t G = (size_t)c->ng[0] * c->ng[1] * c->ng[2];
   const size_t cs = sizeof(cplx<T>);
-  HIPALLOC(c->spec, (size_t)nrhs * c->ng[2] * c->ng[1] * c->kx * cs);
+  HIPALLOC(c->spec, (size_t)nrhs * c->ng[2] * c->ng[1] * c->kxs * cs);
   HIPALLOC(c->sz_grid, (size_t)nrhs * G * sizeof(T));
   HIPALLOC(c->zsum, (size_t)nrhs * (kZsumBlocks + 1));
   HIPALLOC(c->ms, (size_t)nrhs * 3 * n);
@@ -700,7 +721,7 @@ int run_operator_sizes(mpol_ctx* c, const double* m, double* out, double* hr, in
   const bool gen_x = generic_fft(c->plan[0]);
   TileCfg tr = tile_cfg_nc<1>(c->ng[0], gen_x, cs, 0, kLdsTile);
   TileArgs ta{};
-  ta.nx = c->ng[0]; ta.ny = c->ng[1]; ta.nz = c->ng[2]; ta.kx = c->kx;
+  ta.nx = c->ng[0]; ta.ny = c->ng[1]; ta.nz = c->ng[2]; ta.kx = c->kx; ta.kxs = c->kxs;
   ta.ty = tr.ty; ta.tz = tr.tz; ta.rows = tr.rows; ta.ncl = tr.ncl; ta.lstride = tr.lstride;
   ta.tw = tw_ptr<T>(c, 0); ta.plan = c->plan[0];
   const size_t rlds = ((size_t)c->ng[0] + (size_t)tr.ncl * tr.lstride * (gen_x ? 2 : 1)) * cs;
@@ -717,7 +738,7 @@ int run_operator_sizes(mpol_ctx* c, const double* m, double* out, double* hr, in
   const ColCfg cy = col_cfg(c->ng[1], generic_fft(c->plan[1]), c->kx, 1, cs, kLdsCol,
                             c->tune[3] > 0 ? c->tune[3] : (sizeof(T) == 8 ? 8 : 16));
   ColArgs ay{};
-  ay.nx = c->ng[0]; ay.ny = c->ng[1]; ay.nz = c->ng[2]; ay.kx = c->kx; ay.tx = cy.tx; ay.lstride = cy.lstride;
+  ay.nx = c->ng[0]; ay.ny = c->ng[1]; ay.nz = c->ng[2]; ay.kx = c->kx; ay.kxs = c->kxs; ay.tx = cy.tx; ay.lstride = cy.lstride;
   ay.plan = c->plan[1]; ay.tw = tw_ptr<T>(c, 1);
   const dim3 ygrid(cdiv(c->kx, cy.tx), c->ng[2], nrhs);
   if (cy.lds > kLdsMax) return fail(MPOL_EINVAL, "y-pass columns do not fit in LDS");
@@ -730,7 +751,7 @@ int run_operator_sizes(mpol_ctx* c, const double* m, double* out, double* hr, in
   const ColCfg cz = col_cfg(c->ng[2], generic_fft(c->plan[2]), c->kx, 1, cs, kLdsCol,
                             c->tune[4] > 0 ? c->tune[4] : (sizeof(T) == 8 ? 8 : 16));
   ColArgs az{};
-  az.nx = c->ng[0]; az.ny = c->ng[1]; az.nz = c->ng[2]; az.kx = c->kx; az.tx = cz.tx; az.lstride = cz.lstride;
+  az.nx = c->ng[0]; az.ny = c->ng[1]; az.nz = c->ng[2]; az.kx = c->kx; az.kxs = c->kxs; az.tx = cz.tx; az.lstride = cz.lstride;
   az.plan = c->plan[2]; az.tw = tw_ptr<T>(c, 2);
   ScaleParams sp{};
   for (int d = 0; d < 3; ++d) {

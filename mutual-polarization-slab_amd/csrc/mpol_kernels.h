@@ -2,7 +2,9 @@
 //
 // Grid spectrum layout in HBM (one buffer per context, reused every matvec):
 //   spec[b][z][y][kx],  b = rhs*3 + component,  kx in [0, nx/2] (half spectrum
-//   along x: the grids are real), complex<T>.
+//   along x: the grids are real), complex<T>; rows are kxs >= nx/2 + 1 long
+//   (rounded up to 16 elements so that column segments start on 128-byte
+//   boundaries).
 // Particles are kept in "column-bin" order: key = g0z*ny + g0y where g0 is the
 // first grid index of the particle's P-wide stencil on each axis (wrapped).
 #pragma once
@@ -446,6 +448,7 @@ __global__ void k_real_space_quad(int n, int nrhs, int nsets, const double* __re
 // ---------------------------------------------------------------------------
 struct TileArgs {
   int nx, ny, nz, kx;
+  int kxs;           // spectrum row stride (>= kx)
   int ty, tz;        // grid rows (y, z) per tile
   int rows;          // ty*tz, a power of two <= 16
   int ncl;           // complex LDS lines = ceil(3*rows/2)
@@ -648,7 +651,7 @@ __device__ inline void store_half_spectrum(const TileArgs& a, const cplx<T>* buf
     cplx<TS> v;
     if ((l & 1) == 0) v = cmk<TS>((TS)((Z.x + Zc.x) * (T)0.5), (TS)((Z.y + Zc.y) * (T)0.5));
     else v = cmk<TS>((TS)((Z.y - Zc.y) * (T)0.5), (TS)((Zc.x - Z.x) * (T)0.5));
-    spec[(((size_t)(rhs * NC + c) * a.nz + z) * a.ny + y) * a.kx + k] = v;
+    spec[(((size_t)(rhs * NC + c) * a.nz + z) * a.ny + y) * a.kxs + k] = v;
   }
 }
 
@@ -863,12 +866,12 @@ __global__ void __launch_bounds__(256) k_xifft_contract(TileArgs a, const cplx<T
     {
       const int c = la / a.rows, row = la - c * a.rows;
       const int y = ty0 + row % a.ty, z = tz0 + row / a.ty;
-      if (y < a.ny && z < a.nz) A = spec[(((size_t)(rhs * NC + c) * a.nz + z) * a.ny + y) * a.kx + k];
+      if (y < a.ny && z < a.nz) A = spec[(((size_t)(rhs * NC + c) * a.nz + z) * a.ny + y) * a.kxs + k];
     }
     if (lb < nreal) {
       const int c = lb / a.rows, row = lb - c * a.rows;
       const int y = ty0 + row % a.ty, z = tz0 + row / a.ty;
-      if (y < a.ny && z < a.nz) B = spec[(((size_t)(rhs * NC + c) * a.nz + z) * a.ny + y) * a.kx + k];
+      if (y < a.ny && z < a.nz) B = spec[(((size_t)(rhs * NC + c) * a.nz + z) * a.ny + y) * a.kxs + k];
     }
     cplx<T>* line = buf + q * a.lstride;
     const bool selfmirror = (k == 0) || (2 * k == a.nx);
@@ -951,6 +954,7 @@ __global__ void __launch_bounds__(256) k_xifft_contract(TileArgs a, const cplx<T
 // ---------------------------------------------------------------------------
 struct ColArgs {
   int nx, ny, nz, kx;
+  int kxs;          // spectrum row stride (>= kx)
   int tx;           // kx columns per workgroup
   int lstride;
   FftPlan plan;
@@ -966,19 +970,19 @@ __global__ void __launch_bounds__(256) k_fft_y(ColArgs a, cplx<T>* __restrict__ 
   const int kx0 = blockIdx.x * a.tx, z = blockIdx.y, b = blockIdx.z;
   const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
   for (int k = threadIdx.x; k < a.ny; k += blockDim.x) tw[k] = twg[k];
-  cplx<T>* base = spec + ((size_t)b * a.nz + z) * a.ny * a.kx;
+  cplx<T>* base = spec + ((size_t)b * a.nz + z) * a.ny * a.kxs;
   // thread -> (column c, first row); rows step by the threads per column set
   const int c = threadIdx.x % a.tx, ystep = blockDim.x / a.tx;
   const int kx = kx0 + c;
   const bool act = (int)threadIdx.x < ystep * a.tx;
   if (act) {
     for (int y = threadIdx.x / a.tx; y < a.ny; y += ystep)
-      buf[c * a.lstride + pidx(y)] = (kx < a.kx) ? base[(size_t)y * a.kx + kx] : cmk<T>(0, 0);
+      buf[c * a.lstride + pidx(y)] = (kx < a.kx) ? base[(size_t)y * a.kxs + kx] : cmk<T>(0, 0);
   }
   __syncthreads();
   fft_any<T, E, N>(buf, a.lstride, a.tx, a.plan, tw, inverse != 0, scratch);
   if (act && kx < a.kx) {
-    for (int y = threadIdx.x / a.tx; y < a.ny; y += ystep) base[(size_t)y * a.kx + kx] = buf[c * a.lstride + pidx(y)];
+    for (int y = threadIdx.x / a.tx; y < a.ny; y += ystep) base[(size_t)y * a.kxs + kx] = buf[c * a.lstride + pidx(y)];
   }
 }
 
@@ -1057,7 +1061,7 @@ __global__ void __launch_bounds__(1024) k_fft_z_scale(ColArgs a, ScaleParams sp,
   const int kx0 = blockIdx.x * a.tx, y = blockIdx.y, rhs = blockIdx.z;
   const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
   for (int k = threadIdx.x; k < a.nz; k += blockDim.x) tw[k] = twg[k];
-  const size_t plane = (size_t)a.ny * a.kx;
+  const size_t plane = (size_t)a.ny * a.kxs;
   // thread -> (column cc, first z); z steps by the threads per column set
   const int cc_t = threadIdx.x % a.tx, zstep = blockDim.x / a.tx, z_t = threadIdx.x / a.tx;
   const int kx_t = kx0 + cc_t;
@@ -1065,7 +1069,7 @@ __global__ void __launch_bounds__(1024) k_fft_z_scale(ColArgs a, ScaleParams sp,
   if (act) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      const cplx<T>* src = spec + (size_t)(rhs * NC + c) * a.nz * plane + (size_t)y * a.kx + kx_t;
+      const cplx<T>* src = spec + (size_t)(rhs * NC + c) * a.nz * plane + (size_t)y * a.kxs + kx_t;
       cplx<T>* dst = buf + (c * a.tx + cc_t) * a.lstride;
       for (int z = z_t; z < a.nz; z += zstep) dst[pidx(z)] = (kx_t < a.kx) ? src[(size_t)z * plane] : cmk<T>(0, 0);
     }
@@ -1151,7 +1155,7 @@ __global__ void __launch_bounds__(1024) k_fft_z_scale(ColArgs a, ScaleParams sp,
   if (act && kx_t < a.kx) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      cplx<T>* dst = spec + (size_t)(rhs * NC + c) * a.nz * plane + (size_t)y * a.kx + kx_t;
+      cplx<T>* dst = spec + (size_t)(rhs * NC + c) * a.nz * plane + (size_t)y * a.kxs + kx_t;
       const cplx<T>* src = buf + (c * a.tx + cc_t) * a.lstride;
       for (int z = z_t; z < a.nz; z += zstep) dst[(size_t)z * plane] = src[pidx(z)];
     }

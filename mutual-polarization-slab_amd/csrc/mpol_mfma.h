@@ -87,7 +87,63 @@ struct ImgArgs {
   const int2* gs;    // [total] {first stencil node along x, sorted particle index}
   const void* fw;    // [total][P + rows] T: norm * x factors, then the row weights
   const int* perm;   // sorted particle index -> caller's index
+  long long total;   // image entries (slots)
+  void* part;        // contraction partials [nrhs][total][NC] T
 };
+
+// Per-particle slot lists of an image (CSR by sorted particle index), so the
+// contraction can sum each particle's tile partials without atomics.
+static __global__ void k_img_pcount(const int2* __restrict__ gs, int64_t total, int* __restrict__ cnt) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < total) atomicAdd(&cnt[gs[q].y], 1);
+}
+static __global__ void k_img_pscatter(const int2* __restrict__ gs, int64_t total, const int* __restrict__ poff,
+                                      int* __restrict__ cursor, int* __restrict__ pslot) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= total) return;
+  const int s = gs[q].y;
+  pslot[poff[s] + atomicAdd(&cursor[s], 1)] = (int)q;
+}
+// Sort each particle's slots ascending (insertion sort, a few tens of entries)
+// so that the contraction's summation order is fixed.
+static __global__ void k_img_psort(int n, const int* __restrict__ poff, int* __restrict__ pslot) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int* a = pslot + poff[s];
+  const int m = poff[s + 1] - poff[s];
+  for (int i = 1; i < m; ++i) {
+    const int v = a[i];
+    int j = i - 1;
+    while (j >= 0 && a[j] > v) {
+      a[j + 1] = a[j];
+      --j;
+    }
+    a[j + 1] = v;
+  }
+}
+
+// out[rhs][perm[s]][c] += hvol * sum over the slots of particle s of the
+// contraction partials (FP64 sums, fixed order).
+template <typename T, int NC>
+__global__ void k_contract_reduce(int n, long long total, const int* __restrict__ poff, const int* __restrict__ pslot,
+                                  const int* __restrict__ perm, const T* __restrict__ part, double hvol,
+                                  double* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int rhs = blockIdx.y;
+  if (s >= n) return;
+  double acc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) acc[c] = 0.0;
+  const T* pr = part + (size_t)rhs * total * NC;
+  for (int k = poff[s]; k < poff[s + 1]; ++k) {
+    const T* p = pr + (size_t)pslot[k] * NC;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c] += (double)p[c];
+  }
+  double* o = out + ((size_t)rhs * n + perm[s]) * NC;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) o[c] += acc[c] * hvol;
+}
 
 // Candidates of tile (ty0, tz0): the total of the column-bin ranges that
 // tile_ranges enumerates (one thread per tile).
@@ -192,7 +248,7 @@ struct MfmaStage {
   T* fw;     // [ch][P + rows]
   T* m;      // [ch][NC] (spread)
   int* gx;   // [ch]
-  int* idx;  // [ch] sorted particle index (spread) or caller's index (contract)
+  int* idx;  // [ch] (spare)
 };
 
 template <typename T, int NC>
@@ -212,23 +268,34 @@ inline size_t mfma_tile_lds(int nx, size_t csize, int P, int ncl, int lstride, i
 }
 
 // Copy chunk [q, q + nc) of the tile image into LDS; the spread also gathers
-// the moments of right-hand side rhs (WITH_M), the contraction the caller's
-// particle indices.
+// the moments of right-hand side rhs (WITH_M).
 template <typename T, bool WITH_M, int NC>
 __device__ inline void mfma_stage_chunk(const TileArgs& a, const ImgArgs& im, const MfmaStage<T>& st, size_t q,
                                         int nc, int rhs) {
   const int PW = a.P + a.rows;
   const T* src = reinterpret_cast<const T*>(im.fw) + q * PW;
-  for (int k = threadIdx.x; k < nc * PW; k += blockDim.x) st.fw[k] = src[k];
-  for (int u = threadIdx.x; u < nc; u += blockDim.x) {
+  const int tot = nc * PW, nt = blockDim.x, t = threadIdx.x;
+  // batches of 8 loads in flight per thread before the LDS writes
+  for (int k0 = 0; k0 < tot; k0 += 8 * nt) {
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int k = k0 + u * nt + t;
+      v[u] = k < tot ? src[k] : (T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int k = k0 + u * nt + t;
+      if (k < tot) st.fw[k] = v[u];
+    }
+  }
+  for (int u = t; u < nc; u += nt) {
     const int2 g = im.gs[q + u];
     st.gx[u] = g.x;
     if (WITH_M) {
       const double* mp = a.ms + NC * ((size_t)rhs * a.n + g.y);
 #pragma unroll
       for (int c = 0; c < NC; ++c) st.m[NC * u + c] = (T)mp[c];
-    } else {
-      st.idx[u] = im.perm[g.y];
     }
   }
 }
@@ -326,7 +393,7 @@ __global__ void __launch_bounds__(256) k_spread_mfma_xfft(TileArgs a, ImgArgs im
     const long long row = tile_line_row(a, l, ty0, tz0, rhs, NC);
     if (row < 0) continue;
     const cplx<T>* line = buf + (l >> 1) * a.lstride;
-    cplx<T>* dst = spec + row * KX;
+    cplx<T>* dst = spec + row * a.kxs;
     for (int k = lane; k < KX; k += 64) {
       const cplx<T> Z = line[pidx(k)];
       const cplx<T> Zc = cconj(line[pidx(k == 0 ? 0 : N - k)]);
@@ -341,10 +408,10 @@ __global__ void __launch_bounds__(256) k_spread_mfma_xfft(TileArgs a, ImgArgs im
 // Inverse x-FFT + contraction.  Wave w handles blocks of 16 consecutive
 // sorted candidates w, w + 4, ...: A[q][x] over the x span of the block (in
 // unwrapped coordinates, the grid read at x mod nx), then the row weights and
-// a reduction over the R lanes of each component; FP64 atomics into out.
+// a reduction over the R lanes of each component, stored as the partial of
+// image slot q (k_contract_reduce sums each particle's partials).
 template <typename T, int N, int NC, int R>
-__global__ void __launch_bounds__(256) k_xifft_contract_mfma(TileArgs a, ImgArgs im, const cplx<T>* __restrict__ spec,
-                                                             double* __restrict__ out) {
+__global__ void __launch_bounds__(256) k_xifft_contract_mfma(TileArgs a, ImgArgs im, const cplx<T>* __restrict__ spec) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   using MF = Mfma<T>;
   using acc_t = typename MF::acc_t;
@@ -359,27 +426,49 @@ __global__ void __launch_bounds__(256) k_xifft_contract_mfma(TileArgs a, ImgArgs
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
   for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];
+  const size_t qt = (size_t)im.off[tile];
+  const int total = im.off[tile + 1] - (int)qt;
+  // the first chunk of the tile image goes to LDS while the spectrum loads
+  mfma_stage_chunk<T, false, NC>(a, im, st, qt, min(a.ch, total), rhs);
   // pack two half spectra per complex line: Z = A + iB (Hermitian
-  // extension); one wave per complex line, lanes along kx
-  for (int q = wave; q < a.ncl; q += nwave) {
-    const long long ra = tile_line_row(a, 2 * q, ty0, tz0, rhs, NC);
-    const long long rb = 2 * q + 1 < NCOL ? tile_line_row(a, 2 * q + 1, ty0, tz0, rhs, NC) : -1;
-    cplx<T>* line = buf + q * a.lstride;
-    for (int k = lane; k < KX; k += 64) {
-      const cplx<T> A = ra >= 0 ? spec[ra * KX + k] : cmk<T>(0, 0);
-      const cplx<T> B = rb >= 0 ? spec[rb * KX + k] : cmk<T>(0, 0);
-      if (k == 0 || 2 * k == N) {
-        line[pidx(k)] = cmk<T>(A.x, B.x);
-      } else {
-        line[pidx(k)] = cmk<T>(A.x - B.y, A.y + B.x);
-        line[pidx(N - k)] = cmk<T>(A.x + B.y, B.x - A.y);
+  // extension).  Flat (line, kx) index, all of a thread's loads issued before
+  // its LDS writes.
+  {
+    constexpr int NCL = (NCOL + 1) / 2, TOT = NCL * KX, IT = (TOT + 255) / 256, G = 8;
+    for (int u0 = 0; u0 < IT; u0 += G) {
+    cplx<T> va[G], vb[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      const int idx = (u0 + u) * 256 + (int)threadIdx.x;
+      const int q = idx / KX, k = idx - q * KX;
+      va[u] = cmk<T>(0, 0);
+      vb[u] = cmk<T>(0, 0);
+      if (idx < TOT) {
+        const long long ra = tile_line_row(a, 2 * q, ty0, tz0, rhs, NC);
+        const long long rb = 2 * q + 1 < NCOL ? tile_line_row(a, 2 * q + 1, ty0, tz0, rhs, NC) : -1;
+        if (ra >= 0) va[u] = spec[ra * a.kxs + k];
+        if (rb >= 0) vb[u] = spec[rb * a.kxs + k];
       }
+    }
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      const int idx = (u0 + u) * 256 + (int)threadIdx.x;
+      const int q = idx / KX, k = idx - q * KX;
+      if (idx < TOT) {
+        cplx<T>* line = buf + q * a.lstride;
+        const cplx<T> A = va[u], B = vb[u];
+        if (k == 0 || 2 * k == N) {
+          line[pidx(k)] = cmk<T>(A.x, B.x);
+        } else {
+          line[pidx(k)] = cmk<T>(A.x - B.y, A.y + B.x);
+          line[pidx(N - k)] = cmk<T>(A.x + B.y, B.x - A.y);
+        }
+      }
+    }
     }
   }
   __syncthreads();
   fft_any<T, 0, N>(buf, a.lstride, a.ncl, a.plan, tw, true, nullptr);
-  const size_t qt = (size_t)im.off[tile];
-  const int total = im.off[tile + 1] - (int)qt;
   const int i16 = lane & 15, k4 = lane >> 4;
   const int colrow = i16 % R;
   int colc[NB];
@@ -391,11 +480,13 @@ __global__ void __launch_bounds__(256) k_xifft_contract_mfma(TileArgs a, ImgArgs
     const int jl = min(j, NCOL - 1);
     colp[nb] = reinterpret_cast<const T*>(buf + (jl >> 1) * a.lstride) + (jl & 1);
   }
-  double* outr = out + (size_t)rhs * NC * (size_t)a.n;
+  T* partr = reinterpret_cast<T*>(im.part) + (size_t)rhs * im.total * NC;
   for (int base = 0; base < total; base += a.ch) {
     const int nc = min(a.ch, total - base);
-    mfma_stage_chunk<T, false, NC>(a, im, st, qt + base, nc, rhs);
-    __syncthreads();
+    if (base > 0) {
+      mfma_stage_chunk<T, false, NC>(a, im, st, qt + base, nc, rhs);
+      __syncthreads();
+    }
     const int nblk = (nc + 15) >> 4;
     for (int cb = wave; cb < nblk; cb += nwave) {
       const int c0 = cb * 16;
@@ -429,11 +520,10 @@ __global__ void __launch_bounds__(256) k_xifft_contract_mfma(TileArgs a, ImgArgs
         const bool okr = cr < nc;
         const int crq = okr ? cr : c0;
         const T wq = st.fw[crq * PW + P + colrow];
-        const int orig = st.idx[crq];
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
           const T v = group_sum<T>(acc[nb][r] * wq, R);
-          if (colc[nb] < NC && colrow == 0 && okr) atomicAdd(outr + NC * (size_t)orig + colc[nb], (double)v * a.hvol);
+          if (colc[nb] < NC && colrow == 0 && okr) partr[(qt + base + cr) * NC + colc[nb]] = v;
         }
       }
     }

@@ -365,11 +365,11 @@ __global__ void __launch_bounds__(256) k_rows_c2r(TileArgs a, const cplx<T>* __r
     const int la = 2 * q, lb = 2 * q + 1;
     {
       const int y = ty0 + la % a.ty, z = tz0 + la / a.ty;
-      if (y < a.ny && z < a.nz) A = spec[(((size_t)rhs * a.nz + z) * a.ny + y) * a.kx + k];
+      if (y < a.ny && z < a.nz) A = spec[(((size_t)rhs * a.nz + z) * a.ny + y) * a.kxs + k];
     }
     if (lb < a.rows) {
       const int y = ty0 + lb % a.ty, z = tz0 + lb / a.ty;
-      if (y < a.ny && z < a.nz) B = spec[(((size_t)rhs * a.nz + z) * a.ny + y) * a.kx + k];
+      if (y < a.ny && z < a.nz) B = spec[(((size_t)rhs * a.nz + z) * a.ny + y) * a.kxs + k];
     }
     cplx<T>* line = buf + q * a.lstride;
     const bool selfmirror = (k == 0) || (2 * k == a.nx);
