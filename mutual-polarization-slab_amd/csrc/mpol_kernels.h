@@ -976,8 +976,20 @@ __global__ void __launch_bounds__(256) k_fft_y(ColArgs a, cplx<T>* __restrict__ 
   const int kx = kx0 + c;
   const bool act = (int)threadIdx.x < ystep * a.tx;
   if (act) {
-    for (int y = threadIdx.x / a.tx; y < a.ny; y += ystep)
-      buf[c * a.lstride + pidx(y)] = (kx < a.kx) ? base[(size_t)y * a.kxs + kx] : cmk<T>(0, 0);
+    // batches of 8 loads in flight before the LDS writes
+    for (int y0 = threadIdx.x / a.tx; y0 < a.ny; y0 += 8 * ystep) {
+      cplx<T> v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int y = y0 + u * ystep;
+        v[u] = (kx < a.kx && y < a.ny) ? base[(size_t)y * a.kxs + kx] : cmk<T>(0, 0);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int y = y0 + u * ystep;
+        if (y < a.ny) buf[c * a.lstride + pidx(y)] = v[u];
+      }
+    }
   }
   __syncthreads();
   fft_any<T, E, N>(buf, a.lstride, a.tx, a.plan, tw, inverse != 0, scratch);
@@ -1071,7 +1083,20 @@ __global__ void __launch_bounds__(1024) k_fft_z_scale(ColArgs a, ScaleParams sp,
     for (int c = 0; c < NC; ++c) {
       const cplx<T>* src = spec + (size_t)(rhs * NC + c) * a.nz * plane + (size_t)y * a.kxs + kx_t;
       cplx<T>* dst = buf + (c * a.tx + cc_t) * a.lstride;
-      for (int z = z_t; z < a.nz; z += zstep) dst[pidx(z)] = (kx_t < a.kx) ? src[(size_t)z * plane] : cmk<T>(0, 0);
+      // batches of 8 loads in flight before the LDS writes
+      for (int z0 = z_t; z0 < a.nz; z0 += 8 * zstep) {
+        cplx<T> v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int z = z0 + u * zstep;
+          v[u] = (kx_t < a.kx && z < a.nz) ? src[(size_t)z * plane] : cmk<T>(0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int z = z0 + u * zstep;
+          if (z < a.nz) dst[pidx(z)] = v[u];
+        }
+      }
     }
   }
   __syncthreads();
