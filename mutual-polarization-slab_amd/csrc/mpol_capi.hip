@@ -43,7 +43,7 @@ int check_ready(mpol_ctx* c) {
 int dots(mpol_ctx* c, const double* A, size_t stride_i, int cnt, const double* w, int64_t n, int nrhs, double* out) {
   hipLaunchKernelGGL(k_dots, dim3(kDotBlocks, nrhs), dim3(256), 0, c->stream, A, stride_i, cnt, w, n, nrhs,
                      c->gpart.p);
-  hipLaunchKernelGGL(k_dots_finish, dim3(cdiv((int64_t)nrhs * cnt, 256)), dim3(256), 0, c->stream, c->gpart.p,
+  hipLaunchKernelGGL(k_dots_finish, dim3(cdiv((int64_t)nrhs * cnt, 4)), dim3(256), 0, c->stream, c->gpart.p,
                      kDotBlocks, nrhs * cnt, out);
   return MPOL_OK;
 }
@@ -564,9 +564,15 @@ int solve_impl(mpol_ctx* c, int kind, const double* b, const double* x0, double*
     HIPCHK(hipStreamSynchronize(st));
     int64_t nmv64 = 0, nmv32 = 0;
     int ref = 0;
+    // The first residual (from the caller's guess) only seeds the first
+    // correction solve, so it is taken with the FP32 operator; every later
+    // residual, and any that would end the loop, is FP64.
+    bool fp32_residual = o->max_refine > 0;
     for (;; ++ref) {
-      CHECK_STATUS(apply(c, kind, x, c->gAx.p, nrhs, MPOL_FP64, slab));
-      ++nmv64;
+      const int rprec = fp32_residual ? MPOL_FP32 : MPOL_FP64;
+      CHECK_STATUS(apply(c, kind, x, c->gAx.p, nrhs, rprec, slab));
+      if (fp32_residual) ++nmv32;
+      else ++nmv64;
       hipLaunchKernelGGL(k_residual, dim3(cdiv(tot, 256)), dim3(256), 0, st, b, c->gAx.p, c->gd.p, tot);
       dots(c, c->gd.p, 0, 1, c->gd.p, n3, nrhs, c->grn2.p);
       HIPCHK(hipMemcpyAsync(rn2.data(), c->grn2.p, nrhs * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -578,7 +584,15 @@ int solve_impl(mpol_ctx* c, int kind, const double* b, const double* x0, double*
         if (rel[r] <= o->tol) ++conv;
         else if (rn2[r] > 0.0) ratio = std::min(ratio, std::sqrt(bn2[r] / rn2[r]));
       }
-      if (conv == nrhs || ref >= o->max_refine) break;
+      if (fp32_residual) {
+        fp32_residual = false;
+        if (conv == nrhs) {  // certify with an FP64 residual
+          --ref;
+          continue;
+        }
+      } else if (conv == nrhs || ref >= o->max_refine) {
+        break;
+      }
       // inner tolerance relative to the current residual
       double tin = 0.3 * o->tol * ratio;
       tin = std::min(0.1, std::max(tin, 2e-6));

@@ -11,41 +11,55 @@ namespace mpol {
 
 enum RhsState : int { kDone = 0, kIter = 1, kSolve = 2, kWait = 3 };
 
-constexpr int kDotBlocks = 120;  // partial-sum blocks per right-hand side
+constexpr int kDotBlocks = 256;  // partial-sum blocks per right-hand side (one per CU)
 
 // partial[(rhs*cnt + i)*kDotBlocks + blk] = sum over the block's chunk of a_i . w
 // with a_i = A + i*stride_i (rhs offset rhs*n).  Deterministic.
 static __global__ void __launch_bounds__(256) k_dots(const double* __restrict__ A, size_t stride_i, int cnt,
                                               const double* __restrict__ w, int64_t n, int nrhs,
                                               double* __restrict__ partial) {
-  __shared__ double red[256];
+  __shared__ double red[4][4];  // [vector of the group][wave]
   const int rhs = blockIdx.y;
   const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
   const int64_t lo = (int64_t)blockIdx.x * chunk;
   const int64_t hi = lo + chunk < n ? lo + chunk : n;
   const double* wr = w + (size_t)rhs * n;
-  for (int i = 0; i < cnt; ++i) {
-    const double* ar = A + i * stride_i + (size_t)rhs * n;
-    double s = 0.0;
-    for (int64_t k = lo + threadIdx.x; k < hi; k += blockDim.x) s += ar[k] * wr[k];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-      if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-      __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // four dot products per sweep over the chunk (w read once per group)
+  for (int i0 = 0; i0 < cnt; i0 += 4) {
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t k = lo + threadIdx.x; k < hi; k += blockDim.x) {
+      const double wv = wr[k];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (i0 + u < cnt) s[u] += A[(size_t)(i0 + u) * stride_i + (size_t)rhs * n + k] * wv;
     }
-    if (threadIdx.x == 0) partial[((size_t)rhs * cnt + i) * gridDim.x + blockIdx.x] = red[0];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      for (int o = 32; o > 0; o >>= 1) s[u] += __shfl_xor(s[u], o);
+    if (lane == 0) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) red[u][wave] = s[u];
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < 4 && i0 + (int)threadIdx.x < cnt) {
+      const int u = threadIdx.x;
+      partial[((size_t)rhs * cnt + i0 + u) * gridDim.x + blockIdx.x] = (red[u][0] + red[u][1]) + (red[u][2] + red[u][3]);
+    }
     __syncthreads();
   }
 }
 
-// out[rhs*cnt + i] = sum of the partials (fixed order)
+// out[rhs*cnt + i] = sum of the partials (fixed order): one wave per output,
+// lanes load the partials in parallel, then a fixed butterfly.
 static __global__ void k_dots_finish(const double* __restrict__ partial, int nblk, int total, double* __restrict__ out) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (t >= total) return;
   double s = 0.0;
-  for (int b = 0; b < nblk; ++b) s += partial[(size_t)t * nblk + b];
-  out[t] = s;
+  for (int b = lane; b < nblk; b += 64) s += partial[(size_t)t * nblk + b];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (lane == 0) out[t] = s;
 }
 
 // w[rhs] -= sum_i h[rhs*cnt + i] V_i[rhs]   (only rhs in state kIter)
