@@ -133,6 +133,10 @@ def main():
     torch.cuda.synchronize()
     geo_s = time.perf_counter() - t0
     ginfo = ctx.geometry_info()
+    log("autotuning launch shapes (untimed)")
+    from mpol_hip.autotune import tune_launch
+    knobs = {"fp32": tune_launch(ctx, FP32), "fp64": tune_launch(ctx, FP64)}
+    log(f"launch knobs {knobs}")
     slab = 0.0
     phi = 4.0 * np.pi * n / (3.0 * np.prod(cfg["box"]))
     beta = cfg["beta"]
@@ -285,6 +289,7 @@ def main():
         "stage_ms_fp64": {k: round(v, 4) for k, v in stages64.items()},
         "stage_gbs_fp32": stage_gbs,
         "geometry_s": round(geo_s, 3),
+        "launch_knobs": knobs,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -248,6 +248,11 @@ int mpol_matvec_vjp(mpol_ctx* ctx, const double* p_dev, const double* lam_dev, i
  * than 7. */
 int mpol_set_tuning(mpol_ctx* ctx, const int32_t* knobs, int32_t nknobs);
 
+/* The same knobs for the operator in one precision only (MPOL_FP32 or
+ * MPOL_FP64; -1 = both, as mpol_set_tuning).  No counterpart in the
+ * reference; the autotuner (mpol_hip/autotune.py) sets them per precision. */
+int mpol_set_tuning_for(mpol_ctx* ctx, int32_t precision, const int32_t* knobs, int32_t nknobs);
+
 /* Geometry summary (host). */
 int mpol_geometry_info(mpol_ctx* ctx, mpol_geometry_summary* info);
 

@@ -863,8 +863,10 @@ int mpol_quad_solve(mpol_ctx* c, const double* b, const double* x0, double* x, i
   return solve_impl(c, kOpQuad, b, x0, x, nrhs, o, info);
 }
 
-int mpol_set_tuning(mpol_ctx* c, const int32_t* knobs, int32_t nknobs) {
+int mpol_set_tuning_for(mpol_ctx* c, int32_t precision, const int32_t* knobs, int32_t nknobs) {
   if (!c || (nknobs > 0 && !knobs) || nknobs < 0 || nknobs > kTuneKnobs) return fail(MPOL_EINVAL, "bad tuning arguments");
+  if (precision != MPOL_FP32 && precision != MPOL_FP64 && precision != -1)
+    return fail(MPOL_EINVAL, "precision must be MPOL_FP32, MPOL_FP64 or -1 (both)");
   int t[kTuneKnobs] = {0, 0, 0, 0, 0, 0, 0};
   for (int i = 0; i < nknobs; ++i) t[i] = knobs[i];
   const bool pow2_rows = (t[0] == 0 && t[1] == 0) ||
@@ -875,8 +877,15 @@ int mpol_set_tuning(mpol_ctx* c, const int32_t* knobs, int32_t nknobs) {
   if (t[5] != 0 && (t[5] < 64 || t[5] > 1024 || t[5] % 64 != 0))
     return fail(MPOL_EINVAL, "z-pass threads must be a multiple of 64 in [64, 1024]");
   if (t[6] < 0 || t[6] > 2) return fail(MPOL_EINVAL, "engine must be 0 (automatic), 1 (matrix cores) or 2 (vector loops)");
-  for (int i = 0; i < kTuneKnobs; ++i) c->tune[i] = t[i];
+  for (int p = 0; p < 2; ++p) {
+    if (precision != -1 && p != (precision == MPOL_FP32 ? 1 : 0)) continue;
+    for (int i = 0; i < kTuneKnobs; ++i) c->tune[p][i] = t[i];
+  }
   return MPOL_OK;
+}
+
+int mpol_set_tuning(mpol_ctx* c, const int32_t* knobs, int32_t nknobs) {
+  return mpol_set_tuning_for(c, -1, knobs, nknobs);
 }
 
 int mpol_geometry_info(mpol_ctx* c, mpol_geometry_summary* info) {

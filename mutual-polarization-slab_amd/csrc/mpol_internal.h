@@ -232,7 +232,7 @@ struct mpol_ctx {
   // launch-shape overrides (0 = automatic): tile ty, tile tz, particle chunk,
   // y-pass columns, z-pass columns, z-pass threads, spread/contract engine
   // (1 matrix cores, 2 vector loops)
-  int tune[kTuneKnobs] = {0, 0, 0, 0, 0, 0, 0};
+  int tune[2][kTuneKnobs] = {};  // [0] FP64 operator, [1] FP32 operator
   // per-tile candidate images of the matrix-core spread/contract ([0] FP32, [1] FP64)
   TileImage img[2];
   // gmres
@@ -344,15 +344,21 @@ size_t tile_lds(int nx, bool generic, size_t csize, int P, const TileCfg& t, boo
          (size_t)t.ch * ((NC + (size_t)P + t.rows) * (csize / 2) + 2 * sizeof(int));
 }
 
+// Launch-shape knobs of the operator in precision T.
+template <typename T>
+const int* knobs(const mpol_ctx* c) {
+  return c->tune[sizeof(T) == 4 ? 1 : 0];
+}
+
 template <int NC>
-void apply_tile_tuning(const mpol_ctx* c, TileCfg& t) {
-  if (c->tune[0] > 0 && c->tune[1] > 0) {
-    t.ty = c->tune[0];
-    t.tz = c->tune[1];
+void apply_tile_tuning(const int* tn, TileCfg& t) {
+  if (tn[0] > 0 && tn[1] > 0) {
+    t.ty = tn[0];
+    t.tz = tn[1];
     t.rows = t.ty * t.tz;
     t.ncl = (NC * t.rows + 1) / 2;
   }
-  if (c->tune[2] > 0) t.ch = c->tune[2];
+  if (tn[2] > 0) t.ch = tn[2];
 }
 
 // Tile shape and chunk of the matrix-core spread/contract: the largest tile
@@ -364,7 +370,8 @@ template <typename T, int NC>
 TileCfg mfma_tile_cfg(const mpol_ctx* c, bool& ok) {
   const int nx = c->ng[0];
   TileCfg t;
-  ok = c->tune[6] != 2 && is_ct_size(nx) && nx % 16 == 0 && c->P + 16 <= nx;
+  const int* tn = knobs<T>(c);
+  ok = tn[6] != 2 && is_ct_size(nx) && nx % 16 == 0 && c->P + 16 <= nx;
   if (!ok) return t;
   const size_t cs = sizeof(cplx<T>);
   // 4 or 8 rows (the kernels are instantiated for those)
@@ -389,7 +396,7 @@ TileCfg mfma_tile_cfg(const mpol_ctx* c, bool& ok) {
       if (found) break;
     }
   }
-  apply_tile_tuning<NC>(c, t);
+  apply_tile_tuning<NC>(tn, t);
   t.ncl = (NC * t.rows + 1) / 2;
   t.lds = mfma_tile_lds<NC>(nx, cs, c->P, t.ncl, t.lstride, t.ch, t.rows);
   if (t.lds > kLdsMax || (t.rows != 4 && t.rows != 8)) ok = false;
@@ -487,7 +494,7 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   // tile (x axis) kernels
   const bool gen_x = generic_fft(c->plan[0]);
   TileCfg tc = tile_cfg_nc<NC>(c->ng[0], gen_x, cs, c->P, kLdsTile);
-  apply_tile_tuning<NC>(c, tc);
+  apply_tile_tuning<NC>(knobs<T>(c), tc);
   tc.lds = tile_lds<NC>(c->ng[0], gen_x, cs, c->P, tc, false);
   TileArgs ta{};
   ta.nx = c->ng[0]; ta.ny = c->ng[1]; ta.nz = c->ng[2]; ta.kx = c->kx; ta.kxs = c->kxs;
@@ -507,7 +514,7 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   // matrix-core spread/contract over the tile image when the grid suits it
   bool use_mfma = false;
   const TileCfg tm = mfma_tile_cfg<T, NC>(c, use_mfma);
-  if (c->tune[6] == 1 && !use_mfma) return fail(MPOL_EINVAL, "matrix-core spread/contract does not apply to this grid");
+  if (knobs<T>(c)[6] == 1 && !use_mfma) return fail(MPOL_EINVAL, "matrix-core spread/contract does not apply to this grid");
   TileArgs tma = ta;
   ImgArgs ima{};
   dim3 mgrid(1, 1, 1);
@@ -542,7 +549,7 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
 #undef LAUNCH_SPREAD_MFMA
   } else if (gather) {
     TileCfg tg = tile_cfg_nc<NC>(c->ng[0], false, cs, c->P, kLdsTile, true);
-    apply_tile_tuning<NC>(c, tg);
+    apply_tile_tuning<NC>(knobs<T>(c), tg);
     tg.lds = tile_lds<NC>(c->ng[0], false, cs, c->P, tg, true);
     if (tg.lds > kLdsMax) return fail(MPOL_EINVAL, "spread tile does not fit in LDS (reduce tile or chunk)");
     TileArgs tga = ta;
@@ -559,7 +566,7 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
     // the spread tile accumulates in FP64 in both precisions
     using TA = double;
     TileCfg tsc = tile_cfg_nc<NC>(c->ng[0], gen_x, sizeof(cplx<TA>), c->P, kLdsTile);
-    apply_tile_tuning<NC>(c, tsc);
+    apply_tile_tuning<NC>(knobs<T>(c), tsc);
     const size_t slds = tile_lds<NC>(c->ng[0], gen_x, sizeof(cplx<TA>), c->P, tsc, false);
     if (slds > kLdsMax) return fail(MPOL_EINVAL, "spread tile does not fit in LDS (reduce tile or chunk)");
     TileArgs tsa = ta;
@@ -581,7 +588,7 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
 
   // y axis forward
   const ColCfg cy = col_cfg(c->ng[1], generic_fft(c->plan[1]), c->kx, 1, cs, kLdsCol,
-                            c->tune[3] > 0 ? c->tune[3] : (sizeof(T) == 8 ? 8 : 16));  // 128-byte rows
+                            knobs<T>(c)[3] > 0 ? knobs<T>(c)[3] : (sizeof(T) == 8 ? 8 : 16));  // 128-byte rows
   ColArgs ay{};
   ay.nx = c->ng[0]; ay.ny = c->ng[1]; ay.nz = c->ng[2]; ay.kx = c->kx; ay.kxs = c->kxs; ay.tx = cy.tx; ay.lstride = cy.lstride;
   ay.plan = c->plan[1]; ay.tw = tw_ptr<T>(c, 1);
@@ -598,7 +605,7 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
 
   // z axis forward + scale + inverse
   const ColCfg cz = col_cfg(c->ng[2], generic_fft(c->plan[2]), c->kx, NC, cs, kLdsCol,
-                            c->tune[4] > 0 ? c->tune[4] : (sizeof(T) == 8 ? 4 : 8));
+                            knobs<T>(c)[4] > 0 ? knobs<T>(c)[4] : (sizeof(T) == 8 ? 4 : 8));
   ColArgs az{};
   az.nx = c->ng[0]; az.ny = c->ng[1]; az.nz = c->ng[2]; az.kx = c->kx; az.kxs = c->kxs; az.tx = cz.tx; az.lstride = cz.lstride;
   az.plan = c->plan[2]; az.tw = tw_ptr<T>(c, 2);
@@ -622,7 +629,7 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
     const int e = is_ct_size(c->ng[2]) ? ct_plan(c->ng[2]).e : c->plan[2].e;
     if (e > 0) zthreads = std::min(1024, std::max(512, NC * cz.tx * (c->ng[2] / e)));
     zthreads = (zthreads + 63) / 64 * 64;
-    if (c->tune[5] > 0) zthreads = c->tune[5];
+    if (knobs<T>(c)[5] > 0) zthreads = knobs<T>(c)[5];
   }
 #define LAUNCH_FFTZ(E, N, ...)                                                                  \
   do {                                                                                          \
@@ -736,7 +743,7 @@ int run_operator_sizes(mpol_ctx* c, const double* m, double* out, double* hr, in
 #undef LAUNCH_R2C
   // y forward, z forward + scale + inverse, y inverse
   const ColCfg cy = col_cfg(c->ng[1], generic_fft(c->plan[1]), c->kx, 1, cs, kLdsCol,
-                            c->tune[3] > 0 ? c->tune[3] : (sizeof(T) == 8 ? 8 : 16));
+                            knobs<T>(c)[3] > 0 ? knobs<T>(c)[3] : (sizeof(T) == 8 ? 8 : 16));
   ColArgs ay{};
   ay.nx = c->ng[0]; ay.ny = c->ng[1]; ay.nz = c->ng[2]; ay.kx = c->kx; ay.kxs = c->kxs; ay.tx = cy.tx; ay.lstride = cy.lstride;
   ay.plan = c->plan[1]; ay.tw = tw_ptr<T>(c, 1);
@@ -749,7 +756,7 @@ int run_operator_sizes(mpol_ctx* c, const double* m, double* out, double* hr, in
   } while (0)
   DISPATCH_FFT(c->ng[1], ay.plan.e, LAUNCH_FFTY, 0);
   const ColCfg cz = col_cfg(c->ng[2], generic_fft(c->plan[2]), c->kx, 1, cs, kLdsCol,
-                            c->tune[4] > 0 ? c->tune[4] : (sizeof(T) == 8 ? 8 : 16));
+                            knobs<T>(c)[4] > 0 ? knobs<T>(c)[4] : (sizeof(T) == 8 ? 8 : 16));
   ColArgs az{};
   az.nx = c->ng[0]; az.ny = c->ng[1]; az.nz = c->ng[2]; az.kx = c->kx; az.kxs = c->kxs; az.tx = cz.tx; az.lstride = cz.lstride;
   az.plan = c->plan[2]; az.tw = tw_ptr<T>(c, 2);

@@ -4,6 +4,7 @@ Submodules:
   _lib     -- ctypes binding of libmpol.so (include/mpol.h)
   context  -- Context: one geometry on one GPU (tables, precompute, matvec, GMRES)
   build    -- hipcc build of libmpol.so for gfx950
+  autotune -- launch-shape and Ewald-grid autotuner
 """
 from .context import (Context, SolveResult, default_tables, real_space_table, real_space_table_quad,
                       real_space_table_sizes,

@@ -196,13 +196,15 @@ class Context:
 
     TUNING_KNOBS = ("tile_ty", "tile_tz", "chunk", "y_cols", "z_cols", "z_threads", "engine")
 
-    def set_tuning(self, **knobs):
-        """Override launch shapes (0 = automatic); see TUNING_KNOBS."""
+    def set_tuning(self, precision=None, **knobs):
+        """Override launch shapes (0 = automatic); see TUNING_KNOBS.  precision
+        (FP32 or FP64) restricts them to the operator in that precision."""
         unknown = set(knobs) - set(self.TUNING_KNOBS)
         if unknown:
             raise ValueError(f"unknown tuning knobs {sorted(unknown)}")
         vals = np.array([int(knobs.get(k, 0)) for k in self.TUNING_KNOBS], dtype=np.int32)
-        _lib.check(self._lib.mpol_set_tuning(self._ctx, _lib.i32ptr(vals), vals.size))
+        prec = -1 if precision is None else int(precision)
+        _lib.check(self._lib.mpol_set_tuning_for(self._ctx, prec, _lib.i32ptr(vals), vals.size))
 
     def geometry_info(self) -> dict:
         s = _lib.GeometrySummary()
