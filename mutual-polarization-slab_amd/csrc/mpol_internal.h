@@ -17,6 +17,7 @@
 #include "mpol_kernels.h"
 #include "mpol_mfma.h"
 #include "mpol_sizes.h"
+#include "mpol_zreg.h"
 
 namespace mpol {
 void real_space_tables(double xi, const double* r, int64_t n, double* out);
@@ -636,7 +637,16 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
     set_lds(k_fft_z_scale<T, E, N, NC>, cz.lds);                                                \
     hipLaunchKernelGGL((k_fft_z_scale<T, E, N, NC>), zgrid, dim3(zthreads), cz.lds, st, az, sp, spec); \
   } while (0)
-  DISPATCH_FFT(c->ng[2], az.plan.e, LAUNCH_FFTZ, 0);
+  // nz = 256 dipole grids: transforms in registers (unless a z column width is forced)
+  if (NC == 3 && c->ng[2] == 256 && knobs<T>(c)[4] == 0) {
+    constexpr int TXR = sizeof(T) == 4 ? 16 : 8;
+    ColArgs ar = az;
+    ar.tx = TXR;
+    hipLaunchKernelGGL((k_fft_z_scale_reg256<T, TXR>), dim3(cdiv(c->kx, TXR), c->ng[1], nrhs), dim3(TXR * 16), 0, st,
+                       ar, sp, spec);
+  } else {
+    DISPATCH_FFT(c->ng[2], az.plan.e, LAUNCH_FFTZ, 0);
+  }
   if (ev) HIPCHK(hipEventRecord(ev[5], st));
   if (stop_after <= 4) return MPOL_OK;
 

@@ -322,6 +322,9 @@ __global__ void __launch_bounds__(256) k_spread_mfma_xfft(TileArgs a, ImgArgs im
   const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
   for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];
   for (int k = threadIdx.x; k < a.ncl * a.lstride; k += blockDim.x) buf[k] = cmk<T>(0, 0);
+  // the cleared lines and twiddles must be complete before any other thread
+  // uses them (a tile without candidates goes straight to the FFT)
+  __syncthreads();
   const size_t qt = (size_t)im.off[tile];
   const int total = im.off[tile + 1] - (int)qt;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
@@ -363,8 +366,9 @@ __global__ void __launch_bounds__(256) k_spread_mfma_xfft(TileArgs a, ImgArgs im
           const int cq = ok ? cand : qa;
           int o = x0 + i16 - st.gx[cq];
           if (o < 0) o += N;
-          const T f = st.fw[cq * PW + min(o, P - 1)];
-          const T av = (ok && o < P) ? f : (T)0;
+          // the load stays unconditional (a multiply, not a select) so that the
+          // loop body has no branch and the accumulators stay in place
+          const T av = st.fw[cq * PW + min(o, P - 1)] * (T)((ok && o < P) ? 1 : 0);
           const T wv = st.fw[cq * PW + P + colrow];
 #pragma unroll
           for (int nb = 0; nb < NB; ++nb) {
@@ -502,8 +506,7 @@ __global__ void __launch_bounds__(256) k_xifft_contract_mfma(TileArgs a, ImgArgs
       for (int xx = xs; xx < xe; xx += 4) {
         const int xk = xx + k4;
         const int o = xk - gxc;
-        const T f = st.fw[cq * PW + min(max(o, 0), P - 1)];
-        const T av = (ok && o >= 0 && o < P) ? f : (T)0;
+        const T av = st.fw[cq * PW + min(max(o, 0), P - 1)] * (T)((ok && o >= 0 && o < P) ? 1 : 0);
         const int xw = xk >= N ? xk - N : xk;
         const int px = 2 * pidx(xw);
 #pragma unroll
