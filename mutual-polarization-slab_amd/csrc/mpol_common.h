@@ -39,6 +39,31 @@ __device__ __forceinline__ cplx<T> cmul_pi(cplx<T> a) { return cmk(-a.y, a.x); }
 __device__ __forceinline__ int pidx(int a) { return a + (a >> 4); }
 __host__ __device__ inline int padded_len(int n) { return (n + (n >> 4)) | 1; }
 
+// Sum over groups of R consecutive lanes (R a power of two <= 16, groups
+// inside one 16-lane DPP row); every lane of a group gets the group's sum.
+// Uses row mirrors and quad permutations (no LDS traffic).
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+constexpr int kDppRowMirror = 0x140, kDppRowHalfMirror = 0x141, kDppSwap1 = 0xB1, kDppSwap2 = 0x4E;
+
+template <typename T>
+__device__ __forceinline__ T group_sum(T v, int R) {
+  if (R >= 16) v += dpp_mov<kDppRowMirror>(v);
+  if (R >= 8) v += dpp_mov<kDppRowHalfMirror>(v);
+  if (R >= 4) v += dpp_mov<kDppSwap2>(v);
+  if (R >= 2) v += dpp_mov<kDppSwap1>(v);
+  return v;
+}
+
 // Frequency of natural-order FFT index i on an n-point grid: i for
 // i <= floor((n-1)/2), else i - n (even n: index n/2 maps to -n/2).
 __host__ __device__ inline int fft_freq(int i, int n) { return (i <= (n - 1) / 2) ? i : i - n; }

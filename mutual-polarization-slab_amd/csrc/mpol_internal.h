@@ -475,7 +475,7 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
     hipLaunchKernelGGL(k_sum_finish, dim3(cdiv(nrhs, 256)), dim3(256), 0, st, zpart, kZsumBlocks, nrhs, c->zsum.p);
     if (ev) HIPCHK(hipEventRecord(ev[1], st));
     const double V = c->box[0] * c->box[1] * c->box[2];
-    hipLaunchKernelGGL((k_real_space<T>), dim3(cdiv(n, 256), nrhs), dim3(256), 0, st, n, nrhs, c->nsets, m, out,
+    hipLaunchKernelGGL((k_real_space<T>), dim3(cdiv((size_t)n * 16, 256), nrhs), dim3(256), 0, st, n, nrhs, c->nsets, m, out,
                        c->diag.p, c->cplx_eps ? c->diag_im.p : nullptr, c->rs_off.p, c->rs_j.p, pA, pBA, prh,
                        c->zsum.p, slab / V, hr != nullptr ? 1 : 0, hr, hslab);
   } else {
@@ -483,7 +483,7 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
     const T* pq;
     if constexpr (sizeof(T) == 8) pq = c->rs_Q.p;
     else pq = c->rs_Q32.p;
-    hipLaunchKernelGGL((k_real_space_quad<T>), dim3(cdiv(n, 256), nrhs), dim3(256), 0, st, n, nrhs, c->nsets, m,
+    hipLaunchKernelGGL((k_real_space_quad<T>), dim3(cdiv((size_t)n * 16, 256), nrhs), dim3(256), 0, st, n, nrhs, c->nsets, m,
                        out, c->diag.p, c->diag_q.p, c->cplx_eps ? c->diag_im.p : nullptr,
                        c->cplx_eps ? c->diag_qim.p : nullptr, c->rs_off.p, c->rs_j.p, pA, pBA, prh, pq,
                        hr != nullptr ? 1 : 0, hr);
@@ -717,7 +717,7 @@ int run_operator_sizes(mpol_ctx* c, const double* m, double* out, double* hr, in
     pA = c->sz_A32.p; pBA = c->sz_BA32.p; prh = c->rs_rh32.p;
   }
   if (hr) HIPALLOC(c->sz_slab, (size_t)nrhs * 3 * n);
-  hipLaunchKernelGGL((k_real_space<T>), dim3(cdiv(n, 256), nrhs), dim3(256), 0, st, n, nrhs, c->nsets, m, out,
+  hipLaunchKernelGGL((k_real_space<T>), dim3(cdiv((size_t)n * 16, 256), nrhs), dim3(256), 0, st, n, nrhs, c->nsets, m, out,
                      c->sz_diag.p, c->cplx_eps ? c->sz_diag_im.p : nullptr, c->rs_off.p, c->rs_j.p, pA, pBA, prh,
                      c->zsum.p, 0.0, hr != nullptr ? 1 : 0, hr, hr ? c->sz_slab.p : nullptr);
   hipLaunchKernelGGL(k_permute_m<3>, dim3(cdiv(n, 256), nrhs), dim3(256), 0, st, m, c->perm.p, n, c->ms.p);

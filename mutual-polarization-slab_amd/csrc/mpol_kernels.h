@@ -267,21 +267,27 @@ static __global__ void k_sum_finish(const double* __restrict__ partial, int nblk
 // (real part, imaginary part) of one complex system, and the diagonal
 // couples them: out_re += d_re m_re - d_im m_im, out_im += d_re m_im + d_im m_re.
 // Permittivity sets are per right-hand side (real) or per pair (complex).
+// Sixteen lanes per particle: lane g of the group takes pairs off[i] + g,
+// off[i] + g + 16, ... (consecutive lanes read consecutive pair entries), then
+// a fixed lane-sum; lane 0 adds the diagonal and writes.
 template <typename T>
-__global__ void k_real_space(int n, int nrhs, int nsets, const double* __restrict__ m, double* __restrict__ out,
-                             const double* __restrict__ diag, const double* __restrict__ diag_im,
-                             const int* __restrict__ off, const int* __restrict__ pj, const T* __restrict__ pA,
-                             const T* __restrict__ pBA, const T* __restrict__ prh, const double* __restrict__ zsum,
-                             double slab_over_v, int write_diag_only_parts, double* __restrict__ hr_out,
-                             double* __restrict__ slab_out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_real_space(int n, int nrhs, int nsets, const double* __restrict__ m,
+                                                    double* __restrict__ out, const double* __restrict__ diag,
+                                                    const double* __restrict__ diag_im, const int* __restrict__ off,
+                                                    const int* __restrict__ pj, const T* __restrict__ pA,
+                                                    const T* __restrict__ pBA, const T* __restrict__ prh,
+                                                    const double* __restrict__ zsum, double slab_over_v,
+                                                    int write_diag_only_parts, double* __restrict__ hr_out,
+                                                    double* __restrict__ slab_out) {
+  const int g = threadIdx.x & 15;
+  const int i = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 4);
   const int rhs = blockIdx.y;
-  if (i >= n || rhs >= nrhs) return;
+  const bool valid = i < n && rhs < nrhs;  // every lane takes part in the lane sum
+  const int ii = valid ? i : 0;
   const double* mr = m + (size_t)rhs * n * 3;
-  const int set = (nsets == 1) ? 0 : (diag_im ? rhs >> 1 : rhs);
-  const double dg = diag[(size_t)set * n + i];
   T a0 = 0, a1 = 0, a2 = 0;
-  for (int q = off[i]; q < off[i + 1]; ++q) {
+  const int q0 = off[ii], q1 = valid ? off[ii + 1] : q0;
+  for (int q = q0 + g; q < q1; q += 16) {
     const int j = pj[q];
     const T m0 = (T)mr[3 * j], m1 = (T)mr[3 * j + 1], m2 = (T)mr[3 * j + 2];
     const T r0 = prh[3 * (size_t)q], r1 = prh[3 * (size_t)q + 1], r2 = prh[3 * (size_t)q + 2];
@@ -291,6 +297,12 @@ __global__ void k_real_space(int n, int nrhs, int nsets, const double* __restric
     a1 += A * m1 + c * r1;
     a2 += A * m2 + c * r2;
   }
+  a0 = group_sum<T>(a0, 16);
+  a1 = group_sum<T>(a1, 16);
+  a2 = group_sum<T>(a2, 16);
+  if (!valid || g != 0) return;
+  const int set = (nsets == 1) ? 0 : (diag_im ? rhs >> 1 : rhs);
+  const double dg = diag[(size_t)set * n + i];
   double h0 = dg * mr[3 * i] + (double)a0;
   double h1 = dg * mr[3 * i + 1] + (double)a1;
   double h2 = dg * mr[3 * i + 2] + (double)a2;
@@ -357,21 +369,25 @@ static __global__ void k_pairs_quad(PairParams p, const double* __restrict__ x, 
 // E_i, G_i from the moments of the neighbours (RealSpace.m:55-96 restated
 // owner-computes) plus the diagonal: dielectric 3/(4 pi (eps-1)) S and
 // 15/(8 pi (eps-1)) Q with the self terms field_dip_1(0) S and
-// grad_quad_2(0) Q / 2 (folded into dg_s, dg_q).
+// grad_quad_2(0) Q / 2 (folded into dg_s, dg_q).  Sixteen lanes per
+// particle over its pair range, as in k_real_space.
 template <typename T>
-__global__ void k_real_space_quad(int n, int nrhs, int nsets, const double* __restrict__ m, double* __restrict__ out,
+__global__ void __launch_bounds__(256) k_real_space_quad(int n, int nrhs, int nsets, const double* __restrict__ m, double* __restrict__ out,
                                   const double* __restrict__ dg_s, const double* __restrict__ dg_q,
                                   const double* __restrict__ dgi_s, const double* __restrict__ dgi_q,
                                   const int* __restrict__ off, const int* __restrict__ pj,
                                   const T* __restrict__ pA, const T* __restrict__ pBA, const T* __restrict__ prh,
                                   const T* __restrict__ pq, int parts, double* __restrict__ hr_out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 15;
+  const int i = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 4);
   const int rhs = blockIdx.y;
-  if (i >= n || rhs >= nrhs) return;
+  const bool valid = i < n && rhs < nrhs;  // every lane takes part in the lane sum
+  const int ii = valid ? i : 0;
   const double* mr = m + (size_t)rhs * n * 8;
   T e[3] = {0, 0, 0}, g[5] = {0, 0, 0, 0, 0};
   const T half = (T)0.5;
-  for (int q = off[i]; q < off[i + 1]; ++q) {
+  const int q0 = off[ii], q1e = valid ? off[ii + 1] : q0;
+  for (int q = q0 + lane; q < q1e; q += 16) {
     const int j = pj[q];
     const double* mj = mr + 8 * (size_t)j;
     const T S0 = (T)mj[0], S1 = (T)mj[1], S2 = (T)mj[2];
@@ -410,6 +426,11 @@ __global__ void k_real_space_quad(int n, int nrhs, int nsets, const double* __re
       g[p] += v;
     }
   }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) e[c] = group_sum<T>(e[c], 16);
+#pragma unroll
+  for (int c = 0; c < 5; ++c) g[c] = group_sum<T>(g[c], 16);
+  if (!valid || lane != 0) return;
   const int set = (nsets == 1) ? 0 : (dgi_s ? rhs >> 1 : rhs);
   const double ds = dg_s[(size_t)set * n + i], dq = dg_q[(size_t)set * n + i];
   const double* mi = mr + 8 * (size_t)i;

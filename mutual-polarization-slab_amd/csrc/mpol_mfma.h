@@ -57,31 +57,6 @@ struct Mfma<double> {
   static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 
-// Sum over groups of R consecutive lanes (R a power of two <= 16, groups
-// inside one 16-lane DPP row); every lane of a group gets the group's sum.
-// Uses row mirrors and quad permutations (no LDS traffic).
-template <int CTRL>
-__device__ __forceinline__ float dpp_mov(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
-}
-template <int CTRL>
-__device__ __forceinline__ double dpp_mov(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xF, 0xF, true);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, true);
-  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-constexpr int kDppRowMirror = 0x140, kDppRowHalfMirror = 0x141, kDppSwap1 = 0xB1, kDppSwap2 = 0x4E;
-
-template <typename T>
-__device__ __forceinline__ T group_sum(T v, int R) {
-  if (R >= 16) v += dpp_mov<kDppRowMirror>(v);
-  if (R >= 8) v += dpp_mov<kDppRowHalfMirror>(v);
-  if (R >= 4) v += dpp_mov<kDppSwap2>(v);
-  if (R >= 2) v += dpp_mov<kDppSwap1>(v);
-  return v;
-}
-
 struct ImgArgs {
   const int* off;    // [ntiles + 1] first entry of each tile
   const int2* gs;    // [total] {first stencil node along x, sorted particle index}
