@@ -48,6 +48,29 @@ int dots(mpol_ctx* c, const double* A, size_t stride_i, int cnt, const double* w
   return MPOL_OK;
 }
 
+// Copy the per-rhs solver state and relative residuals to the host (through
+// page-locked buffers) and wait for them.
+int read_state(mpol_ctx* c, int nrhs, std::vector<int>& state, std::vector<double>& rel) {
+  if (c->hstate.reserve((size_t)nrhs) != hipSuccess || c->hrel.reserve((size_t)nrhs) != hipSuccess)
+    return fail(MPOL_ENOMEM, "page-locked host allocation failed");
+  HIPCHK(hipMemcpyAsync(c->hstate.p, c->gstate.p, nrhs * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(c->hrel.p, c->grel.p, nrhs * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  std::copy(c->hstate.p, c->hstate.p + nrhs, state.begin());
+  std::copy(c->hrel.p, c->hrel.p + nrhs, rel.begin());
+  return MPOL_OK;
+}
+
+// Copy cnt doubles from the device to host memory (through a page-locked
+// buffer) and wait for them.
+int read_doubles(mpol_ctx* c, const double* dev, int cnt, double* host) {
+  if (c->hscr.reserve((size_t)cnt) != hipSuccess) return fail(MPOL_ENOMEM, "page-locked host allocation failed");
+  HIPCHK(hipMemcpyAsync(c->hscr.p, dev, cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  std::copy(c->hscr.p, c->hscr.p + cnt, host);
+  return MPOL_OK;
+}
+
 // Restarted GMRES on x (in: initial guess, out: solution) for A x = b.
 // nrhs here counts systems; a system's vector holds all its right-hand sides
 // (kind components per particle, times 2 with complex permittivities).
@@ -90,8 +113,7 @@ int gmres_core(mpol_ctx* c, int kind, const double* b, double* x, int nrhs, doub
   // ||b||
   dots(c, b, 0, 1, b, n3, nrhs, c->grn2.p);
   std::vector<double> bn2(nrhs);
-  HIPCHK(hipMemcpyAsync(bn2.data(), c->grn2.p, nrhs * sizeof(double), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  CHECK_STATUS(read_doubles(c, c->grn2.p, nrhs, bn2.data()));
   std::vector<double> bn(nrhs);
   std::vector<int> state(nrhs);
   for (int r = 0; r < nrhs; ++r) {
@@ -116,9 +138,7 @@ int gmres_core(mpol_ctx* c, int kind, const double* b, double* x, int nrhs, doub
     dots(c, c->gR.p, 0, 1, c->gR.p, n3, nrhs, c->grn2.p);
     hipLaunchKernelGGL(k_cycle_start, dim3(sgrid), dim3(256), 0, st, nrhs, restart, c->grn2.p, c->gbn.p, tol,
                        c->gstate.p, c->gg.p, c->gbeta.p, c->grel.p);
-    HIPCHK(hipMemcpyAsync(state.data(), c->gstate.p, nrhs * sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(rel.data(), c->grel.p, nrhs * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    CHECK_STATUS(read_state(c, nrhs, state, rel));
     all_done = std::all_of(state.begin(), state.end(), [](int s) { return s == kDone; });
     if (all_done || outer == maxit) break;
     hipLaunchKernelGGL(k_scale_into, vgrid, dim3(256), 0, st, c->gR.p, c->gbeta.p, c->gV.p, n3, c->gstate.p);
@@ -126,33 +146,35 @@ int gmres_core(mpol_ctx* c, int kind, const double* b, double* x, int nrhs, doub
       double* vj = c->gV.p + (size_t)j * sv;
       CHECK_STATUS(apply(c, kind, vj, c->gW.p, nrhs, precision, slab));
       ++*nmv;
-      // classical Gram-Schmidt, twice
-      dots(c, c->gV.p, sv, j + 1, c->gW.p, n3, nrhs, c->gh1.p);
-      hipLaunchKernelGGL(k_sub_basis, vgrid, dim3(256), 0, st, c->gV.p, sv, j + 1, c->gh1.p, c->gW.p, n3,
-                         c->gstate.p);
-      dots(c, c->gV.p, sv, j + 1, c->gW.p, n3, nrhs, c->gh2.p);
-      hipLaunchKernelGGL(k_sub_basis, vgrid, dim3(256), 0, st, c->gV.p, sv, j + 1, c->gh2.p, c->gW.p, n3,
-                         c->gstate.p);
-      dots(c, c->gW.p, 0, 1, c->gW.p, n3, nrhs, c->grn2.p);
-      hipLaunchKernelGGL(k_arnoldi_finish, dim3(sgrid), dim3(256), 0, st, nrhs, restart, j, c->gh1.p, c->gh2.p,
-                         c->grn2.p, c->gH.p, c->gstate.p);
+      // classical Gram-Schmidt, twice; each subtraction pass also forms the
+      // partial sums of the next reduction (second-pass coefficients, ||w||^2)
+      const int cj = j + 1;
+      const dim3 dgrid(kDotBlocks, nrhs);
+      dots(c, c->gV.p, sv, cj, c->gW.p, n3, nrhs, c->gh1.p);
+      hipLaunchKernelGGL(k_sub_dots, dgrid, dim3(256), 0, st, c->gV.p, sv, cj, c->gh1.p, c->gW.p, n3, c->gstate.p, 0,
+                         c->gpart.p);
+      hipLaunchKernelGGL(k_dots_finish, dim3(cdiv((int64_t)nrhs * cj, 4)), dim3(256), 0, st, c->gpart.p, kDotBlocks,
+                         nrhs * cj, c->gh2.p);
+      hipLaunchKernelGGL(k_sub_dots, dgrid, dim3(256), 0, st, c->gV.p, sv, cj, c->gh2.p, c->gW.p, n3, c->gstate.p, 1,
+                         c->gpart.p);
+      hipLaunchKernelGGL(k_dots_finish, dim3(cdiv(nrhs, 4)), dim3(256), 0, st, c->gpart.p, kDotBlocks, nrhs,
+                         c->grn2.p);
       hipLaunchKernelGGL(k_normalize_next, vgrid, dim3(256), 0, st, c->gW.p, c->gV.p + (size_t)(j + 1) * sv,
-                         c->gH.p, restart, j, n3, c->gstate.p);
-      hipLaunchKernelGGL(k_givens, dim3(sgrid), dim3(256), 0, st, nrhs, restart, j, c->gH.p, c->gcs.p, c->gsn.p,
-                         c->gg.p, c->gbn.p, tol, c->gstate.p, c->gkdim.p, c->grel.p);
-      hipLaunchKernelGGL(k_backsolve, dim3(sgrid), dim3(256), 0, st, nrhs, restart, c->gH.p, c->gg.p,
-                         c->gkdim.p, c->gstate.p, c->gy.p);
-      hipLaunchKernelGGL(k_update_x, vgrid, dim3(256), 0, st, c->gV.p, sv, c->gy.p, c->gkdim.p, c->gstate.p,
-                         restart, x, n3);
-      hipLaunchKernelGGL(k_solve_to_wait, dim3(sgrid), dim3(256), 0, st, nrhs, c->gstate.p);
-      HIPCHK(hipMemcpyAsync(state.data(), c->gstate.p, nrhs * sizeof(int), hipMemcpyDeviceToHost, st));
-      HIPCHK(hipStreamSynchronize(st));
+                         c->grn2.p, n3, c->gstate.p);
+      hipLaunchKernelGGL(k_arnoldi_step, dim3(sgrid), dim3(256), 0, st, nrhs, restart, j, c->gh1.p, c->gh2.p,
+                         c->grn2.p, c->gH.p, c->gcs.p, c->gsn.p, c->gg.p, c->gbn.p, tol, c->gstate.p, c->gkdim.p,
+                         c->grel.p, c->gy.p);
+      CHECK_STATUS(read_state(c, nrhs, state, rel));
+      if (std::any_of(state.begin(), state.end(), [](int s) { return s == kSolve; })) {
+        // a cycle ended for some right-hand sides: x += V y, then they wait for the next cycle
+        hipLaunchKernelGGL(k_update_x, vgrid, dim3(256), 0, st, c->gV.p, sv, c->gy.p, c->gkdim.p, c->gstate.p,
+                           restart, x, n3);
+        hipLaunchKernelGGL(k_solve_to_wait, dim3(sgrid), dim3(256), 0, st, nrhs, c->gstate.p);
+      }
       if (std::none_of(state.begin(), state.end(), [](int s) { return s == kIter; })) break;
     }
     if (inner) {
       // every right-hand side whose estimated residual met tol is done
-      HIPCHK(hipMemcpyAsync(rel.data(), c->grel.p, nrhs * sizeof(double), hipMemcpyDeviceToHost, st));
-      HIPCHK(hipStreamSynchronize(st));
       bool est_done = true;
       for (int r = 0; r < nrhs; ++r)
         if (state[r] != kDone && !(rel[r] <= tol)) est_done = false;
@@ -560,8 +582,7 @@ int solve_impl(mpol_ctx* c, int kind, const double* b, const double* x0, double*
     HIPALLOC(c->gpart, (size_t)nrhs * (o->restart + 1) * kDotBlocks);
     std::vector<double> bn2(nrhs), rn2(nrhs);
     dots(c, b, 0, 1, b, n3, nrhs, c->grn2.p);
-    HIPCHK(hipMemcpyAsync(bn2.data(), c->grn2.p, nrhs * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    CHECK_STATUS(read_doubles(c, c->grn2.p, nrhs, bn2.data()));
     int64_t nmv64 = 0, nmv32 = 0;
     int ref = 0;
     // The first residual (from the caller's guess) only seeds the first
@@ -575,8 +596,7 @@ int solve_impl(mpol_ctx* c, int kind, const double* b, const double* x0, double*
       else ++nmv64;
       hipLaunchKernelGGL(k_residual, dim3(cdiv(tot, 256)), dim3(256), 0, st, b, c->gAx.p, c->gd.p, tot);
       dots(c, c->gd.p, 0, 1, c->gd.p, n3, nrhs, c->grn2.p);
-      HIPCHK(hipMemcpyAsync(rn2.data(), c->grn2.p, nrhs * sizeof(double), hipMemcpyDeviceToHost, st));
-      HIPCHK(hipStreamSynchronize(st));
+      CHECK_STATUS(read_doubles(c, c->grn2.p, nrhs, rn2.data()));
       double ratio = 1e300;
       conv = 0;
       for (int r = 0; r < nrhs; ++r) {

@@ -13,26 +13,21 @@ enum RhsState : int { kDone = 0, kIter = 1, kSolve = 2, kWait = 3 };
 
 constexpr int kDotBlocks = 256;  // partial-sum blocks per right-hand side (one per CU)
 
-// partial[(rhs*cnt + i)*kDotBlocks + blk] = sum over the block's chunk of a_i . w
-// with a_i = A + i*stride_i (rhs offset rhs*n).  Deterministic.
-static __global__ void __launch_bounds__(256) k_dots(const double* __restrict__ A, size_t stride_i, int cnt,
-                                              const double* __restrict__ w, int64_t n, int nrhs,
-                                              double* __restrict__ partial) {
-  __shared__ double red[4][4];  // [vector of the group][wave]
-  const int rhs = blockIdx.y;
-  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
-  const int64_t lo = (int64_t)blockIdx.x * chunk;
-  const int64_t hi = lo + chunk < n ? lo + chunk : n;
-  const double* wr = w + (size_t)rhs * n;
+// Partial dot products of one block's chunk [lo, hi) of right-hand side rhs:
+// partial[(rhs*cnt + i)*gridDim.x + blockIdx.x] = sum over the chunk of
+// a_i . w, a_i = A + i*stride_i + rhs*n (four vectors per sweep, w read once
+// per group).  Thread t handles elements lo + t, lo + t + blockDim.x, ...
+__device__ __forceinline__ void block_dots(const double* __restrict__ A, size_t stride_i, int cnt,
+                                           const double* __restrict__ wr, size_t roff, int64_t lo, int64_t hi,
+                                           int rhs, double* __restrict__ partial, double (*red)[4]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  // four dot products per sweep over the chunk (w read once per group)
   for (int i0 = 0; i0 < cnt; i0 += 4) {
     double s[4] = {0.0, 0.0, 0.0, 0.0};
     for (int64_t k = lo + threadIdx.x; k < hi; k += blockDim.x) {
       const double wv = wr[k];
 #pragma unroll
       for (int u = 0; u < 4; ++u)
-        if (i0 + u < cnt) s[u] += A[(size_t)(i0 + u) * stride_i + (size_t)rhs * n + k] * wv;
+        if (i0 + u < cnt) s[u] += A[(size_t)(i0 + u) * stride_i + roff + k] * wv;
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u)
@@ -50,8 +45,26 @@ static __global__ void __launch_bounds__(256) k_dots(const double* __restrict__ 
   }
 }
 
-// out[rhs*cnt + i] = sum of the partials (fixed order): one wave per output,
-// lanes load the partials in parallel, then a fixed butterfly.
+// Chunk of block blockIdx.x when n elements are split over gridDim.x blocks.
+__device__ __forceinline__ void block_chunk(int64_t n, int64_t& lo, int64_t& hi) {
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  lo = (int64_t)blockIdx.x * chunk;
+  hi = lo + chunk < n ? lo + chunk : n;
+}
+
+// partial dot products of a_i . w (i < cnt) for every rhs; grid (blocks, nrhs)
+static __global__ void __launch_bounds__(256) k_dots(const double* __restrict__ A, size_t stride_i, int cnt,
+                                              const double* __restrict__ w, int64_t n, int nrhs,
+                                              double* __restrict__ partial) {
+  __shared__ double red[4][4];  // [vector of the group][wave]
+  const int rhs = blockIdx.y;
+  int64_t lo, hi;
+  block_chunk(n, lo, hi);
+  block_dots(A, stride_i, cnt, w + (size_t)rhs * n, (size_t)rhs * n, lo, hi, rhs, partial, red);
+}
+
+// out[t] = sum of the nblk partials of output t (fixed order): one wave per
+// output, lanes load the partials in parallel, then a fixed butterfly.
 static __global__ void k_dots_finish(const double* __restrict__ partial, int nblk, int total, double* __restrict__ out) {
   const int t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -62,16 +75,31 @@ static __global__ void k_dots_finish(const double* __restrict__ partial, int nbl
   if (lane == 0) out[t] = s;
 }
 
-// w[rhs] -= sum_i h[rhs*cnt + i] V_i[rhs]   (only rhs in state kIter)
-static __global__ void k_sub_basis(const double* __restrict__ V, size_t stride_i, int cnt, const double* __restrict__ h,
-                            double* __restrict__ w, int64_t n, const int* __restrict__ state) {
+// One Gram-Schmidt pass fused with the next reduction, per rhs in kIter
+// (grid (blocks, nrhs), chunks as in k_dots):
+//   w[rhs] -= sum_{i<cnt} h[rhs*cnt + i] V_i[rhs]
+// then the partial sums of V_i . w (i < cnt; next_norm = 0, the second pass's
+// coefficients) or of w . w (next_norm = 1, ||w||^2) over the block's chunk.
+// A thread re-reads only the elements of w that it wrote itself.
+static __global__ void __launch_bounds__(256) k_sub_dots(const double* __restrict__ V, size_t stride_i, int cnt,
+                                                  const double* __restrict__ h, double* __restrict__ w, int64_t n,
+                                                  const int* __restrict__ state, int next_norm,
+                                                  double* __restrict__ partial) {
+  __shared__ double red[4][4];
   const int rhs = blockIdx.y;
-  if (state[rhs] != kIter) return;
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  double acc = w[(size_t)rhs * n + k];
-  for (int i = 0; i < cnt; ++i) acc -= h[rhs * cnt + i] * V[i * stride_i + (size_t)rhs * n + k];
-  w[(size_t)rhs * n + k] = acc;
+  if (state[rhs] != kIter) return;  // uniform over the block
+  int64_t lo, hi;
+  block_chunk(n, lo, hi);
+  const size_t roff = (size_t)rhs * n;
+  double* wr = w + roff;
+  const double* hr = h + (size_t)rhs * cnt;
+  for (int64_t k = lo + threadIdx.x; k < hi; k += blockDim.x) {
+    double acc = wr[k];
+    for (int i = 0; i < cnt; ++i) acc -= hr[i] * V[(size_t)i * stride_i + roff + k];
+    wr[k] = acc;
+  }
+  if (next_norm) block_dots(wr - roff, 0, 1, wr, roff, lo, hi, rhs, partial, red);
+  else block_dots(V, stride_i, cnt, wr, roff, lo, hi, rhs, partial, red);
 }
 
 // r = b - Ax
@@ -123,34 +151,32 @@ static __global__ void k_scale_into(const double* __restrict__ r, const double* 
   v[(size_t)rhs * n + k] = (state[rhs] == kIter && bt != 0.0) ? r[(size_t)rhs * n + k] / bt : 0.0;
 }
 
-// Store h (+ h2) into column j of H and normalise w into V_{j+1}; hnorm2 = ||w||^2.
-static __global__ void k_arnoldi_finish(int nrhs, int restart, int j, const double* __restrict__ h1,
-                                 const double* __restrict__ h2, const double* __restrict__ hnorm2,
-                                 double* __restrict__ H, const int* __restrict__ state) {
+// V_{j+1} = w / ||w|| (hnorm2 = ||w||^2) for rhs in kIter.
+static __global__ void k_normalize_next(const double* __restrict__ w, double* __restrict__ vnext,
+                                        const double* __restrict__ hnorm2, int64_t n, const int* __restrict__ state) {
+  const int rhs = blockIdx.y;
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n || state[rhs] != kIter) return;
+  const double hn = sqrt(hnorm2[rhs]);
+  vnext[(size_t)rhs * n + k] = hn != 0.0 ? w[(size_t)rhs * n + k] / hn : 0.0;
+}
+
+// One Arnoldi step of the small problem, per right-hand side in kIter:
+// column j of H from the two Gram-Schmidt passes (h1 + h2) and ||w||, the
+// Givens rotations on that column, the decision whether the cycle ends
+// (estimated relative residual |g_{j+1}| / ||b|| in relest), and for a cycle
+// that ends, the back substitution of the rotated H y = g.
+static __global__ void k_arnoldi_step(int nrhs, int restart, int j, const double* __restrict__ h1,
+                                      const double* __restrict__ h2, const double* __restrict__ hnorm2,
+                                      double* __restrict__ H, double* __restrict__ cs, double* __restrict__ sn,
+                                      double* __restrict__ g, const double* __restrict__ bnorm, double tol,
+                                      int* __restrict__ state, int* __restrict__ kdim, double* __restrict__ relest,
+                                      double* __restrict__ y) {
   const int rhs = blockIdx.x * blockDim.x + threadIdx.x;
   if (rhs >= nrhs || state[rhs] != kIter) return;
   double* Hr = H + (size_t)rhs * (restart + 1) * restart;
   for (int i = 0; i <= j; ++i) Hr[i * restart + j] = h1[rhs * (j + 1) + i] + h2[rhs * (j + 1) + i];
   Hr[(j + 1) * restart + j] = sqrt(hnorm2[rhs]);
-}
-
-static __global__ void k_normalize_next(const double* __restrict__ w, double* __restrict__ vnext, const double* __restrict__ H,
-                                 int restart, int j, int64_t n, const int* __restrict__ state) {
-  const int rhs = blockIdx.y;
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n || state[rhs] != kIter) return;
-  const double hn = H[(size_t)rhs * (restart + 1) * restart + (size_t)(j + 1) * restart + j];
-  vnext[(size_t)rhs * n + k] = hn != 0.0 ? w[(size_t)rhs * n + k] / hn : 0.0;
-}
-
-// Givens rotations on column j; decide whether the cycle ends for each rhs;
-// relest[rhs] = the estimated relative residual |g_{j+1}| / ||b||.
-static __global__ void k_givens(int nrhs, int restart, int j, double* __restrict__ H, double* __restrict__ cs,
-                         double* __restrict__ sn, double* __restrict__ g, const double* __restrict__ bnorm,
-                         double tol, int* __restrict__ state, int* __restrict__ kdim, double* __restrict__ relest) {
-  const int rhs = blockIdx.x * blockDim.x + threadIdx.x;
-  if (rhs >= nrhs || state[rhs] != kIter) return;
-  double* Hr = H + (size_t)rhs * (restart + 1) * restart;
   double* c = cs + (size_t)rhs * restart;
   double* s = sn + (size_t)rhs * restart;
   double* gr = g + (size_t)rhs * (restart + 1);
@@ -176,26 +202,16 @@ static __global__ void k_givens(int nrhs, int restart, int j, double* __restrict
   gr[j + 1] = -s[j] * gj;
   const double rel = fabs(gr[j + 1]) / bnorm[rhs];
   relest[rhs] = rel;
-  if (rel <= tol || j == restart - 1 || breakdown) {
-    state[rhs] = kSolve;
-    kdim[rhs] = j + 1;
-  }
-}
-
-// Back substitution of the rotated (upper triangular) H y = g for rhs in kSolve.
-static __global__ void k_backsolve(int nrhs, int restart, const double* __restrict__ H, const double* __restrict__ g,
-                            const int* __restrict__ kdim, const int* __restrict__ state, double* __restrict__ y) {
-  const int rhs = blockIdx.x * blockDim.x + threadIdx.x;
-  if (rhs >= nrhs || state[rhs] != kSolve) return;
-  const double* Hr = H + (size_t)rhs * (restart + 1) * restart;
-  const double* gr = g + (size_t)rhs * (restart + 1);
+  if (!(rel <= tol || j == restart - 1 || breakdown)) return;
+  state[rhs] = kSolve;
+  const int k = j + 1;
+  kdim[rhs] = k;
   double* yr = y + (size_t)rhs * restart;
-  const int k = kdim[rhs];
   for (int i = k - 1; i >= 0; --i) {
-    double s = gr[i];
-    for (int l = i + 1; l < k; ++l) s -= Hr[i * restart + l] * yr[l];
+    double sum = gr[i];
+    for (int l = i + 1; l < k; ++l) sum -= Hr[i * restart + l] * yr[l];
     const double d = Hr[i * restart + i];
-    yr[i] = d != 0.0 ? s / d : 0.0;
+    yr[i] = d != 0.0 ? sum / d : 0.0;
   }
 }
 

@@ -76,6 +76,34 @@ struct DevBuf {
   ~DevBuf() { release(); }
 };
 
+// Page-locked host buffer (grows only), so device-to-host copies of solver
+// state are queued on the stream without a staging wait.
+template <typename T>
+struct HostBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  hipError_t reserve(size_t cnt) {
+    if (cnt <= n && p) return hipSuccess;
+    release();
+    if (cnt == 0) return hipSuccess;
+    hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&p), cnt * sizeof(T), hipHostMallocDefault);
+    if (e != hipSuccess) {
+      p = nullptr;
+      n = 0;
+      return e;
+    }
+    n = cnt;
+    return hipSuccess;
+  }
+  void release() {
+    if (p) (void)hipDeviceSynchronize();
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~HostBuf() { release(); }
+};
+
 inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
 inline FftPlan make_plan(int n) {
@@ -239,6 +267,8 @@ struct mpol_ctx {
   // gmres
   DevBuf<double> gV, gW, gR, gAx, gH, gcs, gsn, gg, gy, gh1, gh2, gpart, gbn, gbeta, grel, grn2, gx, gd;
   DevBuf<int> gstate, gkdim;
+  HostBuf<int> hstate;  // solver state read back each Arnoldi step
+  HostBuf<double> hrel, hscr;
   // truncated fixed-dipole forces: cell list
   DevBuf<int> f_cell, f_count, f_start, f_cursor, f_members;
 };
@@ -470,9 +500,11 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
     pA = c->rs_A32.p; pBA = c->rs_BA32.p; prh = c->rs_rh32.p;
   }
   if constexpr (NC == kOpDipole) {
-    double* zpart = c->zsum.p + nrhs;
-    hipLaunchKernelGGL(k_sum_z, dim3(kZsumBlocks, nrhs), dim3(256), 0, st, m, n, zpart);
-    hipLaunchKernelGGL(k_sum_finish, dim3(cdiv(nrhs, 256)), dim3(256), 0, st, zpart, kZsumBlocks, nrhs, c->zsum.p);
+    if (slab != 0.0) {
+      double* zpart = c->zsum.p + nrhs;
+      hipLaunchKernelGGL(k_sum_z, dim3(kZsumBlocks, nrhs), dim3(256), 0, st, m, n, zpart);
+      hipLaunchKernelGGL(k_sum_finish, dim3(cdiv(nrhs, 256)), dim3(256), 0, st, zpart, kZsumBlocks, nrhs, c->zsum.p);
+    }
     if (ev) HIPCHK(hipEventRecord(ev[1], st));
     const double V = c->box[0] * c->box[1] * c->box[2];
     hipLaunchKernelGGL((k_real_space<T>), dim3(cdiv((size_t)n * 16, 256), nrhs), dim3(256), 0, st, n, nrhs, c->nsets, m, out,
@@ -672,7 +704,7 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   if (use_mfma) {
     DISPATCH_CT(c->ng[0], LAUNCH_CONTRACT_MFMA, 0);
     const TileImage& im = c->img[sizeof(T) == 8 ? 1 : 0];
-    hipLaunchKernelGGL((k_contract_reduce<T, NC>), dim3(cdiv(n, 256), nrhs), dim3(256), 0, st, n, ima.total,
+    hipLaunchKernelGGL((k_contract_reduce<T, NC>), dim3(cdiv((size_t)n * 16, 256), nrhs), dim3(256), 0, st, n, ima.total,
                        im.poff.p, im.pslot.p, c->perm.p, reinterpret_cast<const T*>(ima.part), ta.hvol, out);
   } else {
     DISPATCH_FFT(c->ng[0], ta.plan.e, LAUNCH_CONTRACT, 0);

@@ -313,7 +313,7 @@ __global__ void __launch_bounds__(256) k_real_space(int n, int nrhs, int nsets, 
     h1 += di * mp[3 * i + 1];
     h2 += di * mp[3 * i + 2];
   }
-  const double sz = slab_over_v * zsum[rhs];
+  const double sz = slab_over_v != 0.0 ? slab_over_v * zsum[rhs] : 0.0;  // zsum is not formed without a slab term
   double* o = out + ((size_t)rhs * n + i) * 3;
   if (write_diag_only_parts) {
     // component mode: out <- 0 (wave part accumulates), hr_out <- Hr, slab_out <- H_slab

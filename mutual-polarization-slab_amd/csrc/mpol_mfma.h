@@ -98,23 +98,31 @@ static __global__ void k_img_psort(int n, const int* __restrict__ poff, int* __r
 }
 
 // out[rhs][perm[s]][c] += hvol * sum over the slots of particle s of the
-// contraction partials (FP64 sums, fixed order).
+// contraction partials (FP64 sums in a fixed order: sixteen lanes per
+// particle, lane g taking slots g, g + 16, ... of the sorted slot list, then a
+// fixed lane-sum).
 template <typename T, int NC>
-__global__ void k_contract_reduce(int n, long long total, const int* __restrict__ poff, const int* __restrict__ pslot,
-                                  const int* __restrict__ perm, const T* __restrict__ part, double hvol,
-                                  double* __restrict__ out) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_contract_reduce(int n, long long total, const int* __restrict__ poff,
+                                                         const int* __restrict__ pslot, const int* __restrict__ perm,
+                                                         const T* __restrict__ part, double hvol,
+                                                         double* __restrict__ out) {
+  const int g = threadIdx.x & 15;
+  const int s = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 4);
   const int rhs = blockIdx.y;
-  if (s >= n) return;
+  const bool valid = s < n;  // every lane takes part in the lane sum
   double acc[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) acc[c] = 0.0;
   const T* pr = part + (size_t)rhs * total * NC;
-  for (int k = poff[s]; k < poff[s + 1]; ++k) {
+  const int k0 = valid ? poff[s] : 0, k1 = valid ? poff[s + 1] : 0;
+  for (int k = k0 + g; k < k1; k += 16) {
     const T* p = pr + (size_t)pslot[k] * NC;
 #pragma unroll
     for (int c = 0; c < NC; ++c) acc[c] += (double)p[c];
   }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) acc[c] = group_sum<double>(acc[c], 16);
+  if (!valid || g != 0) return;
   double* o = out + ((size_t)rhs * n + perm[s]) * NC;
 #pragma unroll
   for (int c = 0; c < NC; ++c) o[c] += acc[c] * hvol;
