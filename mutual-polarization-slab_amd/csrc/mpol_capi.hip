@@ -39,12 +39,20 @@ int check_ready(mpol_ctx* c) {
   return MPOL_OK;
 }
 
+// Partial-sum blocks per right-hand side of the reductions over vectors of
+// n elements: at least 1024 elements per block and about 1024 blocks in all
+// (at most kDotBlocks per right-hand side).
+int dot_blocks(int64_t n, int nrhs) {
+  const int64_t by_total = (1024 + nrhs - 1) / nrhs, by_len = (n + 1023) / 1024;
+  return (int)std::max<int64_t>(1, std::min<int64_t>({by_total, by_len, (int64_t)kDotBlocks}));
+}
+
 // squared norms / dot products of `cnt` vectors A_i (stride stride_i) with w, per rhs -> out[rhs*cnt+i]
 int dots(mpol_ctx* c, const double* A, size_t stride_i, int cnt, const double* w, int64_t n, int nrhs, double* out) {
-  hipLaunchKernelGGL(k_dots, dim3(kDotBlocks, nrhs), dim3(256), 0, c->stream, A, stride_i, cnt, w, n, nrhs,
-                     c->gpart.p);
-  hipLaunchKernelGGL(k_dots_finish, dim3(cdiv((int64_t)nrhs * cnt, 4)), dim3(256), 0, c->stream, c->gpart.p,
-                     kDotBlocks, nrhs * cnt, out);
+  const int nb = dot_blocks(n, nrhs);
+  hipLaunchKernelGGL(k_dots, dim3(nb, nrhs), dim3(256), 0, c->stream, A, stride_i, cnt, w, n, nrhs, c->gpart.p);
+  hipLaunchKernelGGL(k_dots_finish, dim3(cdiv((int64_t)nrhs * cnt, 4)), dim3(256), 0, c->stream, c->gpart.p, nb,
+                     nrhs * cnt, out);
   return MPOL_OK;
 }
 
@@ -149,16 +157,16 @@ int gmres_core(mpol_ctx* c, int kind, const double* b, double* x, int nrhs, doub
       // classical Gram-Schmidt, twice; each subtraction pass also forms the
       // partial sums of the next reduction (second-pass coefficients, ||w||^2)
       const int cj = j + 1;
-      const dim3 dgrid(kDotBlocks, nrhs);
+      const int nb = dot_blocks(n3, nrhs);
+      const dim3 dgrid(nb, nrhs);
       dots(c, c->gV.p, sv, cj, c->gW.p, n3, nrhs, c->gh1.p);
       hipLaunchKernelGGL(k_sub_dots, dgrid, dim3(256), 0, st, c->gV.p, sv, cj, c->gh1.p, c->gW.p, n3, c->gstate.p, 0,
                          c->gpart.p);
-      hipLaunchKernelGGL(k_dots_finish, dim3(cdiv((int64_t)nrhs * cj, 4)), dim3(256), 0, st, c->gpart.p, kDotBlocks,
+      hipLaunchKernelGGL(k_dots_finish, dim3(cdiv((int64_t)nrhs * cj, 4)), dim3(256), 0, st, c->gpart.p, nb,
                          nrhs * cj, c->gh2.p);
       hipLaunchKernelGGL(k_sub_dots, dgrid, dim3(256), 0, st, c->gV.p, sv, cj, c->gh2.p, c->gW.p, n3, c->gstate.p, 1,
                          c->gpart.p);
-      hipLaunchKernelGGL(k_dots_finish, dim3(cdiv(nrhs, 4)), dim3(256), 0, st, c->gpart.p, kDotBlocks, nrhs,
-                         c->grn2.p);
+      hipLaunchKernelGGL(k_dots_finish, dim3(cdiv(nrhs, 4)), dim3(256), 0, st, c->gpart.p, nb, nrhs, c->grn2.p);
       hipLaunchKernelGGL(k_normalize_next, vgrid, dim3(256), 0, st, c->gW.p, c->gV.p + (size_t)(j + 1) * sv,
                          c->grn2.p, n3, c->gstate.p);
       hipLaunchKernelGGL(k_arnoldi_step, dim3(sgrid), dim3(256), 0, st, nrhs, restart, j, c->gh1.p, c->gh2.p,

@@ -724,6 +724,8 @@ inline int rhs_per_system(const mpol_ctx* c) { return c->cplx_eps ? 2 : 1; }
 // Polydisperse operator: real space (pair couplings of the two radii,
 // dielectric 3/(4 pi a^3 (eps-1))) + one scalar grid per right-hand side.
 // hr != nullptr: hr <- real space, out <- wave space.
+constexpr size_t kSizesContractLds = 64 * 1024;  // stencil staging of the polydisperse contraction
+
 template <typename T>
 int run_operator_sizes(mpol_ctx* c, const double* m, double* out, double* hr, int nrhs) {
   const int n = c->n;
@@ -736,10 +738,7 @@ int run_operator_sizes(mpol_ctx* c, const double* m, double* out, double* hr, in
   HIPALLOC(c->ms, (size_t)nrhs * 3 * n);
   cplx<T>* spec = reinterpret_cast<cplx<T>*>(c->spec.p);
   T* grid = reinterpret_cast<T*>(c->sz_grid.p);
-  // real space (the slab term is absent: its sum only needs to be defined)
-  double* zpart = c->zsum.p + nrhs;
-  hipLaunchKernelGGL(k_sum_z, dim3(kZsumBlocks, nrhs), dim3(256), 0, st, m, n, zpart);
-  hipLaunchKernelGGL(k_sum_finish, dim3(cdiv(nrhs, 256)), dim3(256), 0, st, zpart, kZsumBlocks, nrhs, c->zsum.p);
+  // real space (no slab term: the real-space kernel does not read zsum)
   const T* pA;
   const T* pBA;
   const T* prh;
@@ -836,7 +835,13 @@ int run_operator_sizes(mpol_ctx* c, const double* m, double* out, double* hr, in
   } while (0)
   DISPATCH_FFT(c->ng[0], ta.plan.e, LAUNCH_C2R, 0);
 #undef LAUNCH_C2R
-  hipLaunchKernelGGL((k_sizes_contract<T>), dim3(n), dim3(256), 0, st, sa, grid, out);
+  const size_t clds = (size_t)c->P * c->P * c->P * (3 * sizeof(T) + sizeof(int));
+  if (clds <= kSizesContractLds) {
+    set_lds(k_sizes_contract_lds<T>, clds);
+    hipLaunchKernelGGL((k_sizes_contract_lds<T>), dim3(n), dim3(256), clds, st, sa, grid, out);
+  } else {
+    hipLaunchKernelGGL((k_sizes_contract<T>), dim3(n), dim3(256), 0, st, sa, grid, out);
+  }
   HIPCHK(hipGetLastError());
   return MPOL_OK;
 }

@@ -318,6 +318,64 @@ __global__ void __launch_bounds__(256) k_sizes_contract(SizesGridArgs a, const T
   }
 }
 
+// Same contraction with the stencil staged once: phase 1 stores w r (3 T) and
+// the node index of every stencil point in LDS (P^3 (3 sizeof(T) + 4) bytes of
+// dynamic LDS); phase 2 gives each wave whole right-hand sides (rhs = wave,
+// wave + 4, ...), lanes over the stencil, a fixed butterfly per wave.
+template <typename T>
+__global__ void __launch_bounds__(256) k_sizes_contract_lds(SizesGridArgs a, const T* __restrict__ grid,
+                                                            double* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int s = blockIdx.x;
+  if (s >= a.n) return;
+  const int P = a.P, P2 = P * P, P3 = P2 * P;
+  T* wr = reinterpret_cast<T*>(smem_raw);  // [3][P3]
+  int* node = reinterpret_cast<int*>(wr + 3 * P3);
+  const int4 rc = a.rec[s];
+  const double4 b = a.base[s];
+  const T rad = (T)a.rad[s];
+  for (int o = threadIdx.x; o < P3; o += blockDim.x) {
+    const int ox = o % P, oy = (o / P) % P, oz = o / P2;
+    const T r0 = (T)(b.x + (double)(ox - a.half) * a.h[0]);
+    const T r1 = (T)(b.y + (double)(oy - a.half) * a.h[1]);
+    const T r2 = (T)(b.z + (double)(oz - a.half) * a.h[2]);
+    const T d = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
+    const T w = sizes_weight<T>(d, rad, (T)a.xi, (T)a.eta);
+    int gx = rc.x + ox, gy = rc.y + oy, gz = rc.z + oz;
+    gx -= (gx >= a.nx) ? a.nx : 0;
+    gy -= (gy >= a.ny) ? a.ny : 0;
+    gz -= (gz >= a.nz) ? a.nz : 0;
+    wr[o] = w * r0;
+    wr[P3 + o] = w * r1;
+    wr[2 * P3 + o] = w * r2;
+    node[o] = (gz * a.ny + gy) * a.nx + gx;
+  }
+  __syncthreads();
+  const size_t plane = (size_t)a.nx * a.ny * a.nz;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  for (int rhs = wave; rhs < a.nrhs; rhs += nwave) {
+    const T* g = grid + rhs * plane;
+    T acc0 = 0, acc1 = 0, acc2 = 0;
+    for (int o = lane; o < P3; o += 64) {
+      const T v = g[node[o]];
+      acc0 += v * wr[o];
+      acc1 += v * wr[P3 + o];
+      acc2 += v * wr[2 * P3 + o];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      acc0 += __shfl_xor(acc0, off);
+      acc1 += __shfl_xor(acc1, off);
+      acc2 += __shfl_xor(acc2, off);
+    }
+    if (lane == 0) {
+      double* o3 = out + ((size_t)rhs * a.n + rc.w) * 3;
+      o3[0] += (double)acc0 * a.hvol;
+      o3[1] += (double)acc1 * a.hvol;
+      o3[2] += (double)acc2 * a.hvol;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // x-axis transforms of real grids (scalar field, tiles of ty*tz rows)
 // ---------------------------------------------------------------------------

@@ -36,6 +36,10 @@ def _load():
         lib.fcc_mc.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_uint64, ctypes.c_int, dp, dp]
         lib.min_distance.restype = ctypes.c_double
         lib.min_distance.argtypes = [ctypes.c_int, dp, ctypes.c_double]
+        lib.poly_pack.restype = ctypes.c_double
+        lib.poly_pack.argtypes = [ctypes.c_int, dp, ctypes.c_double, ctypes.c_uint64, ctypes.c_int, dp, dp]
+        lib.poly_min_ratio.restype = ctypes.c_double
+        lib.poly_min_ratio.argtypes = [ctypes.c_int, dp, dp, ctypes.c_double]
         _lib = lib
     return _lib
 
@@ -55,6 +59,54 @@ def hard_spheres(n: int, phi: float, seed: int, sweeps: int = 200):
 def min_distance(x, box) -> float:
     x = np.ascontiguousarray(x, dtype=np.float64)
     return _load().min_distance(x.shape[0], x.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), float(box[0]))
+
+
+def polydisperse(n: int, phi: float, seed: int, sigma: float = 0.2, max_iter: int = 20000):
+    """(x (n,3), a (n,), box (3,)): n non-overlapping spheres with log-normal
+    radii (mean 1, log-standard deviation sigma) at volume fraction phi in a
+    cubic periodic box (uniform random centres, then pairwise overlap
+    removal)."""
+    rng = np.random.default_rng(seed)
+    a = np.exp(rng.normal(-0.5 * sigma * sigma, sigma, n))
+    x = np.zeros((n, 3))
+    box = np.zeros(3)
+    dp = ctypes.POINTER(ctypes.c_double)
+    it = _load().poly_pack(int(n), a.ctypes.data_as(dp), float(phi), int(seed), int(max_iter),
+                           x.ctypes.data_as(dp), box.ctypes.data_as(dp))
+    if it < 0:
+        raise ValueError("cannot build a polydisperse hard-sphere configuration at this volume fraction")
+    return x, a, box
+
+
+def min_gap_ratio(x, a, box) -> float:
+    """Smallest d_ij / (a_i + a_j) over all pairs (>= 1: no overlaps)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    dp = ctypes.POINTER(ctypes.c_double)
+    return _load().poly_min_ratio(x.shape[0], x.ctypes.data_as(dp), a.ctypes.data_as(dp), float(box[0]))
+
+
+def heterogeneous_set(count: int = 256, seed: int = 7, n_range=(500, 8000), phi_range=(0.1, 0.45),
+                      sigma: float = 0.2, n_eps: int = 16, eps_range=(0.1, 100.0)):
+    """Independent polydisperse configurations: N log-uniform in n_range,
+    log-normal radii (mean 1, sigma), phi uniform in phi_range, and an
+    n_eps-point permittivity-ratio spectrum log-spaced over eps_range.
+    Returns (configs [{"x", "a", "box", "phi", "seed"}], eps (n_eps,))."""
+    from concurrent.futures import ThreadPoolExecutor
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]), count)).astype(np.int64)
+    phis = rng.uniform(phi_range[0], phi_range[1], count)
+
+    def one(k):
+        s = int(seed) * 100003 + k
+        x, a, box = polydisperse(int(ns[k]), float(phis[k]), s, sigma)
+        return {"x": x, "a": a, "box": box, "phi": float(phis[k]), "seed": s}
+
+    _load()
+    with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:  # the packer releases the GIL
+        configs = list(ex.map(one, range(count)))
+    eps = np.logspace(np.log10(eps_range[0]), np.log10(eps_range[1]), n_eps)
+    return configs, eps
 
 
 def rsa(n: int, phi: float, seed: int):

@@ -11,7 +11,7 @@
 #include <stdlib.h>
 #include <string.h>
 
-static uint64_t rng_state;
+static __thread uint64_t rng_state; /* per thread: configurations can be built concurrently */
 static inline uint64_t next_u64(void) { /* splitmix64 */
   uint64_t z = (rng_state += 0x9E3779B97F4A7C15ULL);
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
@@ -179,4 +179,137 @@ double min_distance(int n, const double* x, double L) {
   }
   free(c.head); free(c.next); free(c.prev); free(c.cell);
   return sqrt(best);
+}
+
+/* smallest d_ij / (a_i + a_j) over the pairs in neighbouring cells */
+static double pair_ratio_min(const Cells* c, const double* x, const double* a, int n, double L) {
+  double best = 1e300;
+  for (int i = 0; i < n; ++i) {
+    int k = c->cell[i];
+    int ix = k % c->nc, iy = (k / c->nc) % c->nc, iz = k / (c->nc * c->nc);
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          int jx = (ix + dx + c->nc) % c->nc, jy = (iy + dy + c->nc) % c->nc, jz = (iz + dz + c->nc) % c->nc;
+          for (int j = c->head[jx + c->nc * (jy + c->nc * jz)]; j >= 0; j = c->next[j]) {
+            if (j <= i) continue;
+            double d2 = 0.0;
+            for (int q = 0; q < 3; ++q) {
+              double d = x[3 * i + q] - x[3 * j + q];
+              d -= L * rint(d / L);
+              d2 += d * d;
+            }
+            double r = sqrt(d2) / (a[i] + a[j]);
+            if (r < best) best = r;
+          }
+        }
+  }
+  return best;
+}
+
+static void cells_init(Cells* c, int n, double L, double amax) {
+  c->nc = (int)floor(L / (2.0 * amax));
+  if (c->nc < 3) c->nc = 3;
+  c->cs = L / c->nc;
+  const int ncell = c->nc * c->nc * c->nc;
+  c->head = (int*)malloc(sizeof(int) * ncell);
+  c->next = (int*)malloc(sizeof(int) * n);
+  c->prev = (int*)malloc(sizeof(int) * n);
+  c->cell = (int*)malloc(sizeof(int) * n);
+}
+static void cells_fill(Cells* c, const double* x, int n) {
+  const int ncell = c->nc * c->nc * c->nc;
+  for (int k = 0; k < ncell; ++k) c->head[k] = -1;
+  for (int i = 0; i < n; ++i) cell_insert(c, i, cell_of(c, &x[3 * i]));
+}
+static void cells_free(Cells* c) { free(c->head); free(c->next); free(c->prev); free(c->cell); }
+
+/* Polydisperse hard spheres: radii a[0..n), cubic periodic box of side L with
+ * sum(4/3 pi a^3) / L^3 = phi.  Uniform random positions, then overlap
+ * removal: every overlapping pair (d < a_i + a_j) is pushed apart along its
+ * axis to the distance (a_i + a_j)(1 + 1e-6), half each, all pairs at once
+ * (Jacobi), until no pair overlaps.  Returns the number of relaxation passes,
+ * or -1 on bad input or when max_iter passes do not remove every overlap. */
+double poly_pack(int n, const double* a, double phi, uint64_t seed, int max_iter, double* x, double* box_out) {
+  if (n < 2 || !(phi > 0.0) || phi > 0.55) return -1.0;
+  double vol = 0.0, amax = 0.0;
+  for (int i = 0; i < n; ++i) {
+    if (!(a[i] > 0.0)) return -1.0;
+    vol += 4.0 / 3.0 * M_PI * a[i] * a[i] * a[i];
+    if (a[i] > amax) amax = a[i];
+  }
+  const double L = cbrt(vol / phi);
+  if (L < 4.0 * amax) return -1.0; /* contact distance must stay below L/2 */
+  rng_state = seed * 0x2545F4914F6CDD1DULL + 1;
+  for (int i = 0; i < 3 * n; ++i) x[i] = uniform() * L;
+  Cells c;
+  cells_init(&c, n, L, amax);
+  double* dx = (double*)malloc(sizeof(double) * 3 * n);
+  int it = 0;
+  for (; it < max_iter; ++it) {
+    cells_fill(&c, x, n);
+    memset(dx, 0, sizeof(double) * 3 * n);
+    int any = 0;
+    for (int i = 0; i < n; ++i) {
+      int k = c.cell[i];
+      int ix = k % c.nc, iy = (k / c.nc) % c.nc, iz = k / (c.nc * c.nc);
+      for (int ddz = -1; ddz <= 1; ++ddz)
+        for (int ddy = -1; ddy <= 1; ++ddy)
+          for (int ddx = -1; ddx <= 1; ++ddx) {
+            int jx = (ix + ddx + c.nc) % c.nc, jy = (iy + ddy + c.nc) % c.nc, jz = (iz + ddz + c.nc) % c.nc;
+            for (int j = c.head[jx + c.nc * (jy + c.nc * jz)]; j >= 0; j = c.next[j]) {
+              if (j <= i) continue;
+              double r[3], d2 = 0.0;
+              for (int q = 0; q < 3; ++q) {
+                r[q] = x[3 * i + q] - x[3 * j + q];
+                r[q] -= L * rint(r[q] / L);
+                d2 += r[q] * r[q];
+              }
+              const double contact = a[i] + a[j], target = contact * (1.0 + 1e-6);
+              if (d2 >= contact * contact) continue;
+              any = 1;
+              double d = sqrt(d2);
+              double u[3];
+              if (d > 1e-12) {
+                for (int q = 0; q < 3; ++q) u[q] = r[q] / d;
+              } else { /* coincident centres: a random axis */
+                double z = 2.0 * uniform() - 1.0, t = 2.0 * M_PI * uniform(), w = sqrt(1.0 - z * z);
+                u[0] = w * cos(t); u[1] = w * sin(t); u[2] = z;
+              }
+              const double push = 0.5 * (target - d) * 1.0001;
+              for (int q = 0; q < 3; ++q) {
+                dx[3 * i + q] += push * u[q];
+                dx[3 * j + q] -= push * u[q];
+              }
+            }
+          }
+    }
+    if (!any) break;
+    for (int i = 0; i < 3 * n; ++i) {
+      x[i] += dx[i];
+      x[i] -= L * floor(x[i] / L);
+    }
+  }
+  free(dx);
+  cells_free(&c);
+  box_out[0] = box_out[1] = box_out[2] = L;
+  return it < max_iter ? (double)it : -1.0;
+}
+
+/* smallest d_ij / (a_i + a_j) over all pairs (1e300 if none within the cell
+ * range); >= 1 means no overlaps */
+double poly_min_ratio(int n, const double* x, const double* a, double L) {
+  if (n < 2) return 1e300;
+  double amax = 0.0;
+  for (int i = 0; i < n; ++i)
+    if (a[i] > amax) amax = a[i];
+  Cells c;
+  cells_init(&c, n, L, amax);
+  double* w = (double*)malloc(sizeof(double) * 3 * n);
+  for (int i = 0; i < 3 * n; ++i) w[i] = x[i] - L * floor(x[i] / L);
+  cells_fill(&c, w, n);
+  double r = pair_ratio_min(&c, w, a, n, L);
+  free(w);
+  cells_free(&c);
+  return r;
 }
