@@ -136,11 +136,12 @@ int mpol_matvec(mpol_ctx* ctx, const double* m_dev, double* out_dev, int32_t nrh
                 double slab_factor);
 
 /* Diagnostic timing of mpol_matvec (no counterpart in the reference): runs it
- * `repeats` times and returns in stage_ms (host, 7 entries) the mean
+ * `repeats` times and returns in stage_ms (host, 8 entries) the mean
  * milliseconds of each stage measured with HIP events on the context stream:
  * [0] slab sum, [1] real-space sum, [2] spread + forward x-FFT,
  * [3] forward y-FFT, [4] z-FFT + k-space scale + inverse z-FFT,
- * [5] inverse y-FFT, [6] inverse x-FFT + contraction. */
+ * [5] inverse y-FFT, [6] inverse x-FFT + contraction, [7] per-particle sum of
+ * the contraction's tile partials (matrix-core engine; 0 otherwise). */
 int mpol_matvec_timed(mpol_ctx* ctx, const double* m_dev, double* out_dev, int32_t nrhs, int32_t precision,
                       double slab_factor, int32_t repeats, double* stage_ms_host);
 

@@ -338,7 +338,7 @@ const void* base_ptr(mpol_ctx* c) {
 // (dipole operator: hr <- real space, hslab <- slab term, out <- wave space)
 // or NC = 8 (dipole + quadrupole operator: hr <- real space, out <- wave
 // space).  Stage boundaries recorded when `ev` is given (kStages + 1 events).
-constexpr int kStages = 7;
+constexpr int kStages = 8;
 constexpr int kZsumBlocks = 64;
 enum { kOpDipole = 3, kOpQuad = 8, kOpSizes = 103 };
 inline int unknowns_per_particle(int kind) { return kind == kOpQuad ? 8 : 3; }
@@ -430,7 +430,9 @@ TileCfg mfma_tile_cfg(const mpol_ctx* c, bool& ok) {
   apply_tile_tuning<NC>(tn, t);
   t.ncl = (NC * t.rows + 1) / 2;
   t.lds = mfma_tile_lds<NC>(nx, cs, c->P, t.ncl, t.lstride, t.ch, t.rows);
-  if (t.lds > kLdsMax || (t.rows != 4 && t.rows != 8)) ok = false;
+  // 4 or 8 rows; 16 rows (4 x 4 tiles) for the dipole operator only
+  const bool rows_ok = t.rows == 4 || t.rows == 8 || (NC == 3 && t.rows == 16 && t.ty == 4 && t.tz == 4);
+  if (t.lds > kLdsMax || !rows_ok || t.ty > 4 || t.tz > 4) ok = false;
   // candidate slots of the image are int32
   const double redund = (double)(c->P + t.ty) / t.ty * (double)(c->P + t.tz) / t.tz;
   if ((double)c->n * redund >= 2.0e9) ok = false;
@@ -477,6 +479,22 @@ int ensure_tile_image(mpol_ctx* c, const TileArgs& ta, TileImage& im) {
   im.ch = ta.ch;
   im.total = total;
   return MPOL_OK;
+}
+
+// 16-row (4 x 4) tiles of the matrix-core spread / contraction: instantiated
+// for the dipole operator only (NC = 3: three column blocks of 16).
+template <typename T, int N, int NC>
+void launch_mfma16(bool spread, dim3 grid, size_t lds, hipStream_t st, const TileArgs& ta, const ImgArgs& im,
+                   cplx<T>* spec) {
+  if constexpr (NC == 3) {
+    if (spread) {
+      set_lds(k_spread_mfma_xfft<T, N, NC, 16>, lds);
+      hipLaunchKernelGGL((k_spread_mfma_xfft<T, N, NC, 16>), grid, dim3(256), lds, st, ta, im, spec);
+    } else {
+      set_lds(k_xifft_contract_mfma<T, N, NC, 16>, lds);
+      hipLaunchKernelGGL((k_xifft_contract_mfma<T, N, NC, 16>), grid, dim3(256), lds, st, ta, im, spec);
+    }
+  }
 }
 
 template <typename T, int NC>
@@ -570,7 +588,9 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   if (use_mfma) {
 #define LAUNCH_SPREAD_MFMA(N, ...)                                                              \
   do {                                                                                          \
-    if (tm.rows == 8) {                                                                         \
+    if (tm.rows == 16) {                                                                        \
+      launch_mfma16<T, N, NC>(true, mgrid, tm.lds, st, tma, ima, spec);                         \
+    } else if (tm.rows == 8) {                                                                  \
       set_lds(k_spread_mfma_xfft<T, N, NC, 8>, tm.lds);                                         \
       hipLaunchKernelGGL((k_spread_mfma_xfft<T, N, NC, 8>), mgrid, dim3(256), tm.lds, st, tma, ima, spec); \
     } else {                                                                                    \
@@ -693,7 +713,9 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   } while (0)
 #define LAUNCH_CONTRACT_MFMA(N, ...)                                                            \
   do {                                                                                          \
-    if (tm.rows == 8) {                                                                         \
+    if (tm.rows == 16) {                                                                        \
+      launch_mfma16<T, N, NC>(false, mgrid, tm.lds, st, tma, ima, spec);                        \
+    } else if (tm.rows == 8) {                                                                  \
       set_lds(k_xifft_contract_mfma<T, N, NC, 8>, tm.lds);                                      \
       hipLaunchKernelGGL((k_xifft_contract_mfma<T, N, NC, 8>), mgrid, dim3(256), tm.lds, st, tma, ima, spec); \
     } else {                                                                                    \
@@ -703,14 +725,16 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   } while (0)
   if (use_mfma) {
     DISPATCH_CT(c->ng[0], LAUNCH_CONTRACT_MFMA, 0);
+    if (ev) HIPCHK(hipEventRecord(ev[7], st));
     const TileImage& im = c->img[sizeof(T) == 8 ? 1 : 0];
     hipLaunchKernelGGL((k_contract_reduce<T, NC>), dim3(cdiv((size_t)n * 16, 256), nrhs), dim3(256), 0, st, n, ima.total,
                        im.poff.p, im.pslot.p, c->perm.p, reinterpret_cast<const T*>(ima.part), ta.hvol, out);
   } else {
     DISPATCH_FFT(c->ng[0], ta.plan.e, LAUNCH_CONTRACT, 0);
+    if (ev) HIPCHK(hipEventRecord(ev[7], st));
   }
 #undef LAUNCH_CONTRACT_MFMA
-  if (ev) HIPCHK(hipEventRecord(ev[7], st));
+  if (ev) HIPCHK(hipEventRecord(ev[8], st));
 #undef LAUNCH_FFTY
 #undef LAUNCH_FFTZ
 #undef LAUNCH_CONTRACT

@@ -293,7 +293,7 @@ __global__ void __launch_bounds__(256) k_spread_mfma_xfft(TileArgs a, ImgArgs im
   extern __shared__ __align__(16) unsigned char smem_raw[];
   using MF = Mfma<T>;
   using acc_t = typename MF::acc_t;
-  static_assert(R == 4 || R == 8, "tile rows");
+  static_assert(R == 4 || R == 8 || R == 16, "tile rows");
   constexpr int NCOL = NC * R, NB = (NCOL + 15) / 16, NXB = N / 16, KX = N / 2 + 1;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* buf = tw + N;
@@ -402,7 +402,7 @@ __global__ void __launch_bounds__(256) k_xifft_contract_mfma(TileArgs a, ImgArgs
   extern __shared__ __align__(16) unsigned char smem_raw[];
   using MF = Mfma<T>;
   using acc_t = typename MF::acc_t;
-  static_assert(R == 4 || R == 8, "tile rows");
+  static_assert(R == 4 || R == 8 || R == 16, "tile rows");
   constexpr int NCOL = NC * R, NB = (NCOL + 15) / 16, KX = N / 2 + 1;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* buf = tw + N;

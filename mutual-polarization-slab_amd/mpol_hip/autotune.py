@@ -43,6 +43,8 @@ TILE_OPTIONS = (
     {"engine": 1, "tile_ty": 4, "tile_tz": 2, "chunk": 256},
     {"engine": 1, "tile_ty": 4, "tile_tz": 2, "chunk": 64},
     {"engine": 1, "tile_ty": 2, "tile_tz": 2, "chunk": 128},
+    {"engine": 1, "tile_ty": 4, "tile_tz": 4, "chunk": 128},
+    {"engine": 1, "tile_ty": 4, "tile_tz": 4, "chunk": 64},
     {"engine": 2},
 )
 Y_OPTIONS = ({"y_cols": 0}, {"y_cols": 8}, {"y_cols": 16}, {"y_cols": 32})
@@ -86,7 +88,7 @@ def tune_launch(ctx, precision=FP32, nrhs=1, repeats=5, seed=0, force=False):
         results.sort(key=lambda r: r[0])
         best = results[0][1]
 
-    score(TILE_OPTIONS, ("spread_xfft", "xifft_contract"))
+    score(TILE_OPTIONS, ("spread_xfft", "xifft_contract", "contract_reduce"))
     score(Y_OPTIONS, ("fft_y_fwd", "fft_y_inv"))
     score(Z_OPTIONS, ("fft_z_scale",))
     ctx.set_tuning(precision=precision, **best)

@@ -319,7 +319,8 @@ class Context:
     def sizes_solve(self, b, x0, tol, restart=10, maxit=100, mixed=False, max_refine=8):
         return self._solve(self._lib.mpol_sizes_solve, 3, b, x0, tol, restart, maxit, mixed, max_refine, 0.0)
 
-    STAGES = ("slab_sum", "real_space", "spread_xfft", "fft_y_fwd", "fft_z_scale", "fft_y_inv", "xifft_contract")
+    STAGES = ("slab_sum", "real_space", "spread_xfft", "fft_y_fwd", "fft_z_scale", "fft_y_inv", "xifft_contract",
+              "contract_reduce")
 
     def matvec_timed(self, m, precision=FP64, slab_factor=1.0, repeats=10):
         """Mean milliseconds of each operator stage (HIP events on the launch stream)."""
