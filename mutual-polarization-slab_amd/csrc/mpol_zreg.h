@@ -41,8 +41,19 @@ __device__ __forceinline__ void dipole_scale_point(const ColArgs& a, const Scale
   const bool nyq_z = (a.nz % 2 == 0) && (2 * mz == -a.nz);
   const double k0 = mx * sp.twopi_over_l[0], k1 = my * sp.twopi_over_l[1], k2 = mz * sp.twopi_over_l[2];
   const double kk = (k0 * k0 + k1 * k1) + k2 * k2;
-  const double ikn = 1.0 / sqrt(kk);
-  const T u0 = (T)(k0 * ikn), u1 = (T)(k1 * ikn), u2 = (T)(k2 * ikn);
+  T u0, u1, u2;
+  if constexpr (sizeof(T) == 4) {
+    // FP32 operator: hardware reciprocal square root (about 1 ulp)
+    const float ikn = __builtin_amdgcn_rsqf((float)kk);
+    u0 = (float)k0 * ikn;
+    u1 = (float)k1 * ikn;
+    u2 = (float)k2 * ikn;
+  } else {
+    const double ikn = 1.0 / sqrt(kk);
+    u0 = k0 * ikn;
+    u1 = k1 * ikn;
+    u2 = k2 * ikn;
+  }
   double cf;
   if (sp.jt) cf = sp.jt[mx * mx + my * my + mz * mz] * (sp.ex[kx] * sp.ey[y] * sp.ez[kz]) * sp.inv_g;
   else cf = kspace_coef(sp, k0, k1, k2) * sp.inv_g;
