@@ -91,7 +91,7 @@ def heterogeneous_sample(count=64, plain=8, workers=4):
     from mpol_hip import batch
     from workloads.heterogeneous import plain_fp64
     configs, eps = workloads.heterogeneous_set(count=count, seed=7)
-    grids, _, tune_s = batch.tune_grids(configs, eps, tol=1e-3, workers=workers)
+    grids, _, tune_s = batch.tune_grids(configs, eps, tol=1e-3)
     res = batch.solve_configurations(configs, eps, tol=1e-3, gmres_tol=1e-6, precision="mixed", workers=workers,
                                      tune=False, grids=grids)
     idx = list(range(0, count, count // plain))[:plain]
@@ -138,6 +138,7 @@ def main():
     ap.add_argument("--sweeps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-heterogeneous", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path",
@@ -285,6 +286,11 @@ def main():
                "host_cpus": len(os.sched_getaffinity(0))}
         del mv_per_solve
 
+    adj = None
+    if not args.no_adjoint:
+        log("adjoint forces and dU/dbeta vs central finite differences (32 particles, h = 1e-4)")
+        from workloads import adjoint_check
+        adj, _ = adjoint_check.run(particles=32, h=1e-4, solve_tol=1e-12, sweeps=args.sweeps, verbose=False)
     het = None
     if not args.no_heterogeneous:
         log("heterogeneous polydisperse configurations: batched mixed vs one right-hand side at a time in FP64")
@@ -326,6 +332,7 @@ def main():
         "geometry_s": round(geo_s, 3),
         "launch_knobs": knobs,
         "batched_heterogeneous": het,
+        "adjoint_vs_finite_differences": adj,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -49,11 +49,15 @@ def ewald_setup(box, xi, errortol):
 
 
 class _Setup:
-    def __init__(self, box, xi, errortol, ngrid=None, P=None, tables=None, slab_factor=0.0, restart=10, maxit=100):
+    def __init__(self, box, xi, errortol, ngrid=None, P=None, tables=None, slab_factor=0.0, restart=10, maxit=100,
+                 solve_tol=None, rc=None):
         self.box = np.asarray(box, dtype=np.float64).reshape(3)
         self.xi = float(xi)
-        self.tol = float(errortol)
-        rc, ng, Pd, eta = ewald_setup(self.box, self.xi, self.tol)
+        # errortol sets the Ewald parameters (as in the reference) and, unless
+        # solve_tol is given, the GMRES tolerance
+        self.tol = float(errortol if solve_tol is None else solve_tol)
+        rc_rule, ng, Pd, eta = ewald_setup(self.box, self.xi, float(errortol))
+        rc = rc_rule if rc is None else float(rc)
         if ngrid is not None:
             ng = np.asarray(ngrid, dtype=np.int64).reshape(3)
         if P is not None:
@@ -113,9 +117,13 @@ def _as_dev(a, device):
 
 
 def solve_dipoles(x, eps, box, xi, errortol=1e-10, H0=(0.0, 0.0, 1.0), slab_factor=0.0, ngrid=None, P=None,
-                  tables=None):
+                  tables=None, solve_tol=None, rc=None):
     """Induced dipoles (nrhs, N, 3) for unit fields H0 ((3,) or (nrhs, 3)),
-    differentiable with respect to x and eps when they require grad."""
+    differentiable with respect to x and eps when they require grad.
+    errortol derives rc, the grid, P and eta by the reference's rule (ngrid,
+    P and rc, when given, replace the derived ones); solve_tol (default
+    errortol) is the GMRES relative tolerance of the forward and adjoint
+    solves."""
     dev = x.device if isinstance(x, torch.Tensor) and x.is_cuda else require_device()
     xt = _as_dev(x, dev)
     n = xt.shape[0]
@@ -124,24 +132,24 @@ def solve_dipoles(x, eps, box, xi, errortol=1e-10, H0=(0.0, 0.0, 1.0), slab_fact
         et = et.expand(n)
     H = torch.as_tensor(np.asarray(H0, dtype=np.float64).reshape(-1, 3), device=dev)
     b = H[:, None, :].expand(H.shape[0], n, 3).contiguous()
-    setup = _Setup(box, xi, errortol, ngrid, P, tables, slab_factor)
+    setup = _Setup(box, xi, errortol, ngrid, P, tables, slab_factor, solve_tol=solve_tol, rc=rc)
     return PolarizationSolve.apply(xt, et.contiguous(), b, setup)
 
 
 def polarization_energy(x, eps, box, xi, errortol=1e-10, H0=(0.0, 0.0, 1.0), slab_factor=0.0, ngrid=None, P=None,
-                        tables=None):
+                        tables=None, solve_tol=None, rc=None):
     """U = -1/2 H0 . sum_i p_i for the dipoles induced by the unit field H0
     (a differentiable scalar tensor)."""
-    p = solve_dipoles(x, eps, box, xi, errortol, H0, slab_factor, ngrid, P, tables)
+    p = solve_dipoles(x, eps, box, xi, errortol, H0, slab_factor, ngrid, P, tables, solve_tol, rc)
     H = torch.as_tensor(np.asarray(H0, dtype=np.float64).reshape(1, 1, 3), device=p.device)
     return -0.5 * (p * H).sum()
 
 
 def polarization_forces(x, eps, box, xi, errortol=1e-10, H0=(0.0, 0.0, 1.0), slab_factor=0.0, ngrid=None, P=None,
-                        tables=None):
+                        tables=None, solve_tol=None, rc=None):
     """(F = -dU/dx (N,3), U) as numpy arrays."""
     dev = x.device if isinstance(x, torch.Tensor) and x.is_cuda else require_device()
     xt = _as_dev(x, dev).detach().clone().requires_grad_(True)
-    U = polarization_energy(xt, eps, box, xi, errortol, H0, slab_factor, ngrid, P, tables)
+    U = polarization_energy(xt, eps, box, xi, errortol, H0, slab_factor, ngrid, P, tables, solve_tol, rc)
     (g,) = torch.autograd.grad(U, xt)
     return (-g).cpu().numpy(), float(U.item())

@@ -85,6 +85,10 @@ int read_doubles(mpol_ctx* c, const double* dev, int cnt, double* host) {
 // Krylov dimension of the FP32 correction solves of the mixed scheme (at
 // least the caller's restart): one cycle usually reaches the inner tolerance.
 constexpr int kInnerRestart = 32;
+// Smallest relative tolerance of a correction solve, and its restart cycles
+// (a correction that stalls returns to the FP64 residual, which decides).
+constexpr double kInnerFloor = 5e-7;
+constexpr int kInnerCycles = 4;
 
 // Restarted GMRES.  inner = true is the correction solve of the mixed scheme:
 // x starts at zero (the first residual is b, no operator application) and
@@ -621,15 +625,17 @@ int solve_impl(mpol_ctx* c, int kind, const double* b, const double* x0, double*
       } else if (conv == nrhs || ref >= o->max_refine) {
         break;
       }
-      // inner tolerance relative to the current residual
+      // inner tolerance relative to the current residual, not below
+      // kInnerFloor: the FP32 operator's rounding (~1e-7 relative) limits how
+      // far its own residual estimate can usefully fall
       double tin = 0.3 * o->tol * ratio;
-      tin = std::min(0.1, std::max(tin, 2e-6));
+      tin = std::min(0.1, std::max(tin, kInnerFloor));
       // residual -> rhs of the correction solve (copy into gx), correction into gd
       hipLaunchKernelGGL(k_copy, dim3(cdiv(tot, 256)), dim3(256), 0, st, c->gd.p, c->gx.p, tot);
       HIPCHK(hipMemsetAsync(c->gd.p, 0, tot * sizeof(double), st));
       int64_t nmv = 0;
-      CHECK_STATUS(gmres_core(c, kind, c->gx.p, c->gd.p, nrhs, tin, std::max(o->restart, kInnerRestart), o->maxit,
-                              MPOL_FP32, slab, &nmv, nullptr, nullptr, true));
+      CHECK_STATUS(gmres_core(c, kind, c->gx.p, c->gd.p, nrhs, tin, std::max(o->restart, kInnerRestart),
+                              std::min(o->maxit, kInnerCycles), MPOL_FP32, slab, &nmv, nullptr, nullptr, true));
       nmv32 += nmv;
       hipLaunchKernelGGL(k_axpy_all, dim3(cdiv(tot, 256)), dim3(256), 0, st, 1.0, c->gd.p, x, tot);
     }

@@ -239,9 +239,11 @@ def solve_configurations(configs, eps, fields=None, *, tol=1e-3, gmres_tol=None,
         seconds=seconds, rhs=count * nk * nf, tune_seconds=tune_s, report=[r[3] for r in results])
 
 
-def tune_grids(configs, eps, fields=None, *, tol=1e-3, workers=4, sizes=GRID_SIZES, repeats=2, device=None):
+def tune_grids(configs, eps, fields=None, *, tol=1e-3, workers=1, sizes=GRID_SIZES, repeats=2, device=None):
     """Grid size per configuration (fastest batched FP32 operator among the
     candidates), without solving: (grids (count,), timings [(G, ms)] per
-    configuration, seconds).  Pass grids to solve_configurations."""
+    configuration, seconds).  One worker by default, so that the timings are
+    not shared with other configurations' work.  Pass grids to
+    solve_configurations."""
     return solve_configurations(configs, eps, fields, tol=tol, workers=workers, tune=True, sizes=sizes,
                                 repeats=repeats, device=device, _tune_only=True)

@@ -86,7 +86,7 @@ def main(argv=None):
     print(f"[heterogeneous] {len(configs)} configurations packed in {pack_s:.1f} s", file=sys.stderr, flush=True)
     grids, tune_s = None, 0.0
     if not args.no_tune:
-        grids, _, tune_s = batch.tune_grids(configs, eps, tol=args.tol, workers=args.workers)
+        grids, _, tune_s = batch.tune_grids(configs, eps, tol=args.tol)
         print(f"[heterogeneous] grids chosen in {tune_s:.1f} s", file=sys.stderr, flush=True)
     res = batch.solve_configurations(configs, eps, tol=args.tol, gmres_tol=args.gmres_tol, precision="mixed",
                                      workers=args.workers, tune=False, grids=grids)
