@@ -139,6 +139,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-heterogeneous", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-other-workloads", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path",
@@ -291,6 +292,11 @@ def main():
         log("adjoint forces and dU/dbeta vs central finite differences (32 particles, h = 1e-4)")
         from workloads import adjoint_check
         adj, _ = adjoint_check.run(particles=32, h=1e-4, solve_tol=1e-12, sweeps=args.sweeps, verbose=False)
+    others = None
+    if not args.no_other_workloads:
+        log("other full-size workloads: three-field tensor (N=1e4, 128^3), dipole+quadrupole (N=2e4, 160^3)")
+        from workloads import configs as other_configs
+        others = {"tensor": other_configs.tensor(sweeps=args.sweeps), "quadrupole": other_configs.quadrupole()}
     het = None
     if not args.no_heterogeneous:
         log("heterogeneous polydisperse configurations: batched mixed vs one right-hand side at a time in FP64")
@@ -333,6 +339,7 @@ def main():
         "launch_knobs": knobs,
         "batched_heterogeneous": het,
         "adjoint_vs_finite_differences": adj,
+        "other_workloads": others,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
