@@ -427,12 +427,20 @@ TileCfg mfma_tile_cfg(const mpol_ctx* c, bool& ok) {
       if (found) break;
     }
   }
+  const TileCfg chosen = t;
   apply_tile_tuning<NC>(tn, t);
+  // 4 or 8 rows; 16 rows (4 x 4 tiles) for the dipole operator only: a tile
+  // shape tuned on the dipole operator that this operator lacks keeps the
+  // default shape (with the tuned chunk)
+  const bool rows_ok = t.rows == 4 || t.rows == 8 || (NC == 3 && t.rows == 16 && t.ty == 4 && t.tz == 4);
+  if (!rows_ok || t.ty > 4 || t.tz > 4) {
+    const int ch = t.ch;
+    t = chosen;
+    t.ch = ch;
+  }
   t.ncl = (NC * t.rows + 1) / 2;
   t.lds = mfma_tile_lds<NC>(nx, cs, c->P, t.ncl, t.lstride, t.ch, t.rows);
-  // 4 or 8 rows; 16 rows (4 x 4 tiles) for the dipole operator only
-  const bool rows_ok = t.rows == 4 || t.rows == 8 || (NC == 3 && t.rows == 16 && t.ty == 4 && t.tz == 4);
-  if (t.lds > kLdsMax || !rows_ok || t.ty > 4 || t.tz > 4) ok = false;
+  if (t.lds > kLdsMax) ok = false;
   // candidate slots of the image are int32
   const double redund = (double)(c->P + t.ty) / t.ty * (double)(c->P + t.tz) / t.tz;
   if ((double)c->n * redund >= 2.0e9) ok = false;

@@ -277,3 +277,16 @@ def test_quad_spread_contract_engines_agree(gpu, tile, chunk):
     assert _rel(out[1, 64], out[2, 64]) < 1e-13
     assert _rel(out[1, 32], out[2, 64]) < 2e-6
     assert _rel(out[1, 64][0], _oracle(c, m[0]).real) < 1e-12
+
+
+def test_quad_operator_keeps_default_tile_for_dipole_only_shape(gpu):
+    """16-row tiles exist for the dipole operator only: a context tuned to
+    them (the launch tuner times the dipole operator) still applies the
+    quadrupole operator on the matrix cores with its default tile shape."""
+    c = Case(gpu, n=60, phi=0.05, ngrid=[64, 40, 36], P=9)
+    m = _moments(c.n, 9, k=2)
+    c.ctx.set_tuning(engine=1)
+    ref = c.ctx.quad_matvec(m).cpu().numpy()
+    c.ctx.set_tuning(tile_ty=4, tile_tz=4, chunk=64, engine=1)
+    out = c.ctx.quad_matvec(m).cpu().numpy()
+    assert _rel(out, ref) < 1e-13
