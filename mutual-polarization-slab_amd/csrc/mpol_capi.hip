@@ -79,6 +79,13 @@ int read_doubles(mpol_ctx* c, const double* dev, int cnt, double* host) {
   return MPOL_OK;
 }
 
+// Sets the operator gate of a context for one scope.
+struct GateScope {
+  mpol_ctx* c;
+  GateScope(mpol_ctx* ctx, const int* state) : c(ctx) { c->gate = state; }
+  ~GateScope() { c->gate = nullptr; }
+};
+
 // Restarted GMRES on x (in: initial guess, out: solution) for A x = b.
 // nrhs here counts systems; a system's vector holds all its right-hand sides
 // (kind components per particle, times 2 with complex permittivities).
@@ -156,7 +163,11 @@ int gmres_core(mpol_ctx* c, int kind, const double* b, double* x, int nrhs, doub
     hipLaunchKernelGGL(k_scale_into, vgrid, dim3(256), 0, st, c->gR.p, c->gbeta.p, c->gV.p, n3, c->gstate.p);
     for (int j = 0; j < restart; ++j) {
       double* vj = c->gV.p + (size_t)j * sv;
-      CHECK_STATUS(apply(c, kind, vj, c->gW.p, nrhs, precision, slab));
+      {
+        // systems whose cycle has ended (or that converged) skip the operator
+        GateScope gs(c, c->gstate.p);
+        CHECK_STATUS(apply(c, kind, vj, c->gW.p, nrhs, precision, slab));
+      }
       ++*nmv;
       // classical Gram-Schmidt, twice; each subtraction pass also forms the
       // partial sums of the next reduction (second-pass coefficients, ||w||^2)

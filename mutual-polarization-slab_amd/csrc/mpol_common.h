@@ -64,6 +64,17 @@ __device__ __forceinline__ T group_sum(T v, int R) {
   return v;
 }
 
+// Per-system state of the batched GMRES.
+enum RhsState : int { kDone = 0, kIter = 1, kSolve = 2, kWait = 3 };
+
+// Operator batch entry b (a right-hand side, or a right-hand side's grid
+// component) is skipped when a gate is given and its system, b / gdiv, is
+// not in an Arnoldi step: inside a cycle the systems that have finished it
+// cost no operator work.
+__device__ __forceinline__ bool rhs_idle(const int* gate, int gdiv, int b) {
+  return gate != nullptr && gate[b / gdiv] != kIter;
+}
+
 // Frequency of natural-order FFT index i on an n-point grid: i for
 // i <= floor((n-1)/2), else i - n (even n: index n/2 maps to -n/2).
 __host__ __device__ inline int fft_freq(int i, int n) { return (i <= (n - 1) / 2) ? i : i - n; }

@@ -7,9 +7,9 @@
 
 #include <hip/hip_runtime.h>
 
-namespace mpol {
+#include "mpol_common.h"
 
-enum RhsState : int { kDone = 0, kIter = 1, kSolve = 2, kWait = 3 };
+namespace mpol {
 
 constexpr int kDotBlocks = 256;  // partial-sum blocks per right-hand side (one per CU)
 

@@ -267,6 +267,9 @@ struct mpol_ctx {
   // gmres
   DevBuf<double> gV, gW, gR, gAx, gH, gcs, gsn, gg, gy, gh1, gh2, gpart, gbn, gbeta, grel, grn2, gx, gd;
   DevBuf<int> gstate, gkdim;
+  // while set (inside an Arnoldi step), the per-system GMRES state: operator
+  // work of systems not in kIter is skipped
+  const int* gate = nullptr;
   HostBuf<int> hstate;  // solver state read back each Arnoldi step
   HostBuf<double> hrel, hscr;
   // truncated fixed-dipole forces: cell list
@@ -515,6 +518,8 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   cplx<T>* spec = reinterpret_cast<cplx<T>*>(c->spec.p);
   const int n = c->n;
   hipStream_t st = c->stream;
+  const int* gate = c->gate;
+  const int rps = c->cplx_eps ? 2 : 1;  // operator right-hand sides per system
 
   if (ev) HIPCHK(hipEventRecord(ev[0], st));
   const T* pA;
@@ -535,7 +540,7 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
     const double V = c->box[0] * c->box[1] * c->box[2];
     hipLaunchKernelGGL((k_real_space<T>), dim3(cdiv((size_t)n * 16, 256), nrhs), dim3(256), 0, st, n, nrhs, c->nsets, m, out,
                        c->diag.p, c->cplx_eps ? c->diag_im.p : nullptr, c->rs_off.p, c->rs_j.p, pA, pBA, prh,
-                       c->zsum.p, slab / V, hr != nullptr ? 1 : 0, hr, hslab);
+                       c->zsum.p, slab / V, hr != nullptr ? 1 : 0, hr, hslab, gate, rps);
   } else {
     if (ev) HIPCHK(hipEventRecord(ev[1], st));
     const T* pq;
@@ -544,10 +549,11 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
     hipLaunchKernelGGL((k_real_space_quad<T>), dim3(cdiv((size_t)n * 16, 256), nrhs), dim3(256), 0, st, n, nrhs, c->nsets, m,
                        out, c->diag.p, c->diag_q.p, c->cplx_eps ? c->diag_im.p : nullptr,
                        c->cplx_eps ? c->diag_qim.p : nullptr, c->rs_off.p, c->rs_j.p, pA, pBA, prh, pq,
-                       hr != nullptr ? 1 : 0, hr);
+                       hr != nullptr ? 1 : 0, hr, gate, rps);
   }
   HIPALLOC(c->ms, (size_t)nrhs * NC * n);
-  hipLaunchKernelGGL(k_permute_m<NC>, dim3(cdiv(n, 256), nrhs), dim3(256), 0, st, m, c->perm.p, n, c->ms.p);
+  hipLaunchKernelGGL(k_permute_m<NC>, dim3(cdiv(n, 256), nrhs), dim3(256), 0, st, m, c->perm.p, n, c->ms.p, gate,
+                     rps);
   if (ev) HIPCHK(hipEventRecord(ev[2], st));
 
   // tile (x axis) kernels
@@ -567,6 +573,8 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   ta.norm = c->norm;
   ta.tw = tw_ptr<T>(c, 0); ta.plan = c->plan[0];
   ta.hvol = c->h[0] * c->h[1] * c->h[2];
+  ta.gate = gate;
+  ta.gdiv = rps;
   const dim3 tgrid(cdiv(c->ng[1], tc.ty), cdiv(c->ng[2], tc.tz), nrhs);
   if (tc.lds > kLdsMax) return fail(MPOL_EINVAL, "spread/contract tile does not fit in LDS (reduce tile or chunk)");
 
@@ -653,6 +661,8 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   ColArgs ay{};
   ay.nx = c->ng[0]; ay.ny = c->ng[1]; ay.nz = c->ng[2]; ay.kx = c->kx; ay.kxs = c->kxs; ay.tx = cy.tx; ay.lstride = cy.lstride;
   ay.plan = c->plan[1]; ay.tw = tw_ptr<T>(c, 1);
+  ay.gate = gate;
+  ay.gdiv = NC * rps;  // batch index = rhs * NC + component
   const dim3 ygrid(cdiv(c->kx, cy.tx), c->ng[2], NC * nrhs);
   if (cy.lds > kLdsMax) return fail(MPOL_EINVAL, "y-pass columns do not fit in LDS");
 #define LAUNCH_FFTY(E, N, INV)                                                                  \
@@ -670,6 +680,8 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   ColArgs az{};
   az.nx = c->ng[0]; az.ny = c->ng[1]; az.nz = c->ng[2]; az.kx = c->kx; az.kxs = c->kxs; az.tx = cz.tx; az.lstride = cz.lstride;
   az.plan = c->plan[2]; az.tw = tw_ptr<T>(c, 2);
+  az.gate = gate;
+  az.gdiv = rps;
   ScaleParams sp{};
   for (int d = 0; d < 3; ++d) {
     sp.twopi_over_l[d] = 2.0 * kPi / c->box[d];
@@ -736,7 +748,8 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
     if (ev) HIPCHK(hipEventRecord(ev[7], st));
     const TileImage& im = c->img[sizeof(T) == 8 ? 1 : 0];
     hipLaunchKernelGGL((k_contract_reduce<T, NC>), dim3(cdiv((size_t)n * 16, 256), nrhs), dim3(256), 0, st, n, ima.total,
-                       im.poff.p, im.pslot.p, c->perm.p, reinterpret_cast<const T*>(ima.part), ta.hvol, out);
+                       im.poff.p, im.pslot.p, c->perm.p, reinterpret_cast<const T*>(ima.part), ta.hvol, out, gate,
+                       rps);
   } else {
     DISPATCH_FFT(c->ng[0], ta.plan.e, LAUNCH_CONTRACT, 0);
     if (ev) HIPCHK(hipEventRecord(ev[7], st));
@@ -770,6 +783,8 @@ int run_operator_sizes(mpol_ctx* c, const double* m, double* out, double* hr, in
   HIPALLOC(c->ms, (size_t)nrhs * 3 * n);
   cplx<T>* spec = reinterpret_cast<cplx<T>*>(c->spec.p);
   T* grid = reinterpret_cast<T*>(c->sz_grid.p);
+  const int* gate = c->gate;
+  const int rps = c->cplx_eps ? 2 : 1;
   // real space (no slab term: the real-space kernel does not read zsum)
   const T* pA;
   const T* pBA;
@@ -782,11 +797,13 @@ int run_operator_sizes(mpol_ctx* c, const double* m, double* out, double* hr, in
   if (hr) HIPALLOC(c->sz_slab, (size_t)nrhs * 3 * n);
   hipLaunchKernelGGL((k_real_space<T>), dim3(cdiv((size_t)n * 16, 256), nrhs), dim3(256), 0, st, n, nrhs, c->nsets, m, out,
                      c->sz_diag.p, c->cplx_eps ? c->sz_diag_im.p : nullptr, c->rs_off.p, c->rs_j.p, pA, pBA, prh,
-                     c->zsum.p, 0.0, hr != nullptr ? 1 : 0, hr, hr ? c->sz_slab.p : nullptr);
-  hipLaunchKernelGGL(k_permute_m<3>, dim3(cdiv(n, 256), nrhs), dim3(256), 0, st, m, c->perm.p, n, c->ms.p);
+                     c->zsum.p, 0.0, hr != nullptr ? 1 : 0, hr, hr ? c->sz_slab.p : nullptr, gate, rps);
+  hipLaunchKernelGGL(k_permute_m<3>, dim3(cdiv(n, 256), nrhs), dim3(256), 0, st, m, c->perm.p, n, c->ms.p, gate, rps);
   // spread into real grids
   HIPCHK(hipMemsetAsync(grid, 0, (size_t)nrhs * G * sizeof(T), st));
   SizesGridArgs sa{};
+  sa.gate = gate;
+  sa.gdiv = rps;
   sa.nx = c->ng[0]; sa.ny = c->ng[1]; sa.nz = c->ng[2];
   sa.P = c->P; sa.half = c->half; sa.n = n; sa.nrhs = nrhs;
   for (int d = 0; d < 3; ++d) sa.h[d] = c->h[d];
@@ -804,6 +821,8 @@ int run_operator_sizes(mpol_ctx* c, const double* m, double* out, double* hr, in
   ta.nx = c->ng[0]; ta.ny = c->ng[1]; ta.nz = c->ng[2]; ta.kx = c->kx; ta.kxs = c->kxs;
   ta.ty = tr.ty; ta.tz = tr.tz; ta.rows = tr.rows; ta.ncl = tr.ncl; ta.lstride = tr.lstride;
   ta.tw = tw_ptr<T>(c, 0); ta.plan = c->plan[0];
+  ta.gate = gate;
+  ta.gdiv = rps;
   const size_t rlds = ((size_t)c->ng[0] + (size_t)tr.ncl * tr.lstride * (gen_x ? 2 : 1)) * cs;
   if (rlds > kLdsMax) return fail(MPOL_EINVAL, "row transform tile does not fit in LDS");
   const dim3 tgrid(cdiv(c->ng[1], tr.ty), cdiv(c->ng[2], tr.tz), nrhs);
@@ -820,6 +839,8 @@ int run_operator_sizes(mpol_ctx* c, const double* m, double* out, double* hr, in
   ColArgs ay{};
   ay.nx = c->ng[0]; ay.ny = c->ng[1]; ay.nz = c->ng[2]; ay.kx = c->kx; ay.kxs = c->kxs; ay.tx = cy.tx; ay.lstride = cy.lstride;
   ay.plan = c->plan[1]; ay.tw = tw_ptr<T>(c, 1);
+  ay.gate = gate;
+  ay.gdiv = rps;
   const dim3 ygrid(cdiv(c->kx, cy.tx), c->ng[2], nrhs);
   if (cy.lds > kLdsMax) return fail(MPOL_EINVAL, "y-pass columns do not fit in LDS");
 #define LAUNCH_FFTY(E, N, INV)                                                                  \
@@ -833,6 +854,8 @@ int run_operator_sizes(mpol_ctx* c, const double* m, double* out, double* hr, in
   ColArgs az{};
   az.nx = c->ng[0]; az.ny = c->ng[1]; az.nz = c->ng[2]; az.kx = c->kx; az.kxs = c->kxs; az.tx = cz.tx; az.lstride = cz.lstride;
   az.plan = c->plan[2]; az.tw = tw_ptr<T>(c, 2);
+  az.gate = gate;
+  az.gdiv = rps;
   ScaleParams sp{};
   for (int d = 0; d < 3; ++d) {
     sp.twopi_over_l[d] = 2.0 * kPi / c->box[d];

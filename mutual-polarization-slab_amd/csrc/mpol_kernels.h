@@ -138,10 +138,10 @@ static __global__ void k_geo_scatter(GeoParams g, const int* __restrict__ bin_ke
 // quadrupole) into sorted order.
 template <int NC>
 __global__ void k_permute_m(const double* __restrict__ m, const int* __restrict__ perm, int n,
-                            double* __restrict__ ms) {
+                            double* __restrict__ ms, const int* __restrict__ gate, int gdiv) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int rhs = blockIdx.y;
-  if (s >= n) return;
+  if (s >= n || rhs_idle(gate, gdiv, rhs)) return;
   const size_t o = (size_t)rhs * n;
   const int i = perm[s];
 #pragma unroll
@@ -278,10 +278,12 @@ __global__ void __launch_bounds__(256) k_real_space(int n, int nrhs, int nsets, 
                                                     const T* __restrict__ pBA, const T* __restrict__ prh,
                                                     const double* __restrict__ zsum, double slab_over_v,
                                                     int write_diag_only_parts, double* __restrict__ hr_out,
-                                                    double* __restrict__ slab_out) {
+                                                    double* __restrict__ slab_out, const int* __restrict__ gate,
+                                                    int gdiv) {
   const int g = threadIdx.x & 15;
   const int i = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 4);
   const int rhs = blockIdx.y;
+  if (rhs_idle(gate, gdiv, rhs)) return;  // uniform over the block
   const bool valid = i < n && rhs < nrhs;  // every lane takes part in the lane sum
   const int ii = valid ? i : 0;
   const double* mr = m + (size_t)rhs * n * 3;
@@ -377,10 +379,12 @@ __global__ void __launch_bounds__(256) k_real_space_quad(int n, int nrhs, int ns
                                   const double* __restrict__ dgi_s, const double* __restrict__ dgi_q,
                                   const int* __restrict__ off, const int* __restrict__ pj,
                                   const T* __restrict__ pA, const T* __restrict__ pBA, const T* __restrict__ prh,
-                                  const T* __restrict__ pq, int parts, double* __restrict__ hr_out) {
+                                  const T* __restrict__ pq, int parts, double* __restrict__ hr_out,
+                                  const int* __restrict__ gate, int gdiv) {
   const int lane = threadIdx.x & 15;
   const int i = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 4);
   const int rhs = blockIdx.y;
+  if (rhs_idle(gate, gdiv, rhs)) return;  // uniform over the block
   const bool valid = i < n && rhs < nrhs;  // every lane takes part in the lane sum
   const int ii = valid ? i : 0;
   const double* mr = m + (size_t)rhs * n * 8;
@@ -489,6 +493,8 @@ struct TileArgs {
   const void* tw;    // nx twiddles
   FftPlan plan;
   double hvol;       // prod(h), trapezoid factor of the contraction
+  const int* gate;   // optional per-system state (rhs_idle): tiles of idle systems exit
+  int gdiv;          // right-hand sides per system
 };
 
 // Build the candidate particle ranges of a tile into shared arrays.
@@ -687,6 +693,7 @@ __global__ void __launch_bounds__(256) k_spread_xfft(TileArgs a, const double* _
   unsigned char* stage_base = reinterpret_cast<unsigned char*>(buf + a.ncl * a.lstride * (E == 0 && N == 0 ? 2 : 1));
   const Stage<T> st = stage_layout<T, NC>(stage_base, a.ch, a.P, a.rows);
   const int ty0 = blockIdx.x * a.ty, tz0 = blockIdx.y * a.tz, rhs = blockIdx.z;
+  if (rhs_idle(a.gate, a.gdiv, rhs)) return;
   const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
   for (int k = threadIdx.x; k < a.nx; k += blockDim.x) tw[k] = twg[k];
   for (int k = threadIdx.x; k < a.ncl * a.lstride; k += blockDim.x) buf[k] = cmk<T>(0, 0);
@@ -785,6 +792,7 @@ __global__ void __launch_bounds__(256) k_spread_gather_xfft(TileArgs a, const do
   const Stage<T> st = stage_layout<T, NC>(stage_base, a.ch, a.P, a.rows);
   int* cnt = st.orig + a.ch;     // nx + 1 bucket starts
   const int ty0 = blockIdx.x * a.ty, tz0 = blockIdx.y * a.tz, rhs = blockIdx.z;
+  if (rhs_idle(a.gate, a.gdiv, rhs)) return;
   const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
   for (int k = threadIdx.x; k < a.nx; k += blockDim.x) tw[k] = twg[k];
   for (int k = threadIdx.x; k < a.ncl * a.lstride; k += blockDim.x) buf[k] = cmk<T>(0, 0);
@@ -875,6 +883,7 @@ __global__ void __launch_bounds__(256) k_xifft_contract(TileArgs a, const cplx<T
   unsigned char* stage_base = reinterpret_cast<unsigned char*>(buf + a.ncl * a.lstride * (E == 0 && N == 0 ? 2 : 1));
   const Stage<T> st = stage_layout<T, NC>(stage_base, a.ch, a.P, a.rows);
   const int ty0 = blockIdx.x * a.ty, tz0 = blockIdx.y * a.tz, rhs = blockIdx.z;
+  if (rhs_idle(a.gate, a.gdiv, rhs)) return;
   const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
   for (int k = threadIdx.x; k < a.nx; k += blockDim.x) tw[k] = twg[k];
   // pack two half spectra per complex line: Z = A + iB (Hermitian extension)
@@ -980,6 +989,8 @@ struct ColArgs {
   int lstride;
   FftPlan plan;
   const void* tw;   // twiddles for the transformed axis
+  const int* gate;  // optional per-system state (rhs_idle) of blockIdx.z / gdiv
+  int gdiv;
 };
 
 template <typename T, int E, int N>
@@ -989,6 +1000,7 @@ __global__ void __launch_bounds__(256) k_fft_y(ColArgs a, cplx<T>* __restrict__ 
   cplx<T>* buf = tw + a.ny;
   cplx<T>* scratch = buf + a.tx * a.lstride;
   const int kx0 = blockIdx.x * a.tx, z = blockIdx.y, b = blockIdx.z;
+  if (rhs_idle(a.gate, a.gdiv, b)) return;
   const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
   for (int k = threadIdx.x; k < a.ny; k += blockDim.x) tw[k] = twg[k];
   cplx<T>* base = spec + ((size_t)b * a.nz + z) * a.ny * a.kxs;
@@ -1092,6 +1104,7 @@ __global__ void __launch_bounds__(1024) k_fft_z_scale(ColArgs a, ScaleParams sp,
   cplx<T>* buf = tw + a.nz;
   cplx<T>* scratch = buf + NC * a.tx * a.lstride;
   const int kx0 = blockIdx.x * a.tx, y = blockIdx.y, rhs = blockIdx.z;
+  if (rhs_idle(a.gate, a.gdiv, rhs)) return;
   const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
   for (int k = threadIdx.x; k < a.nz; k += blockDim.x) tw[k] = twg[k];
   const size_t plane = (size_t)a.ny * a.kxs;

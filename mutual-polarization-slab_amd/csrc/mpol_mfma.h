@@ -105,10 +105,12 @@ template <typename T, int NC>
 __global__ void __launch_bounds__(256) k_contract_reduce(int n, long long total, const int* __restrict__ poff,
                                                          const int* __restrict__ pslot, const int* __restrict__ perm,
                                                          const T* __restrict__ part, double hvol,
-                                                         double* __restrict__ out) {
+                                                         double* __restrict__ out, const int* __restrict__ gate,
+                                                         int gdiv) {
   const int g = threadIdx.x & 15;
   const int s = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 4);
   const int rhs = blockIdx.y;
+  if (rhs_idle(gate, gdiv, rhs)) return;  // uniform over the block
   const bool valid = s < n;  // every lane takes part in the lane sum
   double acc[NC];
 #pragma unroll
@@ -302,6 +304,7 @@ __global__ void __launch_bounds__(256) k_spread_mfma_xfft(TileArgs a, ImgArgs im
   __shared__ int s_xr[4 * NXB];
   const int tile = blockIdx.x + blockIdx.y * gridDim.x;
   const int ty0 = blockIdx.x * a.ty, tz0 = blockIdx.y * a.tz, rhs = blockIdx.z;
+  if (rhs_idle(a.gate, a.gdiv, rhs)) return;
   const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
   for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];
   for (int k = threadIdx.x; k < a.ncl * a.lstride; k += blockDim.x) buf[k] = cmk<T>(0, 0);
@@ -410,6 +413,7 @@ __global__ void __launch_bounds__(256) k_xifft_contract_mfma(TileArgs a, ImgArgs
   const MfmaStage<T> st = mfma_stage_layout<T, NC>(reinterpret_cast<unsigned char*>(buf + a.ncl * a.lstride), a.ch, PW);
   const int tile = blockIdx.x + blockIdx.y * gridDim.x;
   const int ty0 = blockIdx.x * a.ty, tz0 = blockIdx.y * a.tz, rhs = blockIdx.z;
+  if (rhs_idle(a.gate, a.gdiv, rhs)) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
   for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];

@@ -215,6 +215,8 @@ struct SizesGridArgs {
   const int4* rec;    // sorted: first stencil node x, y, z; caller's index
   const double4* base;  // sorted: node0*h - x per axis
   const double* rad;  // sorted radii
+  const int* gate;    // optional per-system state (rhs_idle): idle right-hand sides are skipped
+  int gdiv;
 };
 
 // Weight w with grid value w (r.p) for r = node - x: K(d)/d, or the
@@ -259,6 +261,7 @@ __global__ void __launch_bounds__(256) k_sizes_spread(SizesGridArgs a, const dou
     gz -= (gz >= a.nz) ? a.nz : 0;
     const size_t node = ((size_t)gz * a.ny + gy) * a.nx + gx;
     for (int rhs = 0; rhs < a.nrhs; ++rhs) {
+      if (rhs_idle(a.gate, a.gdiv, rhs)) continue;
       const double* p = ms + 3 * ((size_t)rhs * a.n + s);
       atomicAdd(grid + rhs * plane + node, w * (r0 * (T)p[0] + r1 * (T)p[1] + r2 * (T)p[2]));
     }
@@ -280,6 +283,7 @@ __global__ void __launch_bounds__(256) k_sizes_contract(SizesGridArgs a, const T
   const size_t plane = (size_t)a.nx * a.ny * a.nz;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   for (int rhs = 0; rhs < a.nrhs; ++rhs) {
+    if (rhs_idle(a.gate, a.gdiv, rhs)) continue;  // uniform over the block
     T acc0 = 0, acc1 = 0, acc2 = 0;
     const T* g = grid + rhs * plane;
     for (int o = threadIdx.x; o < P3; o += blockDim.x) {
@@ -354,6 +358,7 @@ __global__ void __launch_bounds__(256) k_sizes_contract_lds(SizesGridArgs a, con
   const size_t plane = (size_t)a.nx * a.ny * a.nz;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   for (int rhs = wave; rhs < a.nrhs; rhs += nwave) {
+    if (rhs_idle(a.gate, a.gdiv, rhs)) continue;
     const T* g = grid + rhs * plane;
     T acc0 = 0, acc1 = 0, acc2 = 0;
     for (int o = lane; o < P3; o += 64) {
@@ -387,6 +392,7 @@ __global__ void __launch_bounds__(256) k_rows_r2c(TileArgs a, const T* __restric
   cplx<T>* buf = tw + a.nx;
   cplx<T>* scratch = buf + a.ncl * a.lstride;
   const int ty0 = blockIdx.x * a.ty, tz0 = blockIdx.y * a.tz, rhs = blockIdx.z;
+  if (rhs_idle(a.gate, a.gdiv, rhs)) return;
   const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
   for (int k = threadIdx.x; k < a.nx; k += blockDim.x) tw[k] = twg[k];
   const T* g = grid + (size_t)rhs * a.nx * a.ny * a.nz;
@@ -414,6 +420,7 @@ __global__ void __launch_bounds__(256) k_rows_c2r(TileArgs a, const cplx<T>* __r
   cplx<T>* buf = tw + a.nx;
   cplx<T>* scratch = buf + a.ncl * a.lstride;
   const int ty0 = blockIdx.x * a.ty, tz0 = blockIdx.y * a.tz, rhs = blockIdx.z;
+  if (rhs_idle(a.gate, a.gdiv, rhs)) return;
   const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
   for (int k = threadIdx.x; k < a.nx; k += blockDim.x) tw[k] = twg[k];
   const int tot = a.ncl * a.kx;

@@ -78,6 +78,7 @@ __global__ void __launch_bounds__(TX * 16) k_fft_z_scale_reg256(ColArgs a, Scale
   __shared__ cplx<T> xb[TX * LS];
   const int col = threadIdx.x % TX, j = threadIdx.x / TX;
   const int kx = blockIdx.x * TX + col, y = blockIdx.y, rhs = blockIdx.z;
+  if (rhs_idle(a.gate, a.gdiv, rhs)) return;
   const bool valid = kx < a.kx;
   const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
   for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];
