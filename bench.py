@@ -81,12 +81,14 @@ def cpu_baseline(cfg, perp, para, rv, gpu_fp64_matvec):
     return {"precompute_s": t1 - t0, "matvec_s": t2 - t1, "gpu_fp64_matvec_relerr": err}
 
 
-def heterogeneous_sample(count=64, plain=8, workers=4):
+def heterogeneous_sample(count=256, plain=16, workers=8):
     """Many independent polydisperse configurations (N log-uniform in
     [500, 8000], phi in [0.1, 0.45], 16-point permittivity spectrum x 3
-    fields each): the batched mixed-precision driver over the whole set
-    against `plain` of them solved one right-hand side at a time in FP64 on
-    one stream.  Grid choice is timed apart from the solves."""
+    fields each; 256 of them by default, 12,288 right-hand sides): the
+    batched mixed-precision driver over the whole set against `plain` of
+    them (every count/plain-th) solved one right-hand side at a time in FP64
+    on one stream, compared by rate and, scaled by particle count, as the
+    whole set's time.  Grid choice is timed apart from the solves."""
     import workloads
     from mpol_hip import batch
     from workloads.heterogeneous import plain_fp64
@@ -99,12 +101,16 @@ def heterogeneous_sample(count=64, plain=8, workers=4):
     live = int(np.sum(np.abs(eps - 1.0) > 1e-12))
     rate = count * live * 3 / res.seconds  # eps = 1 points (zero dipoles, no solve) not counted
     plain_rate = nr / secs
+    # the sample's share of the set's particles scales its time to the whole set
+    share = sum(configs[i]["x"].shape[0] for i in idx) / sum(c["x"].shape[0] for c in configs)
     return {"configs": count, "particles": int(sum(c["x"].shape[0] for c in configs)), "rhs": res.rhs,
             "rhs_solved": count * live * 3, "ewald_tol": 1e-3, "gmres_tol": 1e-6, "workers": workers,
             "batched_mixed_rhs_per_s": round(rate, 2), "seconds": round(res.seconds, 3),
             "grid_tuning_s": round(tune_s, 3),
             "fp64_one_rhs_at_a_time_rhs_per_s": round(plain_rate, 2), "fp64_one_rhs_sample_configs": len(idx),
             "speedup_batched_mixed_vs_fp64_one_rhs": round(rate / plain_rate, 2),
+            "fp64_one_rhs_whole_set_seconds_extrapolated": round(secs / share, 2),
+            "speedup_whole_set_extrapolated_by_particles": round(secs / share / res.seconds, 2),
             "grids": {int(g): int(c) for g, c in zip(*np.unique(grids, return_counts=True))},
             "converged_fraction": float(res.converged.sum() / max(1, count * live * 3))}
 

@@ -174,7 +174,7 @@ class _Worker:
         self.ctx.close()
 
 
-def solve_configurations(configs, eps, fields=None, *, tol=1e-3, gmres_tol=None, precision="mixed", workers=4,
+def solve_configurations(configs, eps, fields=None, *, tol=1e-3, gmres_tol=None, precision="mixed", workers=8,
                          tune=True, sizes=GRID_SIZES, repeats=2, grids=None, device=None, _tune_only=False):
     """Solve every configuration over the spectrum eps (Nk,) x fields (nf,3)
     (default the three axes) with the polydisperse operator; see the module

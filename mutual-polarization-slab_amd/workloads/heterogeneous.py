@@ -66,7 +66,7 @@ def plain_fp64(configs, eps, grids, tol, gmres_tol):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--count", type=int, default=256)
-    ap.add_argument("--workers", type=int, default=4)
+    ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--check", type=int, default=4, help="configurations re-solved in FP64 for the error")
     ap.add_argument("--tol", type=float, default=1e-3, help="Ewald rule tolerance")
     ap.add_argument("--gmres-tol", type=float, default=1e-6)
