@@ -86,6 +86,16 @@ struct GateScope {
   ~GateScope() { c->gate = nullptr; }
 };
 
+// Page-locked double slots for the per-step state and residual estimates and
+// their events (the Arnoldi loop reads step j - 1 while step j runs).
+int ensure_step_reads(mpol_ctx* c, int nrhs) {
+  if (c->hstep.reserve((size_t)2 * nrhs) != hipSuccess || c->hsteprel.reserve((size_t)2 * nrhs) != hipSuccess)
+    return fail(MPOL_ENOMEM, "page-locked host allocation failed");
+  for (int k = 0; k < 2; ++k)
+    if (!c->step_ev[k]) HIPCHK(hipEventCreateWithFlags(&c->step_ev[k], hipEventDisableTiming));
+  return MPOL_OK;
+}
+
 // Restarted GMRES on x (in: initial guess, out: solution) for A x = b.
 // nrhs here counts systems; a system's vector holds all its right-hand sides
 // (kind components per particle, times 2 with complex permittivities).
@@ -161,6 +171,29 @@ int gmres_core(mpol_ctx* c, int kind, const double* b, double* x, int nrhs, doub
     all_done = std::all_of(state.begin(), state.end(), [](int s) { return s == kDone; });
     if (all_done || outer == maxit) break;
     hipLaunchKernelGGL(k_scale_into, vgrid, dim3(256), 0, st, c->gR.p, c->gbeta.p, c->gV.p, n3, c->gstate.p);
+    // The host stays one Arnoldi step ahead of the state it reads: step j is
+    // enqueued before the state after step j - 1 is waited for.  Every
+    // kernel of a step is gated per system (kIter), so a step enqueued for a
+    // system whose cycle had just ended does nothing for it; the x update
+    // acts on kSolve systems only and moves them to kWait, so enqueuing it
+    // again is harmless.
+    CHECK_STATUS(ensure_step_reads(c, nrhs));
+    bool stop = false;
+    auto take_state = [&](int jj) -> int {
+      HIPCHK(hipEventSynchronize(c->step_ev[jj & 1]));
+      std::copy(c->hstep.p + (size_t)(jj & 1) * nrhs, c->hstep.p + (size_t)(jj & 1) * nrhs + nrhs, state.begin());
+      std::copy(c->hsteprel.p + (size_t)(jj & 1) * nrhs, c->hsteprel.p + (size_t)(jj & 1) * nrhs + nrhs,
+                rel.begin());
+      if (std::any_of(state.begin(), state.end(), [](int s) { return s == kSolve; })) {
+        // a cycle ended for some right-hand sides: x += V y, then they wait for the next cycle
+        hipLaunchKernelGGL(k_update_x, vgrid, dim3(256), 0, st, c->gV.p, sv, c->gy.p, c->gkdim.p, c->gstate.p,
+                           restart, x, n3);
+        hipLaunchKernelGGL(k_solve_to_wait, dim3(sgrid), dim3(256), 0, st, nrhs, c->gstate.p);
+      }
+      stop = std::none_of(state.begin(), state.end(), [](int s) { return s == kIter; });
+      return MPOL_OK;
+    };
+    int pending = -1;  // step whose state copy is in flight
     for (int j = 0; j < restart; ++j) {
       double* vj = c->gV.p + (size_t)j * sv;
       {
@@ -187,15 +220,22 @@ int gmres_core(mpol_ctx* c, int kind, const double* b, double* x, int nrhs, doub
       hipLaunchKernelGGL(k_arnoldi_step, dim3(sgrid), dim3(256), 0, st, nrhs, restart, j, c->gh1.p, c->gh2.p,
                          c->grn2.p, c->gH.p, c->gcs.p, c->gsn.p, c->gg.p, c->gbn.p, tol, c->gstate.p, c->gkdim.p,
                          c->grel.p, c->gy.p);
-      CHECK_STATUS(read_state(c, nrhs, state, rel));
-      if (std::any_of(state.begin(), state.end(), [](int s) { return s == kSolve; })) {
-        // a cycle ended for some right-hand sides: x += V y, then they wait for the next cycle
-        hipLaunchKernelGGL(k_update_x, vgrid, dim3(256), 0, st, c->gV.p, sv, c->gy.p, c->gkdim.p, c->gstate.p,
-                           restart, x, n3);
-        hipLaunchKernelGGL(k_solve_to_wait, dim3(sgrid), dim3(256), 0, st, nrhs, c->gstate.p);
+      HIPCHK(hipMemcpyAsync(c->hstep.p + (size_t)(j & 1) * nrhs, c->gstate.p, nrhs * sizeof(int),
+                            hipMemcpyDeviceToHost, st));
+      HIPCHK(hipMemcpyAsync(c->hsteprel.p + (size_t)(j & 1) * nrhs, c->grel.p, nrhs * sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+      HIPCHK(hipEventRecord(c->step_ev[j & 1], st));
+      if (pending >= 0) {
+        CHECK_STATUS(take_state(pending));
+        if (stop) {
+          --*nmv;  // step j had no iterating system left
+          pending = -1;
+          break;
+        }
       }
-      if (std::none_of(state.begin(), state.end(), [](int s) { return s == kIter; })) break;
+      pending = j;
     }
+    if (pending >= 0) CHECK_STATUS(take_state(pending));
     if (inner) {
       // every right-hand side whose estimated residual met tol is done
       bool est_done = true;

@@ -201,6 +201,10 @@ constexpr size_t kLdsCol = 112 * 1024;
 }  // namespace mpol
 
 struct mpol_ctx {
+  ~mpol_ctx() {
+    for (hipEvent_t& e : step_ev)
+      if (e) (void)hipEventDestroy(e);
+  }
   int device = 0;
   hipStream_t stream = nullptr;  // launch stream; nullptr is the default stream
   // tables
@@ -272,6 +276,9 @@ struct mpol_ctx {
   const int* gate = nullptr;
   HostBuf<int> hstate;  // solver state read back each Arnoldi step
   HostBuf<double> hrel, hscr;
+  HostBuf<int> hstep;          // two slots of per-step state (read one step behind)
+  HostBuf<double> hsteprel;
+  hipEvent_t step_ev[2] = {nullptr, nullptr};
   // truncated fixed-dipole forces: cell list
   DevBuf<int> f_cell, f_count, f_start, f_cursor, f_members;
 };
