@@ -216,8 +216,10 @@ def maxwell_garnett_guess(eps_p, phi, Hhat):
 
 
 def run_capacitance(x, box, eps_p, xi, H0, errortol: float = 1e-5, slab_factor: float = 1.0,
-                    use_scipy_gmres: bool = True, verbose: int = 0, *, precision: str = "fp64"):
-    """(C (F,) complex128, p (N,3,F) complex128) for one frame or a trajectory."""
+                    use_scipy_gmres: bool = True, verbose: int = 0, *, precision: str = "fp64", p_guess=None):
+    """(C (F,) complex128, p (N,3,F) complex128) for one frame or a trajectory.
+    p_guess ((N,3) or (N,3,F); not in the reference) replaces the
+    Maxwell-Garnett initial guess, e.g. the previous step's dipoles."""
     x = np.asarray(x, dtype=np.float64)
     box = np.asarray(box, dtype=np.float64).reshape(3)
     eps_p = np.asarray(eps_p, dtype=np.float64).reshape(-1)
@@ -235,8 +237,17 @@ def run_capacitance(x, box, eps_p, xi, H0, errortol: float = 1e-5, slab_factor: 
     if verbose >= 1:
         print(f"RunCapacitance: N={n} frames={nf} xi={xi} errortol={errortol:g} slab_factor={slab_factor:g}",
               flush=True)
+    if p_guess is not None:
+        p_guess = np.asarray(p_guess)
+        if p_guess.ndim == 2:
+            p_guess = p_guess[:, :, None]
+        if p_guess.shape[0] != n or p_guess.shape[1] != 3 or p_guess.shape[2] not in (1, nf):
+            raise ValueError(f"p_guess must have shape (N,3) or (N,3,F); got {p_guess.shape}")
     for f in range(nf):
-        guess = maxwell_garnett_guess(eps_p, phi, Hhat)
+        if p_guess is None:
+            guess = maxwell_garnett_guess(eps_p, phi, Hhat)
+        else:
+            guess = p_guess[:, :, min(f, p_guess.shape[2] - 1)]
         C[f], p[:, :, f] = capacitance(x[:, :, f], eps_p, box, guess, xi, errortol, perp, para, rtable, offset,
                                        offsetxyz, Hhat, slab_factor=slab_factor, use_scipy_gmres=use_scipy_gmres,
                                        verbose=verbose, precision=precision)

@@ -62,7 +62,8 @@ class MutualPolarizationDipoleForceCached(hoomd.md.force.Custom):
 
     def __init__(self, *, eps_p: float, xi: float, errtol: float, slab_factor: float, Hhat, r_cut: float,
                  prefactor: float, field_strength: float, pol_every_steps: int = 1, use_scipy_gmres: bool = True,
-                 periodic_z: bool = True, use_cell_list: bool = True, precision: str = "fp64"):
+                 periodic_z: bool = True, use_cell_list: bool = True, precision: str = "fp64",
+                 warm_start: bool = False):
         super().__init__()
         self.eps_p = float(eps_p)
         self.xi = float(xi)
@@ -77,6 +78,11 @@ class MutualPolarizationDipoleForceCached(hoomd.md.force.Custom):
         self.use_cell_list = bool(use_cell_list)
         self.pol_every_steps = max(1, int(pol_every_steps))
         self.precision = precision
+        # not in the reference: start each solve from the cached dipoles
+        # (hoomd_force.py:183-188 there caches them but never reuses them)
+        self.warm_start = bool(warm_start)
+        self.last_solve_info: Optional[dict] = None
+        self._solved: Optional[np.ndarray] = None
         self.calls = 0
         self.pol_solves = 0
         self.last_dipoles: Optional[np.ndarray] = None
@@ -90,15 +96,24 @@ class MutualPolarizationDipoleForceCached(hoomd.md.force.Custom):
         have = 0 if self._by_tag is None else self._by_tag.shape[0]
         if self._by_tag is None or need > have:
             grown = np.zeros((max(need, have), 3), dtype=np.float64)
+            solved = np.zeros(max(need, have), dtype=bool)
             if self._by_tag is not None:
                 grown[:have] = self._by_tag
+                solved[:have] = self._solved
             self._by_tag = grown
+            self._solved = solved
 
     def _solve_dipoles(self, pos: np.ndarray, boxL: np.ndarray, tags: np.ndarray) -> None:
         eps = np.full(pos.shape[0], self.eps_p, dtype=np.float64)
+        guess = None
+        if (self.warm_start and self.field_strength != 0.0 and self._solved is not None and tags.size
+                and bool(self._solved[tags].all())):
+            guess = self._by_tag[tags] / self.field_strength
         _, p = capacitance.run_capacitance(pos, boxL, eps, self.xi, self.Hhat.copy(), errortol=self.errtol,
                                            slab_factor=self.slab_factor, verbose=0,
-                                           use_scipy_gmres=self.use_scipy_gmres, precision=self.precision)
+                                           use_scipy_gmres=self.use_scipy_gmres, precision=self.precision,
+                                           p_guess=guess)
+        self.last_solve_info = dict(capacitance.magnetic_dipole.last_info or {})
         p = np.asarray(p)
         if p.ndim == 3:
             p0 = np.real(p[:, :, 0])
@@ -108,6 +123,7 @@ class MutualPolarizationDipoleForceCached(hoomd.md.force.Custom):
             raise RuntimeError(f"Unexpected p shape from run_capacitance: {p.shape}")
         self._grow_cache(tags)
         self._by_tag[tags] = p0.astype(np.float64) * self.field_strength
+        self._solved[tags] = True
         self.pol_solves += 1
 
     def set_forces(self, timestep):

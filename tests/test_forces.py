@@ -385,3 +385,31 @@ def test_gpu_hoomd_mutual_polarization_cached(dfe, monkeypatch):
     assert _rel(system.arrays.force, f_before[perm]) < 1e-12
     force.set_forces(3)
     assert force.pol_solves == 2
+
+
+@pytest.mark.gpu
+def test_gpu_hoomd_warm_start_reuses_cached_dipoles(dfe, monkeypatch):
+    """warm_start=True (not in the reference) starts each solve from the
+    cached dipoles: after a small move it needs fewer operator applications
+    than a cold start and gives the same dipoles to the solve tolerance."""
+    from helpers import rsa
+
+    x, box = rsa(60, 0.2, 13)
+    tags = np.arange(60)
+    kw = dict(eps_p=10.0, xi=0.9, errtol=1e-8, slab_factor=0.0, Hhat=[0.0, 0.0, 1.0], r_cut=3.5,
+              prefactor=1.0, field_strength=2.0, pol_every_steps=1)
+    moved = x + 0.01 * np.random.default_rng(4).standard_normal(x.shape)
+    counts, dips = {}, {}
+    for warm in (False, True):
+        system = _FakeSystem(x.copy(), tags, box)
+        hf = _hoomd_force_module(monkeypatch, system)
+        force = hf.MutualPolarizationDipoleForceCached(warm_start=warm, **kw)
+        force.set_forces(0)
+        system.pos = moved.copy()
+        force.set_forces(1)
+        assert force.pol_solves == 2
+        info = force.last_solve_info
+        counts[warm] = info["matvecs_fp64"] + info["matvecs_fp32"]
+        dips[warm] = force.last_dipoles.copy()
+    assert counts[True] < counts[False]
+    assert _rel(dips[True], dips[False]) < 1e-6
