@@ -264,6 +264,12 @@ int mpol_geometry_info(mpol_ctx* ctx, mpol_geometry_summary* info);
  * ("CPU - Ewald test/ewald.py":604-931). */
 int mpol_real_space_table(double xi, const double* r_host, int64_t n, double* out_host);
 
+/* The same six couplings without the overlap terms of r < 2: the field at a
+ * point (not a sphere) from a sphere's dipole, as the field-map tables are
+ * built.  Replaces RealSpaceTable (Compute_Field/RealSpaceTable.m:1-163,
+ * whose (r < 2) regularisation lines are commented out). */
+int mpol_real_space_table_field(double xi, const double* r_host, int64_t n, double* out_host);
+
 /* The eight couplings the quadrupole method adds, at separations r (host, n
  * values > 0), each with its r = 0 self term first: out (host) is 8 x (n+1)
  * in the order pot_quad, field_quad_1, field_quad_2, field_quad_3,

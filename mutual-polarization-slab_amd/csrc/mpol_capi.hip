@@ -997,6 +997,15 @@ int mpol_real_space_table(double xi, const double* r, int64_t n, double* out) {
   return MPOL_OK;
 }
 
+int mpol_real_space_table_field(double xi, const double* r, int64_t n, double* out) {
+  if (!r || !out || n < 0) return fail(MPOL_EINVAL, "bad table arguments");
+  if (!(xi > 0.0)) return fail(MPOL_EINVAL, "xi must be positive");
+  for (int64_t i = 0; i < n; ++i)
+    if (!(r[i] > 0.0)) return fail(MPOL_EINVAL, "table separations must be positive");
+  real_space_tables(xi, r, n, out, false);
+  return MPOL_OK;
+}
+
 int mpol_real_space_table_sizes(double xi, const double* r, int64_t n, const double* radii, int32_t M,
                                 double* perp, double* para) {
   if (!r || !radii || !perp || !para || n < 1 || M < 1) return fail(MPOL_EINVAL, "bad table arguments");

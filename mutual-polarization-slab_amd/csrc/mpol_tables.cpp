@@ -327,8 +327,10 @@ void real_space_tables_quad(double xi, const double* r, int64_t n, double* out) 
   }
 }
 
-// out: 6 x (n+1); entry 0 of each row is the self term.
-void real_space_tables(double xi, const double* r, int64_t n, double* out) {
+// out: 6 x (n+1); entry 0 of each row is the self term.  regularize = false
+// leaves out the [r < 2] overlap terms, as the field-map tables do
+// (Compute_Field/RealSpaceTable.m: the couplings of a point to a sphere).
+void real_space_tables(double xi, const double* r, int64_t n, double* out, bool regularize) {
   const std::vector<Coupling> cp = couplings(xi);
   const double e4 = std::exp(-4.0 * xi * xi);
   const double tail = std::erfc(2.0 * xi) / (4.0 * kPi);
@@ -349,7 +351,7 @@ void real_space_tables(double xi, const double* r, int64_t n, double* out) {
       const Coupling& c = cp[t];
       double v = c.ep.eval(rv) * ep + c.em.eval(rv) * em + c.e0.eval(rv) * e0 + c.fp.eval(rv) * fp +
                  c.fm.eval(rv) * fm + c.f0.eval(rv) * f0;
-      if (rv < 2.0) v += regularisation(c.reg, rv);
+      if (regularize && rv < 2.0) v += regularisation(c.reg, rv);
       out[t * (n + 1) + 1 + i] = v;
     }
   }

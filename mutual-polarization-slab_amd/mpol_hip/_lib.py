@@ -59,6 +59,7 @@ SIGNATURES = [
     ("mpol_set_tuning_for", ctypes.c_int, [_VP, ctypes.c_int32, _I32P, ctypes.c_int32]),
     ("mpol_geometry_info", ctypes.c_int, [_VP, ctypes.POINTER(GeometrySummary)]),
     ("mpol_real_space_table", ctypes.c_int, [ctypes.c_double, _DP, ctypes.c_int64, _DP]),
+    ("mpol_real_space_table_field", ctypes.c_int, [ctypes.c_double, _DP, ctypes.c_int64, _DP]),
     ("mpol_compute_forces", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int64, _DP, ctypes.c_double, ctypes.c_double,
                                            ctypes.c_int32, ctypes.c_int32, _VP, _VP]),
     ("mpol_set_permittivity_complex", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int32]),

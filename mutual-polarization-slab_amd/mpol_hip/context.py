@@ -35,6 +35,16 @@ def real_space_table(r, xi):
     return tuple(out[k] for k in range(6))
 
 
+def real_space_table_field(r, xi):
+    """The six couplings of real_space_table without the overlap terms of
+    r < 2 (the field at a point from a sphere's dipole; field-map tables)."""
+    r = np.ascontiguousarray(np.asarray(r, dtype=np.float64).reshape(-1))
+    out = np.zeros((6, r.size + 1), dtype=np.float64)
+    lib = _lib.load()
+    _lib.check(lib.mpol_real_space_table_field(float(xi), _lib.dptr(r), r.size, _lib.dptr(out)))
+    return tuple(out[k] for k in range(6))
+
+
 def real_space_table_quad(r, xi):
     """The eight quadrupole couplings (self term first) in the order
     pot_quad, field_quad_1..3, grad_quad_1..4, from the library's FP64 host

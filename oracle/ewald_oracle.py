@@ -278,9 +278,11 @@ def _rational(defn, r):
     return scale * np.polyval(coeffs[::-1], r) / r ** k
 
 
-def real_space_table(r, xi):
+def real_space_table(r, xi, regularize=True):
     """The six real-space couplings of unit spheres at separations r (r > 0),
-    each with its r = 0 self term prepended ([E]ewald.py:604-931)."""
+    each with its r = 0 self term prepended ([E]ewald.py:604-931).
+    regularize=False leaves out the (r < 2) overlap terms, as the field-map
+    tables do (Compute_Field/RealSpaceTable.m, regpoly lines commented out)."""
     r = np.asarray(r, dtype=np.float64).reshape(-1)
     x = float(xi)
     defs = _coupling_defs(x)
@@ -290,7 +292,7 @@ def real_space_table(r, xi):
     fp = _erfc((r + 2.0) * x)
     fm = _erfc((r - 2.0) * x)
     f0 = _erfc(r * x)
-    overlap = (r < 2.0)
+    overlap = (r < 2.0) & bool(regularize)
     out = {}
     for name in TABLE_NAMES:
         d = defs[name]
@@ -408,3 +410,59 @@ def field(geo: OperatorGeometry, m, lambdap, slab_factor=1.0, complex_eps=False)
     """A m = Hk + Hr - H_slab ([E]ewald.py:1236)."""
     Hk, Hr, slab = field_components(geo, m, lambdap, slab_factor, complex_eps)
     return Hk + Hr - slab
+
+
+# ---------------------------------------------------------------------------
+# field at probe points (Compute_Field/FieldMap.m, MagneticField.m, RealSpace.m)
+# ---------------------------------------------------------------------------
+def field_map(x_E, x_p, p, box, xi, regularized_tables=False):
+    """E (N_E,3) complex at the probe points x_E from the dipoles p at x_p:
+    errortol 1e-3 and the parameter rule (FieldMap.m:17-36), tables without
+    overlap terms (FieldMap.m:23-25), the particles' grid contracted at the
+    probes (MagneticField.m:33-60) plus the real-space sum over probe-particle
+    pairs with 0 < d < rc (RealSpace.m:23-57, brute force over all pairs here
+    in place of NeighborList_Types), and E = -(Hk + Hr) (FieldMap.m:55).
+    regularized_tables=True swaps in the sphere-sphere tables (a check that
+    the choice matters, not the reference's field map)."""
+    x_E = np.asarray(x_E, dtype=np.float64).reshape(-1, 3)
+    x_p = np.asarray(x_p, dtype=np.float64).reshape(-1, 3)
+    p = np.asarray(p, dtype=np.complex128).reshape(-1, 3)
+    box = np.asarray(box, dtype=np.float64).reshape(3)
+    errortol = 1e-3
+    r = default_rtable()[1:]
+    tabs = real_space_table(r, xi, regularize=bool(regularized_tables))
+    perp, para, rvals = tabs[2], tabs[3], np.concatenate(([0.0], r))
+    rc = np.sqrt(-np.log(errortol)) / xi
+    kcut = 2 * xi * np.sqrt(-np.log(errortol))
+    Ngrid = np.ceil(1 + box * kcut / PI).astype(np.int64)
+    h = box / Ngrid
+    P = int(np.ceil(-2 * np.log(errortol) / PI))
+    eta = P * (h * xi) ** 2 / PI
+    if np.any(rc > box / 2):
+        raise ValueError("Real space cutoff larger than half the box size")
+    offset, offsetxyz = pre_calculations(P, h)
+    lin_p, w_p, _ = prepare_grid_kernel(x_p, Ngrid, h, xi, eta, P, offset, offsetxyz)
+    H = spread_fast(p, Ngrid, lin_p, w_p)
+    fH = np.fft.fftshift(np.fft.fftn(H, axes=(0, 1, 2)), axes=(0, 1, 2))
+    khat, coef = precompute_kspace(Ngrid, box, xi, eta)
+    Ht = np.fft.ifftn(np.fft.ifftshift(scale_fast(fH, khat, coef), axes=(0, 1, 2)), axes=(0, 1, 2))
+    lin_e, _, wc_e = prepare_grid_kernel(x_E, Ngrid, h, xi, eta, P, offset, offsetxyz)
+    Hk = contract_fast(Ht, Ngrid, lin_e, wc_e)
+    Hr = np.zeros((x_E.shape[0], 3), dtype=np.complex128)
+    for i in range(x_E.shape[0]):
+        rr = x_E[i][None, :] - x_p
+        rr = rr - box[None, :] * np.trunc(2.0 * rr / box[None, :])
+        d = np.sqrt(np.sum(rr * rr, axis=1))
+        keep = (d < rc) & (d > 0)
+        if not np.any(keep):
+            continue
+        d, rr, mj = d[keep], rr[keep], p[keep]
+        rhat = rr / d[:, None]
+        idx = np.searchsorted(rvals, d, side="left")
+        lo, hi = rvals[idx - 1], rvals[idx]
+        t = (d - lo) / (hi - lo)
+        A = perp[idx - 1] * (1.0 - t) + perp[idx] * t
+        B = para[idx - 1] * (1.0 - t) + para[idx] * t
+        proj = np.sum(mj * rhat, axis=1)
+        Hr[i] = np.sum(A[:, None] * (mj - proj[:, None] * rhat) + (B * proj)[:, None] * rhat, axis=0)
+    return -(Hk + Hr)
