@@ -148,7 +148,7 @@ def main():
     ap.add_argument("--no-other-workloads", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path",
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); there is no multi-GPU path",
               file=sys.stderr)
         sys.exit(2)
 

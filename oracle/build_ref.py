@@ -3,8 +3,8 @@
 Compiles "CPU - HOOMD simulation test/dipole_forces_ext/_dipole_forces_ext.cpp"
 from the reference tree where it lies (pybind11 headers + Python headers, the
 same flags as the reference's setup script: -O3 -std=c++17 -fopenmp) into
-oracle/_ref/.  The output is git-ignored but travels to the GPU box with the
-tree.  Only tests import it.
+oracle/_ref/.  The output is git-ignored but is kept with the tree, so test
+machines without the reference sources still have it.  Only tests import it.
 
     python oracle/build_ref.py [/root/reference]
 """
