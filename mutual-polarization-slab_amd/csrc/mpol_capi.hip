@@ -500,6 +500,7 @@ int mpol_set_geometry(mpol_ctx* c, const double* x, int64_t n, const double box[
   c->have_geo = true;
   c->quad_pairs_valid = false;
   c->sz_pairs_valid = false;
+  c->sz_nodes_valid = false;
   return MPOL_OK;
 }
 
@@ -866,6 +867,7 @@ int mpol_set_sizes(mpol_ctx* c, const double* radius, const int32_t* type, int32
   c->sz_ntypes = ntypes;
   c->have_sizes = true;
   c->sz_pairs_valid = false;
+  c->sz_nodes_valid = false;
   c->sz_diag_valid = false;
   return MPOL_OK;
 }

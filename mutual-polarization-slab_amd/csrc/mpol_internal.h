@@ -257,6 +257,14 @@ struct mpol_ctx {
   DevBuf<int> sz_type, sz_col;
   bool sz_pairs_valid = false, sz_diag_valid = false;
   DevBuf<unsigned char> sz_grid;
+  // node-major spread: per-node segments of sorted particle indices and their
+  // w r entries ([0] FP32, [1] FP64), particle-major dipoles of the batch
+  bool sz_nodes_valid = false, sz_nw_valid[2] = {false, false};
+  int sz_entries = 0;
+  DevBuf<int> sz_ncnt, sz_noff, sz_nidx;
+  DevBuf<float4> sz_nw32;
+  DevBuf<double4> sz_nw64;
+  DevBuf<unsigned char> sz_pm;
   // vector-Jacobian product workspaces
   DevBuf<double> vjp_in, vjp_out, vjp_grid;
   // workspaces
@@ -773,6 +781,50 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
 // Right-hand sides per system: 2 (real, imaginary part) with complex permittivities.
 inline int rhs_per_system(const mpol_ctx* c) { return c->cplx_eps ? 2 : 1; }
 
+// Node-major spread structure of the polydisperse operator (index once per
+// geometry and radii, entries once per precision).
+template <typename T>
+int ensure_sizes_nodes(mpol_ctx* c, const SizesGridArgs& sa) {
+  hipStream_t st = c->stream;
+  const int nodes = sa.nx * sa.ny * sa.nz;
+  if (!c->sz_nodes_valid) {
+    HIPALLOC(c->sz_ncnt, (size_t)nodes);
+    HIPALLOC(c->sz_noff, (size_t)nodes + 1);
+    HIPCHK(hipMemsetAsync(c->sz_ncnt.p, 0, (size_t)nodes * sizeof(int), st));
+    hipLaunchKernelGGL(k_sizes_node_count, dim3(sa.n), dim3(256), 0, st, sa, c->sz_ncnt.p);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, c->sz_ncnt.p, c->sz_noff.p, nodes);
+    int total = 0;
+    HIPCHK(hipMemcpyAsync(&total, c->sz_noff.p + nodes, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    HIPALLOC(c->sz_nidx, (size_t)std::max(1, total));
+    HIPCHK(hipMemsetAsync(c->sz_ncnt.p, 0, (size_t)nodes * sizeof(int), st));
+    hipLaunchKernelGGL(k_sizes_node_fill, dim3(sa.n), dim3(256), 0, st, sa, c->sz_noff.p, c->sz_ncnt.p,
+                       c->sz_nidx.p);
+    hipLaunchKernelGGL(k_sizes_node_sort, dim3(cdiv(nodes, 256)), dim3(256), 0, st, nodes, c->sz_noff.p,
+                       c->sz_nidx.p);
+    HIPCHK(hipGetLastError());
+    c->sz_entries = total;
+    c->sz_nodes_valid = true;
+    c->sz_nw_valid[0] = c->sz_nw_valid[1] = false;
+  }
+  const int pi = sizeof(T) == 8 ? 1 : 0;
+  if (!c->sz_nw_valid[pi]) {
+    typename Vec4<T>::type* ent;
+    if constexpr (sizeof(T) == 8) {
+      HIPALLOC(c->sz_nw64, (size_t)std::max(1, c->sz_entries));
+      ent = c->sz_nw64.p;
+    } else {
+      HIPALLOC(c->sz_nw32, (size_t)std::max(1, c->sz_entries));
+      ent = c->sz_nw32.p;
+    }
+    hipLaunchKernelGGL((k_sizes_node_weights<T>), dim3(cdiv(nodes, 256)), dim3(256), 0, st, sa, c->sz_noff.p,
+                       c->sz_nidx.p, ent);
+    HIPCHK(hipGetLastError());
+    c->sz_nw_valid[pi] = true;
+  }
+  return MPOL_OK;
+}
+
 // Polydisperse operator: real space (pair couplings of the two radii,
 // dielectric 3/(4 pi a^3 (eps-1))) + one scalar grid per right-hand side.
 // hr != nullptr: hr <- real space, out <- wave space.
@@ -805,9 +857,7 @@ int run_operator_sizes(mpol_ctx* c, const double* m, double* out, double* hr, in
   hipLaunchKernelGGL((k_real_space<T>), dim3(cdiv((size_t)n * 16, 256), nrhs), dim3(256), 0, st, n, nrhs, c->nsets, m, out,
                      c->sz_diag.p, c->cplx_eps ? c->sz_diag_im.p : nullptr, c->rs_off.p, c->rs_j.p, pA, pBA, prh,
                      c->zsum.p, 0.0, hr != nullptr ? 1 : 0, hr, hr ? c->sz_slab.p : nullptr, gate, rps);
-  hipLaunchKernelGGL(k_permute_m<3>, dim3(cdiv(n, 256), nrhs), dim3(256), 0, st, m, c->perm.p, n, c->ms.p, gate, rps);
   // spread into real grids
-  HIPCHK(hipMemsetAsync(grid, 0, (size_t)nrhs * G * sizeof(T), st));
   SizesGridArgs sa{};
   sa.gate = gate;
   sa.gdiv = rps;
@@ -820,7 +870,29 @@ int run_operator_sizes(mpol_ctx* c, const double* m, double* out, double* hr, in
   sa.rec = c->rec.p;
   sa.base = c->base64.p;
   sa.rad = c->sz_rad_sorted.p;
-  hipLaunchKernelGGL((k_sizes_spread<T>), dim3(n), dim3(256), 0, st, sa, c->ms.p, grid);
+  const int P3 = c->P * c->P * c->P;
+  if (knobs<T>(c)[6] != 2 && (double)n * P3 < 2.0e9 && (double)G < 2.0e9) {
+    // gather by node: every live right-hand side's grid is written whole
+    CHECK_STATUS(ensure_sizes_nodes<T>(c, sa));
+    using V = typename Vec4<T>::type;
+    HIPALLOC(c->sz_pm, (size_t)nrhs * n * sizeof(V));
+    V* pm = reinterpret_cast<V*>(c->sz_pm.p);
+    const V* ent;
+    if constexpr (sizeof(T) == 8) ent = c->sz_nw64.p;
+    else ent = c->sz_nw32.p;
+    hipLaunchKernelGGL((k_sizes_permute_pm<T>), dim3(cdiv((size_t)n * nrhs, 256)), dim3(256), 0, st, m, c->perm.p,
+                       n, nrhs, pm, gate, rps);
+    constexpr int R = sizeof(T) == 8 ? 8 : 16;
+    hipLaunchKernelGGL((k_sizes_spread_nodes<T, R>), dim3(cdiv(G, 256), cdiv(nrhs, R)), dim3(256), 0, st, sa,
+                       c->sz_noff.p, ent, pm, grid);
+  } else {
+    // per-particle scatter with atomics (engine knob 2, or index sizes beyond
+    // 32 bits)
+    hipLaunchKernelGGL(k_permute_m<3>, dim3(cdiv(n, 256), nrhs), dim3(256), 0, st, m, c->perm.p, n, c->ms.p, gate,
+                       rps);
+    HIPCHK(hipMemsetAsync(grid, 0, (size_t)nrhs * G * sizeof(T), st));
+    hipLaunchKernelGGL((k_sizes_spread<T>), dim3(n), dim3(256), 0, st, sa, c->ms.p, grid);
+  }
   // x transform of the real rows
   const bool gen_x = generic_fft(c->plan[0]);
   TileCfg tr = tile_cfg_nc<1>(c->ng[0], gen_x, cs, 0, kLdsTile);

@@ -268,6 +268,168 @@ __global__ void __launch_bounds__(256) k_sizes_spread(SizesGridArgs a, const dou
   }
 }
 
+// ---------------------------------------------------------------------------
+// node-major spread (a gather: each node written once, no atomics)
+// ---------------------------------------------------------------------------
+// Built once per geometry and radii: for every grid node the sorted particles
+// whose stencil covers it, in ascending order (a fixed summation order), each
+// with w r of that node as one 4-vector {w r_x, w r_y, w r_z, particle bits}.
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+  using type = float4;
+  __device__ static float bits(int s) { return __int_as_float(s); }
+  __device__ static int index(float v) { return __float_as_int(v); }
+};
+template <>
+struct Vec4<double> {
+  using type = double4;
+  __device__ static double bits(int s) { return __longlong_as_double((long long)s); }
+  __device__ static int index(double v) { return (int)__double_as_longlong(v); }
+};
+
+// Stencil point o of sorted particle s -> node index (z, y, x order) and the
+// per-axis stencil offsets.
+__device__ inline int sizes_node(const SizesGridArgs& a, int4 rc, int o, int& ox, int& oy, int& oz) {
+  const int P = a.P;
+  ox = o % P;
+  oy = (o / P) % P;
+  oz = o / (P * P);
+  int gx = rc.x + ox, gy = rc.y + oy, gz = rc.z + oz;
+  gx -= (gx >= a.nx) ? a.nx : 0;  // P <= n on each axis: one wrap at most
+  gy -= (gy >= a.ny) ? a.ny : 0;
+  gz -= (gz >= a.nz) ? a.nz : 0;
+  return (gz * a.ny + gy) * a.nx + gx;
+}
+
+// Pass 1: stencil hits per node (one workgroup per sorted particle).
+static __global__ void __launch_bounds__(256) k_sizes_node_count(SizesGridArgs a, int* __restrict__ cnt) {
+  const int s = blockIdx.x;
+  if (s >= a.n) return;
+  const int4 rc = a.rec[s];
+  const int P3 = a.P * a.P * a.P;
+  int ox, oy, oz;
+  for (int o = threadIdx.x; o < P3; o += blockDim.x) atomicAdd(&cnt[sizes_node(a, rc, o, ox, oy, oz)], 1);
+}
+
+// Pass 2: each hit's particle into its node's segment (arbitrary order).
+static __global__ void __launch_bounds__(256) k_sizes_node_fill(SizesGridArgs a, const int* __restrict__ off,
+                                                                int* __restrict__ cursor, int* __restrict__ pidx) {
+  const int s = blockIdx.x;
+  if (s >= a.n) return;
+  const int4 rc = a.rec[s];
+  const int P3 = a.P * a.P * a.P;
+  int ox, oy, oz;
+  for (int o = threadIdx.x; o < P3; o += blockDim.x) {
+    const int node = sizes_node(a, rc, o, ox, oy, oz);
+    pidx[off[node] + atomicAdd(&cursor[node], 1)] = s;
+  }
+}
+
+// Pass 3: sort every node's segment (insertion sort: a few tens of entries).
+static __global__ void k_sizes_node_sort(int nodes, const int* __restrict__ off, int* __restrict__ pidx) {
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= nodes) return;
+  const int lo = off[node], hi = off[node + 1];
+  for (int k = lo + 1; k < hi; ++k) {
+    const int v = pidx[k];
+    int j = k - 1;
+    while (j >= lo && pidx[j] > v) {
+      pidx[j + 1] = pidx[j];
+      --j;
+    }
+    pidx[j + 1] = v;
+  }
+}
+
+// Pass 4: w r per entry in precision T, with the same expressions as the
+// per-particle kernels (r = node0 h - x + (o - half) h on each axis).
+template <typename T>
+__global__ void k_sizes_node_weights(SizesGridArgs a, const int* __restrict__ off, const int* __restrict__ pidx,
+                                     typename Vec4<T>::type* __restrict__ ent) {
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= a.nx * a.ny * a.nz) return;
+  const int gx = node % a.nx, gy = (node / a.nx) % a.ny, gz = node / (a.nx * a.ny);
+  for (int k = off[node]; k < off[node + 1]; ++k) {
+    const int s = pidx[k];
+    const int4 rc = a.rec[s];
+    const double4 b = a.base[s];
+    int ox = gx - rc.x, oy = gy - rc.y, oz = gz - rc.z;
+    ox += (ox < 0) ? a.nx : 0;
+    oy += (oy < 0) ? a.ny : 0;
+    oz += (oz < 0) ? a.nz : 0;
+    const T r0 = (T)(b.x + (double)(ox - a.half) * a.h[0]);
+    const T r1 = (T)(b.y + (double)(oy - a.half) * a.h[1]);
+    const T r2 = (T)(b.z + (double)(oz - a.half) * a.h[2]);
+    const T d = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
+    const T w = sizes_weight<T>(d, (T)a.rad[s], (T)a.xi, (T)a.eta);
+    typename Vec4<T>::type e;
+    e.x = w * r0;
+    e.y = w * r1;
+    e.z = w * r2;
+    e.w = Vec4<T>::bits(s);
+    ent[k] = e;
+  }
+}
+
+// Dipoles of the right-hand sides, particle-major in sorted order:
+// pm[s][rhs] = {p_x, p_y, p_z, 0} (m is [rhs][i][3] in the caller's order).
+template <typename T>
+__global__ void k_sizes_permute_pm(const double* __restrict__ m, const int* __restrict__ perm, int n, int nrhs,
+                                   typename Vec4<T>::type* __restrict__ pm, const int* __restrict__ gate,
+                                   int gdiv) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * nrhs) return;
+  const int s = (int)(t / nrhs), rhs = (int)(t - (int64_t)s * nrhs);
+  if (rhs_idle(gate, gdiv, rhs)) return;
+  const double* q = m + 3 * ((size_t)rhs * n + perm[s]);
+  typename Vec4<T>::type v;
+  v.x = (T)q[0];
+  v.y = (T)q[1];
+  v.z = (T)q[2];
+  v.w = (T)0;
+  pm[t] = v;
+}
+
+// grid[rhs][node] = sum over the node's entries of (w r) . p for R
+// right-hand sides per thread (blockIdx.y picks the group); one thread per
+// node, consecutive nodes on consecutive lanes.  Idle right-hand sides are
+// not written.
+template <typename T, int R>
+__global__ void __launch_bounds__(256) k_sizes_spread_nodes(SizesGridArgs a, const int* __restrict__ off,
+                                                            const typename Vec4<T>::type* __restrict__ ent,
+                                                            const typename Vec4<T>::type* __restrict__ pm,
+                                                            T* __restrict__ grid) {
+  using V = typename Vec4<T>::type;
+  const int nodes = a.nx * a.ny * a.nz;
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r0 = blockIdx.y * R;
+  bool live = false;
+#pragma unroll
+  for (int r = 0; r < R; ++r) live |= (r0 + r < a.nrhs) && !rhs_idle(a.gate, a.gdiv, r0 + r);
+  if (node >= nodes || !live) return;
+  T acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = (T)0;
+  const int hi = off[node + 1];
+  for (int k = off[node]; k < hi; ++k) {
+    const V e = ent[k];
+    const V* q = pm + (size_t)Vec4<T>::index(e.w) * a.nrhs + r0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (r0 + r < a.nrhs) {
+        const V v = q[r];
+        acc[r] += e.x * v.x + e.y * v.y + e.z * v.z;
+      }
+    }
+  }
+  const size_t plane = (size_t)nodes;
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (r0 + r < a.nrhs && !rhs_idle(a.gate, a.gdiv, r0 + r)) grid[(size_t)(r0 + r) * plane + node] = acc[r];
+}
+
 // One workgroup per particle: E_i += prod(h) sum over its stencil of
 // w r E~(node) for each right-hand side (out[rhs][i][3], caller's order).
 template <typename T>

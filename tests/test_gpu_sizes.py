@@ -97,6 +97,36 @@ def test_sizes_operator_continuous_radii_fp32_and_batched(gpu):
     assert _rel(o32, o64) < 2e-6
 
 
+def test_sizes_spread_gather_matches_scatter_and_follows_geometry(gpu):
+    """The per-node gather spread (default) against the per-particle atomic
+    scatter (engine knob 2) on 20 right-hand sides (partial groups of 16 and
+    8), then after moving the particles and changing the radii (the node
+    index and its weights must be rebuilt)."""
+    from mpol_hip import FP32, FP64
+    x, box, a = _config(40, 0.15, 8, radii=None)
+    c = Case(gpu, x, box, a, closed_form=True)
+    p = np.random.default_rng(4).standard_normal((20, c.n, 3))
+    c.ctx.set_permittivity(np.stack([c.eps * (1.0 + 0.1 * k) for k in range(20)]))
+    for prec, bar in ((FP64, 1e-13), (FP32, 1e-6)):
+        c.ctx.set_tuning(precision=prec, engine=0)
+        gather = c.ctx.sizes_matvec(p, precision=prec).cpu().numpy()
+        c.ctx.set_tuning(precision=prec, engine=2)
+        scatter = c.ctx.sizes_matvec(p, precision=prec).cpu().numpy()
+        c.ctx.set_tuning(precision=prec, engine=0)
+        assert _rel(gather, scatter) < bar, prec
+    rng = np.random.default_rng(9)
+    x2 = np.mod(x + 0.05 * rng.standard_normal(x.shape), box)
+    a2 = a * rng.uniform(0.9, 1.0, a.size)
+    geo = so.SizesGeometry(x2, a2, box, c.xi, c.tol, None, None, None, exact=True)
+    c.ctx.set_geometry(x2, box, geo.Ngrid, geo.P, c.xi, np.full(3, geo.eta), geo.rc)
+    c.ctx.set_sizes(a2)
+    c.ctx.set_permittivity(c.eps)
+    out = c.ctx.sizes_matvec(p[:1]).cpu().numpy()[0]
+    assert _rel(out, so.field(geo, p[0], c.eps).real) < 1e-12
+    out32 = c.ctx.sizes_matvec(p[:1], precision=FP32).cpu().numpy()[0]
+    assert _rel(out32, out) < 2e-6
+
+
 def test_sizes_operator_complex_permittivity(gpu):
     x, box, a = _config(18, 0.1, 8)
     rng = np.random.default_rng(4)
