@@ -882,9 +882,13 @@ int run_operator_sizes(mpol_ctx* c, const double* m, double* out, double* hr, in
     else ent = c->sz_nw32.p;
     hipLaunchKernelGGL((k_sizes_permute_pm<T>), dim3(cdiv((size_t)n * nrhs, 256)), dim3(256), 0, st, m, c->perm.p,
                        n, nrhs, pm, gate, rps);
-    constexpr int R = sizeof(T) == 8 ? 8 : 16;
-    hipLaunchKernelGGL((k_sizes_spread_nodes<T, R>), dim3(cdiv(G, 256), cdiv(nrhs, R)), dim3(256), 0, st, sa,
-                       c->sz_noff.p, ent, pm, grid);
+#define LAUNCH_SPREAD_NODES(W)                                                                           \
+  hipLaunchKernelGGL((k_sizes_spread_nodes<T, W>), dim3(cdiv((int64_t)G * W, 256), cdiv(nrhs, W)), dim3(256), 0, st, \
+                     sa, c->sz_noff.p, ent, pm, grid)
+    if (nrhs <= 4) LAUNCH_SPREAD_NODES(4);
+    else if (nrhs <= 8) LAUNCH_SPREAD_NODES(8);
+    else LAUNCH_SPREAD_NODES(16);
+#undef LAUNCH_SPREAD_NODES
   } else {
     // per-particle scatter with atomics (engine knob 2, or index sizes beyond
     // 32 bits)

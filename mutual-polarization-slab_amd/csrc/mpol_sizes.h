@@ -392,42 +392,33 @@ __global__ void k_sizes_permute_pm(const double* __restrict__ m, const int* __re
   pm[t] = v;
 }
 
-// grid[rhs][node] = sum over the node's entries of (w r) . p for R
-// right-hand sides per thread (blockIdx.y picks the group); one thread per
-// node, consecutive nodes on consecutive lanes.  Idle right-hand sides are
-// not written.
-template <typename T, int R>
+// grid[rhs][node] = sum over the node's entries of (w r) . p.  W lanes per
+// node, one right-hand side each (blockIdx.y picks the group of W), 64 / W
+// consecutive nodes per wave: an entry's dipoles of the W right-hand sides
+// are W consecutive 4-vectors of pm (one or a few cache lines), and the
+// nodes of a wave share most particles.  Idle right-hand sides are not
+// written.
+template <typename T, int W>
 __global__ void __launch_bounds__(256) k_sizes_spread_nodes(SizesGridArgs a, const int* __restrict__ off,
                                                             const typename Vec4<T>::type* __restrict__ ent,
                                                             const typename Vec4<T>::type* __restrict__ pm,
                                                             T* __restrict__ grid) {
   using V = typename Vec4<T>::type;
   const int nodes = a.nx * a.ny * a.nz;
-  const int node = blockIdx.x * blockDim.x + threadIdx.x;
-  const int r0 = blockIdx.y * R;
-  bool live = false;
-#pragma unroll
-  for (int r = 0; r < R; ++r) live |= (r0 + r < a.nrhs) && !rhs_idle(a.gate, a.gdiv, r0 + r);
-  if (node >= nodes || !live) return;
-  T acc[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) acc[r] = (T)0;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int node = (int)(t / W);
+  const int rhs = blockIdx.y * W + (int)(t % W);
+  if (node >= nodes || rhs >= a.nrhs || rhs_idle(a.gate, a.gdiv, rhs)) return;
+  const V* q = pm + rhs;
+  T acc = (T)0;
   const int hi = off[node + 1];
+#pragma unroll 4
   for (int k = off[node]; k < hi; ++k) {
     const V e = ent[k];
-    const V* q = pm + (size_t)Vec4<T>::index(e.w) * a.nrhs + r0;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      if (r0 + r < a.nrhs) {
-        const V v = q[r];
-        acc[r] += e.x * v.x + e.y * v.y + e.z * v.z;
-      }
-    }
+    const V v = q[(size_t)Vec4<T>::index(e.w) * a.nrhs];
+    acc += e.x * v.x + e.y * v.y + e.z * v.z;
   }
-  const size_t plane = (size_t)nodes;
-#pragma unroll
-  for (int r = 0; r < R; ++r)
-    if (r0 + r < a.nrhs && !rhs_idle(a.gate, a.gdiv, r0 + r)) grid[(size_t)(r0 + r) * plane + node] = acc[r];
+  grid[(size_t)rhs * nodes + node] = acc;
 }
 
 // One workgroup per particle: E_i += prod(h) sum over its stencil of
