@@ -262,9 +262,9 @@ struct mpol_ctx {
   bool sz_nodes_valid = false, sz_nw_valid[2] = {false, false};
   int sz_entries = 0;
   DevBuf<int> sz_ncnt, sz_noff, sz_nidx;
-  DevBuf<float4> sz_nw32;
-  DevBuf<double4> sz_nw64;
-  DevBuf<unsigned char> sz_pm;
+  DevBuf<float4> sz_nw32, sz_pw32;
+  DevBuf<double4> sz_nw64, sz_pw64;  // node-major (nw) and particle-major (pw) entries
+  DevBuf<unsigned char> sz_pm, sz_gridT;
   // vector-Jacobian product workspaces
   DevBuf<double> vjp_in, vjp_out, vjp_grid;
   // workspaces
@@ -810,15 +810,22 @@ int ensure_sizes_nodes(mpol_ctx* c, const SizesGridArgs& sa) {
   const int pi = sizeof(T) == 8 ? 1 : 0;
   if (!c->sz_nw_valid[pi]) {
     typename Vec4<T>::type* ent;
+    typename Vec4<T>::type* pent;
+    const size_t pcount = (size_t)sa.n * sa.P * sa.P * sa.P;
     if constexpr (sizeof(T) == 8) {
       HIPALLOC(c->sz_nw64, (size_t)std::max(1, c->sz_entries));
+      HIPALLOC(c->sz_pw64, pcount);
       ent = c->sz_nw64.p;
+      pent = c->sz_pw64.p;
     } else {
       HIPALLOC(c->sz_nw32, (size_t)std::max(1, c->sz_entries));
+      HIPALLOC(c->sz_pw32, pcount);
       ent = c->sz_nw32.p;
+      pent = c->sz_pw32.p;
     }
     hipLaunchKernelGGL((k_sizes_node_weights<T>), dim3(cdiv(nodes, 256)), dim3(256), 0, st, sa, c->sz_noff.p,
                        c->sz_nidx.p, ent);
+    hipLaunchKernelGGL((k_sizes_particle_weights<T>), dim3(sa.n), dim3(256), 0, st, sa, pent);
     HIPCHK(hipGetLastError());
     c->sz_nw_valid[pi] = true;
   }
@@ -871,7 +878,10 @@ int run_operator_sizes(mpol_ctx* c, const double* m, double* out, double* hr, in
   sa.base = c->base64.p;
   sa.rad = c->sz_rad_sorted.p;
   const int P3 = c->P * c->P * c->P;
-  if (knobs<T>(c)[6] != 2 && (double)n * P3 < 2.0e9 && (double)G < 2.0e9) {
+  // node-major spread and particle-major contraction from stored entries
+  // (2 x 16 B per stencil point in FP32, 2 x 32 B in FP64)
+  const bool by_entries = knobs<T>(c)[6] != 2 && (double)n * P3 <= 4.0e8 && (double)G < 2.0e9;
+  if (by_entries) {
     // gather by node: every live right-hand side's grid is written whole
     CHECK_STATUS(ensure_sizes_nodes<T>(c, sa));
     using V = typename Vec4<T>::type;
@@ -890,8 +900,8 @@ int run_operator_sizes(mpol_ctx* c, const double* m, double* out, double* hr, in
     else LAUNCH_SPREAD_NODES(16);
 #undef LAUNCH_SPREAD_NODES
   } else {
-    // per-particle scatter with atomics (engine knob 2, or index sizes beyond
-    // 32 bits)
+    // per-particle scatter with atomics (engine knob 2, or more stencil
+    // points than the entries are kept for)
     hipLaunchKernelGGL(k_permute_m<3>, dim3(cdiv(n, 256), nrhs), dim3(256), 0, st, m, c->perm.p, n, c->ms.p, gate,
                        rps);
     HIPCHK(hipMemsetAsync(grid, 0, (size_t)nrhs * G * sizeof(T), st));
@@ -974,7 +984,23 @@ int run_operator_sizes(mpol_ctx* c, const double* m, double* out, double* hr, in
   DISPATCH_FFT(c->ng[0], ta.plan.e, LAUNCH_C2R, 0);
 #undef LAUNCH_C2R
   const size_t clds = (size_t)c->P * c->P * c->P * (3 * sizeof(T) + sizeof(int));
-  if (clds <= kSizesContractLds) {
+  if (by_entries) {
+    HIPALLOC(c->sz_gridT, (size_t)nrhs * G * sizeof(T));
+    T* gT = reinterpret_cast<T*>(c->sz_gridT.p);
+    hipLaunchKernelGGL((k_sizes_grid_transpose<T>), dim3(cdiv(G, 64), cdiv(nrhs, 16)), dim3(256), 0, st, grid, (int)G,
+                       nrhs, gT);
+    using V = typename Vec4<T>::type;
+    const V* pent;
+    if constexpr (sizeof(T) == 8) pent = c->sz_pw64.p;
+    else pent = c->sz_pw32.p;
+#define LAUNCH_CONTRACT_NODES(W)                                                                          \
+  hipLaunchKernelGGL((k_sizes_contract_nodes<T, W>), dim3(cdiv((int64_t)n * 64, 256), cdiv(nrhs, W)), dim3(256), 0, \
+                     st, sa, pent, gT, out)
+    if (nrhs <= 4) LAUNCH_CONTRACT_NODES(4);
+    else if (nrhs <= 8) LAUNCH_CONTRACT_NODES(8);
+    else LAUNCH_CONTRACT_NODES(16);
+#undef LAUNCH_CONTRACT_NODES
+  } else if (clds <= kSizesContractLds) {
     set_lds(k_sizes_contract_lds<T>, clds);
     hipLaunchKernelGGL((k_sizes_contract_lds<T>), dim3(n), dim3(256), clds, st, sa, grid, out);
   } else {

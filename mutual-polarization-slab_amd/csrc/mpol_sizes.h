@@ -421,6 +421,101 @@ __global__ void __launch_bounds__(256) k_sizes_spread_nodes(SizesGridArgs a, con
   grid[(size_t)rhs * nodes + node] = acc;
 }
 
+// Particle-major entries for the contraction: pent[s][o] = {w r_x, w r_y,
+// w r_z, node bits} for stencil point o of sorted particle s (one workgroup
+// per particle; the same expressions as the per-particle kernels).
+template <typename T>
+__global__ void __launch_bounds__(256) k_sizes_particle_weights(SizesGridArgs a,
+                                                                typename Vec4<T>::type* __restrict__ pent) {
+  const int s = blockIdx.x;
+  if (s >= a.n) return;
+  const int4 rc = a.rec[s];
+  const double4 b = a.base[s];
+  const T rad = (T)a.rad[s];
+  const int P3 = a.P * a.P * a.P;
+  for (int o = threadIdx.x; o < P3; o += blockDim.x) {
+    int ox, oy, oz;
+    const int node = sizes_node(a, rc, o, ox, oy, oz);
+    const T r0 = (T)(b.x + (double)(ox - a.half) * a.h[0]);
+    const T r1 = (T)(b.y + (double)(oy - a.half) * a.h[1]);
+    const T r2 = (T)(b.z + (double)(oz - a.half) * a.h[2]);
+    const T d = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
+    const T w = sizes_weight<T>(d, rad, (T)a.xi, (T)a.eta);
+    typename Vec4<T>::type e;
+    e.x = w * r0;
+    e.y = w * r1;
+    e.z = w * r2;
+    e.w = Vec4<T>::bits(node);
+    pent[(size_t)s * P3 + o] = e;
+  }
+}
+
+// gT[node][rhs] = grid[rhs][node]: tiles of 64 nodes x 16 right-hand sides
+// through LDS (reads along nodes, writes along right-hand sides).
+template <typename T>
+__global__ void __launch_bounds__(256) k_sizes_grid_transpose(const T* __restrict__ grid, int nodes, int nrhs,
+                                                              T* __restrict__ gT) {
+  __shared__ T tile[16][65];
+  const int n0 = blockIdx.x * 64, r0 = blockIdx.y * 16;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int i = ty; i < 16; i += 4) {
+    const int r = r0 + i, node = n0 + tx;
+    if (r < nrhs && node < nodes) tile[i][tx] = grid[(size_t)r * nodes + node];
+  }
+  __syncthreads();
+  const int ri = threadIdx.x & 15, nj = threadIdx.x >> 4;
+  for (int j = nj; j < 64; j += 16) {
+    const int r = r0 + ri, node = n0 + j;
+    if (r < nrhs && node < nodes) gT[(size_t)node * nrhs + r] = tile[ri][j];
+  }
+}
+
+// E_i += prod(h) sum over particle i's stencil of (w r) g(node) with the
+// transposed grid: one wave per (particle, group of W right-hand sides), lane
+// = k W + r for right-hand side r of the group and stencil split k (points
+// o = k, k + 64/W, ...), so the W values of a node are one contiguous read
+// and a lane walks P^3 W/64 points.  The splits are summed by a fixed
+// butterfly, then in FP64.  out[rhs][i][3] in the caller's order.
+template <typename T, int W>
+__global__ void __launch_bounds__(256) k_sizes_contract_nodes(SizesGridArgs a,
+                                                              const typename Vec4<T>::type* __restrict__ pent,
+                                                              const T* __restrict__ gT, double* __restrict__ out) {
+  using V = typename Vec4<T>::type;
+  constexpr int K = 64 / W;
+  const int lane = threadIdx.x & 63;
+  const int s = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int r = lane % W, k = lane / W;
+  const int rhs = blockIdx.y * W + r;
+  if (s >= a.n) return;  // uniform over the wave
+  const bool on = rhs < a.nrhs && !rhs_idle(a.gate, a.gdiv, rhs);
+  const int P3 = a.P * a.P * a.P;
+  const V* e = pent + (size_t)s * P3;
+  const T* g = gT + (on ? rhs : 0);
+  T acc0 = 0, acc1 = 0, acc2 = 0;
+  if (on) {
+#pragma unroll 4
+    for (int o = k; o < P3; o += K) {
+      const V w = e[o];
+      const T v = g[(size_t)Vec4<T>::index(w.w) * a.nrhs];
+      acc0 += w.x * v;
+      acc1 += w.y * v;
+      acc2 += w.z * v;
+    }
+  }
+#pragma unroll
+  for (int off = W; off < 64; off <<= 1) {
+    acc0 += __shfl_xor(acc0, off);
+    acc1 += __shfl_xor(acc1, off);
+    acc2 += __shfl_xor(acc2, off);
+  }
+  if (on && k == 0) {
+    double* o3 = out + ((size_t)rhs * a.n + a.rec[s].w) * 3;
+    o3[0] += (double)acc0 * a.hvol;
+    o3[1] += (double)acc1 * a.hvol;
+    o3[2] += (double)acc2 * a.hvol;
+  }
+}
+
 // One workgroup per particle: E_i += prod(h) sum over its stencil of
 // w r E~(node) for each right-hand side (out[rhs][i][3], caller's order).
 template <typename T>

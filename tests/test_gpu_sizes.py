@@ -97,11 +97,13 @@ def test_sizes_operator_continuous_radii_fp32_and_batched(gpu):
     assert _rel(o32, o64) < 2e-6
 
 
-def test_sizes_spread_gather_matches_scatter_and_follows_geometry(gpu):
-    """The per-node gather spread (default) against the per-particle atomic
-    scatter (engine knob 2) on 20 right-hand sides (partial groups of 16 and
-    8), then after moving the particles and changing the radii (the node
-    index and its weights must be rebuilt)."""
+def test_sizes_entry_kernels_match_particle_kernels_and_follow_geometry(gpu):
+    """The operator from stored entries (default: per-node gather spread,
+    particle-major contraction of the transposed grid) against the
+    per-particle kernels (engine knob 2: atomic scatter, LDS-staged
+    contraction) on 20 right-hand sides (a partial group of 16), then after
+    moving the particles and changing the radii (the entries must be
+    rebuilt)."""
     from mpol_hip import FP32, FP64
     x, box, a = _config(40, 0.15, 8, radii=None)
     c = Case(gpu, x, box, a, closed_form=True)
