@@ -40,10 +40,11 @@ int check_ready(mpol_ctx* c) {
 }
 
 // Partial-sum blocks per right-hand side of the reductions over vectors of
-// n elements: at least 1024 elements per block and about 1024 blocks in all
-// (at most kDotBlocks per right-hand side).
+// n elements: at least 256 elements per block (one per thread) and about
+// 1024 blocks in all (at most kDotBlocks per right-hand side), so that short
+// batched vectors still keep enough loads in flight.
 int dot_blocks(int64_t n, int nrhs) {
-  const int64_t by_total = (1024 + nrhs - 1) / nrhs, by_len = (n + 1023) / 1024;
+  const int64_t by_total = (1024 + nrhs - 1) / nrhs, by_len = (n + 255) / 256;
   return (int)std::max<int64_t>(1, std::min<int64_t>({by_total, by_len, (int64_t)kDotBlocks}));
 }
 

@@ -13,34 +13,46 @@ namespace mpol {
 
 constexpr int kDotBlocks = 256;  // partial-sum blocks per right-hand side (one per CU)
 
+// Vectors whose per-wave sums are staged in LDS between two barriers, and
+// vectors read together in one sweep over a block's chunk.
+constexpr int kDotGroup = 64;
+constexpr int kDotSweep = 8;
+
 // Partial dot products of one block's chunk [lo, hi) of right-hand side rhs:
 // partial[(rhs*cnt + i)*gridDim.x + blockIdx.x] = sum over the chunk of
-// a_i . w, a_i = A + i*stride_i + rhs*n (four vectors per sweep, w read once
-// per group).  Thread t handles elements lo + t, lo + t + blockDim.x, ...
+// a_i . w, a_i = A + i*stride_i + rhs*n (kDotSweep vectors per sweep, w read
+// once per sweep).  Thread t handles elements lo + t, lo + t + blockDim.x, ...;
+// each wave's sums go to LDS (red[vector][wave]) for up to kDotGroup vectors
+// between two barriers, so the sweeps of a group follow each other without
+// waiting; the four wave sums are added in a fixed order.
 __device__ __forceinline__ void block_dots(const double* __restrict__ A, size_t stride_i, int cnt,
                                            const double* __restrict__ wr, size_t roff, int64_t lo, int64_t hi,
                                            int rhs, double* __restrict__ partial, double (*red)[4]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int i0 = 0; i0 < cnt; i0 += 4) {
-    double s[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int64_t k = lo + threadIdx.x; k < hi; k += blockDim.x) {
-      const double wv = wr[k];
+  for (int g0 = 0; g0 < cnt; g0 += kDotGroup) {
+    const int gc = min(kDotGroup, cnt - g0);
+    for (int i0 = 0; i0 < gc; i0 += kDotSweep) {
+      double s[kDotSweep];
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (i0 + u < cnt) s[u] += A[(size_t)(i0 + u) * stride_i + roff + k] * wv;
-    }
+      for (int u = 0; u < kDotSweep; ++u) s[u] = 0.0;
+      for (int64_t k = lo + threadIdx.x; k < hi; k += blockDim.x) {
+        const double wv = wr[k];
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
-      for (int o = 32; o > 0; o >>= 1) s[u] += __shfl_xor(s[u], o);
-    if (lane == 0) {
+        for (int u = 0; u < kDotSweep; ++u)
+          if (i0 + u < gc) s[u] += A[(size_t)(g0 + i0 + u) * stride_i + roff + k] * wv;
+      }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) red[u][wave] = s[u];
+      for (int u = 0; u < kDotSweep; ++u)
+        for (int o = 32; o > 0; o >>= 1) s[u] += __shfl_xor(s[u], o);
+      if (lane == 0) {
+#pragma unroll
+        for (int u = 0; u < kDotSweep; ++u)
+          if (i0 + u < gc) red[i0 + u][wave] = s[u];
+      }
     }
     __syncthreads();
-    if ((int)threadIdx.x < 4 && i0 + (int)threadIdx.x < cnt) {
-      const int u = threadIdx.x;
-      partial[((size_t)rhs * cnt + i0 + u) * gridDim.x + blockIdx.x] = (red[u][0] + red[u][1]) + (red[u][2] + red[u][3]);
-    }
+    for (int u = threadIdx.x; u < gc; u += blockDim.x)
+      partial[((size_t)rhs * cnt + g0 + u) * gridDim.x + blockIdx.x] = (red[u][0] + red[u][1]) + (red[u][2] + red[u][3]);
     __syncthreads();
   }
 }
@@ -56,7 +68,7 @@ __device__ __forceinline__ void block_chunk(int64_t n, int64_t& lo, int64_t& hi)
 static __global__ void __launch_bounds__(256) k_dots(const double* __restrict__ A, size_t stride_i, int cnt,
                                               const double* __restrict__ w, int64_t n, int nrhs,
                                               double* __restrict__ partial) {
-  __shared__ double red[4][4];  // [vector of the group][wave]
+  __shared__ double red[kDotGroup][4];  // [vector of the group][wave]
   const int rhs = blockIdx.y;
   int64_t lo, hi;
   block_chunk(n, lo, hi);
@@ -85,7 +97,7 @@ static __global__ void __launch_bounds__(256) k_sub_dots(const double* __restric
                                                   const double* __restrict__ h, double* __restrict__ w, int64_t n,
                                                   const int* __restrict__ state, int next_norm,
                                                   double* __restrict__ partial) {
-  __shared__ double red[4][4];
+  __shared__ double red[kDotGroup][4];
   const int rhs = blockIdx.y;
   if (state[rhs] != kIter) return;  // uniform over the block
   int64_t lo, hi;
@@ -175,31 +187,31 @@ static __global__ void k_arnoldi_step(int nrhs, int restart, int j, const double
   const int rhs = blockIdx.x * blockDim.x + threadIdx.x;
   if (rhs >= nrhs || state[rhs] != kIter) return;
   double* Hr = H + (size_t)rhs * (restart + 1) * restart;
-  for (int i = 0; i <= j; ++i) Hr[i * restart + j] = h1[rhs * (j + 1) + i] + h2[rhs * (j + 1) + i];
-  Hr[(j + 1) * restart + j] = sqrt(hnorm2[rhs]);
   double* c = cs + (size_t)rhs * restart;
   double* s = sn + (size_t)rhs * restart;
   double* gr = g + (size_t)rhs * (restart + 1);
+  const double* h1r = h1 + (size_t)rhs * (j + 1);
+  const double* h2r = h2 + (size_t)rhs * (j + 1);
+  // the earlier rotations on the new column h1 + h2: rotation i mixes entries
+  // i and i + 1, so entry i + 1 enters unrotated and the running entry stays
+  // in a register (no store-to-load chain through H)
+  double a = h1r[0] + h2r[0];
   for (int i = 0; i < j; ++i) {
-    const double a = Hr[i * restart + j], b = Hr[(i + 1) * restart + j];
+    const double b = h1r[i + 1] + h2r[i + 1];
     Hr[i * restart + j] = c[i] * a + s[i] * b;
-    Hr[(i + 1) * restart + j] = -s[i] * a + c[i] * b;
+    a = -s[i] * a + c[i] * b;
   }
-  const double a = Hr[j * restart + j], b = Hr[(j + 1) * restart + j];
+  const double b = sqrt(hnorm2[rhs]);
   const bool breakdown = (b == 0.0);
   const double den = sqrt(a * a + b * b);
-  if (den == 0.0) {
-    c[j] = 1.0;
-    s[j] = 0.0;
-  } else {
-    c[j] = a / den;
-    s[j] = b / den;
-  }
-  Hr[j * restart + j] = c[j] * a + s[j] * b;
+  const double cj = den == 0.0 ? 1.0 : a / den, sj = den == 0.0 ? 0.0 : b / den;
+  c[j] = cj;
+  s[j] = sj;
+  Hr[j * restart + j] = cj * a + sj * b;
   Hr[(j + 1) * restart + j] = 0.0;
   const double gj = gr[j];
-  gr[j] = c[j] * gj;
-  gr[j + 1] = -s[j] * gj;
+  gr[j] = cj * gj;
+  gr[j + 1] = -sj * gj;
   const double rel = fabs(gr[j + 1]) / bnorm[rhs];
   relest[rhs] = rel;
   if (!(rel <= tol || j == restart - 1 || breakdown)) return;
