@@ -19,7 +19,10 @@ Ngrid = ceil(1 + L qc/pi), isotropic eta, P = ceil(2/mean(h) - 2 ln tol/pi))
 is G, subject to rc <= L/2.  With tune=True each candidate's batched FP32
 operator application is timed on the configuration itself and the fastest is
 kept; the rule's tolerance, and so the accuracy class, is the same for all
-candidates.
+candidates.  Candidates are timed in increasing size and, with early_stop,
+the search ends at the first one slower than the best so far (a larger grid
+trades real-space pairs for grid work, so the time is usually unimodal in
+the size).
 
 solve_configurations(configs, eps, fields) -> BatchResult
     configs: [{"x": (N,3), "a": (N,), "box": (3,)}], eps: (Nk,) permittivity
@@ -118,7 +121,8 @@ class _Worker:
         ctx.set_sizes(a)
         ctx.set_permittivity(sets)
 
-    def run(self, cfg, eps, fields, tol, gmres_tol, precision, tune, sizes, repeats, grid=None, tune_only=False):
+    def run(self, cfg, eps, fields, tol, gmres_tol, precision, tune, sizes, repeats, grid=None, tune_only=False,
+            early_stop=True):
         torch = self.torch
         x = np.asarray(cfg["x"], dtype=np.float64)
         a = np.asarray(cfg["a"], dtype=np.float64).reshape(-1)
@@ -147,6 +151,7 @@ class _Worker:
                 best_ms = math.inf
                 for cand in cands:
                     self._setup(x, a, box, cand, sets)
+                    current = cand
                     self.ctx.sizes_matvec(gt, precision=FP32)  # warm-up
                     self.stream.synchronize()
                     t1 = time.perf_counter()
@@ -157,7 +162,8 @@ class _Worker:
                     report.append((int(cand[1][0]), ms))
                     if ms < best_ms:
                         best, best_ms = cand, ms
-                current = cands[-1]
+                    elif early_stop:
+                        break
                 self.tune_s += time.perf_counter() - t0
             if tune_only:
                 return None, best, None, report
@@ -175,13 +181,15 @@ class _Worker:
 
 
 def solve_configurations(configs, eps, fields=None, *, tol=1e-3, gmres_tol=None, precision="mixed", workers=8,
-                         tune=True, sizes=GRID_SIZES, repeats=2, grids=None, device=None, _tune_only=False):
+                         tune=True, sizes=GRID_SIZES, repeats=2, grids=None, device=None, early_stop=True,
+                         _tune_only=False):
     """Solve every configuration over the spectrum eps (Nk,) x fields (nf,3)
     (default the three axes) with the polydisperse operator; see the module
     docstring.  tol is the Ewald rule's tolerance, gmres_tol (default tol) the
     relative residual of the solves; precision "mixed" (FP32 operator inside
     FP64 refinement) or "fp64".  grids (count,), if given, fixes each
-    configuration's grid size instead of choosing it."""
+    configuration's grid size instead of choosing it; early_stop ends a
+    configuration's grid search at the first candidate slower than the best."""
     import torch
 
     from .context import require_device
@@ -207,7 +215,7 @@ def solve_configurations(configs, eps, fields=None, *, tol=1e-3, gmres_tol=None,
                 return
             try:
                 results[i] = w.run(configs[i], eps, fields, tol, gmres_tol, precision, tune, sizes, repeats,
-                                   None if grids is None else grids[i], _tune_only)
+                                   None if grids is None else grids[i], _tune_only, early_stop)
             except Exception as e:  # surfaced on the calling thread
                 errors.append(e)
                 return
@@ -239,11 +247,12 @@ def solve_configurations(configs, eps, fields=None, *, tol=1e-3, gmres_tol=None,
         seconds=seconds, rhs=count * nk * nf, tune_seconds=tune_s, report=[r[3] for r in results])
 
 
-def tune_grids(configs, eps, fields=None, *, tol=1e-3, workers=1, sizes=GRID_SIZES, repeats=2, device=None):
+def tune_grids(configs, eps, fields=None, *, tol=1e-3, workers=1, sizes=GRID_SIZES, repeats=2, device=None,
+               early_stop=True):
     """Grid size per configuration (fastest batched FP32 operator among the
-    candidates), without solving: (grids (count,), timings [(G, ms)] per
-    configuration, seconds).  One worker by default, so that the timings are
-    not shared with other configurations' work.  Pass grids to
+    candidates timed), without solving: (grids (count,), timings [(G, ms)]
+    per configuration, seconds).  One worker by default, so that the timings
+    are not shared with other configurations' work.  Pass grids to
     solve_configurations."""
     return solve_configurations(configs, eps, fields, tol=tol, workers=workers, tune=True, sizes=sizes,
-                                repeats=repeats, device=device, _tune_only=True)
+                                repeats=repeats, device=device, early_stop=early_stop, _tune_only=True)

@@ -13,7 +13,7 @@ the FP64 operator throughout, and with --plain-sample K, K configurations
 solved one right-hand side at a time in FP64 on one stream (the way the
 reference solves them), extrapolated to the set by particle count.
 
-    python -m workloads.heterogeneous [--count 256] [--workers 4] [--check 4]
+    python -m workloads.heterogeneous [--count 256] [--workers 8] [--check 4]
 """
 from __future__ import annotations
 

@@ -52,3 +52,22 @@ def test_batch_mixed_precision_close_to_fp64(gpu):
     for i in range(2):
         assert _rel(r32.p[i], r64.p[i]) < 1e-5
         assert r32.matvecs[i][1] > 0  # the bulk ran on the FP32 operator
+
+
+def test_tune_grids_reports_and_early_stop(gpu):
+    """tune_grids returns, per configuration, the fastest timed candidate;
+    without early_stop every candidate is timed, with it the search ends at
+    the first candidate slower than the best so far."""
+    configs = _configs()[:2]
+    eps = np.array([0.5, 4.0])
+    full, rep_full, _ = batch.tune_grids(configs, eps, early_stop=False, device=gpu)
+    short, rep_short, _ = batch.tune_grids(configs, eps, device=gpu)
+    for i, c in enumerate(configs):
+        sizes_ok = [int(cand[1][0]) for cand in batch.grid_candidates(c["box"], 1e-3)]
+        assert [g for g, _ in rep_full[i]] == sizes_ok
+        assert full[i] == min(rep_full[i], key=lambda t: t[1])[0]
+        times = [ms for _, ms in rep_short[i]]
+        assert [g for g, _ in rep_short[i]] == sizes_ok[:len(times)]
+        # every candidate timed before the last one improved on the best so far
+        assert all(times[k] < min(times[:k], default=np.inf) for k in range(len(times) - 1))
+        assert short[i] == min(rep_short[i], key=lambda t: t[1])[0]
