@@ -892,9 +892,11 @@ int run_operator_sizes(mpol_ctx* c, const double* m, double* out, double* hr, in
     else ent = c->sz_nw32.p;
     hipLaunchKernelGGL((k_sizes_permute_pm<T>), dim3(cdiv((size_t)n * nrhs, 256)), dim3(256), 0, st, m, c->perm.p,
                        n, nrhs, pm, gate, rps);
-#define LAUNCH_SPREAD_NODES(W)                                                                           \
-  hipLaunchKernelGGL((k_sizes_spread_nodes<T, W>), dim3(cdiv((int64_t)G * W, 256), cdiv(nrhs, W)), dim3(256), 0, st, \
-                     sa, c->sz_noff.p, ent, pm, grid)
+    // one right-hand side per lane: more waves in flight hide the entry ->
+    // dipole load chain better than several right-hand sides per lane
+#define LAUNCH_SPREAD_NODES(W)                                                                         \
+  hipLaunchKernelGGL((k_sizes_spread_nodes<T, W, 1>), dim3(cdiv((int64_t)G * W, 256), cdiv(nrhs, W)), dim3(256), 0, \
+                     st, sa, c->sz_noff.p, ent, pm, grid)
     if (nrhs <= 4) LAUNCH_SPREAD_NODES(4);
     else if (nrhs <= 8) LAUNCH_SPREAD_NODES(8);
     else LAUNCH_SPREAD_NODES(16);
@@ -985,20 +987,22 @@ int run_operator_sizes(mpol_ctx* c, const double* m, double* out, double* hr, in
 #undef LAUNCH_C2R
   const size_t clds = (size_t)c->P * c->P * c->P * (3 * sizeof(T) + sizeof(int));
   if (by_entries) {
-    HIPALLOC(c->sz_gridT, (size_t)nrhs * G * sizeof(T));
+    const int ldg = (nrhs + 3) / 4 * 4;  // aligned 4-vector reads of a node's right-hand sides
+    HIPALLOC(c->sz_gridT, (size_t)ldg * G * sizeof(T));
     T* gT = reinterpret_cast<T*>(c->sz_gridT.p);
     hipLaunchKernelGGL((k_sizes_grid_transpose<T>), dim3(cdiv(G, 64), cdiv(nrhs, 16)), dim3(256), 0, st, grid, (int)G,
-                       nrhs, gT);
+                       nrhs, ldg, gT);
     using V = typename Vec4<T>::type;
     const V* pent;
     if constexpr (sizeof(T) == 8) pent = c->sz_pw64.p;
     else pent = c->sz_pw32.p;
-#define LAUNCH_CONTRACT_NODES(W)                                                                          \
-  hipLaunchKernelGGL((k_sizes_contract_nodes<T, W>), dim3(cdiv((int64_t)n * 64, 256), cdiv(nrhs, W)), dim3(256), 0, \
-                     st, sa, pent, gT, out)
-    if (nrhs <= 4) LAUNCH_CONTRACT_NODES(4);
-    else if (nrhs <= 8) LAUNCH_CONTRACT_NODES(8);
-    else LAUNCH_CONTRACT_NODES(16);
+    // a wave: 64 / (K W) particles x K stencil splits x W lanes of 4 right-hand sides
+#define LAUNCH_CONTRACT_NODES(W, K)                                                                        \
+  hipLaunchKernelGGL((k_sizes_contract_nodes<T, W, K>), dim3(cdiv(cdiv(n, 64 / ((K) * (W))) * (int64_t)64, 256),  \
+                     cdiv(nrhs, 4 * (W))), dim3(256), 0, st, sa, pent, gT, ldg, out)
+    if (nrhs <= 4) LAUNCH_CONTRACT_NODES(1, 32);
+    else if (nrhs <= 8) LAUNCH_CONTRACT_NODES(2, 16);
+    else LAUNCH_CONTRACT_NODES(4, 8);
 #undef LAUNCH_CONTRACT_NODES
   } else if (clds <= kSizesContractLds) {
     set_lds(k_sizes_contract_lds<T>, clds);

@@ -393,12 +393,12 @@ __global__ void k_sizes_permute_pm(const double* __restrict__ m, const int* __re
 }
 
 // grid[rhs][node] = sum over the node's entries of (w r) . p.  W lanes per
-// node, one right-hand side each (blockIdx.y picks the group of W), 64 / W
-// consecutive nodes per wave: an entry's dipoles of the W right-hand sides
-// are W consecutive 4-vectors of pm (one or a few cache lines), and the
-// nodes of a wave share most particles.  Idle right-hand sides are not
-// written.
-template <typename T, int W>
+// node with RL consecutive right-hand sides each (blockIdx.y picks the group
+// of W RL), 64 / W consecutive nodes per wave: an entry's dipoles of a lane
+// are RL consecutive 4-vectors of pm, the nodes of a wave share most
+// particles, and an entry load serves W RL right-hand sides.  Idle
+// right-hand sides are not written.
+template <typename T, int W, int RL>
 __global__ void __launch_bounds__(256) k_sizes_spread_nodes(SizesGridArgs a, const int* __restrict__ off,
                                                             const typename Vec4<T>::type* __restrict__ ent,
                                                             const typename Vec4<T>::type* __restrict__ pm,
@@ -407,18 +407,35 @@ __global__ void __launch_bounds__(256) k_sizes_spread_nodes(SizesGridArgs a, con
   const int nodes = a.nx * a.ny * a.nz;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int node = (int)(t / W);
-  const int rhs = blockIdx.y * W + (int)(t % W);
-  if (node >= nodes || rhs >= a.nrhs || rhs_idle(a.gate, a.gdiv, rhs)) return;
-  const V* q = pm + rhs;
-  T acc = (T)0;
+  const int r0 = (blockIdx.y * W + (int)(t % W)) * RL;
+  if (node >= nodes || r0 >= a.nrhs) return;
+  bool live[RL];
+  bool any = false;
+#pragma unroll
+  for (int u = 0; u < RL; ++u) {
+    live[u] = r0 + u < a.nrhs && !rhs_idle(a.gate, a.gdiv, r0 + u);
+    any |= live[u];
+  }
+  if (!any) return;
+  T acc[RL];
+#pragma unroll
+  for (int u = 0; u < RL; ++u) acc[u] = (T)0;
   const int hi = off[node + 1];
-#pragma unroll 4
+#pragma unroll 2
   for (int k = off[node]; k < hi; ++k) {
     const V e = ent[k];
-    const V v = q[(size_t)Vec4<T>::index(e.w) * a.nrhs];
-    acc += e.x * v.x + e.y * v.y + e.z * v.z;
+    const V* q = pm + (size_t)Vec4<T>::index(e.w) * a.nrhs + r0;
+#pragma unroll
+    for (int u = 0; u < RL; ++u) {
+      if (r0 + u < a.nrhs) {  // a particle's row ends at nrhs
+        const V v = q[u];
+        acc[u] += e.x * v.x + e.y * v.y + e.z * v.z;
+      }
+    }
   }
-  grid[(size_t)rhs * nodes + node] = acc;
+#pragma unroll
+  for (int u = 0; u < RL; ++u)
+    if (live[u]) grid[(size_t)(r0 + u) * nodes + node] = acc[u];
 }
 
 // Particle-major entries for the contraction: pent[s][o] = {w r_x, w r_y,
@@ -450,11 +467,12 @@ __global__ void __launch_bounds__(256) k_sizes_particle_weights(SizesGridArgs a,
   }
 }
 
-// gT[node][rhs] = grid[rhs][node]: tiles of 64 nodes x 16 right-hand sides
-// through LDS (reads along nodes, writes along right-hand sides).
+// gT[node][rhs] = grid[rhs][node] (row stride ldg >= nrhs, a multiple of 4):
+// tiles of 64 nodes x 16 right-hand sides through LDS (reads along nodes,
+// writes along right-hand sides).
 template <typename T>
 __global__ void __launch_bounds__(256) k_sizes_grid_transpose(const T* __restrict__ grid, int nodes, int nrhs,
-                                                              T* __restrict__ gT) {
+                                                              int ldg, T* __restrict__ gT) {
   __shared__ T tile[16][65];
   const int n0 = blockIdx.x * 64, r0 = blockIdx.y * 16;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
@@ -466,53 +484,66 @@ __global__ void __launch_bounds__(256) k_sizes_grid_transpose(const T* __restric
   const int ri = threadIdx.x & 15, nj = threadIdx.x >> 4;
   for (int j = nj; j < 64; j += 16) {
     const int r = r0 + ri, node = n0 + j;
-    if (r < nrhs && node < nodes) gT[(size_t)node * nrhs + r] = tile[ri][j];
+    if (r < nrhs && node < nodes) gT[(size_t)node * ldg + r] = tile[ri][j];
   }
 }
 
 // E_i += prod(h) sum over particle i's stencil of (w r) g(node) with the
-// transposed grid: one wave per (particle, group of W right-hand sides), lane
-// = k W + r for right-hand side r of the group and stencil split k (points
-// o = k, k + 64/W, ...), so the W values of a node are one contiguous read
-// and a lane walks P^3 W/64 points.  The splits are summed by a fixed
-// butterfly, then in FP64.  out[rhs][i][3] in the caller's order.
-template <typename T, int W>
+// transposed grid.  A wave holds 64 / (K W) particles; for each, K stencil
+// splits (points o = k, k + K, ...) x W lanes of RL consecutive right-hand
+// sides (blockIdx.y picks the group of W RL), so a lane reads a node's RL
+// values as one aligned 4-vector and an entry load serves W RL right-hand
+// sides.  The splits are summed by a fixed butterfly, then in FP64.
+// out[rhs][i][3] in the caller's order.
+template <typename T, int W, int K>
 __global__ void __launch_bounds__(256) k_sizes_contract_nodes(SizesGridArgs a,
                                                               const typename Vec4<T>::type* __restrict__ pent,
-                                                              const T* __restrict__ gT, double* __restrict__ out) {
+                                                              const T* __restrict__ gT, int ldg,
+                                                              double* __restrict__ out) {
   using V = typename Vec4<T>::type;
-  constexpr int K = 64 / W;
+  constexpr int RL = 4, PPW = 64 / (K * W);
+  static_assert(PPW * K * W == 64, "lanes");
   const int lane = threadIdx.x & 63;
-  const int s = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  const int r = lane % W, k = lane / W;
-  const int rhs = blockIdx.y * W + r;
-  if (s >= a.n) return;  // uniform over the wave
-  const bool on = rhs < a.nrhs && !rhs_idle(a.gate, a.gdiv, rhs);
+  const int64_t wv = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int p = lane / (K * W), k = (lane / W) % K, rl = lane % W;
+  const int s = (int)(wv * PPW) + p;
+  const int r0 = (blockIdx.y * W + rl) * RL;
+  const bool on = s < a.n && r0 < a.nrhs;  // every lane takes part in the butterfly
   const int P3 = a.P * a.P * a.P;
-  const V* e = pent + (size_t)s * P3;
-  const T* g = gT + (on ? rhs : 0);
-  T acc0 = 0, acc1 = 0, acc2 = 0;
+  T acc[RL][3];
+#pragma unroll
+  for (int u = 0; u < RL; ++u) acc[u][0] = acc[u][1] = acc[u][2] = (T)0;
   if (on) {
-#pragma unroll 4
+    const V* e = pent + (size_t)s * P3;
+    const T* g = gT + r0;
+#pragma unroll 2
     for (int o = k; o < P3; o += K) {
       const V w = e[o];
-      const T v = g[(size_t)Vec4<T>::index(w.w) * a.nrhs];
-      acc0 += w.x * v;
-      acc1 += w.y * v;
-      acc2 += w.z * v;
+      const V v = *reinterpret_cast<const V*>(g + (size_t)Vec4<T>::index(w.w) * ldg);
+      const T vv[RL] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int u = 0; u < RL; ++u) {
+        acc[u][0] += w.x * vv[u];
+        acc[u][1] += w.y * vv[u];
+        acc[u][2] += w.z * vv[u];
+      }
     }
   }
 #pragma unroll
-  for (int off = W; off < 64; off <<= 1) {
-    acc0 += __shfl_xor(acc0, off);
-    acc1 += __shfl_xor(acc1, off);
-    acc2 += __shfl_xor(acc2, off);
-  }
-  if (on && k == 0) {
-    double* o3 = out + ((size_t)rhs * a.n + a.rec[s].w) * 3;
-    o3[0] += (double)acc0 * a.hvol;
-    o3[1] += (double)acc1 * a.hvol;
-    o3[2] += (double)acc2 * a.hvol;
+  for (int off = W; off < K * W; off <<= 1)
+#pragma unroll
+    for (int u = 0; u < RL; ++u)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) acc[u][c] += __shfl_xor(acc[u][c], off);
+  if (!on || k != 0) return;
+  const int i = a.rec[s].w;
+#pragma unroll
+  for (int u = 0; u < RL; ++u) {
+    const int rhs = r0 + u;
+    if (rhs >= a.nrhs || rhs_idle(a.gate, a.gdiv, rhs)) continue;
+    double* o3 = out + ((size_t)rhs * a.n + i) * 3;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o3[c] += (double)acc[u][c] * a.hvol;
   }
 }
 
