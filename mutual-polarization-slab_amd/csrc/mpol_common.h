@@ -54,6 +54,9 @@ __device__ __forceinline__ double dpp_mov(double v) {
   return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 constexpr int kDppRowMirror = 0x140, kDppRowHalfMirror = 0x141, kDppSwap1 = 0xB1, kDppSwap2 = 0x4E;
+// row_newbcast:n (0x150 + n): every lane of a 16-lane row reads lane n of
+// its row
+constexpr int kDppRowBcast = 0x150;
 
 template <typename T>
 __device__ __forceinline__ T group_sum(T v, int R) {

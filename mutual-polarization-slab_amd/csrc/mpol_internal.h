@@ -899,7 +899,9 @@ int run_operator_sizes(mpol_ctx* c, const double* m, double* out, double* hr, in
                      st, sa, c->sz_noff.p, ent, pm, grid)
     if (nrhs <= 4) LAUNCH_SPREAD_NODES(4);
     else if (nrhs <= 8) LAUNCH_SPREAD_NODES(8);
-    else LAUNCH_SPREAD_NODES(16);
+    else
+      hipLaunchKernelGGL((k_sizes_spread_rows<T>), dim3(cdiv((int64_t)G * 16, 256), cdiv(nrhs, 16)), dim3(256), 0, st,
+                         sa, c->sz_noff.p, ent, pm, grid);
 #undef LAUNCH_SPREAD_NODES
   } else {
     // per-particle scatter with atomics (engine knob 2, or more stencil

@@ -438,6 +438,53 @@ __global__ void __launch_bounds__(256) k_sizes_spread_nodes(SizesGridArgs a, con
     if (live[u]) grid[(size_t)(r0 + u) * nodes + node] = acc[u];
 }
 
+// The same gather with 16 right-hand sides per node, one DPP row per node
+// (lane r = right-hand side r of the group): the row's lanes load 16
+// consecutive entries of the node at once, one each, and broadcast them in
+// turn (row_newbcast), so an entry is read once per row instead of once per
+// lane.  Every lane runs the wave-uniform entry loop (the broadcasts need
+// the whole row); the sum per (node, right-hand side) is over the node's
+// entries in ascending order, as in k_sizes_spread_nodes.
+template <typename T>
+__global__ void __launch_bounds__(256) k_sizes_spread_rows(SizesGridArgs a, const int* __restrict__ off,
+                                                           const typename Vec4<T>::type* __restrict__ ent,
+                                                           const typename Vec4<T>::type* __restrict__ pm,
+                                                           T* __restrict__ grid) {
+  using V = typename Vec4<T>::type;
+  const int nodes = a.nx * a.ny * a.nz;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int node = (int)(t >> 4), r = (int)(t & 15);
+  const int rhs = blockIdx.y * 16 + r;
+  const bool in = node < nodes;
+  const int lo = in ? off[node] : 0;
+  const int cnt = in ? off[node + 1] - lo : 0;
+  const bool on = in && rhs < a.nrhs && !rhs_idle(a.gate, a.gdiv, rhs);
+  const V* q = pm + (on ? rhs : 0);
+  T acc = (T)0;
+  for (int j0 = 0; __any(j0 < cnt); j0 += 16) {
+    V mine;
+    mine.x = mine.y = mine.z = mine.w = (T)0;
+    if (j0 + r < cnt) mine = ent[lo + j0 + r];
+#define SPREAD_ROW_STEP(J)                                                   \
+  {                                                                          \
+    const T ex = dpp_mov<kDppRowBcast + (J)>(mine.x);                        \
+    const T ey = dpp_mov<kDppRowBcast + (J)>(mine.y);                        \
+    const T ez = dpp_mov<kDppRowBcast + (J)>(mine.z);                        \
+    const T ew = dpp_mov<kDppRowBcast + (J)>(mine.w);                        \
+    if (on && j0 + (J) < cnt) {                                              \
+      const V v = q[(size_t)Vec4<T>::index(ew) * a.nrhs];                    \
+      acc += ex * v.x + ey * v.y + ez * v.z;                                 \
+    }                                                                        \
+  }
+    SPREAD_ROW_STEP(0) SPREAD_ROW_STEP(1) SPREAD_ROW_STEP(2) SPREAD_ROW_STEP(3)
+    SPREAD_ROW_STEP(4) SPREAD_ROW_STEP(5) SPREAD_ROW_STEP(6) SPREAD_ROW_STEP(7)
+    SPREAD_ROW_STEP(8) SPREAD_ROW_STEP(9) SPREAD_ROW_STEP(10) SPREAD_ROW_STEP(11)
+    SPREAD_ROW_STEP(12) SPREAD_ROW_STEP(13) SPREAD_ROW_STEP(14) SPREAD_ROW_STEP(15)
+#undef SPREAD_ROW_STEP
+  }
+  if (on) grid[(size_t)rhs * nodes + node] = acc;
+}
+
 // Particle-major entries for the contraction: pent[s][o] = {w r_x, w r_y,
 // w r_z, node bits} for stencil point o of sorted particle s (one workgroup
 // per particle; the same expressions as the per-particle kernels).
