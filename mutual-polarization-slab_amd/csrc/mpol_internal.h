@@ -945,7 +945,6 @@ int run_operator_sizes(mpol_ctx* c, const double* m, double* out, double* hr, in
     set_lds(k_fft_y<T, E, N>, cy.lds);                                                          \
     hipLaunchKernelGGL((k_fft_y<T, E, N>), ygrid, dim3(256), cy.lds, st, ay, spec, INV);       \
   } while (0)
-  DISPATCH_FFT(c->ng[1], ay.plan.e, LAUNCH_FFTY, 0);
   const ColCfg cz = col_cfg(c->ng[2], generic_fft(c->plan[2]), c->kx, 1, cs, kLdsCol,
                             knobs<T>(c)[4] > 0 ? knobs<T>(c)[4] : (sizeof(T) == 8 ? 8 : 16));
   ColArgs az{};
@@ -976,9 +975,20 @@ int run_operator_sizes(mpol_ctx* c, const double* m, double* out, double* hr, in
     set_lds(k_fft_z_scale<T, E, N, 1>, cz.lds);                                                 \
     hipLaunchKernelGGL((k_fft_z_scale<T, E, N, 1>), zgrid, dim3(zthreads), cz.lds, st, az, sp, spec); \
   } while (0)
-  DISPATCH_FFT(c->ng[2], az.plan.e, LAUNCH_FFTZ, 0);
+  // ny == nz == 32: forward y, forward z + scale + inverse z and inverse y
+  // in one kernel per group of y-z planes (engine knob 2: the separate passes)
+  constexpr int kPlaneTx = sizeof(T) == 8 ? 2 : 4;
+  if (c->ng[1] == 32 && c->ng[2] == 32 && knobs<T>(c)[6] != 2) {
+    const size_t plds = (32 + 2 * (size_t)kPlaneTx * 32 * padded_len(32)) * cs;
+    set_lds(k_sizes_yz_plane<T, 32, kPlaneTx>, plds);
+    hipLaunchKernelGGL((k_sizes_yz_plane<T, 32, kPlaneTx>), dim3(cdiv(c->kx, kPlaneTx), 1, nrhs), dim3(256), plds, st,
+                       ay, sp, spec);
+  } else {
+    DISPATCH_FFT(c->ng[1], ay.plan.e, LAUNCH_FFTY, 0);
+    DISPATCH_FFT(c->ng[2], az.plan.e, LAUNCH_FFTZ, 0);
+    DISPATCH_FFT(c->ng[1], ay.plan.e, LAUNCH_FFTY, 1);
+  }
 #undef LAUNCH_FFTZ
-  DISPATCH_FFT(c->ng[1], ay.plan.e, LAUNCH_FFTY, 1);
 #undef LAUNCH_FFTY
 #define LAUNCH_C2R(E, N, ...)                                                                   \
   do {                                                                                          \

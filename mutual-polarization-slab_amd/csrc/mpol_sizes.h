@@ -708,6 +708,74 @@ __global__ void __launch_bounds__(256) k_sizes_contract_lds(SizesGridArgs a, con
 }
 
 // ---------------------------------------------------------------------------
+// y and z passes of a small cubic-section grid in one kernel
+// ---------------------------------------------------------------------------
+// For ny == nz == N (compile-time) a workgroup holds the whole y-z plane of
+// TX kx columns of one right-hand side in LDS and does forward y, forward z,
+// the scalar scale exp(-(1-eta) q^2/(4 xi^2))/q^2 (times 1/G), inverse z and
+// inverse y, transposing the plane in LDS between the axes: the half
+// spectrum is read and written once instead of three times.  The same line
+// transforms and scale expressions as the separate y and z passes.
+// A[(c N + z) ls + pidx(y)] holds y-lines, B[(c N + y) ls + pidx(z)] z-lines.
+template <typename T, int N, int TX>
+__global__ void __launch_bounds__(256) k_sizes_yz_plane(ColArgs a, ScaleParams sp, cplx<T>* __restrict__ spec) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int ls = padded_len(N);
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* A = tw + N;
+  cplx<T>* B = A + TX * N * ls;
+  const int kx0 = blockIdx.x * TX, rhs = blockIdx.z;
+  if (rhs_idle(a.gate, a.gdiv, rhs)) return;  // uniform over the block
+  const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
+  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];
+  cplx<T>* base = spec + (size_t)rhs * N * N * a.kxs;
+  constexpr int TOT = TX * N * N;
+  // consecutive threads on consecutive kx of a row
+  for (int t = threadIdx.x; t < TOT; t += blockDim.x) {
+    const int c = t % TX, y = (t / TX) % N, z = t / (TX * N);
+    const int kx = kx0 + c;
+    A[(c * N + z) * ls + pidx(y)] = kx < a.kx ? base[((size_t)z * N + y) * a.kxs + kx] : cmk<T>(0, 0);
+  }
+  __syncthreads();
+  fft_ct<T, N>(A, ls, TX * N, tw, false);
+  for (int t = threadIdx.x; t < TOT; t += blockDim.x) {
+    const int z = t % N, y = (t / N) % N, c = t / (N * N);
+    B[(c * N + y) * ls + pidx(z)] = A[(c * N + z) * ls + pidx(y)];
+  }
+  __syncthreads();
+  fft_ct<T, N>(B, ls, TX * N, tw, false);
+  for (int t = threadIdx.x; t < TOT; t += blockDim.x) {
+    const int z = t % N, y = (t / N) % N, c = t / (N * N);
+    const int kx = kx0 + c;
+    if (kx >= a.kx) continue;
+    const int mx = fft_freq(kx, a.nx), my = fft_freq(y, N), mz = fft_freq(z, N);
+    const double k0 = mx * sp.twopi_over_l[0], k1 = my * sp.twopi_over_l[1], k2 = mz * sp.twopi_over_l[2];
+    const double kk = (k0 * k0 + k1 * k1) + k2 * k2;
+    cplx<T>* f = B + (c * N + y) * ls + pidx(z);
+    if (kk == 0.0) {
+      *f = cmk<T>(0, 0);
+      continue;
+    }
+    const double cf = (sp.ex[kx] * sp.ey[y] * sp.ez[z]) / kk * sp.inv_g;
+    const cplx<T> F0 = *f;
+    *f = cmk<T>((T)cf * F0.x, (T)cf * F0.y);
+  }
+  __syncthreads();
+  fft_ct<T, N>(B, ls, TX * N, tw, true);
+  for (int t = threadIdx.x; t < TOT; t += blockDim.x) {
+    const int y = t % N, z = (t / N) % N, c = t / (N * N);
+    A[(c * N + z) * ls + pidx(y)] = B[(c * N + y) * ls + pidx(z)];
+  }
+  __syncthreads();
+  fft_ct<T, N>(A, ls, TX * N, tw, true);
+  for (int t = threadIdx.x; t < TOT; t += blockDim.x) {
+    const int c = t % TX, y = (t / TX) % N, z = t / (TX * N);
+    const int kx = kx0 + c;
+    if (kx < a.kx) base[((size_t)z * N + y) * a.kxs + kx] = A[(c * N + z) * ls + pidx(y)];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // x-axis transforms of real grids (scalar field, tiles of ty*tz rows)
 // ---------------------------------------------------------------------------
 // grid[rhs][z][y][x] rows -> half spectrum spec[rhs][z][y][kx]

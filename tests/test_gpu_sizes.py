@@ -129,6 +129,35 @@ def test_sizes_entry_kernels_match_particle_kernels_and_follow_geometry(gpu):
     assert _rel(out32, out) < 2e-6
 
 
+@pytest.mark.parametrize("ngrid", [(32, 32, 32), (40, 32, 32)])
+def test_sizes_operator_yz_plane_grids(gpu, ngrid):
+    """ny = nz = 32 takes the one-kernel y/z passes (per y-z plane): FP64
+    against the oracle on 18 right-hand sides (partial column groups: kx = 17
+    and 21), FP32 against FP64, and the separate passes (engine knob 2) equal
+    to it."""
+    from mpol_hip import FP32, FP64
+    x, box, a = _config(36, 0.12, 13, radii=None)
+    xi, tol = 0.9, 1e-6
+    geo = so.SizesGeometry(x, a, box, xi, tol, None, None, None, Ngrid=np.array(ngrid), P=8, exact=True)
+    from mpol_hip import Context
+    ctx = Context(gpu)
+    ctx.set_tables(np.zeros(2), np.zeros(2), np.array([0.0, 2 * geo.rc]))
+    ctx.set_geometry(x, box, geo.Ngrid, geo.P, xi, np.full(3, geo.eta), geo.rc)
+    ctx.set_sizes(a)
+    rng = np.random.default_rng(21)
+    eps = np.stack([rng.uniform(2.0, 30.0, x.shape[0]) for _ in range(18)])
+    ctx.set_permittivity(eps)
+    p = rng.standard_normal((18, x.shape[0], 3))
+    o64 = ctx.sizes_matvec(p, precision=FP64).cpu().numpy()
+    for k in (0, 7, 17):
+        assert _rel(o64[k], so.field(geo, p[k], eps[k]).real) < 1e-12, k
+    o32 = ctx.sizes_matvec(p, precision=FP32).cpu().numpy()
+    assert _rel(o32, o64) < 2e-6
+    ctx.set_tuning(engine=2)
+    sep = ctx.sizes_matvec(p, precision=FP64).cpu().numpy()
+    assert _rel(sep, o64) < 1e-13
+
+
 def test_sizes_operator_complex_permittivity(gpu):
     x, box, a = _config(18, 0.1, 8)
     rng = np.random.default_rng(4)
