@@ -977,12 +977,14 @@ int run_operator_sizes(mpol_ctx* c, const double* m, double* out, double* hr, in
   } while (0)
   // ny == nz == 32: forward y, forward z + scale + inverse z and inverse y
   // in one kernel per group of y-z planes (engine knob 2: the separate passes)
-  constexpr int kPlaneTx = sizeof(T) == 8 ? 2 : 4;
+  constexpr int kPlaneTx = 2;
   if (c->ng[1] == 32 && c->ng[2] == 32 && knobs<T>(c)[6] != 2) {
     const size_t plds = (32 + 2 * (size_t)kPlaneTx * 32 * padded_len(32)) * cs;
+    // one thread per line pair position: kPlaneTx * 32 lines of 32 / ct_plan(32).e threads
+    constexpr int threads = kPlaneTx * 32 * (32 / ct_plan(32).e);
     set_lds(k_sizes_yz_plane<T, 32, kPlaneTx>, plds);
-    hipLaunchKernelGGL((k_sizes_yz_plane<T, 32, kPlaneTx>), dim3(cdiv(c->kx, kPlaneTx), 1, nrhs), dim3(256), plds, st,
-                       ay, sp, spec);
+    hipLaunchKernelGGL((k_sizes_yz_plane<T, 32, kPlaneTx>), dim3(cdiv(c->kx, kPlaneTx), 1, nrhs), dim3(threads), plds,
+                       st, ay, sp, spec);
   } else {
     DISPATCH_FFT(c->ng[1], ay.plan.e, LAUNCH_FFTY, 0);
     DISPATCH_FFT(c->ng[2], az.plan.e, LAUNCH_FFTZ, 0);
