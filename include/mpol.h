@@ -246,8 +246,9 @@ int mpol_matvec_vjp(mpol_ctx* ctx, const double* p_dev, const double* lam_dev, i
  * z-pass columns per workgroup (<= 64), z-pass threads (multiple of 64),
  * spread/contract engine (1 matrix cores over per-tile candidate images,
  * 2 vector loops; for the polydisperse operator 2 selects the per-particle
- * scatter instead of the per-node gather).  0 keeps the automatic choice;
- * nknobs may be shorter than 7. */
+ * kernels (atomic scatter, LDS-staged contraction) and the separate y/z
+ * passes instead of the stored-entry gather/contraction and the one-kernel
+ * y/z passes).  0 keeps the automatic choice; nknobs may be shorter than 7. */
 int mpol_set_tuning(mpol_ctx* ctx, const int32_t* knobs, int32_t nknobs);
 
 /* The same knobs for the operator in one precision only (MPOL_FP32 or

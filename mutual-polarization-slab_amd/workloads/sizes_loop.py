@@ -9,8 +9,9 @@ batched operator applications and one mixed solve on one stream.
                                    [--reps 20] [--engine 0|2] [--seed 11]
 
 (run from mutual-polarization-slab_amd/, or with it on sys.path).  engine 2
-selects the per-particle kernels (atomic spread, LDS-staged contraction)
-instead of the stored-entry ones.  Prints one line per step.
+selects the per-particle kernels (atomic spread, LDS-staged contraction) and
+the separate y/z passes instead of the stored-entry kernels and the
+one-kernel y/z passes.  Prints one line per step.
 """
 from __future__ import annotations
 
