@@ -67,6 +67,17 @@ __device__ __forceinline__ T group_sum(T v, int R) {
   return v;
 }
 
+// Workgroups go to the 8 XCDs round-robin by linear id.  xcd_block maps a
+// linear id to a logical block so that each XCD runs one contiguous range of
+// logical blocks (a bijection on [0, total)): blocks over neighbouring
+// particles then share that XCD's L2.
+__device__ __forceinline__ int xcd_block(int lin, int total) {
+  constexpr int kXcd = 8;
+  const int q = total / kXcd, r = total % kXcd;
+  const int x = lin % kXcd, k = lin / kXcd;
+  return x * q + min(x, r) + k;
+}
+
 // Per-system state of the batched GMRES.
 enum RhsState : int { kDone = 0, kIter = 1, kSolve = 2, kWait = 3 };
 

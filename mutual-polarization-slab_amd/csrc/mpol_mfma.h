@@ -108,8 +108,10 @@ __global__ void __launch_bounds__(256) k_contract_reduce(int n, long long total,
                                                          double* __restrict__ out, const int* __restrict__ gate,
                                                          int gdiv) {
   const int g = threadIdx.x & 15;
-  const int s = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 4);
-  const int rhs = blockIdx.y;
+  // XCD-contiguous blocks: the partials of neighbouring particles stay in one L2
+  const int lb = xcd_block(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+  const int s = (int)(((size_t)(lb % gridDim.x) * blockDim.x + threadIdx.x) >> 4);
+  const int rhs = lb / gridDim.x;
   if (rhs_idle(gate, gdiv, rhs)) return;  // uniform over the block
   const bool valid = s < n;  // every lane takes part in the lane sum
   double acc[NC];

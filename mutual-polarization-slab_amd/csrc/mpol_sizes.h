@@ -550,11 +550,14 @@ __global__ void __launch_bounds__(256) k_sizes_contract_nodes(SizesGridArgs a,
   using V = typename Vec4<T>::type;
   constexpr int RL = 4, PPW = 64 / (K * W);
   static_assert(PPW * K * W == 64, "lanes");
+  // XCD-contiguous blocks: the grid nodes a block reads stay in one L2
+  const int lb = xcd_block(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+  const int bx = lb % gridDim.x, by = lb / gridDim.x;
   const int lane = threadIdx.x & 63;
-  const int64_t wv = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t wv = ((int64_t)bx * blockDim.x + threadIdx.x) >> 6;
   const int p = lane / (K * W), k = (lane / W) % K, rl = lane % W;
   const int s = (int)(wv * PPW) + p;
-  const int r0 = (blockIdx.y * W + rl) * RL;
+  const int r0 = (by * W + rl) * RL;
   const bool on = s < a.n && r0 < a.nrhs;  // every lane takes part in the butterfly
   const int P3 = a.P * a.P * a.P;
   T acc[RL][3];

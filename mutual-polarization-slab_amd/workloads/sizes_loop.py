@@ -6,7 +6,7 @@ by the many-configuration driver's rule at tolerance 1e-3, then repeated
 batched operator applications and one mixed solve on one stream.
 
     python -m workloads.sizes_loop [--n 3000] [--phi 0.3] [--grid 32] [--precision fp32|fp64]
-                                   [--reps 20] [--engine 0|2] [--seed 11]
+                                   [--reps 20] [--engine 0|2] [--seed 11] [--copy-mb 0]
 
 (run from mutual-polarization-slab_amd/, or with it on sys.path).  engine 2
 selects the per-particle kernels (atomic spread, LDS-staged contraction) and
@@ -36,6 +36,8 @@ def main(argv=None):
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--engine", type=int, default=0)
     ap.add_argument("--seed", type=int, default=11)
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="first copy a float32 device buffer of this many MiB (counter calibration)")
     args = ap.parse_args(argv)
 
     import torch
@@ -43,6 +45,13 @@ def main(argv=None):
     import workloads
     from mpol_hip import FP32, FP64, batch
 
+    if args.copy_mb > 0:
+        src = torch.ones(args.copy_mb << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        print(f"copied {args.copy_mb} MiB device to device", flush=True)
     x, a, box = workloads.polydisperse(args.n, args.phi, args.seed)
     eps = np.logspace(-1.0, 2.0, 16)
     live = eps[np.abs(eps - 1.0) > 1e-12]
