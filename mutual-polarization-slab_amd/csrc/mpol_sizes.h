@@ -727,7 +727,10 @@ __global__ void __launch_bounds__(256) k_sizes_yz_plane(ColArgs a, ScaleParams s
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* A = tw + N;
   cplx<T>* B = A + TX * N * ls;
-  const int kx0 = blockIdx.x * TX, rhs = blockIdx.z;
+  // XCD-contiguous blocks: the column groups of one plane share the spectrum
+  // rows' cache lines in one L2 (grid (kx groups, 1, right-hand sides))
+  const int lb = xcd_block(blockIdx.x + gridDim.x * blockIdx.z, gridDim.x * gridDim.z);
+  const int kx0 = (lb % gridDim.x) * TX, rhs = lb / gridDim.x;
   if (rhs_idle(a.gate, a.gdiv, rhs)) return;  // uniform over the block
   const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
   for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];
