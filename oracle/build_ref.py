@@ -1,4 +1,4 @@
-"""Build the reference's own C++ force kernel as a checker (test infrastructure).
+"""Build the reference's own C++ force kernel as a test reference (test infrastructure).
 
 Compiles "CPU - HOOMD simulation test/dipole_forces_ext/_dipole_forces_ext.cpp"
 from the reference tree where it lies (pybind11 headers + Python headers, the

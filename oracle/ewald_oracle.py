@@ -2,7 +2,7 @@
 
 TEST INFRASTRUCTURE ONLY.  Nothing in the product imports this module; only
 ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
-``bench.py`` may use it, and only as the checker (or the timed CPU baseline),
+``bench.py`` may use it, and only as the test reference (or the timed CPU baseline),
 never as the thing measured or shipped.
 
 This is a clean NumPy restatement of the reference operator

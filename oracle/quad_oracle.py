@@ -2,7 +2,7 @@
 spectral-Ewald operator of Capacitance_Quadrupole/.
 
 TEST INFRASTRUCTURE ONLY (see ewald_oracle.py's header): only tests/ may use
-it, as the checker.
+it, as the test reference.
 
 A NumPy restatement of the reference's MATLAB, same data flow: spread of the
 3 dipole and 5 quadrupole components with the Gaussian kernel of the dipole

@@ -1,7 +1,7 @@
 """FP64 CPU oracle for the polydisperse dipole operator of Capacitance_Sizes/.
 
 TEST INFRASTRUCTURE ONLY (see ewald_oracle.py's header): only tests/ may use
-it, as the checker.
+it, as the test reference.
 
 A NumPy restatement of the reference's MATLAB, same data flow: the dipoles
 are spread "as charges" onto ONE scalar grid through a radius-dependent
