@@ -405,8 +405,14 @@ int mpol_set_geometry(mpol_ctx* c, const double* x, int64_t n, const double box[
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, c->bin_count.p, c->bin_start.p, nbins);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, c->cell_count.p, c->cell_start.p, (int)ncells);
   hipLaunchKernelGGL(k_geo_scatter, dim3(pg), dim3(256), 0, st, g, c->bin_key.p, c->bin_start.p, c->bin_cursor.p,
-                     c->cell_key.p, c->cell_start.p, c->cell_cursor.p, c->base.p, c->g0u.p, c->perm.p, c->rec.p,
-                     c->base64.p, c->base32.p, c->cell_members.p);
+                     c->cell_key.p, c->cell_start.p, c->cell_cursor.p, c->perm.p, c->cell_members.p);
+  // caller's order inside every bin and cell: the sorted order, and with it
+  // every summation order downstream, is the same on every call
+  hipLaunchKernelGGL(k_sort_segments, dim3(cdiv(nbins, 256)), dim3(256), 0, st, nbins, c->bin_start.p, c->perm.p);
+  hipLaunchKernelGGL(k_sort_segments, dim3(cdiv(ncells, 256)), dim3(256), 0, st, (int)ncells, c->cell_start.p,
+                     c->cell_members.p);
+  hipLaunchKernelGGL(k_geo_records, dim3(pg), dim3(256), 0, st, (int)n, c->perm.p, c->base.p, c->g0u.p, c->rec.p,
+                     c->base64.p, c->base32.p);
   PairParams pp{};
   pp.n = (int)n;
   for (int d = 0; d < 3; ++d) {

@@ -110,27 +110,53 @@ static __global__ void k_scan(const int* __restrict__ in, int* __restrict__ out,
   if (t == nt - 1) out[n] = part[t];
 }
 
-// Pass 2: scatter into bin order.  Per sorted particle s: rec[s] = (first
-// stencil node on x, y, z, caller's index) and base[s] = node0*h - x on each
-// axis (FP64 and FP32 copies), from which the kernels recompute the Gaussian
-// factors exp(-2 xi^2 (base_d + (o - half) h_d)^2 / eta_d).
+// Pass 2: scatter the caller's indices into bin order (perm) and cell order
+// (cell_members); the places inside a bin or cell come from atomics, so the
+// segments are sorted next (k_sort_segments) for an order that does not
+// depend on timing.
 static __global__ void k_geo_scatter(GeoParams g, const int* __restrict__ bin_key, const int* __restrict__ bin_start,
                               int* __restrict__ bin_cursor, const int* __restrict__ cell_key,
                               const int* __restrict__ cell_start, int* __restrict__ cell_cursor,
-                              const double* __restrict__ base, const int* __restrict__ g0_unsorted,
-                              int* __restrict__ perm, int4* __restrict__ rec, double4* __restrict__ base64,
-                              float4* __restrict__ base32, int* __restrict__ cell_members) {
+                              int* __restrict__ perm, int* __restrict__ cell_members) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= g.n) return;
   const int key = bin_key[i];
-  const int s = bin_start[key] + atomicAdd(&bin_cursor[key], 1);
-  perm[s] = i;
+  perm[bin_start[key] + atomicAdd(&bin_cursor[key], 1)] = i;
+  const int ck = cell_key[i];
+  cell_members[cell_start[ck] + atomicAdd(&cell_cursor[ck], 1)] = i;
+}
+
+// Ascending order inside every segment [start[k], start[k+1]) of vals (one
+// thread per segment, insertion sort: segments hold a few entries).
+static __global__ void k_sort_segments(int nseg, const int* __restrict__ start, int* __restrict__ vals) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nseg) return;
+  const int lo = start[k], hi = start[k + 1];
+  for (int a = lo + 1; a < hi; ++a) {
+    const int v = vals[a];
+    int b = a - 1;
+    while (b >= lo && vals[b] > v) {
+      vals[b + 1] = vals[b];
+      --b;
+    }
+    vals[b + 1] = v;
+  }
+}
+
+// Pass 3: per sorted particle s: rec[s] = (first stencil node on x, y, z,
+// caller's index) and base[s] = node0*h - x on each axis (FP64 and FP32
+// copies), from which the kernels recompute the Gaussian factors
+// exp(-2 xi^2 (base_d + (o - half) h_d)^2 / eta_d).
+static __global__ void k_geo_records(int n, const int* __restrict__ perm, const double* __restrict__ base,
+                                     const int* __restrict__ g0_unsorted, int4* __restrict__ rec,
+                                     double4* __restrict__ base64, float4* __restrict__ base32) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int i = perm[s];
   rec[s] = make_int4(g0_unsorted[3 * i], g0_unsorted[3 * i + 1], g0_unsorted[3 * i + 2], i);
   const double b0 = base[3 * i], b1 = base[3 * i + 1], b2 = base[3 * i + 2];
   base64[s] = make_double4(b0, b1, b2, 0.0);
   base32[s] = make_float4((float)b0, (float)b1, (float)b2, 0.0f);
-  const int ck = cell_key[i];
-  cell_members[cell_start[ck] + atomicAdd(&cell_cursor[ck], 1)] = i;
 }
 
 // ms[rhs][s] = m[rhs][perm[s]]: dipoles in stencil-bin order, once per matvec.

@@ -71,3 +71,19 @@ def test_tune_grids_reports_and_early_stop(gpu):
         # every candidate timed before the last one improved on the best so far
         assert all(times[k] < min(times[:k], default=np.inf) for k in range(len(times) - 1))
         assert short[i] == min(rep_short[i], key=lambda t: t[1])[0]
+
+
+def test_batch_is_deterministic(gpu):
+    """The polydisperse kernels and the GMRES reductions have fixed
+    summation orders (no atomics): two runs of the driver on concurrent
+    worker streams give bitwise the same dipoles."""
+    configs = _configs()
+    eps = np.array([0.3, 5.0, 40.0])
+    grids = [32] * len(configs)
+    r1 = batch.solve_configurations(configs, eps, gmres_tol=1e-7, precision="mixed", workers=3, tune=False,
+                                    grids=grids, device=gpu)
+    r2 = batch.solve_configurations(configs, eps, gmres_tol=1e-7, precision="mixed", workers=3, tune=False,
+                                    grids=grids, device=gpu)
+    for a, b in zip(r1.p, r2.p):
+        assert np.array_equal(a, b)
+    assert np.array_equal(r1.matvecs, r2.matvecs)
