@@ -1087,6 +1087,9 @@ int mpol_compute_forces(mpol_ctx* c, const double* pos, const double* dip, int64
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, c->f_count.p, c->f_start.p, (int)ncells);
     hipLaunchKernelGGL(k_force_scatter, dim3(blocks), dim3(256), 0, st, (int)n, c->f_cell.p, c->f_start.p,
                        c->f_cursor.p, c->f_members.p);
+    // caller's order inside every cell: the pair sums run in the same order on every call
+    hipLaunchKernelGGL(k_sort_segments, dim3(cdiv(ncells, 256)), dim3(256), 0, st, (int)ncells, c->f_start.p,
+                       c->f_members.p);
   }
   hipLaunchKernelGGL(k_forces, dim3(blocks), dim3(256), 0, st, p, pos, dip, c->f_cell.p, c->f_start.p,
                      c->f_members.p, forces, pe);
