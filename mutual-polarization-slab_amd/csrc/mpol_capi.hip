@@ -255,6 +255,59 @@ int gmres_core(mpol_ctx* c, int kind, const double* b, double* x, int nrhs, doub
   return MPOL_OK;
 }
 
+// Argument checks of compute_forces (the reference's messages) and the
+// launch parameters: cells no smaller than r_cut, the all-pairs loop when
+// the cells would outnumber max(1024, 8 N).
+int force_setup(mpol_ctx* c, const double* pos, const double* dip, int64_t n, const double box[3], double r_cut,
+                double prefactor, int32_t periodic_z, int32_t use_cell_list, double* forces, double* pe,
+                ForceParams& p, int64_t& ncells) {
+  if (!c || !box) return fail(MPOL_EINVAL, "null argument");
+  if (n < 0 || n > (1LL << 30)) return fail(MPOL_EINVAL, "Invalid shapes: pos(N,3), dip(N,3), box(3) required.");
+  if (n > 0 && (!pos || !dip || !forces || !pe)) return fail(MPOL_EINVAL, "null argument");
+  if (!(box[0] > 0.0 && box[1] > 0.0 && box[2] > 0.0)) return fail(MPOL_EINVAL, "Box lengths must be positive.");
+  if (!(r_cut > 0.0)) return fail(MPOL_EINVAL, "r_cut must be > 0.");
+  if (r_cut >= std::min(0.5 * box[0], 0.5 * box[1]))
+    return fail(MPOL_EINVAL, "r_cut too large for minimum-image periodic x,y. Require r_cut < 0.5*min(Lx,Ly).");
+  if (periodic_z && r_cut >= 0.5 * box[2])
+    return fail(MPOL_EINVAL,
+                "r_cut too large for minimum-image periodic z. Require r_cut < 0.5*Lz when periodic_z=True.");
+  p.n = (int)n;
+  double cells = 1.0;
+  for (int d = 0; d < 3; ++d) {
+    p.box[d] = box[d];
+    // cells no smaller than r_cut, so every neighbour within r_cut is in an adjacent cell
+    const double nc = std::max(1.0, std::min(std::floor(box[d] / r_cut), 1e9));
+    p.nc[d] = (int)nc;
+    cells *= nc;
+  }
+  p.rc2 = r_cut * r_cut;
+  p.pref = prefactor;
+  p.periodic_z = periodic_z ? 1 : 0;
+  // very dilute systems with tiny r_cut: the all-pairs loop rather than more
+  // cells than a few per particle
+  p.use_cells =
+      (use_cell_list && cells <= (double)std::max<int64_t>(1024, 8 * n) && cells < (double)(1LL << 30)) ? 1 : 0;
+  ncells = p.use_cells ? (int64_t)cells : 0;
+  return MPOL_OK;
+}
+
+// Cell list (counts, scan, placement, then the caller's order inside every
+// cell, so that the pair sums run in the same order on every call) and the
+// pair sums.  count and cursor must hold zeros.
+int force_launch(const ForceParams& p, int64_t ncells, const double* pos, const double* dip, int* cell,
+                 int* members, int* count, int* cursor, int* start, double* forces, double* pe, hipStream_t st) {
+  const int blocks = cdiv(p.n, 256);
+  if (p.use_cells) {
+    hipLaunchKernelGGL(k_force_cells, dim3(blocks), dim3(256), 0, st, p, pos, cell, count);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, count, start, (int)ncells);
+    hipLaunchKernelGGL(k_force_scatter, dim3(blocks), dim3(256), 0, st, p.n, cell, start, cursor, members);
+    sort_segments((int)ncells, start, members, st);
+  }
+  hipLaunchKernelGGL(k_forces, dim3(blocks), dim3(256), 0, st, p, pos, dip, cell, start, members, forces, pe);
+  HIPCHK(hipGetLastError());
+  return MPOL_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -408,9 +461,8 @@ int mpol_set_geometry(mpol_ctx* c, const double* x, int64_t n, const double box[
                      c->cell_key.p, c->cell_start.p, c->cell_cursor.p, c->perm.p, c->cell_members.p);
   // caller's order inside every bin and cell: the sorted order, and with it
   // every summation order downstream, is the same on every call
-  hipLaunchKernelGGL(k_sort_segments, dim3(cdiv(nbins, 256)), dim3(256), 0, st, nbins, c->bin_start.p, c->perm.p);
-  hipLaunchKernelGGL(k_sort_segments, dim3(cdiv(ncells, 256)), dim3(256), 0, st, (int)ncells, c->cell_start.p,
-                     c->cell_members.p);
+  sort_segments(nbins, c->bin_start.p, c->perm.p, st);
+  sort_segments((int)ncells, c->cell_start.p, c->cell_members.p, st);
   hipLaunchKernelGGL(k_geo_records, dim3(pg), dim3(256), 0, st, (int)n, c->perm.p, c->base.p, c->g0u.p, c->rec.p,
                      c->base64.p, c->base32.p);
   PairParams pp{};
@@ -1046,35 +1098,12 @@ int mpol_real_space_table_quad(double xi, const double* r, int64_t n, double* ou
 int mpol_compute_forces(mpol_ctx* c, const double* pos, const double* dip, int64_t n, const double box[3],
                         double r_cut, double prefactor, int32_t periodic_z, int32_t use_cell_list, double* forces,
                         double* pe) {
-  if (!c || !box) return fail(MPOL_EINVAL, "null argument");
-  if (n < 0 || n > (1LL << 30)) return fail(MPOL_EINVAL, "Invalid shapes: pos(N,3), dip(N,3), box(3) required.");
-  if (n > 0 && (!pos || !dip || !forces || !pe)) return fail(MPOL_EINVAL, "null argument");
-  if (!(box[0] > 0.0 && box[1] > 0.0 && box[2] > 0.0)) return fail(MPOL_EINVAL, "Box lengths must be positive.");
-  if (!(r_cut > 0.0)) return fail(MPOL_EINVAL, "r_cut must be > 0.");
-  if (r_cut >= std::min(0.5 * box[0], 0.5 * box[1]))
-    return fail(MPOL_EINVAL, "r_cut too large for minimum-image periodic x,y. Require r_cut < 0.5*min(Lx,Ly).");
-  if (periodic_z && r_cut >= 0.5 * box[2])
-    return fail(MPOL_EINVAL,
-                "r_cut too large for minimum-image periodic z. Require r_cut < 0.5*Lz when periodic_z=True.");
+  ForceParams p{};
+  int64_t ncells = 0;
+  CHECK_STATUS(force_setup(c, pos, dip, n, box, r_cut, prefactor, periodic_z, use_cell_list, forces, pe, p, ncells));
   if (n == 0) return MPOL_OK;
   HIPCHK(hipSetDevice(c->device));
   hipStream_t st = c->stream;
-  ForceParams p{};
-  p.n = (int)n;
-  for (int d = 0; d < 3; ++d) {
-    p.box[d] = box[d];
-    // cells no smaller than r_cut, so every neighbour within r_cut is in an adjacent cell
-    const int nc = (int)std::floor(box[d] / r_cut);
-    p.nc[d] = nc < 1 ? 1 : nc;
-  }
-  p.rc2 = r_cut * r_cut;
-  p.pref = prefactor;
-  p.periodic_z = periodic_z ? 1 : 0;
-  const int64_t ncells = (int64_t)p.nc[0] * p.nc[1] * p.nc[2];
-  // very dilute systems with tiny r_cut: fall back to the all-pairs loop
-  // rather than allocate more cells than a few per particle
-  p.use_cells = (use_cell_list && ncells <= std::max<int64_t>(1024, 8 * n) && ncells < (1LL << 30)) ? 1 : 0;
-  const int blocks = cdiv(n, 256);
   if (p.use_cells) {
     HIPALLOC(c->f_cell, (size_t)n);
     HIPALLOC(c->f_members, (size_t)n);
@@ -1083,18 +1112,9 @@ int mpol_compute_forces(mpol_ctx* c, const double* pos, const double* dip, int64
     HIPALLOC(c->f_start, (size_t)ncells + 1);
     HIPCHK(hipMemsetAsync(c->f_count.p, 0, ncells * sizeof(int), st));
     HIPCHK(hipMemsetAsync(c->f_cursor.p, 0, ncells * sizeof(int), st));
-    hipLaunchKernelGGL(k_force_cells, dim3(blocks), dim3(256), 0, st, p, pos, c->f_cell.p, c->f_count.p);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, c->f_count.p, c->f_start.p, (int)ncells);
-    hipLaunchKernelGGL(k_force_scatter, dim3(blocks), dim3(256), 0, st, (int)n, c->f_cell.p, c->f_start.p,
-                       c->f_cursor.p, c->f_members.p);
-    // caller's order inside every cell: the pair sums run in the same order on every call
-    hipLaunchKernelGGL(k_sort_segments, dim3(cdiv(ncells, 256)), dim3(256), 0, st, (int)ncells, c->f_start.p,
-                       c->f_members.p);
   }
-  hipLaunchKernelGGL(k_forces, dim3(blocks), dim3(256), 0, st, p, pos, dip, c->f_cell.p, c->f_start.p,
-                     c->f_members.p, forces, pe);
-  HIPCHK(hipGetLastError());
-  return MPOL_OK;
+  return force_launch(p, ncells, pos, dip, c->f_cell.p, c->f_members.p, c->f_count.p, c->f_cursor.p, c->f_start.p,
+                      forces, pe, st);
 }
 
 }  // extern "C"

@@ -106,6 +106,13 @@ struct HostBuf {
 
 inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
+// Ascending order inside every segment [start[k], start[k+1]) of vals: short
+// segments one thread each, long ones one workgroup each.
+inline void sort_segments(int nseg, const int* start, int* vals, hipStream_t st) {
+  hipLaunchKernelGGL(k_sort_segments, dim3(cdiv(nseg, 256)), dim3(256), 0, st, nseg, start, vals);
+  hipLaunchKernelGGL(k_sort_segments_big, dim3(std::min(nseg, 1024)), dim3(256), 0, st, nseg, start, vals);
+}
+
 inline FftPlan make_plan(int n) {
   FftPlan p{};
   p.n = n;

@@ -126,12 +126,15 @@ static __global__ void k_geo_scatter(GeoParams g, const int* __restrict__ bin_ke
   cell_members[cell_start[ck] + atomicAdd(&cell_cursor[ck], 1)] = i;
 }
 
-// Ascending order inside every segment [start[k], start[k+1]) of vals (one
-// thread per segment, insertion sort: segments hold a few entries).
+// Ascending order inside every segment [start[k], start[k+1]) of vals.
+// Segments of at most kSortSmall entries: one thread each, insertion sort;
+// longer ones are left to k_sort_segments_big.
+constexpr int kSortSmall = 32;
 static __global__ void k_sort_segments(int nseg, const int* __restrict__ start, int* __restrict__ vals) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nseg) return;
   const int lo = start[k], hi = start[k + 1];
+  if (hi - lo > kSortSmall) return;
   for (int a = lo + 1; a < hi; ++a) {
     const int v = vals[a];
     int b = a - 1;
@@ -140,6 +143,39 @@ static __global__ void k_sort_segments(int nseg, const int* __restrict__ start, 
       --b;
     }
     vals[b + 1] = v;
+  }
+}
+
+// Segments longer than kSortSmall: one workgroup per segment (grid-stride
+// over segments), an in-place bitonic network in global memory.  The
+// network's first step of every merge compares i with the mirror entry of
+// its block, so every comparison is ascending and the entries past the end
+// (padding to a power of two) act as +infinity: a comparison whose upper
+// index is past the end is skipped.  O(m log^2 m) work per segment.
+static __global__ void k_sort_segments_big(int nseg, const int* __restrict__ start, int* __restrict__ vals) {
+  for (int k = blockIdx.x; k < nseg; k += gridDim.x) {
+    const int lo = start[k], m = start[k + 1] - lo;
+    if (m <= kSortSmall) continue;
+    int* v = vals + lo;
+    int M = 1;
+    while (M < m) M <<= 1;
+    for (int blk = 2; blk <= M; blk <<= 1) {
+      for (int j = blk >> 1; j > 0; j >>= 1) {
+        for (int t = threadIdx.x; t < (M >> 1); t += blockDim.x) {
+          const int i = (t / j) * 2 * j + (t % j);
+          // first step of a merge: mirror partner inside the block of size blk
+          const int l = (j == (blk >> 1)) ? ((i / blk) * blk + blk - 1 - (i % blk)) : (i + j);
+          if (l < m) {
+            const int a = v[i], b = v[l];
+            if (a > b) {
+              v[i] = b;
+              v[l] = a;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
   }
 }
 
