@@ -303,6 +303,21 @@ int mpol_compute_forces(mpol_ctx* ctx, const double* pos_dev, const double* dip_
                         const double box_host[3], double r_cut, double prefactor, int32_t periodic_z,
                         int32_t use_cell_list, double* forces_dev, double* pe_dev);
 
+/* The same truncated forces without any device allocation: no counterpart
+ * in the reference; for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops such as a HOOMD step, harnesses that forbid
+ * hipMalloc).  Arguments, checks, results (bit for bit) and launches are those
+ * of mpol_compute_forces, except that its cell-list buffers live in a fixed
+ * per-GPU scratch array of the library (27 MiB, held until the process ends)
+ * instead of the context's buffers, and it clears them with a fill kernel, not
+ * a memset.  n at most 262144 (2^18); larger n returns MPOL_EINVAL before any
+ * launch.  Every call on one GPU shares that scratch: calls that overlap on one
+ * GPU (several streams, threads or contexts) race and must be serialised by
+ * the caller. */
+int mpol_compute_forces_noalloc(mpol_ctx* ctx, const double* pos_dev, const double* dip_dev, int64_t n,
+                                const double box_host[3], double r_cut, double prefactor, int32_t periodic_z,
+                                int32_t use_cell_list, double* forces_dev, double* pe_dev);
+
 #ifdef __cplusplus
 }
 #endif

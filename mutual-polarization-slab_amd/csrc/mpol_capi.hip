@@ -6,6 +6,8 @@
 // operator launch sequences are instantiated in mpol_ops_*.hip.
 #include "mpol_internal.h"
 
+#include <mutex>
+
 namespace mpol {
 thread_local std::string g_err;
 
@@ -16,6 +18,39 @@ int fail(int code, const std::string& msg) {
 }  // namespace mpol
 
 namespace {
+
+// Scratch of mpol_compute_forces_noalloc, one copy per GPU (held until the
+// process ends): cell index and cell members of at most kNoAllocMaxN
+// particles, then the per-cell counts, placement cursors and starts of at
+// most kNoAllocMaxCells cells -- 2 x 1 MiB + 3 x 8 MiB of int32.  Every call
+// on a GPU shares its copy, so calls that overlap on one GPU race.
+constexpr int64_t kNoAllocMaxN = 1 << 18;
+constexpr int64_t kNoAllocMaxCells = 8 * kNoAllocMaxN;  // the cell-list rule allows at most max(1024, 8 N)
+constexpr int64_t kScratchCell = 0;
+constexpr int64_t kScratchMembers = kScratchCell + kNoAllocMaxN;
+constexpr int64_t kScratchCount = kScratchMembers + kNoAllocMaxN;
+constexpr int64_t kScratchCursor = kScratchCount + kNoAllocMaxCells;
+constexpr int64_t kScratchStart = kScratchCursor + kNoAllocMaxCells;
+constexpr int64_t kScratchInts = kScratchStart + kNoAllocMaxCells + 1;
+static __device__ __align__(256) int scratch[kScratchInts];
+
+// Device address of `scratch` on each GPU (the host-side name is only a
+// handle for the runtime), looked up once per device with it current.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int* g_scratch_dev[kMaxDevices] = {};
+
+int scratch_address(int device, int** out) {
+  if (device < 0 || device >= kMaxDevices) return fail(MPOL_EINVAL, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void* ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_dev[device] = static_cast<int*>(ptr);
+  }
+  *out = g_scratch_dev[device];
+  return MPOL_OK;
+}
 
 // Operator `kind` (kOpDipole / kOpQuad) on nsys systems.
 int apply(mpol_ctx* c, int kind, const double* m, double* out, int nsys, int precision, double slab) {
@@ -328,8 +363,11 @@ int mpol_create(mpol_ctx** out, int device, void* stream) {
   HIPCHK(hipGetDeviceCount(&nd));
   if (device < 0 || device >= nd) return fail(MPOL_EINVAL, "device index out of range");
   HIPCHK(hipSetDevice(device));
+  int* scr = nullptr;
+  CHECK_STATUS(scratch_address(device, &scr));
   mpol_ctx* c = new mpol_ctx();
   c->device = device;
+  c->scratch = scr;
   c->stream = reinterpret_cast<hipStream_t>(stream);  // NULL: the device's default stream
   *out = c;
   return MPOL_OK;
@@ -1115,6 +1153,35 @@ int mpol_compute_forces(mpol_ctx* c, const double* pos, const double* dip, int64
   }
   return force_launch(p, ncells, pos, dip, c->f_cell.p, c->f_members.p, c->f_count.p, c->f_cursor.p, c->f_start.p,
                       forces, pe, st);
+}
+
+int mpol_compute_forces_noalloc(mpol_ctx* c, const double* pos, const double* dip, int64_t n, const double box[3],
+                                double r_cut, double prefactor, int32_t periodic_z, int32_t use_cell_list,
+                                double* forces, double* pe) {
+  if (n > kNoAllocMaxN)
+    return fail(MPOL_EINVAL, "mpol_compute_forces_noalloc: N = " + std::to_string(n) + " exceeds its maximum of " +
+                                 std::to_string(kNoAllocMaxN) + " particles (use mpol_compute_forces)");
+  ForceParams p{};
+  int64_t ncells = 0;
+  CHECK_STATUS(force_setup(c, pos, dip, n, box, r_cut, prefactor, periodic_z, use_cell_list, forces, pe, p, ncells));
+  if (n == 0) return MPOL_OK;
+  // force_setup keeps the cells at most max(1024, 8 N) <= kNoAllocMaxCells
+  if (p.use_cells && ncells > kNoAllocMaxCells)
+    return fail(MPOL_EINVAL, "mpol_compute_forces_noalloc: too many cells for its scratch");
+  HIPCHK(hipSetDevice(c->device));
+  if (!c->scratch) return fail(MPOL_ESTATE, "scratch address of this device not set up");
+  hipStream_t st = c->stream;
+  int* cell = c->scratch + kScratchCell;
+  int* members = c->scratch + kScratchMembers;
+  int* count = c->scratch + kScratchCount;
+  int* cursor = c->scratch + kScratchCursor;
+  int* start = c->scratch + kScratchStart;
+  if (p.use_cells) {
+    // counts and cursors start at zero (a fill kernel: no memset node in a captured graph)
+    hipLaunchKernelGGL(k_fill_int, dim3(cdiv(ncells, 256)), dim3(256), 0, st, count, ncells, 0);
+    hipLaunchKernelGGL(k_fill_int, dim3(cdiv(ncells, 256)), dim3(256), 0, st, cursor, ncells, 0);
+  }
+  return force_launch(p, ncells, pos, dip, cell, members, count, cursor, start, forces, pe, st);
 }
 
 }  // extern "C"

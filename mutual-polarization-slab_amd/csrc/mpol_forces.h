@@ -56,6 +56,11 @@ static __global__ void k_force_scatter(int n, const int* __restrict__ cell, cons
   members[start[c] + atomicAdd(&cursor[c], 1)] = i;
 }
 
+static __global__ void k_fill_int(int* __restrict__ a, int64_t n, int v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
 __device__ inline double wrap_rint(double dx, double L) { return dx - L * rint(dx / L); }
 
 // distinct neighbour cells of c along one axis (1..3 of them)

@@ -62,6 +62,8 @@ SIGNATURES = [
     ("mpol_real_space_table_field", ctypes.c_int, [ctypes.c_double, _DP, ctypes.c_int64, _DP]),
     ("mpol_compute_forces", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int64, _DP, ctypes.c_double, ctypes.c_double,
                                            ctypes.c_int32, ctypes.c_int32, _VP, _VP]),
+    ("mpol_compute_forces_noalloc", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int64, _DP, ctypes.c_double,
+                                                   ctypes.c_double, ctypes.c_int32, ctypes.c_int32, _VP, _VP]),
     ("mpol_set_permittivity_complex", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int32]),
     ("mpol_set_quad_tables", ctypes.c_int, [_VP, _DP, ctypes.c_int64]),
     ("mpol_quad_matvec", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int32, ctypes.c_int32]),
