@@ -18,7 +18,11 @@ Het's expressions); the eight new tables by xi-independence of the total
 operator (real + wave space, all four blocks), their bare multipole limits at
 small xi, continuity at r = 2 and their r -> 0 limits against the self terms;
 the k-space blocks by the same xi-independence and by the Hermitian identity
-for real input; the dipole block against ewald_oracle's operator.  The
+for real input; the dipole block against ewald_oracle's operator; the
+quadrupole diagonal (dielectric term 15/(8 pi (eps-1)), RealSpace.m:29, with
+the self terms) by the l = 2 Clausius-Mossotti response of a lone sphere to
+an imposed field gradient, Q = 4 pi/3 (eps-1)/(eps+3/2) g, at four
+permittivities.  The
 reference's MATLAB cannot be run here (no MATLAB/Octave), so there are no
 golden vectors for this operator.
 """

@@ -17,7 +17,10 @@ pinned dipole tables of ewald_oracle and the whole operator equals the
 dipole operator to the spectral tolerance; for mixed radii the total operator
 is independent of xi and symmetric; the tables are
 continuous at contact and at containment; the small-d branch of the kernel
-is its d -> 0 limit.  The MATLAB cannot run here, so there are no golden
+is its d -> 0 limit; and a lone sphere of radius a (one per cubic cell) has
+the Clausius-Mossotti dipole 4 pi a^3 beta/(1 - beta phi) E0 at several radii
+and permittivities, which fixes the dielectric term 3p/(4 pi a^3 (eps-1))
+(RealSpace.m:36) and the self terms.  The MATLAB cannot run here, so there are no golden
 vectors.
 """
 from __future__ import annotations

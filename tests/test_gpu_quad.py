@@ -290,3 +290,34 @@ def test_quad_operator_keeps_default_tile_for_dipole_only_shape(gpu):
     c.ctx.set_tuning(tile_ty=4, tile_tz=4, chunk=64, engine=1)
     out = c.ctx.quad_matvec(m).cpu().numpy()
     assert _rel(out, ref) < 1e-13
+
+
+def test_lone_sphere_quadrupole_clausius_mossotti_gpu(gpu):
+    """mpol_quad_solve on a lone sphere under an imposed traceless field
+    gradient: S = 0 and Q = g / D2, D2 = 3/(4 pi) (eps + 3/2)/(eps - 1) (the
+    l = 2 Clausius-Mossotti response; pins the dielectric term
+    15/(8 pi (eps-1)) with the self terms), four permittivities as four
+    right-hand sides of one solve; and the oracle's dense solve to 1e-10."""
+    from mpol_hip import Context
+
+    from test_quad_oracle import _lone_quad_geometry, lone_quad_expected
+    geo = _lone_quad_geometry()
+    ctx = Context(gpu)
+    ctx.set_tables(geo.tabs[0], geo.tabs[1], geo.rvals)
+    ctx.set_geometry(geo.x, geo.box, geo.Ngrid, geo.P, geo.xi, geo.eta, geo.rc)
+    ctx.set_quad_tables(geo.tabs[2:])
+    epss = np.array([4.0, 0.3, 40.0, 1.5])
+    ctx.set_permittivity(epss[:, None])
+    g = np.array([1.0, 0.3, -0.2, -0.5, 0.4])
+    b = np.zeros((4, 1, 8))
+    b[:, 0, 3:] = g
+    res = ctx.quad_solve(b, np.zeros_like(b), 1e-13, restart=8, maxit=20)
+    out = res.x.cpu().numpy()[:, 0]
+    for k, eps in enumerate(epss):
+        S, Q = out[k, :3], out[k, 3:]
+        ref = lone_quad_expected(eps) * g
+        assert np.max(np.abs(S)) < 1e-10
+        assert np.max(np.abs(Q - ref)) <= 1e-4 * np.max(np.abs(ref)), (eps, Q / g)
+        dense = np.linalg.solve(qo.dense_operator(geo, np.array([eps])), np.concatenate([np.zeros(3), g])).real
+        assert _rel(out[k], dense) < 1e-10, (eps, _rel(out[k], dense))
+    ctx.close()

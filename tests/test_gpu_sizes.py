@@ -232,3 +232,33 @@ def test_sizes_spectrum_complex_and_closed_form(gpu):
             assert _rel(p[:, 3 * d:3 * d + 3, k, 0], ref) < 3e-3  # the solves stop at tol 1e-3
     assert _rel(pc, p) < 1e-2
     assert np.max(np.abs(C.imag)) > 0
+
+
+def test_lone_sphere_clausius_mossotti_gpu(gpu):
+    """mpol_sizes_solve on a lone sphere (one per cubic cell) for several
+    radii and permittivities, all in one batched solve: p = 4 pi a^3 beta /
+    (1 - beta phi) E0 (the Clausius-Mossotti closed form, which pins the
+    dielectric term 3p/(4 pi a^3 (eps-1)) and the self terms at each radius),
+    and the oracle's dense solve to 1e-10."""
+    from mpol_hip import Context
+
+    from test_sizes_oracle import LONE_CASES, lone_sphere_expected
+    L, xi, tol = 20.0, 0.6, 1e-10
+    x = np.array([[3.1, 7.4, 11.9]])
+    box = np.array([L, L, L])
+    for a, eps in LONE_CASES:
+        geo = so.SizesGeometry(x, np.array([a]), box, xi, tol, None, None, None, Ngrid=np.array([40, 40, 40]),
+                               exact=True)
+        ctx = Context(gpu)
+        ctx.set_tables(np.zeros(2), np.zeros(2), np.array([0.0, 2 * geo.rc]))
+        ctx.set_geometry(x, box, geo.Ngrid, geo.P, xi, np.full(3, geo.eta), geo.rc)
+        ctx.set_sizes(np.array([a]))
+        ctx.set_permittivity(np.array([eps]))
+        b = np.array([[[0.0, 0.0, 1.0]]])
+        res = ctx.sizes_solve(b, np.zeros_like(b), 1e-13, restart=3, maxit=20)
+        p = res.x.cpu().numpy()[0, 0]
+        ref = lone_sphere_expected(a, eps, L)
+        assert abs(p[2] - ref) <= 2e-6 * abs(ref), (a, eps, p, ref)
+        dense = np.linalg.solve(so.dense_operator(geo, np.array([eps])), np.array([0.0, 0.0, 1.0])).real
+        assert _rel(p, dense) < 1e-10, (a, eps, _rel(p, dense))
+        ctx.close()

@@ -199,3 +199,43 @@ def test_solve_matches_dense_solve():
     assert np.max(np.abs(S.reshape(-1) - sol[:18])) < 1e-10 * np.max(np.abs(sol))
     assert np.max(np.abs(Q.reshape(-1) - sol[18:])) < 1e-10 * np.max(np.abs(sol))
     assert np.max(np.abs(Q)) > 1e-3 * np.max(np.abs(S))  # quadrupoles are induced
+
+
+# A lone sphere in a cubic periodic box (a simple-cubic lattice) under an
+# imposed traceless field gradient g (right-hand side [0; g]): the inversion
+# symmetry of the lattice decouples S from Q (S = 0), and the quadrupole is
+# the l = 2 response of a dielectric sphere, Q = g / D2 with
+# D2 = 3/(4 pi) (eps + 3/2)/(eps - 1), whose eps-dependence is the l = 2
+# Clausius-Mossotti factor (eps - 1)/(eps + 3/2).  Only the images' rank-4
+# lattice sum, O((a/L)^5) (the cubic splitting of the diagonal and the
+# off-diagonal components, 2e-5 at L = 16), perturbs it.  This fixes the
+# dielectric term 15/(8 pi (eps-1)) (Capacitance_Quadrupole/RealSpace.m:29)
+# against the quadrupole self terms: with 15/(4 pi (eps-1)) D2 would be
+# 30-50 % off.
+def _lone_quad_geometry(L=16.0, xi=0.7, tol=1e-10):
+    rc, ng, P = _params(np.full(3, L), xi, tol)
+    r = np.arange(1e-3, 11.0, 1e-3)
+    tabs, rv = qo.operator_tables(r, xi)
+    return qo.QuadGeometry(np.array([[3.1, 7.4, 11.9]]), np.full(3, L), ng, P, xi, rc, tabs, rv)
+
+
+def lone_quad_expected(eps):
+    return 4.0 * PI / 3.0 * (eps - 1.0) / (eps + 1.5)
+
+
+def test_lone_sphere_quadrupole_clausius_mossotti():
+    geo = _lone_quad_geometry()
+    assert geo.rc < 16.0
+    g = np.array([1.0, 0.3, -0.2, -0.5, 0.4])
+    resp = {}
+    for eps in (4.0, 0.3, 40.0, 1.5):
+        A = qo.dense_operator(geo, np.array([eps]))
+        sol = np.linalg.solve(A, np.concatenate([np.zeros(3), g]))
+        S, Q = sol[:3], sol[3:].real
+        assert np.max(np.abs(S)) < 1e-10
+        ref = lone_quad_expected(eps) * g
+        assert np.max(np.abs(Q - ref)) <= 1e-4 * np.max(np.abs(ref)), (eps, Q / g, lone_quad_expected(eps))
+        resp[eps] = Q
+    # the ratio between permittivities needs no normalisation
+    ratio = resp[4.0] / resp[40.0]
+    assert np.max(np.abs(ratio / (lone_quad_expected(4.0) / lone_quad_expected(40.0)) - 1.0)) < 3e-5

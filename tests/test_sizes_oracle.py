@@ -141,3 +141,35 @@ def test_solve_matches_dense_solve():
     A = so.dense_operator(geo, eps)
     sol = np.linalg.solve(A, np.tile([0.0, 0.0, 1.0], 8)).reshape(8, 3)
     assert np.max(np.abs(p - sol)) < 1e-10 * np.max(np.abs(sol))
+
+
+# A lone sphere of radius a in a cubic periodic box is a simple-cubic lattice
+# of one sphere per cell.  At the dipole level the field of the periodic
+# images at a lattice site (k = 0 excluded) is the Lorentz field p/(3V), so
+# p = 4 pi a^3 beta (E0 + p/(3V)), i.e. p = 4 pi a^3 beta/(1 - beta phi) E0
+# with beta = (eps-1)/(eps+2) and phi = 4 pi a^3/(3V): the Clausius-Mossotti
+# polarisability of a sphere of radius a.  This fixes the diagonal of the
+# operator -- the dielectric term 3p/(4 pi a^3 (eps-1))
+# (Capacitance_Sizes/RealSpace.m:36) together with the radius-dependent self
+# terms -- at every radius: a wrong power of a, or a wrong coefficient, is
+# off by O(1).  No pairs: the images lie beyond rc.
+LONE_CASES = [(0.6, 4.0), (1.0, 4.0), (1.7, 4.0), (1.3, 0.3), (2.2, 40.0)]
+
+
+def lone_sphere_expected(a, eps, L):
+    beta = (eps - 1.0) / (eps + 2.0)
+    phi = 4.0 * np.pi * a ** 3 / (3.0 * L ** 3)
+    return 4.0 * np.pi * a ** 3 * beta / (1.0 - beta * phi)
+
+
+@pytest.mark.parametrize("a,eps", LONE_CASES)
+def test_lone_sphere_clausius_mossotti(a, eps):
+    L, xi, tol = 20.0, 0.6, 1e-10
+    x = np.array([[3.1, 7.4, 11.9]])
+    geo = so.SizesGeometry(x, np.array([a]), np.array([L, L, L]), xi, tol, None, None, None, exact=True)
+    assert geo.rc < L  # the periodic images are outside the real-space sum
+    A = so.dense_operator(geo, np.array([eps]))
+    p = np.linalg.solve(A, np.array([0.0, 0.0, 1.0])).real
+    ref = lone_sphere_expected(a, eps, L)
+    assert abs(p[2] - ref) <= 2e-6 * abs(ref), (p, ref)
+    assert np.max(np.abs(p[:2])) <= 1e-9 * abs(ref)
