@@ -303,6 +303,78 @@ int mpol_compute_forces(mpol_ctx* ctx, const double* pos_dev, const double* dip_
                         const double box_host[3], double r_cut, double prefactor, int32_t periodic_z,
                         int32_t use_cell_list, double* forces_dev, double* pe_dev);
 
+/* ---- Step-by-step functions of the reference's fast path --------------
+ * Stateless (the context gives only the device and stream); every array is
+ * device memory owned by the caller, in the reference's layouts: complex
+ * arrays as interleaved (re, im) FP64 pairs; grids (Nx, Ny, Nz, 3) in C order
+ * (the reference's H[..., c] reshaped in Fortran order from the linear node
+ * index ix + Nx iy + Nx Ny iz); int64 indices.  The operator (mpol_matvec)
+ * never forms these arrays; these entry points serve callers of the
+ * reference's precompute API. */
+
+/* linnode, w_spread, w_contract (n x P^3 each) from positions x (n x 3) and the
+ * stencil offsets offset (int64, P^3 x 3) / offsetxyz (P^3 x 3).
+ * Replaces prepare_grid_kernel ("CPU - Ewald test/ewald.py":233-296). */
+int mpol_grid_kernel(mpol_ctx* ctx, const double* x_dev, int64_t n, const int32_t ngrid[3], const double h[3],
+                     double xi, const double eta[3], int32_t P, const int64_t* offset_dev, const double* offsetxyz_dev,
+                     int64_t* linnode_dev, double* w_spread_dev, double* w_contract_dev);
+
+/* H (complex grid) = sum over particles and stencil points of w m_c at node
+ * linnode; MPOL_EINVAL if a node index lies outside the grid.
+ * Replaces spread_fast ("CPU - Ewald test/ewald.py":299-331) and, with
+ * mpol_grid_kernel, spread (:370-418). */
+int mpol_spread_precomp(mpol_ctx* ctx, const double* m_dev, int64_t n, int32_t P3, const int64_t* linnode_dev,
+                        const double* w_spread_dev, const int32_t ngrid[3], double* H_dev);
+
+/* Hk (complex n x 3) = sum_k w_contract[n, k] H[linnode[n, k], c].
+ * Replaces contract_fast ("CPU - Ewald test/ewald.py":334-368) and, with
+ * mpol_grid_kernel, contract (:421-467). */
+int mpol_contract_precomp(mpol_ctx* ctx, const double* H_dev, const int32_t ngrid[3], const int64_t* linnode_dev,
+                          const double* w_contract_dev, int64_t n, int32_t P3, double* Hk_dev);
+
+/* fHtilde = c(k) (khat . fH) khat for a given wave-vector grid k (real,
+ * Nx x Ny x Nz x 3, fftshift order); zero at the k = 0 entry, non-finite
+ * values mapped as numpy.nan_to_num.
+ * Replaces scale ("CPU - Ewald test/ewald.py":473-516). */
+int mpol_scale_kgrid(mpol_ctx* ctx, const double* fH_dev, const double* k_dev, const int32_t ngrid[3], double xi,
+                     const double eta[3], double* out_dev);
+
+/* khat (complex Nx x Ny x Nz x 3) and Htildecoeff (real Nx x Ny x Nz) on the
+ * fftshift-ordered grid of box.
+ * Replaces precompute_kspace ("CPU - Ewald test/ewald.py":523-582). */
+int mpol_kspace_table(mpol_ctx* ctx, const int32_t ngrid[3], const double box[3], double xi, const double eta[3],
+                      double* khat_dev, double* coeff_dev);
+
+/* fHtilde = (coeff (khat . fH)) khat over G grid points (complex arithmetic).
+ * Replaces scale_fast ("CPU - Ewald test/ewald.py":585-599). */
+int mpol_scale_precomp(mpol_ctx* ctx, const double* fH_dev, const double* khat_dev, const double* coeff_dev,
+                       int64_t G, double* out_dev);
+
+/* Of the m candidate pairs (p1, p2), those with minimum-image d < rc, in their
+ * input order: p1f, p2f, rhat (kept x 3), A, B (arrays of m entries; the
+ * first *kept_host are written); MPOL_ERANGE if a kept separation falls
+ * outside rvals.  Replaces prepare_real_space_pairs
+ * ("CPU - Ewald test/ewald.py":939-1016). */
+int mpol_real_space_pairs(mpol_ctx* ctx, const double* x_dev, const double box[3], const int64_t* p1_dev,
+                          const int64_t* p2_dev, int64_t m, double rc, const double* perp_dev, const double* para_dev,
+                          const double* rvals_dev, int64_t ntab, int64_t* p1f_dev, int64_t* p2f_dev, double* rhat_dev,
+                          double* A_dev, double* B_dev, int64_t* kept_host);
+
+/* Hr (complex n x 3) from the filtered pairs: the dielectric and self terms
+ * plus A m_j + (B - A)(m_j . rhat) rhat at p1.
+ * Replaces real_space_fast ("CPU - Ewald test/ewald.py":1019-1072). */
+int mpol_real_space_precomp(mpol_ctx* ctx, const double* m_dev, const double* lambdap_dev, int64_t n,
+                            const int64_t* p1_dev, const int64_t* p2_dev, const double* rhat_dev, const double* A_dev,
+                            const double* B_dev, int64_t npairs, double self_perp, double* Hr_dev);
+
+/* Hr from the candidate pairs directly (d = 0 skipped, d < rc kept,
+ * interpolation as the reference's loop); MPOL_ERANGE past the table.
+ * Replaces real_space ("CPU - Ewald test/ewald.py":1076-1139). */
+int mpol_real_space_direct(mpol_ctx* ctx, const double* x_dev, const double* m_dev, const double* lambdap_dev,
+                           int64_t n, const double box[3], const int64_t* p1_dev, const int64_t* p2_dev, int64_t ncand,
+                           double rc, const double* perp_dev, const double* para_dev, const double* rvals_dev,
+                           int64_t ntab, double* Hr_dev);
+
 /* The same truncated forces without any device allocation: no counterpart
  * in the reference; for callers that cannot allocate (code under HIP graph
  * capture, tight serving loops such as a HOOMD step, harnesses that forbid

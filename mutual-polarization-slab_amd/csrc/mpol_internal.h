@@ -296,6 +296,7 @@ struct mpol_ctx {
   hipEvent_t step_ev[2] = {nullptr, nullptr};
   // truncated fixed-dipole forces: cell list
   DevBuf<int> f_cell, f_count, f_start, f_cursor, f_members;
+  DevBuf<int> drop_keep, drop_pos, drop_err;  // drop-in pair filter (mpol_dropin.hip)
   int* scratch = nullptr;  // this device's allocation-free scratch (mpol_compute_forces_noalloc)
 };
 

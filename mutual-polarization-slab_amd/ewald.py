@@ -16,17 +16,27 @@ Differences a caller can see, all at rounding level:
     rebuilds the same quantities from x (cell list at rc, min-image d < rc,
     same table interpolation, same Gaussian weights) once per geometry and
     caches them.
+The reference's step-by-step functions (prepare_grid_kernel, spread_fast,
+contract_fast, spread, contract, scale, precompute_kspace, scale_fast,
+prepare_real_space_pairs, real_space_fast, real_space) run on the GPU too
+(mpol_grid_kernel ... mpol_real_space_direct) and return host arrays in the
+reference's dtypes and layouts; sums the reference forms with bincount /
+add.at are formed with atomics, so they agree to rounding.
 Host helpers (cell_list, neighbor_list, pre_calculations, matlab_round,
 besselj_3_over_2) are small NumPy index utilities kept for API parity.
 """
 from __future__ import annotations
 
+import ctypes
 import hashlib
+import threading
 from collections import OrderedDict
 from typing import Tuple
 
 import numpy as np
+import torch
 
+from mpol_hip import _lib
 from mpol_hip import context as _ctxmod
 
 # ---------------------------------------------------------------------------
@@ -111,6 +121,272 @@ def real_space_table(r, xi):
     """The six real-space couplings with the self term prepended, computed by
     the library's FP64 routine ([E]ewald.py:604-931)."""
     return _ctxmod.real_space_table(r, xi)
+
+
+# ---------------------------------------------------------------------------
+# step-by-step functions of the fast path ("CPU - Ewald test/ewald.py":233-1139),
+# each one HIP launch sequence on device copies of its arguments
+# ---------------------------------------------------------------------------
+_UTIL: dict = {}
+_UTIL_LOCK = threading.Lock()
+
+
+def _util():
+    """(context, device) for the stateless entry points of this device."""
+    dev = _ctxmod.require_device()
+    key = (dev.type, dev.index)
+    with _UTIL_LOCK:
+        ctx = _UTIL.get(key)
+        if ctx is None:
+            ctx = _ctxmod.Context(dev)
+            _UTIL[key] = ctx
+    ctx._sync_stream()
+    return ctx, dev
+
+
+def _on(dev, a, dtype):
+    if isinstance(a, torch.Tensor):
+        return a.detach().to(device=dev, dtype=dtype).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(np.asarray(a)), device=dev).to(dtype).contiguous()
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _ng(Ngrid):
+    return np.ascontiguousarray(np.asarray(Ngrid, dtype=np.int32).reshape(3))
+
+
+def _host3(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(3))
+
+
+def _np(t):
+    return t.cpu().numpy()
+
+
+def _check_pairs(n, *idx):
+    """Pair indices must address particles (the kernels index with them)."""
+    for a in idx:
+        a = np.asarray(a)
+        if a.size and (a.min() < 0 or a.max() >= n):
+            raise IndexError(f"pair index out of bounds for {n} particles")
+
+
+def _check_tables(perp, para, rvals):
+    nt = np.asarray(rvals).size
+    if nt < 1 or np.asarray(perp).size < nt or np.asarray(para).size < nt:
+        raise IndexError("perp / para must have at least as many entries as rvals")
+
+
+def _grid_kernel_dev(ctx, dev, x, Ngrid, h, xi, eta, P, offset, offsetxyz):
+    xt = _on(dev, x, torch.float64).reshape(-1, 3)
+    n, P3 = xt.shape[0], int(P) ** 3
+    ot = _on(dev, offset, torch.int64).reshape(-1, 3)
+    oxt = _on(dev, offsetxyz, torch.float64).reshape(-1, 3)
+    if ot.shape[0] != P3 or oxt.shape[0] != P3:
+        raise ValueError("offset/offsetxyz shapes do not match P^3. Recompute PreCalculations.")
+    lin = torch.empty((n, P3), dtype=torch.int64, device=dev)
+    ws = torch.empty((n, P3), dtype=torch.float64, device=dev)
+    wc = torch.empty((n, P3), dtype=torch.float64, device=dev)
+    ng = _ng(Ngrid)
+    _lib.check(ctx._lib.mpol_grid_kernel(ctx._ctx, _p(xt), n, _lib.i32ptr(ng), _lib.dptr(_host3(h)), float(xi),
+                                         _lib.dptr(_host3(eta)), int(P), _p(ot), _p(oxt), _p(lin), _p(ws), _p(wc)))
+    return lin, ws, wc
+
+
+def prepare_grid_kernel(x, Ngrid, h, xi, eta, P, offset, offsetxyz):
+    """(linnode (N,P^3) int64 in F-order node numbering, w_spread, w_contract)
+    ([E]ewald.py:233-296)."""
+    ctx, dev = _util()
+    lin, ws, wc = _grid_kernel_dev(ctx, dev, x, Ngrid, h, xi, eta, P, offset, offsetxyz)
+    return _np(lin), _np(ws), _np(wc)
+
+
+def _spread_dev(ctx, dev, m, Ngrid, lin, ws):
+    ng = _ng(Ngrid)
+    mt = _on(dev, m, torch.complex128).reshape(-1, 3)
+    H = torch.empty((int(ng[0]), int(ng[1]), int(ng[2]), 3), dtype=torch.complex128, device=dev)
+    n = mt.shape[0]
+    P3 = lin.shape[1] if lin.dim() == 2 else 0
+    if lin.shape[0] != n or ws.shape != lin.shape:
+        raise ValueError("linnode / weights do not match the dipoles")
+    _lib.check(ctx._lib.mpol_spread_precomp(ctx._ctx, _p(mt), n, P3, _p(lin), _p(ws), _lib.i32ptr(ng), _p(H)))
+    return H
+
+
+def spread_fast(m, Ngrid, linnode, w_spread):
+    """H (Nx,Ny,Nz,3) complex128 ([E]ewald.py:299-331)."""
+    ctx, dev = _util()
+    lin = _on(dev, linnode, torch.int64)
+    ws = _on(dev, w_spread, torch.float64)
+    return _np(_spread_dev(ctx, dev, m, Ngrid, lin, ws))
+
+
+def _contract_dev(ctx, dev, Htilde, Ngrid, lin, wc):
+    ng = _ng(Ngrid)
+    Ht = _on(dev, Htilde, torch.complex128)
+    if tuple(Ht.shape) != (int(ng[0]), int(ng[1]), int(ng[2]), 3):
+        raise ValueError("Htilde must have shape (Nx, Ny, Nz, 3)")
+    n = lin.shape[0]
+    P3 = lin.shape[1] if lin.dim() == 2 else 0
+    if wc.shape != lin.shape:
+        raise ValueError("linnode / weights shapes differ")
+    Hk = torch.empty((n, 3), dtype=torch.complex128, device=dev)
+    _lib.check(ctx._lib.mpol_contract_precomp(ctx._ctx, _p(Ht), _lib.i32ptr(ng), _p(lin), _p(wc), n, P3, _p(Hk)))
+    return Hk
+
+
+def contract_fast(Htilde, Ngrid, linnode, w_contract):
+    """Hk (N,3) complex128 ([E]ewald.py:334-368)."""
+    ctx, dev = _util()
+    lin = _on(dev, linnode, torch.int64)
+    wc = _on(dev, w_contract, torch.float64)
+    return _np(_contract_dev(ctx, dev, Htilde, Ngrid, lin, wc))
+
+
+def spread(x, m, Ngrid, h, xi, eta, P, offset, offsetxyz):
+    """(Hx, Hy, Hz), each (Nx,Ny,Nz) complex128 ([E]ewald.py:370-418)."""
+    ctx, dev = _util()
+    lin, ws, _ = _grid_kernel_dev(ctx, dev, x, Ngrid, h, xi, eta, P, offset, offsetxyz)
+    H = _np(_spread_dev(ctx, dev, m, Ngrid, lin, ws))
+    return H[..., 0].copy(), H[..., 1].copy(), H[..., 2].copy()
+
+
+def contract(x, Ngrid, h, xi, eta, P, Htilde, offset, offsetxyz):
+    """Hk (N,3) complex128, trapezoidal factor prod(h) included ([E]ewald.py:421-467)."""
+    ctx, dev = _util()
+    lin, _, wc = _grid_kernel_dev(ctx, dev, x, Ngrid, h, xi, eta, P, offset, offsetxyz)
+    return _np(_contract_dev(ctx, dev, Htilde, Ngrid, lin, wc))
+
+
+def scale(fH, k, Ngrid, xi, eta):
+    """fHtilde (Nx,Ny,Nz,3) complex128 for the wave vectors k (Nx,Ny,Nz,3)
+    in fftshift order ([E]ewald.py:473-516)."""
+    ctx, dev = _util()
+    ng = _ng(Ngrid)
+    shape = (int(ng[0]), int(ng[1]), int(ng[2]), 3)
+    ft = _on(dev, fH, torch.complex128)
+    kt = _on(dev, np.real(np.asarray(k)) if not isinstance(k, torch.Tensor) else k, torch.float64)
+    if tuple(ft.shape) != shape or tuple(kt.shape) != shape:
+        raise ValueError("fH and k must have shape (Nx, Ny, Nz, 3)")
+    out = torch.empty(shape, dtype=torch.complex128, device=dev)
+    _lib.check(ctx._lib.mpol_scale_kgrid(ctx._ctx, _p(ft), _p(kt), _lib.i32ptr(ng), float(xi),
+                                         _lib.dptr(_host3(eta)), _p(out)))
+    return _np(out)
+
+
+def precompute_kspace(Ngrid, box, xi, eta):
+    """(khat (Nx,Ny,Nz,3) complex128, Htildecoeff (Nx,Ny,Nz) float64) in
+    fftshift order ([E]ewald.py:523-582)."""
+    ctx, dev = _util()
+    ng = _ng(Ngrid)
+    G3 = (int(ng[0]), int(ng[1]), int(ng[2]))
+    khat = torch.empty(G3 + (3,), dtype=torch.complex128, device=dev)
+    coeff = torch.empty(G3, dtype=torch.float64, device=dev)
+    _lib.check(ctx._lib.mpol_kspace_table(ctx._ctx, _lib.i32ptr(ng), _lib.dptr(_host3(box)), float(xi),
+                                          _lib.dptr(_host3(eta)), _p(khat), _p(coeff)))
+    return _np(khat), _np(coeff)
+
+
+def scale_fast(fH, khat, Htildecoeff):
+    """fHtilde = (Htildecoeff (khat . fH)) khat ([E]ewald.py:585-599)."""
+    ctx, dev = _util()
+    ft = _on(dev, fH, torch.complex128)
+    kh = _on(dev, khat, torch.complex128)
+    cf = _on(dev, np.real(np.asarray(Htildecoeff)) if not isinstance(Htildecoeff, torch.Tensor) else Htildecoeff,
+             torch.float64)
+    if ft.shape != kh.shape or ft.dim() != 4 or ft.shape[3] != 3 or tuple(cf.shape) != tuple(ft.shape[:3]):
+        raise ValueError("fH, khat must be (Nx, Ny, Nz, 3) and Htildecoeff (Nx, Ny, Nz)")
+    out = torch.empty_like(ft)
+    G = ft.shape[0] * ft.shape[1] * ft.shape[2]
+    _lib.check(ctx._lib.mpol_scale_precomp(ctx._ctx, _p(ft), _p(kh), _p(cf), G, _p(out)))
+    return _np(out)
+
+
+def prepare_real_space_pairs(x, box, p1, p2, rc, perp, para, rvals):
+    """(p1f, p2f, rhat (M,3), A, B): candidate pairs with minimum-image d < rc
+    in input order, with interpolated couplings ([E]ewald.py:939-1016)."""
+    p1 = np.asarray(p1, dtype=int).reshape(-1)
+    p2 = np.asarray(p2, dtype=int).reshape(-1)
+    if p1.size == 0:
+        ei = np.array([], dtype=int)
+        ef = np.array([], dtype=np.float64)
+        return ei, ei, ef.reshape((0, 3)), ef, ef
+    if p2.size != p1.size:
+        raise ValueError("p1 and p2 must have the same length")
+    _check_pairs(np.asarray(x).reshape(-1, 3).shape[0], p1, p2)
+    _check_tables(perp, para, rvals)
+    ctx, dev = _util()
+    xt = _on(dev, x, torch.float64).reshape(-1, 3)
+    a1 = _on(dev, p1, torch.int64)
+    a2 = _on(dev, p2, torch.int64)
+    tp = _on(dev, np.real(np.asarray(perp)), torch.float64)
+    tq = _on(dev, np.real(np.asarray(para)), torch.float64)
+    rv = _on(dev, rvals, torch.float64)
+    m = p1.size
+    p1f = torch.empty(m, dtype=torch.int64, device=dev)
+    p2f = torch.empty(m, dtype=torch.int64, device=dev)
+    rhat = torch.empty((m, 3), dtype=torch.float64, device=dev)
+    A = torch.empty(m, dtype=torch.float64, device=dev)
+    B = torch.empty(m, dtype=torch.float64, device=dev)
+    kept = ctypes.c_int64(0)
+    _lib.check(ctx._lib.mpol_real_space_pairs(ctx._ctx, _p(xt), _lib.dptr(_host3(box)), _p(a1), _p(a2), m,
+                                              float(rc), _p(tp), _p(tq), _p(rv), rv.numel(), _p(p1f), _p(p2f),
+                                              _p(rhat), _p(A), _p(B), ctypes.byref(kept)))
+    k = int(kept.value)
+    return (_np(p1f[:k]).astype(int), _np(p2f[:k]).astype(int), _np(rhat[:k]), _np(A[:k]), _np(B[:k]))
+
+
+def real_space_fast(m, lambdap, p1, p2, rhat, A, B, self_perp):
+    """Hr (N,3) complex128 from prepare_real_space_pairs' output
+    ([E]ewald.py:1019-1072)."""
+    ctx, dev = _util()
+    mt = _on(dev, m, torch.complex128).reshape(-1, 3)
+    lam = _on(dev, np.real(np.asarray(lambdap)).reshape(-1), torch.float64)
+    n = mt.shape[0]
+    if lam.numel() != n:
+        raise ValueError("lambdap must have one entry per particle")
+    np_ = int(np.asarray(p1).size)
+    Hr = torch.empty((n, 3), dtype=torch.complex128, device=dev)
+    if np_:
+        _check_pairs(n, p1, p2)
+        a1, a2 = _on(dev, p1, torch.int64), _on(dev, p2, torch.int64)
+        rh = _on(dev, rhat, torch.float64).reshape(-1, 3)
+        At, Bt = _on(dev, A, torch.float64), _on(dev, B, torch.float64)
+        if a2.numel() != np_ or rh.shape[0] != np_ or At.numel() != np_ or Bt.numel() != np_:
+            raise ValueError("p1, p2, rhat, A and B must describe the same pairs")
+        ptrs = (_p(a1), _p(a2), _p(rh), _p(At), _p(Bt))
+    else:
+        ptrs = (None,) * 5
+    _lib.check(ctx._lib.mpol_real_space_precomp(ctx._ctx, _p(mt), _p(lam), n, *ptrs, np_, float(self_perp), _p(Hr)))
+    return _np(Hr)
+
+
+def real_space(x, m, lambdap, box, p1, p2, rc, perp, para, rvals, real_precomp=None):
+    """Hr (N,3) complex128 from the candidate pairs ([E]ewald.py:1076-1139)."""
+    ctx, dev = _util()
+    xt = _on(dev, x, torch.float64).reshape(-1, 3)
+    mt = _on(dev, m, torch.complex128).reshape(-1, 3)
+    lam = _on(dev, np.real(np.asarray(lambdap)).reshape(-1), torch.float64)
+    a1 = _on(dev, np.asarray(p1, dtype=np.int64).reshape(-1), torch.int64)
+    a2 = _on(dev, np.asarray(p2, dtype=np.int64).reshape(-1), torch.int64)
+    tp = _on(dev, np.real(np.asarray(perp)), torch.float64)
+    tq = _on(dev, np.real(np.asarray(para)), torch.float64)
+    rv = _on(dev, rvals, torch.float64)
+    n = mt.shape[0]
+    if xt.shape[0] != n or lam.numel() != n:
+        raise ValueError("x, m and lambdap must describe the same particles")
+    if a1.numel() != a2.numel():
+        raise ValueError("p1 and p2 must have the same length")
+    _check_pairs(n, p1, p2)
+    _check_tables(perp, para, rvals)
+    Hr = torch.empty((n, 3), dtype=torch.complex128, device=dev)
+    _lib.check(ctx._lib.mpol_real_space_direct(ctx._ctx, _p(xt), _p(mt), _p(lam), n, _lib.dptr(_host3(box)),
+                                               _p(a1) if a1.numel() else None, _p(a2) if a2.numel() else None,
+                                               a1.numel(), float(rc), _p(tp), _p(tq), _p(rv), rv.numel(), _p(Hr)))
+    return _np(Hr)
 
 
 # ---------------------------------------------------------------------------

@@ -64,6 +64,20 @@ SIGNATURES = [
                                            ctypes.c_int32, ctypes.c_int32, _VP, _VP]),
     ("mpol_compute_forces_noalloc", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int64, _DP, ctypes.c_double,
                                                    ctypes.c_double, ctypes.c_int32, ctypes.c_int32, _VP, _VP]),
+    ("mpol_grid_kernel", ctypes.c_int, [_VP, _VP, ctypes.c_int64, _I32P, _DP, ctypes.c_double, _DP,
+                                        ctypes.c_int32, _VP, _VP, _VP, _VP, _VP]),
+    ("mpol_spread_precomp", ctypes.c_int, [_VP, _VP, ctypes.c_int64, ctypes.c_int32, _VP, _VP, _I32P, _VP]),
+    ("mpol_contract_precomp", ctypes.c_int, [_VP, _VP, _I32P, _VP, _VP, ctypes.c_int64, ctypes.c_int32, _VP]),
+    ("mpol_scale_kgrid", ctypes.c_int, [_VP, _VP, _VP, _I32P, ctypes.c_double, _DP, _VP]),
+    ("mpol_kspace_table", ctypes.c_int, [_VP, _I32P, _DP, ctypes.c_double, _DP, _VP, _VP]),
+    ("mpol_scale_precomp", ctypes.c_int, [_VP, _VP, _VP, _VP, ctypes.c_int64, _VP]),
+    ("mpol_real_space_pairs", ctypes.c_int, [_VP, _VP, _DP, _VP, _VP, ctypes.c_int64, ctypes.c_double, _VP, _VP,
+                                             _VP, ctypes.c_int64, _VP, _VP, _VP, _VP, _VP,
+                                             ctypes.POINTER(ctypes.c_int64)]),
+    ("mpol_real_space_precomp", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int64, _VP, _VP, _VP, _VP, _VP,
+                                               ctypes.c_int64, ctypes.c_double, _VP]),
+    ("mpol_real_space_direct", ctypes.c_int, [_VP, _VP, _VP, _VP, ctypes.c_int64, _DP, _VP, _VP, ctypes.c_int64,
+                                              ctypes.c_double, _VP, _VP, _VP, ctypes.c_int64, _VP]),
     ("mpol_set_permittivity_complex", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int32]),
     ("mpol_set_quad_tables", ctypes.c_int, [_VP, _DP, ctypes.c_int64]),
     ("mpol_quad_matvec", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int32, ctypes.c_int32]),

@@ -31,6 +31,7 @@ HIP_FLAGS = [
 ]
 
 SOURCES = ["mpol_capi.hip", "mpol_ops_dipole.hip", "mpol_ops_quad.hip", "mpol_ops_sizes.hip", "mpol_adjoint.hip",
+           "mpol_dropin.hip",
            "mpol_tables.cpp"]
 
 

@@ -16,7 +16,15 @@ and records, as small .npz fixtures next to this script:
                           42^3 with Ncell > 3), slab_factor 0 and 1;
   * solve_*.npz        -- magnetic_dipole at tol 1e-12 through SciPy GMRES and
                           the in-file gmres_restarted;
-  * run_*.npz          -- run_capacitance end to end at derived parameters.
+  * run_*.npz          -- run_capacitance end to end at derived parameters;
+  * dropin_n24.npz     -- the step-by-step functions of the fast path
+                          (prepare_grid_kernel, spread_fast, spread,
+                          contract_fast, contract, scale, precompute_kspace,
+                          scale_fast, prepare_real_space_pairs,
+                          real_space_fast, real_space) on a small
+                          anisotropic case with random dipoles and grids.
+
+    python tests/golden/make_golden.py /root/reference dropin   # only that one
 
 Only inputs and outputs are stored; no reference source is copied.
 """
@@ -46,10 +54,62 @@ def rsa(n, phi, seed, radius=1.0):
     return pts, np.array([L, L, L])
 
 
-def main(ref_root):
+def dropin_case(ewald):
+    """The reference's step-by-step functions on 24 spheres in a non-cubic box
+    (8 x 10 x 7 grid, P = 5, coordinates partly negative), with random complex
+    dipoles, grids and permittivities as inputs."""
+    n, xi, P = 24, 0.7, 5
+    x, box = rsa(n, 0.15, 21)
+    box = box * np.array([1.0, 1.15, 0.9])
+    x = x * np.array([1.0, 1.15, 0.9]) - 0.3 * box
+    Ngrid = np.array([8, 10, 7])
+    h = box / Ngrid
+    eta = P * (h * xi) ** 2 / np.pi
+    rc = np.sqrt(-np.log(1e-4)) / xi
+    rng = np.random.default_rng(77)
+    m = rng.standard_normal((n, 3)) + 1j * rng.standard_normal((n, 3))
+    lam = 1.0 + 20.0 * rng.random(n)
+    G = tuple(int(v) for v in Ngrid) + (3,)
+    Htilde = rng.standard_normal(G) + 1j * rng.standard_normal(G)
+    fH = rng.standard_normal(G) + 1j * rng.standard_normal(G)
+    r_full = np.arange(0.001, 10.0 + 1e-12, 0.001, dtype=np.float64)
+    r_table = np.concatenate(([0.0], r_full))
+    _, _, perp, para, _, _ = ewald.real_space_table(r_full, xi)
+    offset, offsetxyz = ewald.pre_calculations(P, h)
+    linnode, w_spread, w_contract = ewald.prepare_grid_kernel(x, Ngrid, h, xi, eta, P, offset, offsetxyz)
+    H_fast = ewald.spread_fast(m, Ngrid, linnode, w_spread)
+    Hx, Hy, Hz = ewald.spread(x, m, Ngrid, h, xi, eta, P, offset, offsetxyz)
+    Hk_fast = ewald.contract_fast(Htilde, Ngrid, linnode, w_contract)
+    Hk = ewald.contract(x, Ngrid, h, xi, eta, P, Htilde, offset, offsetxyz)
+    ks = [np.arange(-int(np.ceil((N - 1) / 2.0)), int(np.floor((N - 1) / 2.0)) + 1) * (2.0 * np.pi / L)
+          for N, L in zip(Ngrid, box)]
+    k = np.stack(np.meshgrid(*ks, indexing="ij"), axis=3)
+    fHt = ewald.scale(fH, k, Ngrid, xi, eta)
+    khat, coeff = ewald.precompute_kspace(Ngrid, box, xi, eta)
+    fHt_fast = ewald.scale_fast(fH, khat, coeff)
+    cell, ncell = ewald.cell_list(x, box, rc)
+    p1, p2 = ewald.neighbor_list(cell, ncell)
+    p1f, p2f, rhat, A, B = ewald.prepare_real_space_pairs(x, box, p1, p2, rc, perp, para, r_table)
+    Hr_fast = ewald.real_space_fast(m, lam, p1f, p2f, rhat, A, B, float(perp[0]))
+    Hr = ewald.real_space(x, m, lam, box, p1, p2, rc, perp, para, r_table)
+    np.savez_compressed(os.path.join(HERE, "dropin_n24.npz"), x=x, box=box, Ngrid=Ngrid, h=h, xi=xi, P=P, eta=eta,
+                        rc=rc, m=m, lam=lam, Htilde=Htilde, fH=fH, k=k, offset=offset, offsetxyz=offsetxyz,
+                        linnode=linnode, w_spread=w_spread, w_contract=w_contract, H_fast=H_fast,
+                        H_loop=np.stack((Hx, Hy, Hz), axis=3), Hk_fast=Hk_fast, Hk_loop=Hk, fHt=fHt, khat=khat,
+                        coeff=coeff, fHt_fast=fHt_fast, p1=p1, p2=p2, p1f=p1f, p2f=p2f, rhat=rhat, A=A, B=B,
+                        Hr_fast=Hr_fast, Hr_loop=Hr)
+
+
+def main(ref_root, only=None):
     sys.path.insert(0, os.path.join(ref_root, "CPU - Ewald test"))
     import ewald  # noqa: E402  (reference module, read-only tree)
     import capacitance  # noqa: E402
+
+    if only == "dropin":
+        dropin_case(ewald)
+        print("golden fixture dropin_n24.npz written to", HERE)
+        return
+    dropin_case(ewald)
 
     r_full = np.arange(0.001, 10.0 + 1e-12, 0.001, dtype=np.float64)
     sub = np.unique(np.concatenate((np.arange(0, 60), np.arange(1985, 2016), np.arange(0, r_full.size, 7),
@@ -131,4 +191,4 @@ def main(ref_root):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference", sys.argv[2] if len(sys.argv) > 2 else None)
