@@ -52,9 +52,16 @@ def ewald_parameters(box, xi, errortol):
 # restarted GMRES with a user matvec ("CPU - Ewald test/capacitance.py":60-199)
 # ---------------------------------------------------------------------------
 def gmres_restarted(matvec: Callable[[np.ndarray], np.ndarray], b, x0, tol: float, restart: int, maxit: int,
-                    verbose: int = 0) -> Tuple[np.ndarray, int, float]:
+                    verbose: int = 0, *, device_matvec: bool = False) -> Tuple[np.ndarray, int, float]:
     """Restarted GMRES (MGS Arnoldi, complex Givens, best-so-far least squares)
-    for a Python matvec; vectors live on the GPU.  Returns (x, flag, relres)."""
+    for a Python matvec; the Krylov vectors live on the GPU.  Returns
+    (x, flag, relres) like the reference.
+
+    The reference's matvec takes and returns NumPy vectors, so by default
+    every application copies the vector to the host and back.  With
+    device_matvec=True the matvec takes and returns complex128 torch tensors
+    on the GPU (for example a wrapper of Context.matvec), and no vector
+    leaves the device."""
     dev = _ctxmod.require_device()
     b_t = torch.as_tensor(np.asarray(b, dtype=np.complex128).reshape(-1), device=dev)
     x = torch.as_tensor(np.asarray(x0, dtype=np.complex128).reshape(-1), device=dev).clone()
@@ -63,6 +70,9 @@ def gmres_restarted(matvec: Callable[[np.ndarray], np.ndarray], b, x0, tol: floa
         return x.cpu().numpy(), 0, 0.0
 
     def mv(v):
+        if device_matvec:
+            out = matvec(v)
+            return torch.as_tensor(out, device=dev).to(torch.complex128).reshape(-1)
         out = matvec(v.cpu().numpy())
         return torch.as_tensor(np.asarray(out, dtype=np.complex128).reshape(-1), device=dev)
 
