@@ -6,7 +6,9 @@
 // operator launch sequences are instantiated in mpol_ops_*.hip.
 #include "mpol_internal.h"
 
+#include <map>
 #include <mutex>
+#include <utility>
 
 namespace mpol {
 thread_local std::string g_err;
@@ -14,6 +16,17 @@ thread_local std::string g_err;
 int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
+}
+
+void raise_lds_limit(const void* kernel, size_t bytes) {
+  static std::mutex mu;
+  static std::map<std::pair<int, const void*>, size_t> limit;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(mu);
+  size_t& cur = limit[{dev, kernel}];
+  if (bytes <= cur) return;
+  if (hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess) cur = bytes;
 }
 }  // namespace mpol
 

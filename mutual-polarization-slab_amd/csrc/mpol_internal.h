@@ -302,10 +302,16 @@ struct mpol_ctx {
 
 namespace mpol {
 
+// Raise a kernel's dynamic-LDS limit to at least `bytes` on the current
+// device.  The limit only grows, and the attribute is set once per (device,
+// kernel, larger size) under a process-wide lock: launches from several
+// threads (one context each) never meet a concurrent attribute change of the
+// kernel they launch after its first use, and launches make no attribute call.
+void raise_lds_limit(const void* kernel, size_t bytes);
+
 template <typename K>
 void set_lds(K kernel, size_t bytes) {
-  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)bytes);
+  raise_lds_limit(reinterpret_cast<const void*>(kernel), bytes);
 }
 
 // Launch KERNEL(E, N, ...) with the compile-time plan for length n when there

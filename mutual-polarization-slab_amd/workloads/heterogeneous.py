@@ -69,7 +69,8 @@ def main(argv=None):
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--check", type=int, default=4, help="configurations re-solved in FP64 for the error")
     ap.add_argument("--tol", type=float, default=1e-3, help="Ewald rule tolerance")
-    ap.add_argument("--gmres-tol", type=float, default=1e-6)
+    ap.add_argument("--gmres-tol", type=float, default=1e-8,
+                    help="GMRES tolerance (1e-8: mixed dipoles within 1e-6 of tight FP64 solves)")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--fp64", action="store_true", help="also time the whole set with the FP64 operator throughout")
