@@ -226,10 +226,14 @@ def maxwell_garnett_guess(eps_p, phi, Hhat):
 
 
 def run_capacitance(x, box, eps_p, xi, H0, errortol: float = 1e-5, slab_factor: float = 1.0,
-                    use_scipy_gmres: bool = True, verbose: int = 0, *, precision: str = "fp64", p_guess=None):
+                    use_scipy_gmres: bool = True, verbose: int = 0, *, precision: str = "fp64", p_guess=None,
+                    frame_workers: int = 4):
     """(C (F,) complex128, p (N,3,F) complex128) for one frame or a trajectory.
     p_guess ((N,3) or (N,3,F); not in the reference) replaces the
-    Maxwell-Garnett initial guess, e.g. the previous step's dipoles."""
+    Maxwell-Garnett initial guess, e.g. the previous step's dipoles.  The
+    frames of a trajectory are solved concurrently by frame_workers contexts
+    on their own streams (mpol_hip.frames), each frame exactly as the
+    one-frame loop solves it; frame_workers=1 runs that loop."""
     x = np.asarray(x, dtype=np.float64)
     box = np.asarray(box, dtype=np.float64).reshape(3)
     eps_p = np.asarray(eps_p, dtype=np.float64).reshape(-1)
@@ -253,14 +257,32 @@ def run_capacitance(x, box, eps_p, xi, H0, errortol: float = 1e-5, slab_factor: 
             p_guess = p_guess[:, :, None]
         if p_guess.shape[0] != n or p_guess.shape[1] != 3 or p_guess.shape[2] not in (1, nf):
             raise ValueError(f"p_guess must have shape (N,3) or (N,3,F); got {p_guess.shape}")
-    for f in range(nf):
+    def guess_of(f):
         if p_guess is None:
-            guess = maxwell_garnett_guess(eps_p, phi, Hhat)
-        else:
-            guess = p_guess[:, :, min(f, p_guess.shape[2] - 1)]
-        C[f], p[:, :, f] = capacitance(x[:, :, f], eps_p, box, guess, xi, errortol, perp, para, rtable, offset,
-                                       offsetxyz, Hhat, slab_factor=slab_factor, use_scipy_gmres=use_scipy_gmres,
-                                       verbose=verbose, precision=precision)
+            return maxwell_garnett_guess(eps_p, phi, Hhat)
+        return p_guess[:, :, min(f, p_guess.shape[2] - 1)]
+
+    cplx_guess = p_guess is not None and np.iscomplexobj(p_guess) and bool(np.any(np.imag(p_guess) != 0))
+    if nf > 1 and frame_workers > 1 and not cplx_guess:
+        from mpol_hip import frames as _frames
+        rc, _, ngrid, h, P, eta = ewald_parameters(box, xi, errortol)
+        if np.any(rc > box / 2.0):
+            raise ValueError("Real-space cutoff exceeds half the box size; tolerance unmet.")
+        guesses = np.stack([np.real(guess_of(f)) for f in range(nf)], axis=2)
+        p, infos = _frames.solve_frames(x, box, eps_p, (rc, ngrid, P, xi, eta), errortol, Hhat, guesses,
+                                        slab_factor=slab_factor, precision=precision, workers=frame_workers)
+        for f in range(nf):
+            C[f] = np.mean(p[:, :, f] @ Hhat)
+            if verbose >= 1:
+                inf = infos[f]
+                print(f"  frame {f + 1}/{nf}: flag={inf['flag']} relres={inf['relres']:.3e} "
+                      f"matvecs={inf['matvecs_fp64'] + inf['matvecs_fp32']} C={np.real(C[f]):.6g}", flush=True)
+        magnetic_dipole.last_info = infos[-1]
+        return C, p
+    for f in range(nf):
+        C[f], p[:, :, f] = capacitance(x[:, :, f], eps_p, box, guess_of(f), xi, errortol, perp, para, rtable,
+                                       offset, offsetxyz, Hhat, slab_factor=slab_factor,
+                                       use_scipy_gmres=use_scipy_gmres, verbose=verbose, precision=precision)
     return C, p
 
 

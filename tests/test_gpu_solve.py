@@ -98,3 +98,31 @@ def test_tensor_spectrum_batched_equals_single_solves(gpu):
     S = capacitance.magnetic_dipole(x, eps[:, 1], np.array([0, 0, 1.0]), box, None, None, ng, h, P, 0.9, eta, rc,
                                     guess, None, None, perp, para, rv, 1e-10, slab_factor=0.0)
     assert relerr(p[:, :, 2, 1], S.real) < 1e-6
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_run_capacitance_trajectory_frames_concurrent(gpu, precision):
+    """A trajectory (N, 3, F): the frames solved concurrently on worker
+    streams give the dipoles and capacitances of the one-frame loop
+    ("CPU - Ewald test/capacitance.py":495-513) -- to rounding in FP64; in
+    mixed precision the FP32 operator's rounding (the 17^3 grid is off the
+    matrix-core path, where the spread's atomics order varies) moves the
+    iterates within the solve tolerance -- and its first frame (the golden
+    run's positions) matches the reference's run_capacitance."""
+    import capacitance
+    d = golden("run_n64_slab1.npz")
+    x0, box = d["x"], d["box"]
+    rng = np.random.default_rng(12)
+    frames = [x0] + [np.mod(x0 + 0.05 * k * rng.standard_normal(x0.shape), box) for k in range(1, 6)]
+    x = np.stack(frames, axis=2)
+    args = (x, box, d["eps"], float(d["xi"]), d["H0"])
+    kw = dict(errortol=float(d["errortol"]), slab_factor=1.0, precision=precision)
+    C1, p1 = capacitance.run_capacitance(*args, frame_workers=1, **kw)
+    C4, p4 = capacitance.run_capacitance(*args, frame_workers=4, **kw)
+    assert p4.shape == (64, 3, 6) and C4.shape == (6,)
+    bar = 1e-12 if precision == "fp64" else 1e-5
+    assert relerr(p4, p1) < bar, relerr(p4, p1)
+    assert np.max(np.abs(C4 - C1) / np.abs(C1)) < bar
+    assert relerr(p4[:, :, 0], d["p"][:, :, 0]) < 1e-4
+    assert abs(C4[0] - d["C"][0]) / abs(d["C"][0]) < 1e-5
+    assert len(set(np.round(np.real(C4), 12))) == 6  # the frames differ
