@@ -279,6 +279,13 @@ int mpol_real_space_table_field(double xi, const double* r_host, int64_t n, doub
  * of RealSpaceTable (Capacitance_Quadrupole/RealSpaceTable.m:1-380). */
 int mpol_real_space_table_quad(double xi, const double* r_host, int64_t n, double* out_host);
 
+/* The same eight quadrupole couplings without the overlap terms of r < 2: the
+ * field and gradient at a point from a sphere's moments, as the field-map
+ * tables are built.  Replaces RealSpaceTable
+ * (Compute_Field_Quadrupole/RealSpaceTable.m:1-380, whose regularisation
+ * lines are commented out). */
+int mpol_real_space_table_quad_field(double xi, const double* r_host, int64_t n, double* out_host);
+
 /* Polydisperse pair tables: for the M radii (host) and separations r (host,
  * n values, r[i] > 0 for i > 0), perp and para (host, M(M+1)/2 x n, pair
  * order as in mpol_set_sizes) get the I - rr and rr couplings; entry 0 of
@@ -286,6 +293,13 @@ int mpol_real_space_table_quad(double xi, const double* r_host, int64_t n, doubl
  * Replaces RealSpaceTable (Capacitance_Sizes/RealSpaceTable.m:1-150). */
 int mpol_real_space_table_sizes(double xi, const double* r_host, int64_t n, const double* radii_host, int32_t M,
                                 double* perp_host, double* para_host);
+
+/* The same pair couplings without the overlap terms and without an r -> 0
+ * entry (every r[i] > 0): the field at a point treated as a sphere of radius
+ * radii[i] from a sphere of radius radii[j], as the field-map tables are built.
+ * Replaces RealSpaceTable (Compute_Field_Sizes/RealSpaceTable.m:1-110). */
+int mpol_real_space_table_sizes_field(double xi, const double* r_host, int64_t n, const double* radii_host,
+                                      int32_t M, double* perp_host, double* para_host);
 
 /* Truncated point-dipole forces and per-particle energies of fixed dipoles:
  * pos, dip (n x 3) and the outputs forces (n x 3), pe (n) are FP64 device

@@ -1137,12 +1137,37 @@ int mpol_real_space_table_sizes(double xi, const double* r, int64_t n, const dou
   return MPOL_OK;
 }
 
+int mpol_real_space_table_sizes_field(double xi, const double* r, int64_t n, const double* radii, int32_t M,
+                                      double* perp, double* para) {
+  if (!r || !radii || !perp || !para || n < 1 || M < 1) return fail(MPOL_EINVAL, "bad table arguments");
+  if (!(xi > 0.0)) return fail(MPOL_EINVAL, "xi must be positive");
+  for (int32_t k = 0; k < M; ++k)
+    if (!(radii[k] > 0.0)) return fail(MPOL_EINVAL, "radii must be positive");
+  for (int64_t i = 0; i < n; ++i)
+    if (!(r[i] > 0.0)) return fail(MPOL_EINVAL, "table separations must be positive");
+  int64_t lin = 0;
+  for (int32_t i = 0; i < M; ++i)
+    for (int32_t j = i; j < M; ++j, ++lin)
+      for (int64_t k = 0; k < n; ++k)
+        sizes_coupling(r[k], radii[i], radii[j], xi, perp[lin * n + k], para[lin * n + k], false);
+  return MPOL_OK;
+}
+
 int mpol_real_space_table_quad(double xi, const double* r, int64_t n, double* out) {
   if (!r || !out || n < 0) return fail(MPOL_EINVAL, "bad table arguments");
   if (!(xi > 0.0)) return fail(MPOL_EINVAL, "xi must be positive");
   for (int64_t i = 0; i < n; ++i)
     if (!(r[i] > 0.0)) return fail(MPOL_EINVAL, "table separations must be positive");
   real_space_tables_quad(xi, r, n, out);
+  return MPOL_OK;
+}
+
+int mpol_real_space_table_quad_field(double xi, const double* r, int64_t n, double* out) {
+  if (!r || !out || n < 0) return fail(MPOL_EINVAL, "bad table arguments");
+  if (!(xi > 0.0)) return fail(MPOL_EINVAL, "xi must be positive");
+  for (int64_t i = 0; i < n; ++i)
+    if (!(r[i] > 0.0)) return fail(MPOL_EINVAL, "table separations must be positive");
+  real_space_tables_quad(xi, r, n, out, false);
   return MPOL_OK;
 }
 

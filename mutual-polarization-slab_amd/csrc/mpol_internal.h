@@ -21,7 +21,7 @@
 
 namespace mpol {
 void real_space_tables(double xi, const double* r, int64_t n, double* out, bool regularize = true);
-void real_space_tables_quad(double xi, const double* r, int64_t n, double* out);
+void real_space_tables_quad(double xi, const double* r, int64_t n, double* out, bool regularize = true);
 }
 
 using namespace mpol;

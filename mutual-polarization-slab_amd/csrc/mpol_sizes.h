@@ -36,7 +36,7 @@ struct SizesPoly {
 //   E_u(r) exp(-(r+u)^2 xi^2) + F_u(r) erfc((r+u) xi),   plus the overlap
 // regularisation, with E_u = poly/(r^3 xi^5), F_u = poly/(r^3 xi^6) scaled.
 __host__ __device__ inline void sizes_coupling(double r, double ai, double aj, double x, double& perp,
-                                               double& para) {
+                                               double& para, bool regularize = true) {
   const double pi = kPi, p32 = pi * sqrt(pi);
   const double s = ai + aj, d = ai - aj;
   const double x2 = x * x, x4 = x2 * x2, x6 = x4 * x2;
@@ -74,7 +74,9 @@ __host__ __device__ inline void sizes_coupling(double r, double ai, double aj, d
     const double ce = 1.0 / (1024 * p32 * den * x4 * x), cf = 1.0 / (2048 * pi * den * x6);
     double v = ce * (e1.eval(r) * ex[0] + e2.eval(r) * ex[1] + e3.eval(r) * ex[2] + e4.eval(r) * ex[3]) +
                cf * (f1.eval(r) * ef[0] + f2.eval(r) * ef[1] + f3.eval(r) * ef[2] + f4.eval(r) * ef[3]);
-    if (r < s && r >= d && r >= -d) {
+    if (!regularize) {
+      // field-map tables: no overlap terms (Compute_Field_Sizes/RealSpaceTable.m)
+    } else if (r < s && r >= d && r >= -d) {
       const double t = s - r;
       v += t * t * t / (128 * pi * den) * (-r3 - 3 * s * r * r + 3 * qp * r + ai3 - 3 * ai2 * aj - 3 * ai * aj2 + aj3);
     } else if (r < -d && r > d) {
@@ -105,7 +107,8 @@ __host__ __device__ inline void sizes_coupling(double r, double ai, double aj, d
     const double ce = 1.0 / (512 * p32 * den * x4 * x), cf = 1.0 / (1024 * pi * den * x6);
     double v = ce * (e1.eval(r) * ex[0] + e2.eval(r) * ex[1] + e3.eval(r) * ex[2] + e4.eval(r) * ex[3]) +
                cf * (f1.eval(r) * ef[0] + f2.eval(r) * ef[1] + f3.eval(r) * ef[2] + f4.eval(r) * ef[3]);
-    if (r < s && r >= d && r >= -d) {
+    if (!regularize) {
+    } else if (r < s && r >= d && r >= -d) {
       const double r2 = r * r;
       v += -1.0 / (64 * pi * den) * (s * s * s * s * qp - 8 * (ai3 + aj3) * r3 + 9 * sq * r2 * r2 - 2 * r3 * r3);
     } else if (r < -d && r > d) {

@@ -302,7 +302,9 @@ std::vector<Coupling> quad_couplings(double x) {
 
 // out: 8 x (n+1) in the order pot_quad, field_quad_1..3, grad_quad_1..4;
 // entry 0 of each row is the self term (only grad_quad_2 has one).
-void real_space_tables_quad(double xi, const double* r, int64_t n, double* out) {
+// regularize = false leaves out the [r < 2] overlap terms, as the field-map
+// tables do (Compute_Field_Quadrupole/RealSpaceTable.m).
+void real_space_tables_quad(double xi, const double* r, int64_t n, double* out, bool regularize) {
   const std::vector<Coupling> cp = quad_couplings(xi);
   const double sp = std::sqrt(kPi), x2 = xi * xi, x5 = x2 * x2 * xi;
   const double self_gq2 = 1.0 / (2 * kPi) *
@@ -321,7 +323,7 @@ void real_space_tables_quad(double xi, const double* r, int64_t n, double* out) 
       const Coupling& c = cp[t];
       double v = c.ep.eval(rv) * ep + c.em.eval(rv) * em + c.e0.eval(rv) * e0 + c.fp.eval(rv) * fp +
                  c.fm.eval(rv) * fm + c.f0.eval(rv) * f0;
-      if (rv < 2.0) v += regularisation(c.reg, rv, xi);
+      if (regularize && rv < 2.0) v += regularisation(c.reg, rv, xi);
       out[t * (n + 1) + 1 + i] = v;
     }
   }

@@ -56,6 +56,31 @@ def real_space_table_quad(r, xi):
     return tuple(out[k] for k in range(8))
 
 
+def real_space_table_quad_field(r, xi):
+    """The eight quadrupole couplings of real_space_table_quad without the
+    overlap terms of r < 2 (field-map tables), self term first."""
+    r = np.ascontiguousarray(np.asarray(r, dtype=np.float64).reshape(-1))
+    out = np.zeros((8, r.size + 1), dtype=np.float64)
+    lib = _lib.load()
+    _lib.check(lib.mpol_real_space_table_quad_field(float(xi), _lib.dptr(r), r.size, _lib.dptr(out)))
+    return tuple(out[k] for k in range(8))
+
+
+def real_space_table_sizes_field(r, radii, xi):
+    """(perp, para), each (len(r), M(M+1)/2): the pair couplings of
+    real_space_table_sizes without the overlap terms and without an r -> 0
+    row (every r > 0), for field maps."""
+    r = np.ascontiguousarray(np.asarray(r, dtype=np.float64).reshape(-1))
+    a = np.ascontiguousarray(np.asarray(radii, dtype=np.float64).reshape(-1))
+    m = a.size * (a.size + 1) // 2
+    perp = np.zeros((m, r.size), dtype=np.float64)
+    para = np.zeros((m, r.size), dtype=np.float64)
+    lib = _lib.load()
+    _lib.check(lib.mpol_real_space_table_sizes_field(float(xi), _lib.dptr(r), r.size, _lib.dptr(a), a.size,
+                                                     _lib.dptr(perp), _lib.dptr(para)))
+    return perp.T.copy(), para.T.copy()
+
+
 def real_space_table_sizes(r, radii, xi):
     """(perp, para), each (len(r), M(M+1)/2) for the M radii: the I - rr and
     rr pair couplings of every radius pair (i <= j), row 0 the r -> 0 value."""

@@ -45,18 +45,23 @@ OPERATOR_TABLES = ("field_dip_1", "field_dip_2", "field_quad_1", "field_quad_2",
                    "grad_quad_1", "grad_quad_2", "grad_quad_3", "grad_quad_4")
 
 
-def _combine(r, x, ep, em, e0, fp, fm, f0, reg):
+def _combine(r, x, ep, em, e0, fp, fm, f0, reg, reg_on=True):
     return (ep * np.exp(-(r + 2) ** 2 * x * x) + em * np.exp(-(r - 2) ** 2 * x * x) + e0 * np.exp(-r * r * x * x)
-            + fp * _erfc((r + 2) * x) + fm * _erfc((r - 2) * x) + f0 * _erfc(r * x) + (r < 2) * reg)
+            + fp * _erfc((r + 2) * x) + fm * _erfc((r - 2) * x) + f0 * _erfc(r * x) + (r < 2) * reg * reg_on)
 
 
-def _quad_tables(r, x):
+def _quad_tables(r, x, regularize=True):
     """The eight couplings Quad/RealSpaceTable.m adds to the dipole ones
-    (:66-85, :173-356), written out as in the reference."""
+    (:66-85, :173-356), written out as in the reference; regularize=False
+    drops the (r < 2) overlap terms, as Compute_Field_Quadrupole/
+    RealSpaceTable.m does (its regpoly lines are commented out)."""
+    def comb(*args):
+        return _combine(*args, reg_on=regularize)
+
     p32 = PI ** 1.5
     out = {}
     # potential / quadrupole (unused by the operator; the reference notes its regulariser is wrong)
-    out["pot_quad"] = _combine(
+    out["pot_quad"] = comb(
         r, x,
         3 / (1024 * p32 * x ** 5 * r ** 3) * (20 * x ** 4 * r ** 5 + 24 * x ** 4 * r ** 4 + 2 * x ** 2 * (10 - 4 * x ** 2) * r ** 3
                                             - 16 * x ** 2 * (2 - x ** 2) * r ** 2 - (15 - 12 * x ** 2 + 32 * x ** 4) * r
@@ -73,7 +78,7 @@ def _quad_tables(r, x):
                                             - 2 * x ** 2 * (3 + 8 * x ** 2) * r ** 2 - 3 * (1 + 4 * x ** 2)),
         15 / (1024 * PI * x ** 6 * r ** 3) * (12 * x ** 4 * r ** 4 - 2 * x ** 2 * (3 + 8 * x ** 2) * r ** 2 + 3 + 12 * x ** 2))
     # field / quadrupole (= field gradient / dipole), rrr.S component
-    out["field_quad_1"] = _combine(
+    out["field_quad_1"] = comb(
         r, x,
         15 / (16384 * p32 * x ** 7 * r ** 4) * (-24 * x ** 6 * r ** 7 + 48 * x ** 6 * r ** 6 - 4 * x ** 4 * (9 - 8 * x ** 2) * r ** 5
                                               + 8 * x ** 4 * (3 - 8 * x ** 2) * r ** 4
@@ -100,7 +105,7 @@ def _quad_tables(r, x):
                                              + 45 - 480 * x ** 2),
         -15 / (4 * PI * r ** 4) + 15 * r ** 2 / (64 * PI) * (1 - 3 * r ** 2 / 16))
     # Ir.S + Sr + rS component
-    out["field_quad_2"] = _combine(
+    out["field_quad_2"] = comb(
         r, x,
         3 / (16384 * p32 * x ** 7 * r ** 4) * (-40 * x ** 6 * r ** 7 + 80 * x ** 6 * r ** 6 - 20 * x ** 4 * (11 - 24 * x ** 2) * r ** 5
                                              + 8 * x ** 4 * (45 + 8 * x ** 2) * r ** 4
@@ -127,7 +132,7 @@ def _quad_tables(r, x):
                                              - 9 + 96 * x ** 2),
         3 / (4 * PI * r ** 4) - 3 * r / (8 * PI) * (1 - 5 * r / 8 + 5 * r ** 3 / 128))
     # Ir.S component
-    out["field_quad_3"] = _combine(
+    out["field_quad_3"] = comb(
         r, x,
         5 / (1024 * p32 * x ** 5 * r ** 2) * (4 * x ** 4 * r ** 5 - 8 * x ** 4 * r ** 4 + 16 * x ** 2 * (1 - 2 * x ** 2) * r ** 3
                                             - 24 * x ** 2 * r ** 2 + 3 * r + 6),
@@ -145,7 +150,7 @@ def _quad_tables(r, x):
     x2, x4, x6, x8, x10 = x ** 2, x ** 4, x ** 6, x ** 8, x ** 10
     gq1_f = (16 * x8 * r ** 8 + (96 * x6 - 128 * x8) * r ** 6 + (72 * x4 - 192 * x6 + 256 * x8) * r ** 4
              - (24 * x2 - 96 * x4 + 256 * x6) * r ** 2 + 9 - 48 * x2 + 192 * x4)
-    out["grad_quad_1"] = _combine(
+    out["grad_quad_1"] = comb(
         r, x,
         75 / (16384 * p32 * x ** 7 * r ** 3) * (8 * x6 * r ** 7 - 16 * x6 * r ** 6 + (44 * x4 - 32 * x6) * r ** 5
                                               - (72 * x4 - 64 * x6) * r ** 4 + (18 * x2 + 32 * x4) * r ** 3
@@ -163,7 +168,7 @@ def _quad_tables(r, x):
     gq2_f = (288 * x10 * r ** 10 + (3600 * x8 - 4800 * x10) * r ** 8 + (10800 * x6 - 28800 * x8 + 38400 * x10) * r ** 6
              + (5400 * x4 - 21600 * x6 + 57600 * x8) * r ** 4 - (1350 * x2 - 7200 * x4 + 28800 * x6) * r ** 2
              + 405 - 2700 * x2 + 14400 * x4)
-    out["grad_quad_2"] = _combine(
+    out["grad_quad_2"] = comb(
         r, x,
         3 / (32768 * p32 * x ** 9 * r ** 5) * (-48 * x8 * r ** 9 + 96 * x8 * r ** 8 - (576 * x6 - 608 * x8) * r ** 7
                                              + (1056 * x6 - 1216 * x8) * r ** 6
@@ -192,7 +197,7 @@ def _quad_tables(r, x):
     gq3_f = (-96 * x10 * r ** 10 + (-720 * x8 + 960 * x10) * r ** 8 - (720 * x6 - 1920 * x8 + 2560 * x10) * r ** 6
              + (360 * x4 - 1440 * x6 + 3840 * x8) * r ** 4 - (270 * x2 - 1440 * x4 + 5760 * x6) * r ** 2
              + 135 - 900 * x2 + 4800 * x4)
-    out["grad_quad_3"] = _combine(
+    out["grad_quad_3"] = comb(
         r, x,
         -15 / (65536 * p32 * x ** 9 * r ** 5) * (48 * x8 * r ** 9 - 96 * x8 * r ** 8 + (336 * x6 - 288 * x8) * r ** 7
                                                - 576 * (x6 - x8) * r ** 6 + (216 * x4 + 144 * x6 + 128 * x8) * r ** 5
@@ -219,7 +224,7 @@ def _quad_tables(r, x):
     gq4_f = (288 * x10 * r ** 10 + (720 * x8 - 960 * x10) * r ** 8 - (720 * x6 - 1920 * x8 + 2560 * x10) * r ** 6
              + (1080 * x4 - 4320 * x6 + 11520 * x8) * r ** 4 - (1350 * x2 - 7200 * x4 + 28800 * x6) * r ** 2
              + 945 - 6300 * x2 + 33600 * x4)
-    out["grad_quad_4"] = _combine(
+    out["grad_quad_4"] = comb(
         r, x,
         -15 / (65536 * p32 * x ** 9 * r ** 5) * (144 * x8 * r ** 9 - 288 * x8 * r ** 8 + (288 * x6 + 96 * x8) * r ** 7
                                                - (288 * x6 + 192 * x8) * r ** 6
@@ -254,24 +259,26 @@ def grad_quad_2_self(x):
                            + 3 * _erfc(2 * x))
 
 
-def real_space_table(r, xi):
+def real_space_table(r, xi, regularize=True):
     """The 14 tables of Quad/RealSpaceTable.m in its output order, evaluated
     at every r given (r = 0 gives non-finite couplings, as in the
     reference); when r[0] == 0 the self terms are prepended (:358-378), so
-    every table then has len(r) + 1 entries."""
+    every table then has len(r) + 1 entries.  regularize=False: the tables
+    of Compute_Field_Quadrupole/RealSpaceTable.m (no overlap terms, no self
+    terms prepended)."""
     r = np.asarray(r, dtype=np.float64).reshape(-1)
     x = float(xi)
     with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
         rp = np.where(r > 0, r, 1.0)
-        six = eo.real_space_table(rp, x)  # shared expressions (self first)
+        six = eo.real_space_table(rp, x, regularize=regularize)  # shared expressions (self first)
         base = dict(zip(eo.TABLE_NAMES, (t[1:] for t in six)))
-        quad = _quad_tables(r, x)
+        quad = _quad_tables(r, x, regularize)
     tabs = {}
     for name in TABLE_NAMES:
         v = base[name] if name in base else quad[name]
         v = np.where(r > 0, v, np.nan)
         tabs[name] = v
-    if r.size and r[0] == 0.0:
+    if r.size and r[0] == 0.0 and regularize:
         e4 = np.exp(-4 * x * x)
         tail = _erfc(2 * x) / (4 * PI)
         self_fd = (-1 + 6 * x * x + (1 - 2 * x * x) * e4) / (16 * PI ** 1.5 * x ** 3) + tail
@@ -496,3 +503,50 @@ def dense_operator(geo: QuadGeometry, lam):
         E, G = field(geo, v[:3 * n].reshape(n, 3), v[3 * n:].reshape(n, 5), lam)
         A[:, k] = np.concatenate([E.reshape(-1), G.reshape(-1)])
     return A
+
+
+# ---------------------------------------------------------------------------
+# field at probe points (Compute_Field_Quadrupole/FieldMap.m, MatrixVectorMultiply.m,
+# RealSpace.m)
+# ---------------------------------------------------------------------------
+def field_map(x_E, x_p, S, Q, box, xi):
+    """E (N_E,3) complex at the probe points x_E from the dipoles S (N_p,3)
+    and quadrupoles Q (N_p,5) at x_p: errortol 1e-3 and the parameter rule
+    (FieldMap.m:17-36), the 14 tables on r = [0, 1e-4:1e-4:10] without the
+    overlap terms (FieldMap.m:23-25; Compute_Field_Quadrupole/RealSpaceTable.m
+    prepends no self terms), the particles' moments spread, scaled by the
+    four k-space blocks and contracted at the probes
+    (MatrixVectorMultiply.m:31-76), the real-space sum over probe-particle
+    pairs with 0 < d < rc (RealSpace.m:27-100, brute force over all pairs in
+    place of NeighborList_Types), and E = -(Ek + Er) (FieldMap.m:49-52)."""
+    x_E = np.asarray(x_E, dtype=np.float64).reshape(-1, 3)
+    x_p = np.asarray(x_p, dtype=np.float64).reshape(-1, 3)
+    S = np.asarray(S, dtype=np.complex128).reshape(-1, 3)
+    Q = np.asarray(Q, dtype=np.complex128).reshape(-1, 5)
+    box = np.asarray(box, dtype=np.float64).reshape(3)
+    errortol = 1e-3
+    rvals = np.concatenate(([0.0], np.arange(1, 100001) * 1e-4))
+    t = dict(zip(TABLE_NAMES, real_space_table(rvals, xi, regularize=False)))
+    tabs = np.stack([t[n] for n in OPERATOR_TABLES])
+    rc, _, Ngrid, h, P, eta = ewald_parameters(box, xi, errortol)
+    if np.any(rc > box / 2):
+        raise ValueError("Real space cutoff larger than half the box size")
+    offset, offsetxyz = eo.pre_calculations(P, h)
+    lin_p, w_p, _ = eo.prepare_grid_kernel(x_p, Ngrid, h, xi, eta, P, offset, offsetxyz)
+    H = spread(np.concatenate([S, Q], axis=1), Ngrid, lin_p, w_p)
+    fH = np.fft.fftn(H, axes=(0, 1, 2))
+    Ht = np.fft.ifftn(scale(fH, kgrid(Ngrid, box), xi, eta), axes=(0, 1, 2))
+    lin_e, _, wc_e = eo.prepare_grid_kernel(x_E, Ngrid, h, xi, eta, P, offset, offsetxyz)
+    Ek = contract(Ht, Ngrid, lin_e, wc_e)[:, :3]
+    Er = np.zeros((x_E.shape[0], 3), dtype=np.complex128)
+    for i in range(x_E.shape[0]):
+        rr = x_E[i][None, :] - x_p
+        rr = rr - box[None, :] * np.trunc(2.0 * rr / box[None, :])
+        d = np.sqrt(np.sum(rr * rr, axis=1))
+        keep = (d < rc) & (d > 0)
+        if not np.any(keep):
+            continue
+        d, rr = d[keep], rr[keep]
+        Ef, _ = pair_terms(S[keep], Q[keep], rr / d[:, None], interpolate(d, tabs, rvals))
+        Er[i] = np.sum(Ef, axis=0)
+    return -(Ek + Er)

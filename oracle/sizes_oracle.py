@@ -68,8 +68,10 @@ def pair_columns(M):
 # ---------------------------------------------------------------------------
 # real-space couplings (Siz/RealSpaceTable.m)
 # ---------------------------------------------------------------------------
-def _couplings(r, ai, aj, x):
-    """(perp, para) at separations r > 0 for radii ai, aj (:34-135)."""
+def _couplings(r, ai, aj, x, regularize=True):
+    """(perp, para) at separations r > 0 for radii ai, aj (:34-135);
+    regularize=False leaves out the overlap terms f_reg, as
+    Compute_Field_Sizes/RealSpaceTable.m does."""
     r = np.asarray(r, dtype=np.float64)
     s, d = ai + aj, ai - aj
     x2, x4, x6 = x * x, x ** 4, x ** 6
@@ -118,7 +120,7 @@ def _couplings(r, ai, aj, x):
     reg[m3] = (r[m3] ** 3 - ai ** 3) / (4 * PI * ai ** 3 * r[m3] ** 3)
     perp = (f1 * np.exp(-(r + s) ** 2 * x2) + f2 * np.exp(-(r - s) ** 2 * x2) + f3 * np.exp(-(r - ai + aj) ** 2 * x2)
             + f4 * np.exp(-(r + ai - aj) ** 2 * x2) + f5 * _erfc((r + s) * x) + f6 * _erfc((r - s) * x)
-            + f7 * _erfc((r - ai + aj) * x) + f8 * _erfc((r + ai - aj) * x) + reg)
+            + f7 * _erfc((r - ai + aj) * x) + f8 * _erfc((r + ai - aj) * x) + reg * regularize)
     # rr component
     cq = 1.0 / (512 * P32 * ai ** 3 * aj ** 3 * r ** 3 * x ** 5)
     b3p = (ai ** 3 - 3 * ai ** 2 * aj - 3 * ai * aj ** 2 + aj ** 3)
@@ -156,7 +158,7 @@ def _couplings(r, ai, aj, x):
     reg[m3] = (r[m3] ** 3 + 2 * ai ** 3) / (4 * PI * ai ** 3 * r[m3] ** 3)
     para = (g1 * np.exp(-(r + s) ** 2 * x2) + g2 * np.exp(-(r - s) ** 2 * x2) + g3 * np.exp(-(r - ai + aj) ** 2 * x2)
             + g4 * np.exp(-(r + ai - aj) ** 2 * x2) + g5 * _erfc((r + s) * x) + g6 * _erfc((r - s) * x)
-            + g7 * _erfc((r - ai + aj) * x) + g8 * _erfc((r + ai - aj) * x) + reg)
+            + g7 * _erfc((r - ai + aj) * x) + g8 * _erfc((r + ai - aj) * x) + reg * regularize)
     return perp, para
 
 
@@ -333,3 +335,99 @@ def dense_operator(geo: SizesGeometry, eps_p):
         v[k] = 1.0
         A[:, k] = field(geo, v.reshape(n, 3), eps_p).reshape(-1)
     return A
+
+
+# ---------------------------------------------------------------------------
+# field at probe points (Compute_Field_Sizes/FieldMap.m, ComputeField.m, RealSpace.m)
+# ---------------------------------------------------------------------------
+def real_space_table_field(r, a_uniq, xi):
+    """(Ep_perp, Ep_para), each (len(r), M(M+1)/2), at every r > 0 given,
+    without the overlap terms and without an r -> 0 row
+    (Compute_Field_Sizes/RealSpaceTable.m)."""
+    r = np.asarray(r, dtype=np.float64).reshape(-1)
+    a = np.asarray(a_uniq, dtype=np.float64).reshape(-1)
+    M = a.size
+    perp = np.zeros((r.size, M * (M + 1) // 2))
+    para = np.zeros_like(perp)
+    lin = 0
+    for i in range(M):
+        for j in range(i, M):
+            perp[:, lin], para[:, lin] = _couplings(r, a[i], a[j], xi, regularize=False)
+            lin += 1
+    return perp, para
+
+
+def field_map(x_E, x_p, a, p, box, xi):
+    """E (N_E,3) complex at the probe points x_E from the dipoles p of the
+    spheres of radii a at x_p: tolerance 1e-3 (FieldMap.m:17-21), pair tables
+    on r = [0.5 : 1e-3 : r_c + 1e-2] without overlap terms for the unique
+    radii (:26-31), every probe treated as a sphere of the smallest radius in
+    the wave-space contraction and in its table column (ComputeField.m:46-51,
+    RealSpace.m:22-25, 56-58), the real-space sum over probe-particle pairs
+    with d < r_c (brute force over all pairs in place of NeighborList_Types;
+    a pair closer than the table start has no interval, as in the
+    reference), and E = -ComputeField (FieldMap.m:40)."""
+    x_E = np.asarray(x_E, dtype=np.float64).reshape(-1, 3)
+    x_p = np.asarray(x_p, dtype=np.float64).reshape(-1, 3)
+    a = np.asarray(a, dtype=np.float64).reshape(-1)
+    p = np.asarray(p, dtype=np.complex128).reshape(-1, 3)
+    box = np.asarray(box, dtype=np.float64).reshape(3)
+    tol = 1e-3
+    rc, _, ng, h, P, eta = ewald_parameters(box, xi, tol)
+    if np.any(rc > box / 2):
+        raise ValueError("Real space cutoff larger than half the box size")
+    a_uniq, a_ind = uniquetol(a)
+    r_table = np.arange(0.5, rc + 10 * tol + 1e-12, tol)
+    Ep_perp, Ep_para = real_space_table_field(r_table, a_uniq, xi)
+    col = pair_columns(a_uniq.size)
+    a_min_ind = a_ind[int(np.argmin(a))]
+    amin = float(np.min(a))
+    # wave space: spread at the particles, contract at the probes with radius min(a)
+    lin = np.arange(P ** 3)
+    c = (P - 1) // 2
+    off = np.stack((lin % P - c, (lin // P) % P - c, lin // (P * P) - c), axis=1)
+
+    def stencil(x, rad):
+        node0 = eo.round_half_away(x / h[None, :]).astype(np.int64)
+        node = np.mod(node0[:, None, :] + off[None, :, :] - 1, ng[None, None, :])
+        linnode = node[..., 0] + ng[0] * (node[..., 1] + ng[1] * node[..., 2])
+        rvec = (node0 * h)[:, None, :] + (off * h)[None, :, :] - x[:, None, :]
+        return linnode, np.stack([kernel(rvec[k], rad[k], xi, eta) for k in range(x.shape[0])])
+
+    G = int(np.prod(ng))
+    lin_p, w_p = stencil(x_p, a)
+    vals = np.einsum("nkc,nc->nk", w_p, p)
+    grid = (np.bincount(lin_p.reshape(-1), weights=np.real(vals).reshape(-1), minlength=G)
+            + 1j * np.bincount(lin_p.reshape(-1), weights=np.imag(vals).reshape(-1), minlength=G))
+    grid = grid.reshape(tuple(ng[::-1])).transpose(2, 1, 0)
+    ks = [np.fft.fftfreq(int(m), d=1.0 / int(m)) * 2 * PI / L for m, L in zip(ng, box)]
+    KX, KY, KZ = np.meshgrid(*ks, indexing="ij")
+    q2 = KX ** 2 + KY ** 2 + KZ ** 2
+    with np.errstate(divide="ignore"):
+        sc = np.where(q2 == 0, 0.0, np.exp(-(1 - eta) * q2 / (4 * xi ** 2)) / q2)
+    Et = np.fft.ifftn(np.fft.fftn(grid) * sc)
+    flat = Et.transpose(2, 1, 0).reshape(-1)
+    lin_e, w_e = stencil(x_E, np.full(x_E.shape[0], amin))
+    Eq = np.prod(h) * np.einsum("nkc,nk->nc", w_e, flat[lin_e])
+    # real space
+    Er = np.zeros((x_E.shape[0], 3), dtype=np.complex128)
+    for i in range(x_E.shape[0]):
+        rr = x_E[i][None, :] - x_p
+        rr = rr - box[None, :] * np.trunc(2.0 * rr / box[None, :])
+        d = np.sqrt(np.sum(rr * rr, axis=1))
+        keep = d < rc
+        if not np.any(keep):
+            continue
+        d, rr, pj = d[keep], rr[keep], p[keep]
+        idx = np.searchsorted(r_table, d, side="left")
+        if np.any(idx <= 0):
+            raise ValueError("a probe lies closer to a particle than the field-map table starts (0.5)")
+        cols = col[a_min_ind, a_ind[keep]]
+        lo, hi = r_table[idx - 1], r_table[idx]
+        wl, wh = (hi - d) / (hi - lo), (d - lo) / (hi - lo)
+        A = Ep_perp[idx - 1, cols] * wl + Ep_perp[idx, cols] * wh
+        B = Ep_para[idx - 1, cols] * wl + Ep_para[idx, cols] * wh
+        rhat = rr / d[:, None]
+        proj = np.sum(pj * rhat, axis=1)
+        Er[i] = np.sum(A[:, None] * (pj - proj[:, None] * rhat) + (B * proj)[:, None] * rhat, axis=0)
+    return -(Eq + Er)
