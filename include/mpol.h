@@ -240,6 +240,21 @@ int mpol_sizes_solve(mpol_ctx* ctx, const double* b_dev, const double* x0_dev, d
 int mpol_matvec_vjp(mpol_ctx* ctx, const double* p_dev, const double* lam_dev, int32_t nrhs, double* grad_x_dev,
                     double* grad_eps_dev);
 
+/* Forces F = -dU/dx (device, N x 3) of the Ewald-summed polarisation energy
+ * U = 1/2 p^T A p - b^T p at solved dipoles p (device, nrhs x N x 3; summed
+ * over the right-hand sides), computed forward: the real-space part from the
+ * pair-force couplings f1 = field_dip_force_1, f2 = field_dip_force_2 (host,
+ * ntab entries on the rvals of mpol_set_tables, as mpol_real_space_table
+ * returns them: f1 = dA/dr = (B - A)/r, f2 = d(B - A)/dr of the couplings A, B
+ * given to mpol_set_tables), interpolated per pair; the wave-space part from
+ * the position derivative of the Gaussian weights.  The reference tabulates
+ * these couplings ("CPU - Ewald test/ewald.py":798-916, self terms :928-929)
+ * but computes no forces from them; its HOOMD forces come from truncated point
+ * dipoles (dipole_forces_ext/hoomd_force.py:192-226).  Real permittivities
+ * only. */
+int mpol_ewald_forces(mpol_ctx* ctx, const double* p_dev, int32_t nrhs, const double* f1_host,
+                      const double* f2_host, double* forces_dev);
+
 /* Launch-shape tuning (no counterpart in the reference; used by the
  * autotuner): knobs[0..6] = spread/contract tile rows along y and z (ty*tz a
  * power of two, each <= 4), particles staged per chunk (32..256), y-pass and

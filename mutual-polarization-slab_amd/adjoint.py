@@ -31,9 +31,10 @@ import numpy as np
 import torch
 
 from mpol_hip import _lib
-from mpol_hip.context import Context, default_tables, require_device
+from mpol_hip.context import Context, default_force_tables, default_tables, require_device
 
-__all__ = ["PolarizationSolve", "solve_dipoles", "polarization_energy", "polarization_forces", "ewald_setup"]
+__all__ = ["PolarizationSolve", "solve_dipoles", "polarization_energy", "polarization_forces", "ewald_forces",
+           "ewald_setup"]
 
 
 def ewald_setup(box, xi, errortol):
@@ -153,3 +154,44 @@ def polarization_forces(x, eps, box, xi, errortol=1e-10, H0=(0.0, 0.0, 1.0), sla
     U = polarization_energy(xt, eps, box, xi, errortol, H0, slab_factor, ngrid, P, tables, solve_tol, rc)
     (g,) = torch.autograd.grad(U, xt)
     return (-g).cpu().numpy(), float(U.item())
+
+
+def ewald_forces(x, eps, box, xi, errortol=1e-10, H0=(0.0, 0.0, 1.0), slab_factor=0.0, ngrid=None, P=None,
+                 tables=None, force_tables=None, solve_tol=None, rc=None):
+    """(F = -dU/dx (N,3), U, p (N,3)) as numpy arrays, computed forward: the
+    dipoles solved once, then mpol_ewald_forces -- the real-space part from
+    the tabulated pair-force couplings field_dip_force_1/2 ("CPU - Ewald
+    test/ewald.py":798-916, which the reference tabulates and never uses),
+    the wave-space part from the Gaussian weights' derivative.  Equal to
+    polarization_forces (the adjoint) up to the interpolation of the tables
+    (the adjoint differentiates the interpolated couplings, this interpolates
+    their derivatives).  force_tables: (f1, f2) on the rvals of tables
+    (default: on default_tables' grid)."""
+    dev = x.device if isinstance(x, torch.Tensor) and x.is_cuda else require_device()
+    xt = _as_dev(x, dev)
+    n = xt.shape[0]
+    et = _as_dev(eps, dev).reshape(-1)
+    if et.numel() == 1:
+        et = et.expand(n).contiguous()
+    setup = _Setup(box, xi, errortol, ngrid, P, tables, slab_factor, solve_tol=solve_tol, rc=rc)
+    if force_tables is None:
+        if tables is not None:
+            raise ValueError("force_tables are needed on the rvals of custom tables")
+        force_tables = default_force_tables(setup.xi)
+    H = torch.as_tensor(np.asarray(H0, dtype=np.float64).reshape(-1, 3), device=dev)
+    b = H[:, None, :].expand(H.shape[0], n, 3).contiguous()
+    c = Context(dev)
+    try:
+        perp, para, rv = setup.tables
+        c.set_tables(perp, para, rv)
+        c.set_geometry(xt, setup.box, setup.ng, setup.P, setup.xi, setup.eta, setup.rc)
+        c.set_permittivity(et)
+        res = c.solve(b, torch.zeros_like(b), setup.tol, restart=setup.restart, maxit=setup.maxit,
+                      slab_factor=setup.slab)
+        if res.flag != 0:
+            raise RuntimeError(f"polarization solve did not converge (relres {res.max_relres:.2e})")
+        F = c.ewald_forces(res.x, force_tables[0], force_tables[1])
+        U = -0.5 * float(torch.sum(b * res.x).item())
+        return F.cpu().numpy(), U, res.x.cpu().numpy()[0]
+    finally:
+        c.close()

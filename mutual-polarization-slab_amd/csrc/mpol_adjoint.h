@@ -79,6 +79,61 @@ static __global__ void k_real_space_vjp(PairParams p, const double* __restrict__
   grad[3 * i + 2] = g2;
 }
 
+// Real-space part of the forces of the Ewald-summed polarisation energy
+// U = 1/2 p^T A p - b^T p at solved dipoles p (F = -dU/dx = -1/2 d(p^T A p)/dx),
+// from the tabulated pair-force couplings f1 = dA/dr = (B - A)/r and
+// f2 = d(B - A)/dr (field_dip_force_1/2 of "CPU - Ewald test/ewald.py":798-916,
+// interpolated on the table grid like the couplings): with u = r_ij/d,
+// a = p_i, b = p_j, the pair energy e = A (a.b) + (B - A)(a.u)(b.u) has
+//   grad_r e = f1 [(a.b) u + (b.u) a + (a.u) b - 2 (a.u)(b.u) u] + f2 (a.u)(b.u) u
+// and F_i = -sum_j grad_r e (each pair enters p^T A p twice).  Owner computes
+// over the CSR pair list; summed over the right-hand sides.
+static __global__ void k_ewald_forces_real(PairParams p, const double* __restrict__ x, const int* __restrict__ off,
+                                           const int* __restrict__ pj, const double* __restrict__ rv,
+                                           const double* __restrict__ f1t, const double* __restrict__ f2t,
+                                           const double* __restrict__ m, int nrhs, double* __restrict__ force) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p.n) return;
+  double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+  const double xi0 = x[3 * i], xi1 = x[3 * i + 1], xi2 = x[3 * i + 2];
+  for (int w = off[i]; w < off[i + 1]; ++w) {
+    const int j = pj[w];
+    double r0, r1, r2, d;
+    {
+#pragma clang fp contract(off)
+      r0 = xi0 - x[3 * j];
+      r1 = xi1 - x[3 * j + 1];
+      r2 = xi2 - x[3 * j + 2];
+      r0 = r0 - p.box[0] * trunc(2.0 * r0 / p.box[0]);
+      r1 = r1 - p.box[1] * trunc(2.0 * r1 / p.box[1]);
+      r2 = r2 - p.box[2] * trunc(2.0 * r2 / p.box[2]);
+      d = sqrt((r0 * r0 + r1 * r1) + r2 * r2);
+    }
+    int idx = table_lower_bound(rv, p.ntab, d);
+    if (idx <= 0) idx = 1;
+    if (idx >= p.ntab) idx = p.ntab - 1;
+    const double lo = rv[idx - 1], hi = rv[idx];
+    const double t = (d - lo) / (hi - lo);
+    const double F1 = f1t[idx - 1] * (1.0 - t) + f1t[idx] * t;
+    const double F2 = f2t[idx - 1] * (1.0 - t) + f2t[idx] * t;
+    const double u0 = r0 / d, u1 = r1 / d, u2 = r2 / d;
+    for (int rhs = 0; rhs < nrhs; ++rhs) {
+      const double* a = m + 3 * ((size_t)rhs * p.n + i);
+      const double* b = m + 3 * ((size_t)rhs * p.n + j);
+      const double ab = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+      const double au = a[0] * u0 + a[1] * u1 + a[2] * u2;
+      const double bu = b[0] * u0 + b[1] * u1 + b[2] * u2;
+      const double radial = F1 * (ab - 2.0 * au * bu) + F2 * au * bu;
+      g0 += radial * u0 + F1 * (bu * a[0] + au * b[0]);
+      g1 += radial * u1 + F1 * (bu * a[1] + au * b[1]);
+      g2 += radial * u2 + F1 * (bu * a[2] + au * b[2]);
+    }
+  }
+  force[3 * i] = -g0;
+  force[3 * i + 1] = -g1;
+  force[3 * i + 2] = -g2;
+}
+
 struct WaveVjpArgs {
   int nx, ny, nz;
   int P, half;
@@ -93,12 +148,12 @@ struct WaveVjpArgs {
 
 // Wave space: one workgroup per particle (sorted order).  grid holds the real
 // field grids of the 2*nrhs inputs [p_0..p_{R-1}, lam_0..lam_{R-1}] as
-// grid[(v*3 + c)][z][y][x].  Adds h^3 sum dW/dx (lam_k . Phi_p + p_k . Phi_lam)
-// to grad (caller's order).
+// grid[(v*3 + c)][z][y][x].  Adds scale * h^3 sum dW/dx (lam_k . Phi_p +
+// p_k . Phi_lam) to grad (caller's order).
 static __global__ void __launch_bounds__(256) k_wave_vjp(WaveVjpArgs a, const double* __restrict__ grid,
                                                          const double* __restrict__ m,
                                                          const double* __restrict__ lam,
-                                                         double* __restrict__ grad) {
+                                                         double* __restrict__ grad, double scale) {
   __shared__ double red[3][4];
   const int s = blockIdx.x;
   if (s >= a.n) return;
@@ -149,7 +204,7 @@ static __global__ void __launch_bounds__(256) k_wave_vjp(WaveVjpArgs a, const do
   if (threadIdx.x < 3) {
     double v = 0.0;
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v += red[threadIdx.x][w];
-    grad[3 * (size_t)k + threadIdx.x] += v * a.hvol;
+    grad[3 * (size_t)k + threadIdx.x] += scale * (v * a.hvol);
   }
 }
 

@@ -4,15 +4,12 @@
 #include "mpol_internal.h"
 #include "mpol_sizes.h"
 
-extern "C" int mpol_matvec_vjp(mpol_ctx* c, const double* m, const double* lam, int32_t nrhs, double* grad_x,
-                               double* grad_eps) {
-  if (!c) return fail(MPOL_EINVAL, "null context");
-  if (!c->have_geo) return fail(MPOL_ESTATE, "geometry not set (call mpol_set_geometry first)");
-  if (c->nsets == 0) return fail(MPOL_ESTATE, "permittivity not set (call mpol_set_permittivity first)");
-  if (c->cplx_eps) return fail(MPOL_ESTATE, "gradients need real permittivities");
-  if (!m || !lam || !grad_x || nrhs < 1) return fail(MPOL_EINVAL, "bad vjp arguments");
-  if (c->nsets != 1 && c->nsets != nrhs) return fail(MPOL_EINVAL, "permittivity sets must be 1 or nrhs");
-  HIPCHK(hipSetDevice(c->device));
+namespace {
+
+// Adds scale * d/dx of the wave-space part of sum_rhs lam^T A(x) p (the
+// Gaussian weights' position derivative at both the spread and the
+// contraction of every particle) to grad (device, N x 3, caller's order).
+int wave_vjp_add(mpol_ctx* c, const double* m, const double* lam, int nrhs, double* grad, double scale) {
   hipStream_t st = c->stream;
   const int n = c->n;
   const size_t vec = (size_t)nrhs * 3 * n;
@@ -46,17 +43,6 @@ extern "C" int mpol_matvec_vjp(mpol_ctx* c, const double* m, const double* lam, 
   } while (0)
   DISPATCH_FFT(c->ng[0], ta.plan.e, LAUNCH_C2R, 0);
 #undef LAUNCH_C2R
-  // real space, then the wave-space weights
-  PairParams pp{};
-  pp.n = n;
-  for (int d = 0; d < 3; ++d) {
-    pp.box[d] = c->box[d];
-    pp.nc[d] = c->nc[d];
-  }
-  pp.rc = c->rc;
-  pp.ntab = c->ntab;
-  hipLaunchKernelGGL(k_real_space_vjp, dim3(cdiv(n, 256)), dim3(256), 0, st, pp, c->x.p, c->rs_off.p, c->rs_j.p,
-                     c->rvals.p, c->perp.p, c->para.p, m, lam, nrhs, grad_x);
   WaveVjpArgs wa{};
   wa.nx = c->ng[0]; wa.ny = c->ng[1]; wa.nz = c->ng[2];
   wa.P = c->P; wa.half = c->half; wa.n = n; wa.nrhs = nrhs;
@@ -68,12 +54,67 @@ extern "C" int mpol_matvec_vjp(mpol_ctx* c, const double* m, const double* lam, 
   wa.hvol = c->h[0] * c->h[1] * c->h[2];
   wa.rec = c->rec.p;
   wa.base = c->base64.p;
-  hipLaunchKernelGGL(k_wave_vjp, dim3(n), dim3(256), 0, st, wa, grid, m, lam, grad_x);
+  hipLaunchKernelGGL(k_wave_vjp, dim3(n), dim3(256), 0, st, wa, grid, m, lam, grad, scale);
+  HIPCHK(hipGetLastError());
+  return MPOL_OK;
+}
+
+PairParams pair_params(const mpol_ctx* c) {
+  PairParams pp{};
+  pp.n = c->n;
+  for (int d = 0; d < 3; ++d) {
+    pp.box[d] = c->box[d];
+    pp.nc[d] = c->nc[d];
+  }
+  pp.rc = c->rc;
+  pp.ntab = c->ntab;
+  return pp;
+}
+
+}  // namespace
+
+extern "C" int mpol_matvec_vjp(mpol_ctx* c, const double* m, const double* lam, int32_t nrhs, double* grad_x,
+                               double* grad_eps) {
+  if (!c) return fail(MPOL_EINVAL, "null context");
+  if (!c->have_geo) return fail(MPOL_ESTATE, "geometry not set (call mpol_set_geometry first)");
+  if (c->nsets == 0) return fail(MPOL_ESTATE, "permittivity not set (call mpol_set_permittivity first)");
+  if (c->cplx_eps) return fail(MPOL_ESTATE, "gradients need real permittivities");
+  if (!m || !lam || !grad_x || nrhs < 1) return fail(MPOL_EINVAL, "bad vjp arguments");
+  if (c->nsets != 1 && c->nsets != nrhs) return fail(MPOL_EINVAL, "permittivity sets must be 1 or nrhs");
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = c->stream;
+  const int n = c->n;
+  // real space (writes grad_x), then the wave-space weights (adds)
+  hipLaunchKernelGGL(k_real_space_vjp, dim3(cdiv(n, 256)), dim3(256), 0, st, pair_params(c), c->x.p, c->rs_off.p,
+                     c->rs_j.p, c->rvals.p, c->perp.p, c->para.p, m, lam, nrhs, grad_x);
+  CHECK_STATUS(wave_vjp_add(c, m, lam, nrhs, grad_x, 1.0));
   if (grad_eps) {
     const int64_t total = (int64_t)c->nsets * n;
     hipLaunchKernelGGL(k_eps_vjp, dim3(cdiv(total, 256)), dim3(256), 0, st, c->eps_re.p, n, c->nsets, nrhs, m, lam,
                        grad_eps);
   }
   HIPCHK(hipGetLastError());
+  return MPOL_OK;
+}
+
+extern "C" int mpol_ewald_forces(mpol_ctx* c, const double* p, int32_t nrhs, const double* f1_host,
+                                 const double* f2_host, double* forces) {
+  if (!c) return fail(MPOL_EINVAL, "null context");
+  if (!c->have_geo) return fail(MPOL_ESTATE, "geometry not set (call mpol_set_geometry first)");
+  if (c->nsets == 0) return fail(MPOL_ESTATE, "permittivity not set (call mpol_set_permittivity first)");
+  if (c->cplx_eps) return fail(MPOL_ESTATE, "forces need real permittivities");
+  if (!p || !f1_host || !f2_host || !forces || nrhs < 1) return fail(MPOL_EINVAL, "bad force arguments");
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = c->stream;
+  const int n = c->n;
+  HIPALLOC(c->ftab1, (size_t)c->ntab);
+  HIPALLOC(c->ftab2, (size_t)c->ntab);
+  HIPCHK(hipMemcpyAsync(c->ftab1.p, f1_host, c->ntab * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->ftab2.p, f2_host, c->ntab * sizeof(double), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_ewald_forces_real, dim3(cdiv(n, 256)), dim3(256), 0, st, pair_params(c), c->x.p,
+                     c->rs_off.p, c->rs_j.p, c->rvals.p, c->ftab1.p, c->ftab2.p, p, nrhs, forces);
+  CHECK_STATUS(wave_vjp_add(c, p, p, nrhs, forces, -0.5));
+  // the host tables are read by the copies above before this call returns
+  HIPCHK(hipStreamSynchronize(st));
   return MPOL_OK;
 }

@@ -297,6 +297,7 @@ struct mpol_ctx {
   // truncated fixed-dipole forces: cell list
   DevBuf<int> f_cell, f_count, f_start, f_cursor, f_members;
   DevBuf<int> drop_keep, drop_pos, drop_err;  // drop-in pair filter (mpol_dropin.hip)
+  DevBuf<double> ftab1, ftab2;                // pair-force tables (mpol_ewald_forces)
   int* scratch = nullptr;  // this device's allocation-free scratch (mpol_compute_forces_noalloc)
 };
 

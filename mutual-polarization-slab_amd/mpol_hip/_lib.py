@@ -91,6 +91,7 @@ SIGNATURES = [
     ("mpol_sizes_solve", ctypes.c_int, [_VP, _VP, _VP, _VP, ctypes.c_int32, ctypes.POINTER(SolveOpts),
                                         ctypes.POINTER(SolveInfo)]),
     ("mpol_matvec_vjp", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int32, _VP, _VP]),
+    ("mpol_ewald_forces", ctypes.c_int, [_VP, _VP, ctypes.c_int32, _DP, _DP, _VP]),
     ("mpol_real_space_table_sizes", ctypes.c_int, [ctypes.c_double, _DP, ctypes.c_int64, _DP, ctypes.c_int32, _DP,
                                                    _DP]),
     ("mpol_real_space_table_sizes_field", ctypes.c_int, [ctypes.c_double, _DP, ctypes.c_int64, _DP,

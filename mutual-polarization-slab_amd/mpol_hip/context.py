@@ -109,6 +109,20 @@ def default_tables(xi):
     return _TABLES[key]
 
 
+_FTABLES: dict = {}
+
+
+def default_force_tables(xi):
+    """field_dip_force_1, field_dip_force_2 on the rvals of default_tables
+    (the pair-force couplings dA/dr and d(B - A)/dr, self entries 0)."""
+    key = float(xi)
+    if key not in _FTABLES:
+        r = np.arange(0.001, 10.0 + 1e-12, 0.001, dtype=np.float64)
+        tabs = real_space_table(r, xi)
+        _FTABLES[key] = (tabs[4], tabs[5])
+    return _FTABLES[key]
+
+
 @dataclass
 class SolveResult:
     x: torch.Tensor
@@ -136,6 +150,7 @@ class Context:
                                              ctypes.c_void_p(stream)))
         self.n = 0
         self.nsets = 0
+        self._ntab = 0
         self.complex = False
         self._keep = []
 
@@ -172,6 +187,7 @@ class Context:
         self._sync_stream()
         _lib.check(self._lib.mpol_set_tables(self._ctx, _lib.dptr(perp), _lib.dptr(para), _lib.dptr(rvals),
                                              rvals.size))
+        self._ntab = rvals.size
 
     def set_geometry(self, x, box, ngrid, P, xi, eta, rc):
         xt = self._dev(x).reshape(-1, 3)
@@ -408,6 +424,22 @@ class Context:
 
     def solve(self, b, x0, tol, restart=10, maxit=100, mixed=False, max_refine=8, slab_factor=1.0):
         return self._solve(self._lib.mpol_solve, 3, b, x0, tol, restart, maxit, mixed, max_refine, slab_factor)
+
+    # -- forces of the Ewald-summed polarisation energy --------------------
+    def ewald_forces(self, p, f1, f2):
+        """F = -dU/dx (N, 3) device tensor at solved dipoles p (nrhs, N, 3):
+        real space from the pair-force tables f1, f2 (field_dip_force_1/2 on
+        the rvals of set_tables), wave space from the Gaussian derivative."""
+        pt = self._as_rhs(p)
+        f1 = np.ascontiguousarray(np.asarray(f1, dtype=np.float64).reshape(-1))
+        f2 = np.ascontiguousarray(np.asarray(f2, dtype=np.float64).reshape(-1))
+        if f1.size != self._ntab or f2.size != self._ntab:
+            raise ValueError("force tables must have one entry per rvals entry of set_tables")
+        out = torch.empty((self.n, 3), dtype=torch.float64, device=self.device)
+        self._sync_stream()
+        _lib.check(self._lib.mpol_ewald_forces(self._ctx, ctypes.c_void_p(pt.data_ptr()), pt.shape[0], _lib.dptr(f1),
+                                               _lib.dptr(f2), ctypes.c_void_p(out.data_ptr())))
+        return out
 
     # -- fixed-dipole forces ----------------------------------------------
     def compute_forces(self, pos, dip, box, r_cut, prefactor=1.0, periodic_z=True, use_cell_list=True):

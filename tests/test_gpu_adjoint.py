@@ -147,3 +147,45 @@ def test_forces_of_separated_spheres_match_frozen_dipole_kernel(gpu):
     Fk, _ = dipole_forces_ext.compute_forces(x - 20.0, p, box, 19.0, 1.0 / (4 * np.pi))
     assert np.max(np.abs(F - Fk)) < 2e-3 * np.max(np.abs(Fk)), (F, Fk)
     assert U < 0
+
+
+def test_forward_ewald_forces_match_adjoint_forces(gpu):
+    """mpol_ewald_forces (real space from the tabulated pair-force couplings
+    field_dip_force_1/2, wave space from the Gaussian derivative) against the
+    adjoint forces, which differentiate the interpolated couplings instead.
+    The slope of a linear interpolant is the derivative at the middle of its
+    interval, so the two differ by A''(d - mid), first order in the table
+    spacing: about 7e-4 of the largest force on the default 1e-3 grid, and a
+    hundred times less on a 1e-5 grid (both converge to the derivative of the
+    exact couplings)."""
+    import adjoint
+    x, box = rsa(40, 0.15, 11)
+    xi, tol = 1.0, 1e-10
+    rng = np.random.default_rng(3)
+    eps = rng.uniform(3.0, 25.0, 40)
+    Fa, Ua = adjoint.polarization_forces(x, eps, box, xi, tol)
+    Ff, Uf, p = adjoint.ewald_forces(x, eps, box, xi, tol)
+    scale = np.max(np.abs(Fa))
+    assert abs(Uf - Ua) <= 1e-12 * abs(Ua)
+    coarse = np.max(np.abs(Ff - Fa)) / scale
+    assert coarse < 2e-3, coarse
+    # fine tables for both: couplings and force couplings on a 1e-5 grid
+    r = np.arange(1e-5, 7.0, 1e-5)
+    t = eo.real_space_table(r, xi)
+    fine = (t[2], t[3], np.concatenate(([0.0], r)))
+    Fa2, _ = adjoint.polarization_forces(x, eps, box, xi, tol, tables=fine)
+    Ff2, _, _ = adjoint.ewald_forces(x, eps, box, xi, tol, tables=fine, force_tables=(t[4], t[5]))
+    fine_err = np.max(np.abs(Ff2 - Fa2)) / scale
+    assert fine_err < 2e-5 and fine_err < 0.05 * coarse, (fine_err, coarse)
+    assert np.max(np.abs(Ff2 - Fa)) < 2e-3 * scale
+
+
+def test_forward_ewald_forces_of_separated_spheres_match_frozen_dipole_kernel(gpu):
+    import adjoint
+    import dipole_forces_ext
+    box = np.array([40.0, 40.0, 40.0])
+    x = np.array([[20.0, 20.0, 17.0], [20.3, 19.6, 20.2], [22.9, 20.4, 21.1]])
+    F, U, p = adjoint.ewald_forces(x, np.full(3, 12.0), box, 0.5, 1e-8)
+    Fk, _ = dipole_forces_ext.compute_forces(x - 20.0, p, box, 19.0, 1.0 / (4 * np.pi))
+    assert np.max(np.abs(F - Fk)) < 2e-3 * np.max(np.abs(Fk)), (F, Fk)
+    assert U < 0

@@ -165,3 +165,25 @@ def test_real_part_equals_hermitian_scale():
     Hk2 = eo.contract_fast(np.real(Ht), geo.Ngrid, lin, wc)
     assert np.max(np.abs(np.real(Hk2) - np.real(Hk))) < 1e-13 * np.max(np.abs(Hk))
     assert np.max(np.abs(np.imag(Hk))) > 1e-12 * np.max(np.abs(Hk))  # the part that is dropped
+
+
+@pytest.mark.parametrize("xi", [0.8, 1.3])
+def test_force_tables_are_the_coupling_derivatives(xi):
+    """field_dip_force_1/2 ("CPU - Ewald test/ewald.py":798-916), tabulated
+    by the reference but used nowhere there, are the r-derivatives of the
+    pair couplings: f1 = dA/dr = (B - A)/r (A and B derive from one scalar
+    potential) and f2 = d(B - A)/dr, inside and outside the overlap region --
+    so the pair force they give is the gradient of the real-space pair energy
+    A (p_i.p_j) + (B - A)(p_i.u)(p_j.u)."""
+    r = np.array([0.5, 1.0, 1.9, 2.1, 2.5, 3.0, 4.0])
+    h = 1e-6
+    t = eo.real_space_table(r, xi)
+    tp = eo.real_space_table(r + h, xi)
+    tm = eo.real_space_table(r - h, xi)
+    A, B, f1, f2 = (t[k][1:] for k in (2, 3, 4, 5))
+    dA = (tp[2][1:] - tm[2][1:]) / (2 * h)
+    dB = (tp[3][1:] - tm[3][1:]) / (2 * h)
+    scale = np.max(np.abs(f1))
+    assert np.max(np.abs(f1 - dA)) < 1e-7 * scale
+    assert np.max(np.abs(f1 - (B - A) / r)) < 1e-12 * scale
+    assert np.max(np.abs(f2 - (dB - dA))) < 1e-7 * np.max(np.abs(f2))
