@@ -29,18 +29,21 @@ from . import capacitance
 from ._forces import compute_forces
 
 
+# HOOMD's local force arrays: the energy attribute is `potential_energy`
+# (`energy` on older builds); torques and virials of these forces are zero.
+_ENERGY_ATTRS = ("potential_energy", "energy")
+_ZEROED_ATTRS = ("torque", "virial")
+
+
 def _write_force_energy_arrays(arrays, f: np.ndarray, e: np.ndarray) -> None:
-    """Force and per-particle energy into HOOMD's local force arrays
-    (energy attribute `potential_energy`, or `energy` on older builds)."""
+    """Force and per-particle energy into HOOMD's local force arrays."""
     arrays.force[:] = f
-    if hasattr(arrays, "potential_energy"):
-        arrays.potential_energy[:] = e
-    elif hasattr(arrays, "energy"):
-        arrays.energy[:] = e
-    if hasattr(arrays, "torque"):
-        arrays.torque[:] = 0.0
-    if hasattr(arrays, "virial"):
-        arrays.virial[:] = 0.0
+    target = next((getattr(arrays, a) for a in _ENERGY_ATTRS if hasattr(arrays, a)), None)
+    if target is not None:
+        target[:] = e
+    for a in _ZEROED_ATTRS:
+        if hasattr(arrays, a):
+            getattr(arrays, a)[:] = 0.0
 
 
 def _evaluate(pos, dip, boxL, r_cut, prefactor, periodic_z, use_cell_list):
