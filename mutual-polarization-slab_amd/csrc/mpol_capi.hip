@@ -105,19 +105,6 @@ int dots(mpol_ctx* c, const double* A, size_t stride_i, int cnt, const double* w
   return MPOL_OK;
 }
 
-// Copy the per-rhs solver state and relative residuals to the host (through
-// page-locked buffers) and wait for them.
-int read_state(mpol_ctx* c, int nrhs, std::vector<int>& state, std::vector<double>& rel) {
-  if (c->hstate.reserve((size_t)nrhs) != hipSuccess || c->hrel.reserve((size_t)nrhs) != hipSuccess)
-    return fail(MPOL_ENOMEM, "page-locked host allocation failed");
-  HIPCHK(hipMemcpyAsync(c->hstate.p, c->gstate.p, nrhs * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipMemcpyAsync(c->hrel.p, c->grel.p, nrhs * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  std::copy(c->hstate.p, c->hstate.p + nrhs, state.begin());
-  std::copy(c->hrel.p, c->hrel.p + nrhs, rel.begin());
-  return MPOL_OK;
-}
-
 // Copy cnt doubles from the device to host memory (through a page-locked
 // buffer) and wait for them.
 int read_doubles(mpol_ctx* c, const double* dev, int cnt, double* host) {
@@ -135,13 +122,22 @@ struct GateScope {
   ~GateScope() { c->gate = nullptr; }
 };
 
-// Page-locked double slots for the per-step state and residual estimates and
-// their events (the Arnoldi loop reads step j - 1 while step j runs).
+// Slot layout of the page-locked state the GMRES kernels post: slots 0 and 1
+// alternate between Arnoldi steps (the host reads step j - 1 while step j
+// runs), slot 2 holds the cycle start.
+constexpr int kCycleSlot = 2;
+
 int ensure_step_reads(mpol_ctx* c, int nrhs) {
-  if (c->hstep.reserve((size_t)2 * nrhs) != hipSuccess || c->hsteprel.reserve((size_t)2 * nrhs) != hipSuccess)
+  const int* had = c->hstep.p;
+  const double* hadrel = c->hsteprel.p;
+  if (c->hstep.reserve((size_t)3 * nrhs) != hipSuccess || c->hsteprel.reserve((size_t)3 * nrhs) != hipSuccess)
     return fail(MPOL_ENOMEM, "page-locked host allocation failed");
-  for (int k = 0; k < 2; ++k)
-    if (!c->step_ev[k]) HIPCHK(hipEventCreateWithFlags(&c->step_ev[k], hipEventDisableTiming));
+  if (c->hstep.p != had || !c->dstep)
+    HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->dstep), c->hstep.p, 0));
+  if (c->hsteprel.p != hadrel || !c->dsteprel)
+    HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->dsteprel), c->hsteprel.p, 0));
+  for (hipEvent_t& e : c->step_ev)
+    if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   return MPOL_OK;
 }
 
@@ -188,20 +184,24 @@ int gmres_core(mpol_ctx* c, int kind, const double* b, double* x, int nrhs, doub
   const dim3 vgrid(cdiv(n3, 256), nrhs);
   const int sgrid = cdiv(nrhs, 256);
 
-  // ||b||
+  CHECK_STATUS(ensure_step_reads(c, nrhs));
+  // ||b|| and the first states stay on the device
   dots(c, b, 0, 1, b, n3, nrhs, c->grn2.p);
-  std::vector<double> bn2(nrhs);
-  CHECK_STATUS(read_doubles(c, c->grn2.p, nrhs, bn2.data()));
-  std::vector<double> bn(nrhs);
-  std::vector<int> state(nrhs);
-  for (int r = 0; r < nrhs; ++r) {
-    bn[r] = std::sqrt(bn2[r]);
-    state[r] = bn[r] > 0.0 ? kWait : kDone;
-  }
-  HIPCHK(hipMemcpyAsync(c->gbn.p, bn.data(), nrhs * sizeof(double), hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(c->gstate.p, state.data(), nrhs * sizeof(int), hipMemcpyHostToDevice, st));
-
+  hipLaunchKernelGGL(k_norms_init, dim3(sgrid), dim3(256), 0, st, nrhs, c->grn2.p, c->gbn.p, c->gstate.p);
+  std::vector<int> state(nrhs, kWait);
   std::vector<double> rel(nrhs, 0.0);
+  // wait for a slot the kernels posted and take its states and residuals
+  auto take_slot = [&](int slot) -> int {
+    HIPCHK(hipEventSynchronize(c->step_ev[slot]));
+    std::copy(c->hstep.p + (size_t)slot * nrhs, c->hstep.p + (size_t)(slot + 1) * nrhs, state.begin());
+    std::copy(c->hsteprel.p + (size_t)slot * nrhs, c->hsteprel.p + (size_t)(slot + 1) * nrhs, rel.begin());
+    return MPOL_OK;
+  };
+  // a cycle that ended on its residual estimate usually ends the solve at
+  // the next cycle start, so that start is read before anything else is
+  // enqueued; any other cycle start is read while the first Arnoldi step of
+  // the new cycle runs
+  bool likely_done = false;
   bool all_done = false;
   for (int outer = 0; outer <= maxit; ++outer) {
     // true residual at the start of each cycle (and once after the last)
@@ -215,25 +215,28 @@ int gmres_core(mpol_ctx* c, int kind, const double* b, double* x, int nrhs, doub
     }
     dots(c, c->gR.p, 0, 1, c->gR.p, n3, nrhs, c->grn2.p);
     hipLaunchKernelGGL(k_cycle_start, dim3(sgrid), dim3(256), 0, st, nrhs, restart, c->grn2.p, c->gbn.p, tol,
-                       c->gstate.p, c->gg.p, c->gbeta.p, c->grel.p);
-    CHECK_STATUS(read_state(c, nrhs, state, rel));
-    all_done = std::all_of(state.begin(), state.end(), [](int s) { return s == kDone; });
-    if (all_done || outer == maxit) break;
+                       c->gstate.p, c->gg.p, c->gbeta.p, c->grel.p, c->dstep + (size_t)kCycleSlot * nrhs,
+                       c->dsteprel + (size_t)kCycleSlot * nrhs);
+    HIPCHK(hipEventRecord(c->step_ev[kCycleSlot], st));
+    bool cycle_read = false;
+    if (likely_done || outer == maxit) {
+      CHECK_STATUS(take_slot(kCycleSlot));
+      cycle_read = true;
+      all_done = std::all_of(state.begin(), state.end(), [](int s) { return s == kDone; });
+      if (all_done || outer == maxit) break;
+    }
     hipLaunchKernelGGL(k_scale_into, vgrid, dim3(256), 0, st, c->gR.p, c->gbeta.p, c->gV.p, n3, c->gstate.p);
     // The host stays one Arnoldi step ahead of the state it reads: step j is
-    // enqueued before the state after step j - 1 is waited for.  Every
-    // kernel of a step is gated per system (kIter), so a step enqueued for a
-    // system whose cycle had just ended does nothing for it; the x update
-    // acts on kSolve systems only and moves them to kWait, so enqueuing it
-    // again is harmless.
-    CHECK_STATUS(ensure_step_reads(c, nrhs));
+    // enqueued before the state after step j - 1 (or the cycle start, for
+    // j = 0) is waited for.  Every kernel of a step is gated per system
+    // (kIter), so a step enqueued for a system whose cycle had just ended
+    // does nothing for it; the x update acts on kSolve systems only and
+    // moves them to kWait, so enqueuing it again is harmless.
     bool stop = false;
+    bool updated = false;  // the x update of this cycle's last step is enqueued
     auto take_state = [&](int jj) -> int {
-      HIPCHK(hipEventSynchronize(c->step_ev[jj & 1]));
-      std::copy(c->hstep.p + (size_t)(jj & 1) * nrhs, c->hstep.p + (size_t)(jj & 1) * nrhs + nrhs, state.begin());
-      std::copy(c->hsteprel.p + (size_t)(jj & 1) * nrhs, c->hsteprel.p + (size_t)(jj & 1) * nrhs + nrhs,
-                rel.begin());
-      if (std::any_of(state.begin(), state.end(), [](int s) { return s == kSolve; })) {
+      CHECK_STATUS(take_slot(jj & 1));
+      if (!updated && std::any_of(state.begin(), state.end(), [](int s) { return s == kSolve; })) {
         // a cycle ended for some right-hand sides: x += V y, then they wait for the next cycle
         hipLaunchKernelGGL(k_update_x, vgrid, dim3(256), 0, st, c->gV.p, sv, c->gy.p, c->gkdim.p, c->gstate.p,
                            restart, x, n3);
@@ -242,7 +245,7 @@ int gmres_core(mpol_ctx* c, int kind, const double* b, double* x, int nrhs, doub
       stop = std::none_of(state.begin(), state.end(), [](int s) { return s == kIter; });
       return MPOL_OK;
     };
-    int pending = -1;  // step whose state copy is in flight
+    int pending = -1;  // step whose state is in flight
     for (int j = 0; j < restart; ++j) {
       double* vj = c->gV.p + (size_t)j * sv;
       {
@@ -268,13 +271,18 @@ int gmres_core(mpol_ctx* c, int kind, const double* b, double* x, int nrhs, doub
                          c->grn2.p, n3, c->gstate.p);
       hipLaunchKernelGGL(k_arnoldi_step, dim3(sgrid), dim3(256), 0, st, nrhs, restart, j, c->gh1.p, c->gh2.p,
                          c->grn2.p, c->gH.p, c->gcs.p, c->gsn.p, c->gg.p, c->gbn.p, tol, c->gstate.p, c->gkdim.p,
-                         c->grel.p, c->gy.p);
-      HIPCHK(hipMemcpyAsync(c->hstep.p + (size_t)(j & 1) * nrhs, c->gstate.p, nrhs * sizeof(int),
-                            hipMemcpyDeviceToHost, st));
-      HIPCHK(hipMemcpyAsync(c->hsteprel.p + (size_t)(j & 1) * nrhs, c->grel.p, nrhs * sizeof(double),
-                            hipMemcpyDeviceToHost, st));
+                         c->grel.p, c->gy.p, c->dstep + (size_t)(j & 1) * nrhs, c->dsteprel + (size_t)(j & 1) * nrhs);
       HIPCHK(hipEventRecord(c->step_ev[j & 1], st));
-      if (pending >= 0) {
+      if (!cycle_read) {
+        // the cycle start, read while step 0 runs
+        CHECK_STATUS(take_slot(kCycleSlot));
+        cycle_read = true;
+        all_done = std::all_of(state.begin(), state.end(), [](int s) { return s == kDone; });
+        if (all_done) {
+          --*nmv;  // step 0 had no iterating system
+          break;
+        }
+      } else if (pending >= 0) {
         CHECK_STATUS(take_state(pending));
         if (stop) {
           --*nmv;  // step j had no iterating system left
@@ -284,14 +292,21 @@ int gmres_core(mpol_ctx* c, int kind, const double* b, double* x, int nrhs, doub
       }
       pending = j;
     }
-    if (pending >= 0) CHECK_STATUS(take_state(pending));
-    if (inner) {
-      // every right-hand side whose estimated residual met tol is done
-      bool est_done = true;
-      for (int r = 0; r < nrhs; ++r)
-        if (state[r] != kDone && !(rel[r] <= tol)) est_done = false;
-      if (est_done) break;
+    if (all_done) break;
+    if (pending == restart - 1) {
+      // the last step ends the cycle of every system still iterating: the
+      // x update goes ahead of the read
+      hipLaunchKernelGGL(k_update_x, vgrid, dim3(256), 0, st, c->gV.p, sv, c->gy.p, c->gkdim.p, c->gstate.p,
+                         restart, x, n3);
+      hipLaunchKernelGGL(k_solve_to_wait, dim3(sgrid), dim3(256), 0, st, nrhs, c->gstate.p);
+      updated = true;
     }
+    if (pending >= 0) CHECK_STATUS(take_state(pending));
+    // the cycle ended on its estimate for every system still iterating
+    likely_done = true;
+    for (int r = 0; r < nrhs; ++r)
+      if (state[r] != kDone && !(rel[r] <= tol)) likely_done = false;
+    if (inner && likely_done) break;  // the caller's FP64 residual decides
   }
   HIPCHK(hipGetLastError());
   int conv = 0;
@@ -752,11 +767,12 @@ int solve_impl(mpol_ctx* c, int kind, const double* b, const double* x0, double*
     HIPALLOC(c->gx, (size_t)tot);
     HIPALLOC(c->gR, (size_t)tot);
     HIPALLOC(c->gAx, (size_t)tot);
-    HIPALLOC(c->grn2, (size_t)nrhs);
+    HIPALLOC(c->gnorm, (size_t)2 * nrhs);
     HIPALLOC(c->gpart, (size_t)nrhs * (o->restart + 1) * kDotBlocks);
-    std::vector<double> bn2(nrhs), rn2(nrhs);
-    dots(c, b, 0, 1, b, n3, nrhs, c->grn2.p);
-    CHECK_STATUS(read_doubles(c, c->grn2.p, nrhs, bn2.data()));
+    // ||b||^2 (gnorm[0, nrhs)), read together with the first residual's
+    std::vector<double> bn2(nrhs), rn2(nrhs), both(2 * (size_t)nrhs);
+    dots(c, b, 0, 1, b, n3, nrhs, c->gnorm.p);
+    bool have_bn2 = false;
     int64_t nmv64 = 0, nmv32 = 0;
     int ref = 0;
     // The first residual (from the caller's guess) only seeds the first
@@ -769,8 +785,15 @@ int solve_impl(mpol_ctx* c, int kind, const double* b, const double* x0, double*
       if (fp32_residual) ++nmv32;
       else ++nmv64;
       hipLaunchKernelGGL(k_residual, dim3(cdiv(tot, 256)), dim3(256), 0, st, b, c->gAx.p, c->gd.p, tot);
-      dots(c, c->gd.p, 0, 1, c->gd.p, n3, nrhs, c->grn2.p);
-      CHECK_STATUS(read_doubles(c, c->grn2.p, nrhs, rn2.data()));
+      dots(c, c->gd.p, 0, 1, c->gd.p, n3, nrhs, c->gnorm.p + nrhs);
+      if (!have_bn2) {
+        CHECK_STATUS(read_doubles(c, c->gnorm.p, 2 * nrhs, both.data()));
+        std::copy(both.begin(), both.begin() + nrhs, bn2.begin());
+        std::copy(both.begin() + nrhs, both.end(), rn2.begin());
+        have_bn2 = true;
+      } else {
+        CHECK_STATUS(read_doubles(c, c->gnorm.p + nrhs, nrhs, rn2.data()));
+      }
       double ratio = 1e300;
       conv = 0;
       for (int r = 0; r < nrhs; ++r) {
@@ -792,9 +815,8 @@ int solve_impl(mpol_ctx* c, int kind, const double* b, const double* x0, double*
       // far its own residual estimate can usefully fall
       double tin = 0.3 * o->tol * ratio;
       tin = std::min(0.1, std::max(tin, kInnerFloor));
-      // residual -> rhs of the correction solve (copy into gx), correction into gd
-      hipLaunchKernelGGL(k_copy, dim3(cdiv(tot, 256)), dim3(256), 0, st, c->gd.p, c->gx.p, tot);
-      HIPCHK(hipMemsetAsync(c->gd.p, 0, tot * sizeof(double), st));
+      // the residual becomes the correction solve's rhs (gx); the correction starts at zero in gd
+      hipLaunchKernelGGL(k_take, dim3(cdiv(tot, 256)), dim3(256), 0, st, c->gd.p, c->gx.p, tot);
       int64_t nmv = 0;
       CHECK_STATUS(gmres_core(c, kind, c->gx.p, c->gd.p, nrhs, tin, std::max(o->restart, kInnerRestart),
                               std::min(o->maxit, kInnerCycles), MPOL_FP32, slab, &nmv, nullptr, nullptr, true));

@@ -131,26 +131,54 @@ static __global__ void k_axpy_all(double a, const double* __restrict__ x, double
   if (k < total) y[k] += a * x[k];
 }
 
+// to = from, then from = 0 (the residual becomes the right-hand side of a
+// correction solve whose correction starts at zero in its place).
+static __global__ void k_take(double* __restrict__ from, double* __restrict__ to, int64_t total) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= total) return;
+  to[k] = from[k];
+  from[k] = 0.0;
+}
+
+// ||b|| from ||b||^2, and every system's first state: kWait, or kDone for b = 0.
+static __global__ void k_norms_init(int nrhs, const double* __restrict__ bnorm2, double* __restrict__ bnorm,
+                                    int* __restrict__ state) {
+  const int rhs = blockIdx.x * blockDim.x + threadIdx.x;
+  if (rhs >= nrhs) return;
+  const double bn = sqrt(bnorm2[rhs]);
+  bnorm[rhs] = bn;
+  state[rhs] = bn > 0.0 ? kWait : kDone;
+}
+
 // Start of a restart cycle: rnorm2[rhs] = ||b - A x||^2.
 // kWait/kIter -> kDone if converged, else kIter with g = [beta, 0, ...].
+// The new state and relative residual also go to post_state / post_rel
+// (page-locked host memory the host reads after an event).
 static __global__ void k_cycle_start(int nrhs, int restart, const double* __restrict__ rnorm2,
-                              const double* __restrict__ bnorm, double tol, int* __restrict__ state,
-                              double* __restrict__ g, double* __restrict__ beta_out, double* __restrict__ relres) {
+                                     const double* __restrict__ bnorm, double tol, int* __restrict__ state,
+                                     double* __restrict__ g, double* __restrict__ beta_out,
+                                     double* __restrict__ relres, int* __restrict__ post_state,
+                                     double* __restrict__ post_rel) {
   const int rhs = blockIdx.x * blockDim.x + threadIdx.x;
   if (rhs >= nrhs) return;
   const double beta = sqrt(rnorm2[rhs]);
   const double rel = bnorm[rhs] > 0.0 ? beta / bnorm[rhs] : 0.0;
   relres[rhs] = rel;
   beta_out[rhs] = beta;
-  if (state[rhs] == kDone) return;
-  if (rel <= tol || beta == 0.0) {
-    state[rhs] = kDone;
-    return;
+  post_rel[rhs] = rel;
+  int s = state[rhs];
+  if (s != kDone) {
+    if (rel <= tol || beta == 0.0) {
+      s = kDone;
+    } else {
+      s = kIter;
+      double* gr = g + (size_t)rhs * (restart + 1);
+      gr[0] = beta;
+      for (int i = 1; i <= restart; ++i) gr[i] = 0.0;
+    }
+    state[rhs] = s;
   }
-  state[rhs] = kIter;
-  double* gr = g + (size_t)rhs * (restart + 1);
-  gr[0] = beta;
-  for (int i = 1; i <= restart; ++i) gr[i] = 0.0;
+  post_state[rhs] = s;
 }
 
 // V_0[rhs] = r[rhs] / beta[rhs] for rhs in kIter.
@@ -177,15 +205,23 @@ static __global__ void k_normalize_next(const double* __restrict__ w, double* __
 // column j of H from the two Gram-Schmidt passes (h1 + h2) and ||w||, the
 // Givens rotations on that column, the decision whether the cycle ends
 // (estimated relative residual |g_{j+1}| / ||b|| in relest), and for a cycle
-// that ends, the back substitution of the rotated H y = g.
+// that ends, the back substitution of the rotated H y = g.  Every system's
+// state and residual estimate after the step also go to post_state /
+// post_rel (page-locked host memory the host reads after an event).
 static __global__ void k_arnoldi_step(int nrhs, int restart, int j, const double* __restrict__ h1,
                                       const double* __restrict__ h2, const double* __restrict__ hnorm2,
                                       double* __restrict__ H, double* __restrict__ cs, double* __restrict__ sn,
                                       double* __restrict__ g, const double* __restrict__ bnorm, double tol,
                                       int* __restrict__ state, int* __restrict__ kdim, double* __restrict__ relest,
-                                      double* __restrict__ y) {
+                                      double* __restrict__ y, int* __restrict__ post_state,
+                                      double* __restrict__ post_rel) {
   const int rhs = blockIdx.x * blockDim.x + threadIdx.x;
-  if (rhs >= nrhs || state[rhs] != kIter) return;
+  if (rhs >= nrhs) return;
+  if (state[rhs] != kIter) {
+    post_state[rhs] = state[rhs];
+    post_rel[rhs] = relest[rhs];
+    return;
+  }
   double* Hr = H + (size_t)rhs * (restart + 1) * restart;
   double* c = cs + (size_t)rhs * restart;
   double* s = sn + (size_t)rhs * restart;
@@ -214,8 +250,13 @@ static __global__ void k_arnoldi_step(int nrhs, int restart, int j, const double
   gr[j + 1] = -sj * gj;
   const double rel = fabs(gr[j + 1]) / bnorm[rhs];
   relest[rhs] = rel;
-  if (!(rel <= tol || j == restart - 1 || breakdown)) return;
+  post_rel[rhs] = rel;
+  if (!(rel <= tol || j == restart - 1 || breakdown)) {
+    post_state[rhs] = kIter;
+    return;
+  }
   state[rhs] = kSolve;
+  post_state[rhs] = kSolve;
   const int k = j + 1;
   kdim[rhs] = k;
   double* yr = y + (size_t)rhs * restart;

@@ -284,16 +284,21 @@ struct mpol_ctx {
   // per-tile candidate images of the matrix-core spread/contract ([0] FP32, [1] FP64)
   TileImage img[2];
   // gmres
-  DevBuf<double> gV, gW, gR, gAx, gH, gcs, gsn, gg, gy, gh1, gh2, gpart, gbn, gbeta, grel, grn2, gx, gd;
+  DevBuf<double> gV, gW, gR, gAx, gH, gcs, gsn, gg, gy, gh1, gh2, gpart, gbn, gbeta, grel, grn2, gx, gd, gnorm;
   DevBuf<int> gstate, gkdim;
   // while set (inside an Arnoldi step), the per-system GMRES state: operator
   // work of systems not in kIter is skipped
   const int* gate = nullptr;
-  HostBuf<int> hstate;  // solver state read back each Arnoldi step
-  HostBuf<double> hrel, hscr;
-  HostBuf<int> hstep;          // two slots of per-step state (read one step behind)
+  HostBuf<double> hscr;  // page-locked staging of small reads
+  // Three slots of per-system GMRES state and residual estimates that the
+  // kernels write into page-locked host memory (two alternating Arnoldi-step
+  // slots, read one step behind, and the cycle-start slot), their device
+  // addresses and the events the host waits on before reading a slot.
+  HostBuf<int> hstep;
   HostBuf<double> hsteprel;
-  hipEvent_t step_ev[2] = {nullptr, nullptr};
+  int* dstep = nullptr;
+  double* dsteprel = nullptr;
+  hipEvent_t step_ev[3] = {nullptr, nullptr, nullptr};
   // truncated fixed-dipole forces: cell list
   DevBuf<int> f_cell, f_count, f_start, f_cursor, f_members;
   DevBuf<int> drop_keep, drop_pos, drop_err;  // drop-in pair filter (mpol_dropin.hip)
