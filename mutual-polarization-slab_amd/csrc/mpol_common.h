@@ -37,7 +37,7 @@ __device__ __forceinline__ cplx<T> cmul_pi(cplx<T> a) { return cmk(-a.y, a.x); }
 // Stockham stores and the row-parallel spread/contract accesses bank-conflict
 // free.
 __device__ __forceinline__ int pidx(int a) { return a + (a >> 4); }
-__host__ __device__ inline int padded_len(int n) { return (n + (n >> 4)) | 1; }
+__host__ __device__ constexpr int padded_len(int n) { return (n + (n >> 4)) | 1; }
 
 // Sum over groups of R consecutive lanes (R a power of two <= 16, groups
 // inside one 16-lane DPP row); every lane of a group gets the group's sum.

@@ -229,41 +229,71 @@ __device__ __forceinline__ long long tile_line_row(const TileArgs& a, int l, int
   return ((long long)(rhs * nc_comp + c) * a.nz + z) * a.ny + y;
 }
 
+// Candidate entries past a chunk's end that the stage fills with zeros: the
+// spread's MFMA steps take four candidates at a time and read up to three
+// entries past the last one without a range check.
+constexpr int kStagePad = 4;
+
+// Moments per staged candidate: NC, zero-padded to the components of the
+// NB = ceil(NC*R/16) column blocks, so that a column past the last line
+// reads a zero moment.
+__host__ __device__ constexpr int mfma_ncp(int nc, int rows) { return ((nc * rows + 15) / 16) * 16 / rows; }
+
 // LDS layout of one staged chunk (after the twiddles and the tile lines).
 template <typename T>
 struct MfmaStage {
-  T* fw;     // [ch][P + rows]
-  T* m;      // [ch][NC] (spread)
-  int* gx;   // [ch]
-  int* idx;  // [ch] (spare)
+  T* fw;     // [ch + kStagePad][P + rows]
+  T* m;      // [ch + kStagePad][ncp] (spread)
+  int* gx;   // [ch + kStagePad]
 };
 
-template <typename T, int NC>
-__device__ inline MfmaStage<T> mfma_stage_layout(unsigned char* base, int ch, int PW) {
+template <typename T>
+__device__ inline MfmaStage<T> mfma_stage_layout(unsigned char* base, int ch, int PW, int ncp) {
   MfmaStage<T> st;
   st.fw = reinterpret_cast<T*>(base);
-  st.m = st.fw + (size_t)ch * PW;
-  st.gx = reinterpret_cast<int*>(st.m + (size_t)ch * NC);
-  st.idx = st.gx + ch;
+  st.m = st.fw + (size_t)(ch + kStagePad) * PW;
+  st.gx = reinterpret_cast<int*>(st.m + (size_t)(ch + kStagePad) * ncp);
   return st;
 }
 
 // LDS bytes of the MFMA tile kernels: twiddles, tile lines, staged chunk.
 template <int NC>
 inline size_t mfma_tile_lds(int nx, size_t csize, int P, int ncl, int lstride, int ch, int rows) {
-  return ((size_t)nx + (size_t)ncl * lstride) * csize + (size_t)ch * ((P + rows + NC) * (csize / 2) + 2 * sizeof(int));
+  return ((size_t)nx + (size_t)ncl * lstride) * csize +
+         (size_t)(ch + kStagePad) * ((P + rows + mfma_ncp(NC, rows)) * (csize / 2) + sizeof(int));
 }
 
 // Copy chunk [q, q + nc) of the tile image into LDS; the spread also gathers
-// the moments of right-hand side rhs (WITH_M).
-template <typename T, bool WITH_M, int NC>
+// the moments of right-hand side rhs and zero-fills kStagePad entries past the
+// chunk (WITH_M).
+template <typename T, bool WITH_M, int NC, int NCP>
 __device__ inline void mfma_stage_chunk(const TileArgs& a, const ImgArgs& im, const MfmaStage<T>& st, size_t q,
                                         int nc, int rhs) {
   const int PW = a.P + a.rows;
   const T* src = reinterpret_cast<const T*>(im.fw) + q * PW;
   const int tot = nc * PW, nt = blockDim.x, t = threadIdx.x;
-  // batches of 8 loads in flight per thread before the LDS writes
-  for (int k0 = 0; k0 < tot; k0 += 8 * nt) {
+  // every load of the chunk issued before the first LDS write: the image
+  // rows (up to kBatch per thread in one batch), this thread's slot record,
+  // then the moments it points to
+  constexpr int kBatch = 16;
+  int2 g = make_int2(0, 0);
+  if (t < nc) g = im.gs[q + t];
+  int k0 = 0;
+  if (tot <= kBatch * nt) {
+    T v[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int k = u * nt + t;
+      v[u] = k < tot ? src[k] : (T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int k = u * nt + t;
+      if (k < tot) st.fw[k] = v[u];
+    }
+    k0 = tot;
+  }
+  for (; k0 < tot; k0 += 8 * nt) {
     T v[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
@@ -277,13 +307,18 @@ __device__ inline void mfma_stage_chunk(const TileArgs& a, const ImgArgs& im, co
     }
   }
   for (int u = t; u < nc; u += nt) {
-    const int2 g = im.gs[q + u];
+    if (u != t) g = im.gs[q + u];
     st.gx[u] = g.x;
     if (WITH_M) {
       const double* mp = a.ms + NC * ((size_t)rhs * a.n + g.y);
 #pragma unroll
-      for (int c = 0; c < NC; ++c) st.m[NC * u + c] = (T)mp[c];
+      for (int c = 0; c < NCP; ++c) st.m[NCP * u + c] = c < NC ? (T)mp[c] : (T)0;
     }
+  }
+  if (WITH_M) {
+    for (int k = t; k < kStagePad * PW; k += nt) st.fw[tot + k] = (T)0;
+    for (int k = t; k < kStagePad * NCP; k += nt) st.m[nc * NCP + k] = (T)0;
+    if (t < kStagePad) st.gx[nc + t] = 0;
   }
 }
 
@@ -291,42 +326,80 @@ __device__ inline void mfma_stage_chunk(const TileArgs& a, const ImgArgs& im, co
 // block holds node x0 + l % 16 (A rows) and columns (c, row) = nb*16 + l % 16
 // of the tile lines (column index = line index c*R + row).  R (tile rows) is
 // a template parameter so that the column blocks, the row of a column and
-// the lane reductions are compile-time.  Needs nx % 16 == 0, P + 16 <= nx.
+// the lane reductions are compile-time.  Needs nx % 16 == 0, P + 16 <= nx,
+// 256 threads (four waves) and lines of padded_len(N) entries.
+//
+// Per MFMA step (four candidates) a lane reads the candidate's first node,
+// one x factor (clamped to the stencil and selected to 0 outside it), its row
+// weight and NB moments: the steps read up to three staged entries past the
+// candidate range, which are either candidates whose stencil misses the x
+// block (their x factor selects to 0) or the zero-filled pad past the chunk.
+// When a wave's accumulators for all its x blocks fit in 64 registers they
+// stay in registers across chunks and are stored into the tile lines once;
+// otherwise each chunk adds its accumulators into the (cleared) lines.
 template <typename T, int N, int NC, int R>
 __global__ void __launch_bounds__(256) k_spread_mfma_xfft(TileArgs a, ImgArgs im, cplx<T>* __restrict__ spec) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   using MF = Mfma<T>;
   using acc_t = typename MF::acc_t;
   static_assert(R == 4 || R == 8 || R == 16, "tile rows");
-  constexpr int NCOL = NC * R, NB = (NCOL + 15) / 16, NXB = N / 16, KX = N / 2 + 1;
+  constexpr int NCOL = NC * R, NB = (NCOL + 15) / 16, NXB = N / 16, NW = 4;
+  constexpr int NCP = mfma_ncp(NC, R), XPW = (NXB + NW - 1) / NW, LS = padded_len(N);
+  constexpr bool kRegAcc = XPW * NB * 4 * (int)sizeof(T) <= 256;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* buf = tw + N;
   const int P = a.P, PW = P + R;
-  const MfmaStage<T> st = mfma_stage_layout<T, NC>(reinterpret_cast<unsigned char*>(buf + a.ncl * a.lstride), a.ch, PW);
+  const MfmaStage<T> st =
+      mfma_stage_layout<T>(reinterpret_cast<unsigned char*>(buf + a.ncl * LS), a.ch, PW, NCP);
   __shared__ int s_xr[4 * NXB];
   const int tile = blockIdx.x + blockIdx.y * gridDim.x;
   const int ty0 = blockIdx.x * a.ty, tz0 = blockIdx.y * a.tz, rhs = blockIdx.z;
   if (rhs_idle(a.gate, a.gdiv, rhs)) return;
-  const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
-  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];
-  for (int k = threadIdx.x; k < a.ncl * a.lstride; k += blockDim.x) buf[k] = cmk<T>(0, 0);
-  // the cleared lines and twiddles must be complete before any other thread
-  // uses them (a tile without candidates goes straight to the FFT)
-  __syncthreads();
   const size_t qt = (size_t)im.off[tile];
   const int total = im.off[tile + 1] - (int)qt;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  // the first chunk of the tile image goes to LDS with the twiddles
+  if (total > 0) mfma_stage_chunk<T, true, NC, NCP>(a, im, st, qt, min(a.ch, total), rhs);
+  const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
+  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];
+  if (!kRegAcc)
+    for (int k = threadIdx.x; k < a.ncl * LS; k += blockDim.x) buf[k] = cmk<T>(0, 0);
+  // the staged chunk, the cleared lines and the twiddles must be complete
+  // before any other thread uses them (a tile without candidates goes
+  // straight to the FFT)
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   const int i16 = lane & 15, k4 = lane >> 4;
   const int colrow = i16 % R;  // row of column nb*16 + i16 for every nb (R | 16)
-  int colc[NB];
+  const int colm = i16 / R;    // component of column i16; block nb adds nb*16/R
+  // one x block's accumulators go to its tile lines (ADD: onto what is there)
+  auto store_block = [&](int x0, const acc_t* acc, bool add) {
 #pragma unroll
-  for (int nb = 0; nb < NB; ++nb) colc[nb] = (nb * 16 + i16) / R;  // >= NC: unused column
+    for (int nb = 0; nb < NB; ++nb) {
+      const int l = nb * 16 + i16;
+      if (l < NCOL) {
+        T* pl = reinterpret_cast<T*>(buf + (l >> 1) * LS) + (l & 1);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          T* d = pl + 2 * pidx(x0 + MF::row(lane, r));
+          *d = add ? *d + acc[nb][r] : acc[nb][r];
+        }
+      }
+    }
+  };
+  acc_t acc[XPW][NB];
+#pragma unroll
+  for (int i = 0; i < XPW; ++i)
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) acc[i][nb] = acc_t{0, 0, 0, 0};
   for (int base = 0; base < total; base += a.ch) {
     const int nc = min(a.ch, total - base);
-    mfma_stage_chunk<T, true, NC>(a, im, st, qt + base, nc, rhs);
-    __syncthreads();
+    if (base > 0) {
+      mfma_stage_chunk<T, true, NC, NCP>(a, im, st, qt + base, nc, rhs);
+      __syncthreads();
+    }
     // candidate ranges of every x block: gx in [x0 - P + 1, x0 + 15] (mod nx),
-    // one range, or two when the window wraps below 0
+    // one range, or two when the window wraps below 0 (the wrapped one first)
     for (int xb = threadIdx.x; xb < NXB; xb += blockDim.x) {
       const int x0 = xb * 16, lo = x0 - P + 1;
       const int hi_idx = gx_lower_bound(st.gx, nc, x0 + 16);
@@ -338,62 +411,71 @@ __global__ void __launch_bounds__(256) k_spread_mfma_xfft(TileArgs a, ImgArgs im
       }
     }
     __syncthreads();
-    for (int xb = wave; xb < NXB; xb += nwave) {
-      const int x0 = xb * 16;
-      acc_t acc[NB];
 #pragma unroll
-      for (int nb = 0; nb < NB; ++nb) acc[nb] = acc_t{0, 0, 0, 0};
+    for (int i = 0; i < XPW; ++i) {
+      const int xb = wave + NW * i;
+      if (xb < NXB) {
+        const int x0 = xb * 16;
+        if (!kRegAcc)
 #pragma unroll
-      for (int sg = 0; sg < 2; ++sg) {
-        const int qa = __builtin_amdgcn_readfirstlane(s_xr[4 * xb + 2 * sg]);
-        const int qb = __builtin_amdgcn_readfirstlane(s_xr[4 * xb + 2 * sg + 1]);
-        for (int q = qa; q < qb; q += 4) {
-          // lanes past the range read candidate qa and contribute 0
-          const int cand = q + k4;
-          const bool ok = cand < qb;
-          const int cq = ok ? cand : qa;
-          int o = x0 + i16 - st.gx[cq];
-          if (o < 0) o += N;
-          // the load stays unconditional (a multiply, not a select) so that the
-          // loop body has no branch and the accumulators stay in place
-          const T av = st.fw[cq * PW + min(o, P - 1)] * (T)((ok && o < P) ? 1 : 0);
-          const T wv = st.fw[cq * PW + P + colrow];
+          for (int nb = 0; nb < NB; ++nb) acc[i][nb] = acc_t{0, 0, 0, 0};
 #pragma unroll
-          for (int nb = 0; nb < NB; ++nb) {
-            const T mv = st.m[cq * NC + min(colc[nb], NC - 1)];
-            const T bv = colc[nb] < NC ? wv * mv : (T)0;
-            acc[nb] = MF::mac(av, bv, acc[nb]);
+        for (int sg = 0; sg < 2; ++sg) {
+          const int qa = __builtin_amdgcn_readfirstlane(s_xr[4 * xb + 2 * sg]);
+          const int qb = __builtin_amdgcn_readfirstlane(s_xr[4 * xb + 2 * sg + 1]);
+          // node offset o = x - gx, plus N for the candidates that wrap below node 0
+          const int xo = x0 + i16 + ((sg == 0 && x0 - P + 1 < 0) ? N : 0);
+          for (int q = qa; q < qb; q += 4) {
+            const int cand = q + k4;
+            const int o = xo - st.gx[cand];
+            const T* f = st.fw + __umul24(cand, PW);
+            // o outside [0, P) (negative ones wrap to large unsigned values)
+            // reads the stencil's last factor and selects 0
+            const T fv = f[min((unsigned)o, (unsigned)(P - 1))];
+            const T av = (unsigned)o < (unsigned)P ? fv : (T)0;
+            const T wv = f[P + colrow];
+            const T* mq = st.m + cand * NCP + colm;
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) acc[i][nb] = MF::mac(av, wv * mq[nb * 16 / R], acc[i][nb]);
           }
         }
-      }
-#pragma unroll
-      for (int nb = 0; nb < NB; ++nb) {
-        if (colc[nb] < NC) {
-          const int l = nb * 16 + i16;
-          T* pl = reinterpret_cast<T*>(buf + (l >> 1) * a.lstride) + (l & 1);
-#pragma unroll
-          for (int r = 0; r < 4; ++r) pl[2 * pidx(x0 + MF::row(lane, r))] += acc[nb][r];
-        }
+        if (!kRegAcc) store_block(x0, acc[i], true);
       }
     }
     __syncthreads();
   }
-  fft_any<T, 0, N>(buf, a.lstride, a.ncl, a.plan, tw, false, nullptr);
-  // half spectrum of the NC*R real lines (packed two per complex line): one
-  // wave per line, lanes along kx
-  for (int l = wave; l < NCOL; l += nwave) {
-    const long long row = tile_line_row(a, l, ty0, tz0, rhs, NC);
-    if (row < 0) continue;
-    const cplx<T>* line = buf + (l >> 1) * a.lstride;
-    cplx<T>* dst = spec + row * a.kxs;
-    for (int k = lane; k < KX; k += 64) {
+  if (kRegAcc) {
+#pragma unroll
+    for (int i = 0; i < XPW; ++i) {
+      const int xb = wave + NW * i;
+      if (xb < NXB) store_block(xb * 16, acc[i], false);
+    }
+    __syncthreads();
+  }
+  fft_any<T, 0, N>(buf, LS, a.ncl, a.plan, tw, false, nullptr);
+  // half spectrum of the NC*R real lines, packed two per complex line (line
+  // pair p): the even line from the Hermitian part, the odd one from the
+  // anti-Hermitian part of Z[k] and conj(Z[N - k]).  One wave per pair, lanes
+  // along kx < N/2, then the kx = N/2 entries of every line.
+  constexpr int NPAIR = NCOL / 2, HALF = N / 2;
+  for (int pr = wave; pr < NPAIR; pr += NW) {
+    const long long re = tile_line_row(a, 2 * pr, ty0, tz0, rhs, NC);
+    const long long ro = tile_line_row(a, 2 * pr + 1, ty0, tz0, rhs, NC);
+    const cplx<T>* line = buf + pr * LS;
+    for (int k = lane; k < HALF; k += 64) {
       const cplx<T> Z = line[pidx(k)];
       const cplx<T> Zc = cconj(line[pidx(k == 0 ? 0 : N - k)]);
-      cplx<T> v;
-      if ((l & 1) == 0) v = cmk<T>((Z.x + Zc.x) * (T)0.5, (Z.y + Zc.y) * (T)0.5);
-      else v = cmk<T>((Z.y - Zc.y) * (T)0.5, (Zc.x - Z.x) * (T)0.5);
-      dst[k] = v;
+      if (re >= 0) spec[re * a.kxs + k] = cmk<T>((Z.x + Zc.x) * (T)0.5, (Z.y + Zc.y) * (T)0.5);
+      if (ro >= 0) spec[ro * a.kxs + k] = cmk<T>((Z.y - Zc.y) * (T)0.5, (Zc.x - Z.x) * (T)0.5);
     }
+  }
+  for (int l = threadIdx.x; l < NCOL; l += blockDim.x) {
+    const long long row = tile_line_row(a, l, ty0, tz0, rhs, NC);
+    if (row < 0) continue;
+    const cplx<T> Z = buf[(l >> 1) * LS + pidx(HALF)];
+    const cplx<T> Zc = cconj(Z);
+    spec[row * a.kxs + HALF] = (l & 1) == 0 ? cmk<T>((Z.x + Zc.x) * (T)0.5, (Z.y + Zc.y) * (T)0.5)
+                                            : cmk<T>((Z.y - Zc.y) * (T)0.5, (Zc.x - Z.x) * (T)0.5);
   }
 }
 
@@ -401,67 +483,81 @@ __global__ void __launch_bounds__(256) k_spread_mfma_xfft(TileArgs a, ImgArgs im
 // sorted candidates w, w + 4, ...: A[q][x] over the x span of the block (in
 // unwrapped coordinates, the grid read at x mod nx), then the row weights and
 // a reduction over the R lanes of each component, stored as the partial of
-// image slot q (k_contract_reduce sums each particle's partials).
+// image slot q (k_contract_reduce sums each particle's partials).  Needs 256
+// threads (four waves) and lines of padded_len(N) entries.
 template <typename T, int N, int NC, int R>
 __global__ void __launch_bounds__(256) k_xifft_contract_mfma(TileArgs a, ImgArgs im, const cplx<T>* __restrict__ spec) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   using MF = Mfma<T>;
   using acc_t = typename MF::acc_t;
   static_assert(R == 4 || R == 8 || R == 16, "tile rows");
-  constexpr int NCOL = NC * R, NB = (NCOL + 15) / 16, KX = N / 2 + 1;
+  constexpr int NCOL = NC * R, NB = (NCOL + 15) / 16, NW = 4, LS = padded_len(N);
+  constexpr int NCP = mfma_ncp(NC, R), NPAIR = NCOL / 2, HALF = N / 2, KPL = (HALF + 63) / 64;
+  // line pairs whose spectrum loads a wave keeps in flight together (32 VGPRs)
+  constexpr int GP = (8 / (KPL * (int)sizeof(T) / 4)) > 0 ? 8 / (KPL * (int)sizeof(T) / 4) : 1;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* buf = tw + N;
   const int P = a.P, PW = P + R;
-  const MfmaStage<T> st = mfma_stage_layout<T, NC>(reinterpret_cast<unsigned char*>(buf + a.ncl * a.lstride), a.ch, PW);
+  const MfmaStage<T> st = mfma_stage_layout<T>(reinterpret_cast<unsigned char*>(buf + a.ncl * LS), a.ch, PW, NCP);
   const int tile = blockIdx.x + blockIdx.y * gridDim.x;
   const int ty0 = blockIdx.x * a.ty, tz0 = blockIdx.y * a.tz, rhs = blockIdx.z;
   if (rhs_idle(a.gate, a.gdiv, rhs)) return;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-  const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
-  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   const size_t qt = (size_t)im.off[tile];
   const int total = im.off[tile + 1] - (int)qt;
   // the first chunk of the tile image goes to LDS while the spectrum loads
-  mfma_stage_chunk<T, false, NC>(a, im, st, qt, min(a.ch, total), rhs);
-  // pack two half spectra per complex line: Z = A + iB (Hermitian
-  // extension).  Flat (line, kx) index, all of a thread's loads issued before
-  // its LDS writes.
-  {
-    constexpr int NCL = (NCOL + 1) / 2, TOT = NCL * KX, IT = (TOT + 255) / 256, G = 8;
-    for (int u0 = 0; u0 < IT; u0 += G) {
-    cplx<T> va[G], vb[G];
+  mfma_stage_chunk<T, false, NC, NCP>(a, im, st, qt, min(a.ch, total), rhs);
+  const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
+  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];
+  // pack the half spectra of line pair pr (real lines 2pr, 2pr + 1) into one
+  // complex line Z = A + iB (Hermitian extension): one wave per pair, lanes
+  // along kx < N/2, GP pairs' loads in flight before their LDS writes; then
+  // the kx = N/2 entries
+  for (int p0 = wave; p0 < NPAIR; p0 += NW * GP) {
+    cplx<T> va[GP][KPL], vb[GP][KPL];
 #pragma unroll
-    for (int u = 0; u < G; ++u) {
-      const int idx = (u0 + u) * 256 + (int)threadIdx.x;
-      const int q = idx / KX, k = idx - q * KX;
-      va[u] = cmk<T>(0, 0);
-      vb[u] = cmk<T>(0, 0);
-      if (idx < TOT) {
-        const long long ra = tile_line_row(a, 2 * q, ty0, tz0, rhs, NC);
-        const long long rb = 2 * q + 1 < NCOL ? tile_line_row(a, 2 * q + 1, ty0, tz0, rhs, NC) : -1;
-        if (ra >= 0) va[u] = spec[ra * a.kxs + k];
-        if (rb >= 0) vb[u] = spec[rb * a.kxs + k];
-      }
-    }
+    for (int g = 0; g < GP; ++g) {
+      const int pr = p0 + NW * g;
+      const long long ra = pr < NPAIR ? tile_line_row(a, 2 * pr, ty0, tz0, rhs, NC) : -1;
+      const long long rb = pr < NPAIR ? tile_line_row(a, 2 * pr + 1, ty0, tz0, rhs, NC) : -1;
 #pragma unroll
-    for (int u = 0; u < G; ++u) {
-      const int idx = (u0 + u) * 256 + (int)threadIdx.x;
-      const int q = idx / KX, k = idx - q * KX;
-      if (idx < TOT) {
-        cplx<T>* line = buf + q * a.lstride;
-        const cplx<T> A = va[u], B = vb[u];
-        if (k == 0 || 2 * k == N) {
-          line[pidx(k)] = cmk<T>(A.x, B.x);
-        } else {
-          line[pidx(k)] = cmk<T>(A.x - B.y, A.y + B.x);
-          line[pidx(N - k)] = cmk<T>(A.x + B.y, B.x - A.y);
+      for (int j = 0; j < KPL; ++j) {
+        const int k = lane + 64 * j;
+        va[g][j] = cmk<T>(0, 0);
+        vb[g][j] = cmk<T>(0, 0);
+        if (k < HALF) {
+          if (ra >= 0) va[g][j] = spec[ra * a.kxs + k];
+          if (rb >= 0) vb[g][j] = spec[rb * a.kxs + k];
         }
       }
     }
+#pragma unroll
+    for (int g = 0; g < GP; ++g) {
+      const int pr = p0 + NW * g;
+      if (pr < NPAIR) {
+        cplx<T>* line = buf + pr * LS;
+#pragma unroll
+        for (int j = 0; j < KPL; ++j) {
+          const int k = lane + 64 * j;
+          if (k < HALF) {
+            const cplx<T> A = va[g][j], B = vb[g][j];
+            line[pidx(k)] = k == 0 ? cmk<T>(A.x, B.x) : cmk<T>(A.x - B.y, A.y + B.x);
+            if (k != 0) line[pidx(N - k)] = cmk<T>(A.x + B.y, B.x - A.y);
+          }
+        }
+      }
     }
   }
+  for (int pr = threadIdx.x; pr < NPAIR; pr += blockDim.x) {
+    const long long ra = tile_line_row(a, 2 * pr, ty0, tz0, rhs, NC);
+    const long long rb = tile_line_row(a, 2 * pr + 1, ty0, tz0, rhs, NC);
+    const T ax = ra >= 0 ? spec[ra * a.kxs + HALF].x : (T)0;
+    const T bx = rb >= 0 ? spec[rb * a.kxs + HALF].x : (T)0;
+    buf[pr * LS + pidx(HALF)] = cmk<T>(ax, bx);
+  }
   __syncthreads();
-  fft_any<T, 0, N>(buf, a.lstride, a.ncl, a.plan, tw, true, nullptr);
+  fft_any<T, 0, N>(buf, LS, a.ncl, a.plan, tw, true, nullptr);
   const int i16 = lane & 15, k4 = lane >> 4;
   const int colrow = i16 % R;
   int colc[NB];
@@ -470,38 +566,52 @@ __global__ void __launch_bounds__(256) k_xifft_contract_mfma(TileArgs a, ImgArgs
   for (int nb = 0; nb < NB; ++nb) {
     const int j = nb * 16 + i16;
     colc[nb] = j / R;
+    // columns past the last line read the last line; their outputs are dropped
     const int jl = min(j, NCOL - 1);
-    colp[nb] = reinterpret_cast<const T*>(buf + (jl >> 1) * a.lstride) + (jl & 1);
+    colp[nb] = reinterpret_cast<const T*>(buf + (jl >> 1) * LS) + (jl & 1);
   }
   T* partr = reinterpret_cast<T*>(im.part) + (size_t)rhs * im.total * NC;
   for (int base = 0; base < total; base += a.ch) {
     const int nc = min(a.ch, total - base);
     if (base > 0) {
-      mfma_stage_chunk<T, false, NC>(a, im, st, qt + base, nc, rhs);
+      mfma_stage_chunk<T, false, NC, NCP>(a, im, st, qt + base, nc, rhs);
       __syncthreads();
     }
+    T* pb = partr + (qt + base) * NC;
     const int nblk = (nc + 15) >> 4;
-    for (int cb = wave; cb < nblk; cb += nwave) {
+    for (int cb = wave; cb < nblk; cb += NW) {
       const int c0 = cb * 16;
       const int cand = c0 + i16;
       const bool ok = cand < nc;
       const int cq = ok ? cand : c0;
-      const int gxc = st.gx[cq];
-      const int xs = __builtin_amdgcn_readfirstlane(st.gx[c0]);
-      const int xe = __builtin_amdgcn_readfirstlane(st.gx[min(c0 + 15, nc - 1)]) + P;
+      // a lane past the chunk's last candidate gets a first node that puts
+      // every node of the span outside its stencil
+      const int gxc = ok ? st.gx[cq] : (1 << 29);
+      const T* f = st.fw + __umul24(cq, PW);
+      // x span [xs, xe) of the block's stencils, widened to multiples of 4
+      // so that no step of four nodes straddles node N (N % 16 == 0); the
+      // added nodes lie outside every stencil of the block
+      const int xs = __builtin_amdgcn_readfirstlane(st.gx[c0]) & ~3;
+      const int xe = (__builtin_amdgcn_readfirstlane(st.gx[min(c0 + 15, nc - 1)]) + P + 3) & ~3;
       acc_t acc[NB];
 #pragma unroll
       for (int nb = 0; nb < NB; ++nb) acc[nb] = acc_t{0, 0, 0, 0};
-      for (int xx = xs; xx < xe; xx += 4) {
-        const int xk = xx + k4;
-        const int o = xk - gxc;
-        const T av = st.fw[cq * PW + min(max(o, 0), P - 1)] * (T)((ok && o >= 0 && o < P) ? 1 : 0);
-        const int xw = xk >= N ? xk - N : xk;
-        const int px = 2 * pidx(xw);
+      // nodes below N, then the ones that wrap to x - N (xe - N < N)
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-          const T g = colp[nb][px];
-          acc[nb] = MF::mac(av, colc[nb] < NC ? g : (T)0, acc[nb]);
+      for (int sg = 0; sg < 2; ++sg) {
+        const int lo = sg == 0 ? xs : N;
+        const int hi = sg == 0 ? min(xe, N) : xe;
+        const int wrap = sg == 0 ? 0 : N;
+        for (int xx = lo; xx < hi; xx += 4) {
+          const int xk = xx + k4;
+          // o outside [0, P) (negative ones wrap to large unsigned values)
+          // reads the stencil's last factor and selects 0
+          const int o = xk - gxc;
+          const T fv = f[min((unsigned)o, (unsigned)(P - 1))];
+          const T av = (unsigned)o < (unsigned)P ? fv : (T)0;
+          const int px = 2 * pidx(xk - wrap);
+#pragma unroll
+          for (int nb = 0; nb < NB; ++nb) acc[nb] = MF::mac(av, colp[nb][px], acc[nb]);
         }
       }
       // out[q][c] = sum_row w_q[row] D[q][(c, row)]: weight, then sum the R
@@ -511,11 +621,11 @@ __global__ void __launch_bounds__(256) k_xifft_contract_mfma(TileArgs a, ImgArgs
         const int cr = c0 + MF::row(lane, r);
         const bool okr = cr < nc;
         const int crq = okr ? cr : c0;
-        const T wq = st.fw[crq * PW + P + colrow];
+        const T wq = st.fw[__umul24(crq, PW) + P + colrow];
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
           const T v = group_sum<T>(acc[nb][r] * wq, R);
-          if (colc[nb] < NC && colrow == 0 && okr) partr[(qt + base + cr) * NC + colc[nb]] = v;
+          if (colc[nb] < NC && colrow == 0 && okr) pb[cr * NC + colc[nb]] = v;
         }
       }
     }
