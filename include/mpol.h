@@ -256,14 +256,16 @@ int mpol_ewald_forces(mpol_ctx* ctx, const double* p_dev, int32_t nrhs, const do
                       const double* f2_host, double* forces_dev);
 
 /* Launch-shape tuning (no counterpart in the reference; used by the
- * autotuner): knobs[0..6] = spread/contract tile rows along y and z (ty*tz a
+ * autotuner): knobs[0..7] = spread/contract tile rows along y and z (ty*tz a
  * power of two, each <= 4), particles staged per chunk (32..256), y-pass and
  * z-pass columns per workgroup (<= 64), z-pass threads (multiple of 64),
  * spread/contract engine (1 matrix cores over per-tile candidate images,
  * 2 vector loops; for the polydisperse operator 2 selects the per-particle
  * kernels (atomic scatter, LDS-staged contraction) and the separate y/z
  * passes instead of the stored-entry gather/contraction and the one-kernel
- * y/z passes).  0 keeps the automatic choice; nknobs may be shorter than 7. */
+ * y/z passes), right-hand sides per matrix-core spread/contract workgroup
+ * (1..64: one staged tile image serves them in turn).  0 keeps the automatic
+ * choice; nknobs may be shorter than 8. */
 int mpol_set_tuning(mpol_ctx* ctx, const int32_t* knobs, int32_t nknobs);
 
 /* The same knobs for the operator in one precision only (MPOL_FP32 or

@@ -1086,7 +1086,7 @@ int mpol_set_tuning_for(mpol_ctx* c, int32_t precision, const int32_t* knobs, in
   if (!c || (nknobs > 0 && !knobs) || nknobs < 0 || nknobs > kTuneKnobs) return fail(MPOL_EINVAL, "bad tuning arguments");
   if (precision != MPOL_FP32 && precision != MPOL_FP64 && precision != -1)
     return fail(MPOL_EINVAL, "precision must be MPOL_FP32, MPOL_FP64 or -1 (both)");
-  int t[kTuneKnobs] = {0, 0, 0, 0, 0, 0, 0};
+  int t[kTuneKnobs] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int i = 0; i < nknobs; ++i) t[i] = knobs[i];
   const bool pow2_rows = (t[0] == 0 && t[1] == 0) ||
                          (t[0] >= 1 && t[0] <= 4 && t[1] >= 1 && t[1] <= 4 && ((t[0] * t[1]) & (t[0] * t[1] - 1)) == 0);
@@ -1096,6 +1096,7 @@ int mpol_set_tuning_for(mpol_ctx* c, int32_t precision, const int32_t* knobs, in
   if (t[5] != 0 && (t[5] < 64 || t[5] > 1024 || t[5] % 64 != 0))
     return fail(MPOL_EINVAL, "z-pass threads must be a multiple of 64 in [64, 1024]");
   if (t[6] < 0 || t[6] > 2) return fail(MPOL_EINVAL, "engine must be 0 (automatic), 1 (matrix cores) or 2 (vector loops)");
+  if (t[7] < 0 || t[7] > 64) return fail(MPOL_EINVAL, "right-hand sides per workgroup must be in [1, 64]");
   for (int p = 0; p < 2; ++p) {
     if (precision != -1 && p != (precision == MPOL_FP32 ? 1 : 0)) continue;
     for (int i = 0; i < kTuneKnobs; ++i) c->tune[p][i] = t[i];

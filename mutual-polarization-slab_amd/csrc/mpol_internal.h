@@ -187,7 +187,8 @@ inline ColCfg col_cfg(int n, bool generic, int kx, int lines_per_col, size_t csi
 
 constexpr size_t kLdsTile = 48 * 1024;
 constexpr size_t kLdsMfma = 80 * 1024;  // two matrix-core tile workgroups per CU
-constexpr int kTuneKnobs = 7;
+constexpr int kTuneKnobs = 8;
+constexpr int kRhsGroup = 4;  // automatic right-hand sides per matrix-core tile workgroup
 
 // Candidates of every tile, sorted by stencil start along x within chunks of
 // ch, for one tile shape and precision (built on first use after
@@ -632,7 +633,11 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
     ima.total = im.total;
     HIPALLOC(im.part, (size_t)nrhs * (size_t)std::max<int64_t>(im.total, 1) * NC * sizeof(T));
     ima.part = im.part.p;
-    mgrid = dim3(cdiv(c->ng[1], tm.ty), cdiv(c->ng[2], tm.tz), nrhs);
+    // right-hand sides per workgroup: one staged tile image serves them in
+    // turn (knob 7; automatic: all of them, up to kRhsGroup)
+    const int rg_knob = knobs<T>(c)[7];
+    tma.rg = std::max(1, std::min(nrhs, rg_knob > 0 ? rg_knob : kRhsGroup));
+    mgrid = dim3(cdiv(c->ng[1], tm.ty), cdiv(c->ng[2], tm.tz), cdiv(nrhs, tma.rg));
   }
 
   // FP32: atomic-free gather spread; FP64: LDS FP64 atomics (faster there)

@@ -545,6 +545,7 @@ struct TileArgs {
   int half;          // (P-1)/2
   int n;             // particles
   int nrhs;
+  int rg;            // right-hand sides per workgroup (matrix-core spread/contract)
   const int4* rec;   // sorted: first stencil node x, y, z; caller's index
   const void* base;  // sorted: node0*h - x per axis (double4 or float4)
   const double* ms;  // sorted dipoles [nrhs][n][3]

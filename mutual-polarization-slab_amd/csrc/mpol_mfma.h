@@ -263,15 +263,17 @@ inline size_t mfma_tile_lds(int nx, size_t csize, int P, int ncl, int lstride, i
          (size_t)(ch + kStagePad) * ((P + rows + mfma_ncp(NC, rows)) * (csize / 2) + sizeof(int));
 }
 
-// Copy chunk [q, q + nc) of the tile image into LDS; the spread also gathers
-// the moments of right-hand side rhs and zero-fills kStagePad entries past the
-// chunk (WITH_M).
+// Copy chunk [q, q + nc) of the tile image into LDS (IMAGE); the spread also
+// gathers the moments of right-hand side rhs and zero-fills kStagePad entries
+// past the chunk (WITH_M).  Without IMAGE only the moments are staged (the
+// image chunk is in LDS from an earlier right-hand side).
 template <typename T, bool WITH_M, int NC, int NCP>
 __device__ inline void mfma_stage_chunk(const TileArgs& a, const ImgArgs& im, const MfmaStage<T>& st, size_t q,
-                                        int nc, int rhs) {
+                                        int nc, int rhs, bool image = true) {
   const int PW = a.P + a.rows;
   const T* src = reinterpret_cast<const T*>(im.fw) + q * PW;
-  const int tot = nc * PW, nt = blockDim.x, t = threadIdx.x;
+  const int nt = blockDim.x, t = threadIdx.x;
+  const int tot = image ? nc * PW : 0;
   // every load of the chunk issued before the first LDS write: the image
   // rows (up to kBatch per thread in one batch), this thread's slot record,
   // then the moments it points to
@@ -308,7 +310,7 @@ __device__ inline void mfma_stage_chunk(const TileArgs& a, const ImgArgs& im, co
   }
   for (int u = t; u < nc; u += nt) {
     if (u != t) g = im.gs[q + u];
-    st.gx[u] = g.x;
+    if (image) st.gx[u] = g.x;
     if (WITH_M) {
       const double* mp = a.ms + NC * ((size_t)rhs * a.n + g.y);
 #pragma unroll
@@ -316,10 +318,20 @@ __device__ inline void mfma_stage_chunk(const TileArgs& a, const ImgArgs& im, co
     }
   }
   if (WITH_M) {
-    for (int k = t; k < kStagePad * PW; k += nt) st.fw[tot + k] = (T)0;
+    if (image) {
+      for (int k = t; k < kStagePad * PW; k += nt) st.fw[nc * PW + k] = (T)0;
+      if (t < kStagePad) st.gx[nc + t] = 0;
+    }
     for (int k = t; k < kStagePad * NCP; k += nt) st.m[nc * NCP + k] = (T)0;
-    if (t < kStagePad) st.gx[nc + t] = 0;
   }
+}
+
+// First right-hand side at or after r0, below r1, whose system is not idle
+// (r1 when there is none).
+__device__ __forceinline__ int next_live_rhs(const TileArgs& a, int r0, int r1) {
+  int r = r0;
+  while (r < r1 && rhs_idle(a.gate, a.gdiv, r)) ++r;
+  return r;
 }
 
 // Spread + forward x-FFT.  Wave w handles x blocks w, w + 4, ...; lane l of a
@@ -353,20 +365,20 @@ __global__ void __launch_bounds__(256) k_spread_mfma_xfft(TileArgs a, ImgArgs im
       mfma_stage_layout<T>(reinterpret_cast<unsigned char*>(buf + a.ncl * LS), a.ch, PW, NCP);
   __shared__ int s_xr[4 * NXB];
   const int tile = blockIdx.x + blockIdx.y * gridDim.x;
-  const int ty0 = blockIdx.x * a.ty, tz0 = blockIdx.y * a.tz, rhs = blockIdx.z;
-  if (rhs_idle(a.gate, a.gdiv, rhs)) return;
+  const int ty0 = blockIdx.x * a.ty, tz0 = blockIdx.y * a.tz;
+  // right-hand sides [r0, r1) in turn; the first live one is staged with
+  // the image, later ones only restage their moments when the tile's
+  // candidates fit in one chunk
+  const int r0 = blockIdx.z * a.rg, r1 = min(r0 + a.rg, a.nrhs);
+  const int rfirst = next_live_rhs(a, r0, r1);
+  if (rfirst >= r1) return;
   const size_t qt = (size_t)im.off[tile];
   const int total = im.off[tile + 1] - (int)qt;
+  const bool one_chunk = total <= a.ch;
   // the first chunk of the tile image goes to LDS with the twiddles
-  if (total > 0) mfma_stage_chunk<T, true, NC, NCP>(a, im, st, qt, min(a.ch, total), rhs);
+  if (total > 0) mfma_stage_chunk<T, true, NC, NCP>(a, im, st, qt, min(a.ch, total), rfirst);
   const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
   for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];
-  if (!kRegAcc)
-    for (int k = threadIdx.x; k < a.ncl * LS; k += blockDim.x) buf[k] = cmk<T>(0, 0);
-  // the staged chunk, the cleared lines and the twiddles must be complete
-  // before any other thread uses them (a tile without candidates goes
-  // straight to the FFT)
-  __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   const int i16 = lane & 15, k4 = lane >> 4;
@@ -387,95 +399,109 @@ __global__ void __launch_bounds__(256) k_spread_mfma_xfft(TileArgs a, ImgArgs im
       }
     }
   };
-  acc_t acc[XPW][NB];
-#pragma unroll
-  for (int i = 0; i < XPW; ++i)
-#pragma unroll
-    for (int nb = 0; nb < NB; ++nb) acc[i][nb] = acc_t{0, 0, 0, 0};
-  for (int base = 0; base < total; base += a.ch) {
-    const int nc = min(a.ch, total - base);
-    if (base > 0) {
-      mfma_stage_chunk<T, true, NC, NCP>(a, im, st, qt + base, nc, rhs);
-      __syncthreads();
-    }
-    // candidate ranges of every x block: gx in [x0 - P + 1, x0 + 15] (mod nx),
-    // one range, or two when the window wraps below 0 (the wrapped one first)
-    for (int xb = threadIdx.x; xb < NXB; xb += blockDim.x) {
-      const int x0 = xb * 16, lo = x0 - P + 1;
-      const int hi_idx = gx_lower_bound(st.gx, nc, x0 + 16);
-      int* r = s_xr + 4 * xb;
-      if (lo >= 0) {
-        r[0] = gx_lower_bound(st.gx, nc, lo); r[1] = hi_idx; r[2] = 0; r[3] = 0;
-      } else {
-        r[0] = gx_lower_bound(st.gx, nc, lo + N); r[1] = nc; r[2] = 0; r[3] = hi_idx;
-      }
-    }
+  for (int rhs = rfirst; rhs < r1; rhs = next_live_rhs(a, rhs + 1, r1)) {
+    const bool first = rhs == rfirst;
+    if (!kRegAcc)
+      for (int k = threadIdx.x; k < a.ncl * LS; k += blockDim.x) buf[k] = cmk<T>(0, 0);
+    // the staged chunk, the cleared lines and the twiddles must be complete
+    // before any other thread uses them (a tile without candidates goes
+    // straight to the FFT)
     __syncthreads();
+    acc_t acc[XPW][NB];
 #pragma unroll
-    for (int i = 0; i < XPW; ++i) {
-      const int xb = wave + NW * i;
-      if (xb < NXB) {
-        const int x0 = xb * 16;
-        if (!kRegAcc)
+    for (int i = 0; i < XPW; ++i)
 #pragma unroll
-          for (int nb = 0; nb < NB; ++nb) acc[i][nb] = acc_t{0, 0, 0, 0};
-#pragma unroll
-        for (int sg = 0; sg < 2; ++sg) {
-          const int qa = __builtin_amdgcn_readfirstlane(s_xr[4 * xb + 2 * sg]);
-          const int qb = __builtin_amdgcn_readfirstlane(s_xr[4 * xb + 2 * sg + 1]);
-          // node offset o = x - gx, plus N for the candidates that wrap below node 0
-          const int xo = x0 + i16 + ((sg == 0 && x0 - P + 1 < 0) ? N : 0);
-          for (int q = qa; q < qb; q += 4) {
-            const int cand = q + k4;
-            const int o = xo - st.gx[cand];
-            const T* f = st.fw + __umul24(cand, PW);
-            // o outside [0, P) (negative ones wrap to large unsigned values)
-            // reads the stencil's last factor and selects 0
-            const T fv = f[min((unsigned)o, (unsigned)(P - 1))];
-            const T av = (unsigned)o < (unsigned)P ? fv : (T)0;
-            const T wv = f[P + colrow];
-            const T* mq = st.m + cand * NCP + colm;
-#pragma unroll
-            for (int nb = 0; nb < NB; ++nb) acc[i][nb] = MF::mac(av, wv * mq[nb * 16 / R], acc[i][nb]);
+      for (int nb = 0; nb < NB; ++nb) acc[i][nb] = acc_t{0, 0, 0, 0};
+    for (int base = 0; base < total; base += a.ch) {
+      const int nc = min(a.ch, total - base);
+      if (!(first && base == 0)) {
+        mfma_stage_chunk<T, true, NC, NCP>(a, im, st, qt + base, nc, rhs, !one_chunk);
+        __syncthreads();
+      }
+      if (first || !one_chunk) {
+        // candidate ranges of every x block: gx in [x0 - P + 1, x0 + 15]
+        // (mod nx), one range, or two when the window wraps below 0 (the
+        // wrapped one first)
+        for (int xb = threadIdx.x; xb < NXB; xb += blockDim.x) {
+          const int x0 = xb * 16, lo = x0 - P + 1;
+          const int hi_idx = gx_lower_bound(st.gx, nc, x0 + 16);
+          int* r = s_xr + 4 * xb;
+          if (lo >= 0) {
+            r[0] = gx_lower_bound(st.gx, nc, lo); r[1] = hi_idx; r[2] = 0; r[3] = 0;
+          } else {
+            r[0] = gx_lower_bound(st.gx, nc, lo + N); r[1] = nc; r[2] = 0; r[3] = hi_idx;
           }
         }
-        if (!kRegAcc) store_block(x0, acc[i], true);
+        __syncthreads();
+      }
+#pragma unroll
+      for (int i = 0; i < XPW; ++i) {
+        const int xb = wave + NW * i;
+        if (xb < NXB) {
+          const int x0 = xb * 16;
+          if (!kRegAcc)
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) acc[i][nb] = acc_t{0, 0, 0, 0};
+#pragma unroll
+          for (int sg = 0; sg < 2; ++sg) {
+            const int qa = __builtin_amdgcn_readfirstlane(s_xr[4 * xb + 2 * sg]);
+            const int qb = __builtin_amdgcn_readfirstlane(s_xr[4 * xb + 2 * sg + 1]);
+            // node offset o = x - gx, plus N for the candidates that wrap below node 0
+            const int xo = x0 + i16 + ((sg == 0 && x0 - P + 1 < 0) ? N : 0);
+            for (int q = qa; q < qb; q += 4) {
+              const int cand = q + k4;
+              const int o = xo - st.gx[cand];
+              const T* f = st.fw + __umul24(cand, PW);
+              // o outside [0, P) (negative ones wrap to large unsigned values)
+              // reads the stencil's last factor and selects 0
+              const T fv = f[min((unsigned)o, (unsigned)(P - 1))];
+              const T av = (unsigned)o < (unsigned)P ? fv : (T)0;
+              const T wv = f[P + colrow];
+              const T* mq = st.m + cand * NCP + colm;
+#pragma unroll
+              for (int nb = 0; nb < NB; ++nb) acc[i][nb] = MF::mac(av, wv * mq[nb * 16 / R], acc[i][nb]);
+            }
+          }
+          if (!kRegAcc) store_block(x0, acc[i], true);
+        }
+      }
+      __syncthreads();
+    }
+    if (kRegAcc) {
+#pragma unroll
+      for (int i = 0; i < XPW; ++i) {
+        const int xb = wave + NW * i;
+        if (xb < NXB) store_block(xb * 16, acc[i], false);
+      }
+      __syncthreads();
+    }
+    fft_any<T, 0, N>(buf, LS, a.ncl, a.plan, tw, false, nullptr);
+    // half spectrum of the NC*R real lines, packed two per complex line
+    // (line pair p): the even line from the Hermitian part, the odd one from
+    // the anti-Hermitian part of Z[k] and conj(Z[N - k]).  One wave per pair,
+    // lanes along kx < N/2, then the kx = N/2 entries of every line.
+    constexpr int NPAIR = NCOL / 2, HALF = N / 2;
+    for (int pr = wave; pr < NPAIR; pr += NW) {
+      const long long re = tile_line_row(a, 2 * pr, ty0, tz0, rhs, NC);
+      const long long ro = tile_line_row(a, 2 * pr + 1, ty0, tz0, rhs, NC);
+      const cplx<T>* line = buf + pr * LS;
+      for (int k = lane; k < HALF; k += 64) {
+        const cplx<T> Z = line[pidx(k)];
+        const cplx<T> Zc = cconj(line[pidx(k == 0 ? 0 : N - k)]);
+        if (re >= 0) spec[re * a.kxs + k] = cmk<T>((Z.x + Zc.x) * (T)0.5, (Z.y + Zc.y) * (T)0.5);
+        if (ro >= 0) spec[ro * a.kxs + k] = cmk<T>((Z.y - Zc.y) * (T)0.5, (Zc.x - Z.x) * (T)0.5);
       }
     }
-    __syncthreads();
-  }
-  if (kRegAcc) {
-#pragma unroll
-    for (int i = 0; i < XPW; ++i) {
-      const int xb = wave + NW * i;
-      if (xb < NXB) store_block(xb * 16, acc[i], false);
+    for (int l = threadIdx.x; l < NCOL; l += blockDim.x) {
+      const long long row = tile_line_row(a, l, ty0, tz0, rhs, NC);
+      if (row < 0) continue;
+      const cplx<T> Z = buf[(l >> 1) * LS + pidx(HALF)];
+      const cplx<T> Zc = cconj(Z);
+      spec[row * a.kxs + HALF] = (l & 1) == 0 ? cmk<T>((Z.x + Zc.x) * (T)0.5, (Z.y + Zc.y) * (T)0.5)
+                                              : cmk<T>((Z.y - Zc.y) * (T)0.5, (Zc.x - Z.x) * (T)0.5);
     }
+    // the lines are read until here; the next right-hand side writes them
     __syncthreads();
-  }
-  fft_any<T, 0, N>(buf, LS, a.ncl, a.plan, tw, false, nullptr);
-  // half spectrum of the NC*R real lines, packed two per complex line (line
-  // pair p): the even line from the Hermitian part, the odd one from the
-  // anti-Hermitian part of Z[k] and conj(Z[N - k]).  One wave per pair, lanes
-  // along kx < N/2, then the kx = N/2 entries of every line.
-  constexpr int NPAIR = NCOL / 2, HALF = N / 2;
-  for (int pr = wave; pr < NPAIR; pr += NW) {
-    const long long re = tile_line_row(a, 2 * pr, ty0, tz0, rhs, NC);
-    const long long ro = tile_line_row(a, 2 * pr + 1, ty0, tz0, rhs, NC);
-    const cplx<T>* line = buf + pr * LS;
-    for (int k = lane; k < HALF; k += 64) {
-      const cplx<T> Z = line[pidx(k)];
-      const cplx<T> Zc = cconj(line[pidx(k == 0 ? 0 : N - k)]);
-      if (re >= 0) spec[re * a.kxs + k] = cmk<T>((Z.x + Zc.x) * (T)0.5, (Z.y + Zc.y) * (T)0.5);
-      if (ro >= 0) spec[ro * a.kxs + k] = cmk<T>((Z.y - Zc.y) * (T)0.5, (Zc.x - Z.x) * (T)0.5);
-    }
-  }
-  for (int l = threadIdx.x; l < NCOL; l += blockDim.x) {
-    const long long row = tile_line_row(a, l, ty0, tz0, rhs, NC);
-    if (row < 0) continue;
-    const cplx<T> Z = buf[(l >> 1) * LS + pidx(HALF)];
-    const cplx<T> Zc = cconj(Z);
-    spec[row * a.kxs + HALF] = (l & 1) == 0 ? cmk<T>((Z.x + Zc.x) * (T)0.5, (Z.y + Zc.y) * (T)0.5)
-                                            : cmk<T>((Z.y - Zc.y) * (T)0.5, (Zc.x - Z.x) * (T)0.5);
   }
 }
 
@@ -500,64 +526,21 @@ __global__ void __launch_bounds__(256) k_xifft_contract_mfma(TileArgs a, ImgArgs
   const int P = a.P, PW = P + R;
   const MfmaStage<T> st = mfma_stage_layout<T>(reinterpret_cast<unsigned char*>(buf + a.ncl * LS), a.ch, PW, NCP);
   const int tile = blockIdx.x + blockIdx.y * gridDim.x;
-  const int ty0 = blockIdx.x * a.ty, tz0 = blockIdx.y * a.tz, rhs = blockIdx.z;
-  if (rhs_idle(a.gate, a.gdiv, rhs)) return;
+  const int ty0 = blockIdx.x * a.ty, tz0 = blockIdx.y * a.tz;
+  // right-hand sides [r0, r1) in turn; the staged image chunk serves all of
+  // them when the tile's candidates fit in one chunk
+  const int r0 = blockIdx.z * a.rg, r1 = min(r0 + a.rg, a.nrhs);
+  const int rfirst = next_live_rhs(a, r0, r1);
+  if (rfirst >= r1) return;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   const size_t qt = (size_t)im.off[tile];
   const int total = im.off[tile + 1] - (int)qt;
+  const bool one_chunk = total <= a.ch;
   // the first chunk of the tile image goes to LDS while the spectrum loads
-  mfma_stage_chunk<T, false, NC, NCP>(a, im, st, qt, min(a.ch, total), rhs);
+  mfma_stage_chunk<T, false, NC, NCP>(a, im, st, qt, min(a.ch, total), rfirst);
   const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
   for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];
-  // pack the half spectra of line pair pr (real lines 2pr, 2pr + 1) into one
-  // complex line Z = A + iB (Hermitian extension): one wave per pair, lanes
-  // along kx < N/2, GP pairs' loads in flight before their LDS writes; then
-  // the kx = N/2 entries
-  for (int p0 = wave; p0 < NPAIR; p0 += NW * GP) {
-    cplx<T> va[GP][KPL], vb[GP][KPL];
-#pragma unroll
-    for (int g = 0; g < GP; ++g) {
-      const int pr = p0 + NW * g;
-      const long long ra = pr < NPAIR ? tile_line_row(a, 2 * pr, ty0, tz0, rhs, NC) : -1;
-      const long long rb = pr < NPAIR ? tile_line_row(a, 2 * pr + 1, ty0, tz0, rhs, NC) : -1;
-#pragma unroll
-      for (int j = 0; j < KPL; ++j) {
-        const int k = lane + 64 * j;
-        va[g][j] = cmk<T>(0, 0);
-        vb[g][j] = cmk<T>(0, 0);
-        if (k < HALF) {
-          if (ra >= 0) va[g][j] = spec[ra * a.kxs + k];
-          if (rb >= 0) vb[g][j] = spec[rb * a.kxs + k];
-        }
-      }
-    }
-#pragma unroll
-    for (int g = 0; g < GP; ++g) {
-      const int pr = p0 + NW * g;
-      if (pr < NPAIR) {
-        cplx<T>* line = buf + pr * LS;
-#pragma unroll
-        for (int j = 0; j < KPL; ++j) {
-          const int k = lane + 64 * j;
-          if (k < HALF) {
-            const cplx<T> A = va[g][j], B = vb[g][j];
-            line[pidx(k)] = k == 0 ? cmk<T>(A.x, B.x) : cmk<T>(A.x - B.y, A.y + B.x);
-            if (k != 0) line[pidx(N - k)] = cmk<T>(A.x + B.y, B.x - A.y);
-          }
-        }
-      }
-    }
-  }
-  for (int pr = threadIdx.x; pr < NPAIR; pr += blockDim.x) {
-    const long long ra = tile_line_row(a, 2 * pr, ty0, tz0, rhs, NC);
-    const long long rb = tile_line_row(a, 2 * pr + 1, ty0, tz0, rhs, NC);
-    const T ax = ra >= 0 ? spec[ra * a.kxs + HALF].x : (T)0;
-    const T bx = rb >= 0 ? spec[rb * a.kxs + HALF].x : (T)0;
-    buf[pr * LS + pidx(HALF)] = cmk<T>(ax, bx);
-  }
-  __syncthreads();
-  fft_any<T, 0, N>(buf, LS, a.ncl, a.plan, tw, true, nullptr);
   const int i16 = lane & 15, k4 = lane >> 4;
   const int colrow = i16 % R;
   int colc[NB];
@@ -570,66 +553,121 @@ __global__ void __launch_bounds__(256) k_xifft_contract_mfma(TileArgs a, ImgArgs
     const int jl = min(j, NCOL - 1);
     colp[nb] = reinterpret_cast<const T*>(buf + (jl >> 1) * LS) + (jl & 1);
   }
-  T* partr = reinterpret_cast<T*>(im.part) + (size_t)rhs * im.total * NC;
-  for (int base = 0; base < total; base += a.ch) {
-    const int nc = min(a.ch, total - base);
-    if (base > 0) {
-      mfma_stage_chunk<T, false, NC, NCP>(a, im, st, qt + base, nc, rhs);
-      __syncthreads();
+  for (int rhs = rfirst; rhs < r1; rhs = next_live_rhs(a, rhs + 1, r1)) {
+    const bool first = rhs == rfirst;
+    // pack the half spectra of line pair pr (real lines 2pr, 2pr + 1) into
+    // one complex line Z = A + iB (Hermitian extension): one wave per pair,
+    // lanes along kx < N/2, GP pairs' loads in flight before their LDS
+    // writes; then the kx = N/2 entries
+    for (int p0 = wave; p0 < NPAIR; p0 += NW * GP) {
+      cplx<T> va[GP][KPL], vb[GP][KPL];
+#pragma unroll
+      for (int g = 0; g < GP; ++g) {
+        const int pr = p0 + NW * g;
+        const long long ra = pr < NPAIR ? tile_line_row(a, 2 * pr, ty0, tz0, rhs, NC) : -1;
+        const long long rb = pr < NPAIR ? tile_line_row(a, 2 * pr + 1, ty0, tz0, rhs, NC) : -1;
+#pragma unroll
+        for (int j = 0; j < KPL; ++j) {
+          const int k = lane + 64 * j;
+          va[g][j] = cmk<T>(0, 0);
+          vb[g][j] = cmk<T>(0, 0);
+          if (k < HALF) {
+            if (ra >= 0) va[g][j] = spec[ra * a.kxs + k];
+            if (rb >= 0) vb[g][j] = spec[rb * a.kxs + k];
+          }
+        }
+      }
+#pragma unroll
+      for (int g = 0; g < GP; ++g) {
+        const int pr = p0 + NW * g;
+        if (pr < NPAIR) {
+          cplx<T>* line = buf + pr * LS;
+#pragma unroll
+          for (int j = 0; j < KPL; ++j) {
+            const int k = lane + 64 * j;
+            if (k < HALF) {
+              const cplx<T> A = va[g][j], B = vb[g][j];
+              line[pidx(k)] = k == 0 ? cmk<T>(A.x, B.x) : cmk<T>(A.x - B.y, A.y + B.x);
+              if (k != 0) line[pidx(N - k)] = cmk<T>(A.x + B.y, B.x - A.y);
+            }
+          }
+        }
+      }
     }
-    T* pb = partr + (qt + base) * NC;
-    const int nblk = (nc + 15) >> 4;
-    for (int cb = wave; cb < nblk; cb += NW) {
-      const int c0 = cb * 16;
-      const int cand = c0 + i16;
-      const bool ok = cand < nc;
-      const int cq = ok ? cand : c0;
-      // a lane past the chunk's last candidate gets a first node that puts
-      // every node of the span outside its stencil
-      const int gxc = ok ? st.gx[cq] : (1 << 29);
-      const T* f = st.fw + __umul24(cq, PW);
-      // x span [xs, xe) of the block's stencils, widened to multiples of 4
-      // so that no step of four nodes straddles node N (N % 16 == 0); the
-      // added nodes lie outside every stencil of the block
-      const int xs = __builtin_amdgcn_readfirstlane(st.gx[c0]) & ~3;
-      const int xe = (__builtin_amdgcn_readfirstlane(st.gx[min(c0 + 15, nc - 1)]) + P + 3) & ~3;
-      acc_t acc[NB];
-#pragma unroll
-      for (int nb = 0; nb < NB; ++nb) acc[nb] = acc_t{0, 0, 0, 0};
-      // nodes below N, then the ones that wrap to x - N (xe - N < N)
-#pragma unroll
-      for (int sg = 0; sg < 2; ++sg) {
-        const int lo = sg == 0 ? xs : N;
-        const int hi = sg == 0 ? min(xe, N) : xe;
-        const int wrap = sg == 0 ? 0 : N;
-        for (int xx = lo; xx < hi; xx += 4) {
-          const int xk = xx + k4;
-          // o outside [0, P) (negative ones wrap to large unsigned values)
-          // reads the stencil's last factor and selects 0
-          const int o = xk - gxc;
-          const T fv = f[min((unsigned)o, (unsigned)(P - 1))];
-          const T av = (unsigned)o < (unsigned)P ? fv : (T)0;
-          const int px = 2 * pidx(xk - wrap);
-#pragma unroll
-          for (int nb = 0; nb < NB; ++nb) acc[nb] = MF::mac(av, colp[nb][px], acc[nb]);
-        }
-      }
-      // out[q][c] = sum_row w_q[row] D[q][(c, row)]: weight, then sum the R
-      // lanes of each component
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int cr = c0 + MF::row(lane, r);
-        const bool okr = cr < nc;
-        const int crq = okr ? cr : c0;
-        const T wq = st.fw[__umul24(crq, PW) + P + colrow];
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-          const T v = group_sum<T>(acc[nb][r] * wq, R);
-          if (colc[nb] < NC && colrow == 0 && okr) pb[cr * NC + colc[nb]] = v;
-        }
-      }
+    for (int pr = threadIdx.x; pr < NPAIR; pr += blockDim.x) {
+      const long long ra = tile_line_row(a, 2 * pr, ty0, tz0, rhs, NC);
+      const long long rb = tile_line_row(a, 2 * pr + 1, ty0, tz0, rhs, NC);
+      const T ax = ra >= 0 ? spec[ra * a.kxs + HALF].x : (T)0;
+      const T bx = rb >= 0 ? spec[rb * a.kxs + HALF].x : (T)0;
+      buf[pr * LS + pidx(HALF)] = cmk<T>(ax, bx);
     }
     __syncthreads();
+    fft_any<T, 0, N>(buf, LS, a.ncl, a.plan, tw, true, nullptr);
+    T* partr = reinterpret_cast<T*>(im.part) + (size_t)rhs * im.total * NC;
+    for (int base = 0; base < total; base += a.ch) {
+      const int nc = min(a.ch, total - base);
+      if (base > 0 || (!first && !one_chunk)) {
+        mfma_stage_chunk<T, false, NC, NCP>(a, im, st, qt + base, nc, rhs);
+        __syncthreads();
+      }
+      T* pb = partr + (qt + base) * NC;
+      const int nblk = (nc + 15) >> 4;
+      for (int cb = wave; cb < nblk; cb += NW) {
+        const int c0 = cb * 16;
+        const int cand = c0 + i16;
+        const bool ok = cand < nc;
+        const int cq = ok ? cand : c0;
+        // a lane past the chunk's last candidate gets a first node that puts
+        // every node of the span outside its stencil
+        const int gxc = ok ? st.gx[cq] : (1 << 29);
+        const T* f = st.fw + __umul24(cq, PW);
+        // x span [xs, xe) of the block's stencils, widened to multiples of 4
+        // so that no step of four nodes straddles node N (N % 16 == 0); the
+        // added nodes lie outside every stencil of the block
+        const int xs = __builtin_amdgcn_readfirstlane(st.gx[c0]) & ~3;
+        const int xe = (__builtin_amdgcn_readfirstlane(st.gx[min(c0 + 15, nc - 1)]) + P + 3) & ~3;
+        acc_t acc[NB];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) acc[nb] = acc_t{0, 0, 0, 0};
+        // nodes below N, then the ones that wrap to x - N (xe - N < N)
+#pragma unroll
+        for (int sg = 0; sg < 2; ++sg) {
+          const int lo = sg == 0 ? xs : N;
+          const int hi = sg == 0 ? min(xe, N) : xe;
+          const int wrap = sg == 0 ? 0 : N;
+          for (int xx = lo; xx < hi; xx += 4) {
+            const int xk = xx + k4;
+            // o outside [0, P) (negative ones wrap to large unsigned values)
+            // reads the stencil's last factor and selects 0
+            const int o = xk - gxc;
+            const T fv = f[min((unsigned)o, (unsigned)(P - 1))];
+            const T av = (unsigned)o < (unsigned)P ? fv : (T)0;
+            const int px = 2 * pidx(xk - wrap);
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) acc[nb] = MF::mac(av, colp[nb][px], acc[nb]);
+          }
+        }
+        // out[q][c] = sum_row w_q[row] D[q][(c, row)]: weight, then sum the R
+        // lanes of each component
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int cr = c0 + MF::row(lane, r);
+          const bool okr = cr < nc;
+          const int crq = okr ? cr : c0;
+          const T wq = st.fw[__umul24(crq, PW) + P + colrow];
+#pragma unroll
+          for (int nb = 0; nb < NB; ++nb) {
+            const T v = group_sum<T>(acc[nb][r] * wq, R);
+            if (colc[nb] < NC && colrow == 0 && okr) pb[cr * NC + colc[nb]] = v;
+          }
+        }
+      }
+      // the lines and the staged chunk are read until here
+      __syncthreads();
+    }
+    // the next right-hand side writes the lines (a tile without candidates
+    // leaves the chunk loop out)
+    if (total == 0) __syncthreads();
   }
 }
 
