@@ -3,7 +3,7 @@ context and applies the dipole operator a few times.
 
     python -m workloads.operator_loop [--workload n1e5_phi0.4] [--precision fp32|fp64]
                                       [--reps 3] [--nrhs 1] [--engine 0|1|2] [--sweeps 20]
-                                      [--copy-mb 256]
+                                      [--chunk 0] [--rhs-group 0] [--copy-mb 256]
 
 (run from mutual-polarization-slab_amd/, or with it on sys.path).  Prints
 one line per step so a profiler run shows progress.
@@ -27,6 +27,8 @@ def main(argv=None):
     ap.add_argument("--nrhs", type=int, default=1)
     ap.add_argument("--engine", type=int, default=0)
     ap.add_argument("--sweeps", type=int, default=20)
+    ap.add_argument("--chunk", type=int, default=0, help="particles staged per chunk (0 = automatic)")
+    ap.add_argument("--rhs-group", type=int, default=0, help="right-hand sides per tile workgroup (0 = automatic)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="first copy a float32 device buffer of this many MiB (counter calibration)")
     args = ap.parse_args(argv)
@@ -43,7 +45,7 @@ def main(argv=None):
     ctx.set_tables(perp, para, rv)
     ctx.set_geometry(cfg["x"], cfg["box"], cfg["ngrid"], cfg["P"], cfg["xi"], cfg["eta"], cfg["rc"])
     ctx.set_permittivity(cfg["eps"])
-    ctx.set_tuning(engine=args.engine)
+    ctx.set_tuning(engine=args.engine, chunk=args.chunk, rhs_group=args.rhs_group)
     prec = FP32 if args.precision == "fp32" else FP64
     if args.copy_mb > 0:
         src = torch.ones(args.copy_mb << 18, dtype=torch.float32, device="cuda")
