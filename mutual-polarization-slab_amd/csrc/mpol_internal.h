@@ -758,8 +758,11 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
     constexpr int TXR = sizeof(T) == 4 ? 16 : 8;
     ColArgs ar = az;
     ar.tx = TXR;
-    hipLaunchKernelGGL((k_fft_z_scale_reg256<T, TXR>), dim3(cdiv(c->kx, TXR), c->ng[1], nrhs), dim3(TXR * 16), 0, st,
-                       ar, sp, spec);
+    const dim3 rgrid(cdiv(c->kx, TXR), c->ng[1], nrhs);
+    if constexpr (sizeof(T) == 4)
+      hipLaunchKernelGGL((k_fft_z_scale_reg256<T, TXR>), rgrid, dim3(TXR * 16), 0, st, ar, sp, spec);
+    else
+      hipLaunchKernelGGL((k_fft_z_scale_reg256_pair<T, TXR>), rgrid, dim3(TXR * 32), 0, st, ar, sp, spec);
   } else {
     DISPATCH_FFT(c->ng[2], az.plan.e, LAUNCH_FFTZ, 0);
   }

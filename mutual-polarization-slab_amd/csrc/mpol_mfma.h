@@ -348,9 +348,12 @@ __device__ __forceinline__ int next_live_rhs(const TileArgs& a, int r0, int r1) 
 // block (their x factor selects to 0) or the zero-filled pad past the chunk.
 // When a wave's accumulators for all its x blocks fit in 64 registers they
 // stay in registers across chunks and are stored into the tile lines once;
-// otherwise each chunk adds its accumulators into the (cleared) lines.
+// otherwise each chunk adds its accumulators into the (cleared) lines.  The
+// register budget is three FP32 / two FP64 workgroups per CU, what the LDS of
+// a 256 (FP32) / 128 (FP64) candidate chunk allows.
 template <typename T, int N, int NC, int R>
-__global__ void __launch_bounds__(256) k_spread_mfma_xfft(TileArgs a, ImgArgs im, cplx<T>* __restrict__ spec) {
+__global__ void __launch_bounds__(256, sizeof(T) == 4 ? 3 : 2)
+    k_spread_mfma_xfft(TileArgs a, ImgArgs im, cplx<T>* __restrict__ spec) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   using MF = Mfma<T>;
   using acc_t = typename MF::acc_t;
@@ -512,7 +515,8 @@ __global__ void __launch_bounds__(256) k_spread_mfma_xfft(TileArgs a, ImgArgs im
 // image slot q (k_contract_reduce sums each particle's partials).  Needs 256
 // threads (four waves) and lines of padded_len(N) entries.
 template <typename T, int N, int NC, int R>
-__global__ void __launch_bounds__(256) k_xifft_contract_mfma(TileArgs a, ImgArgs im, const cplx<T>* __restrict__ spec) {
+__global__ void __launch_bounds__(256, sizeof(T) == 4 ? 1 : 2)
+    k_xifft_contract_mfma(TileArgs a, ImgArgs im, const cplx<T>* __restrict__ spec) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   using MF = Mfma<T>;
   using acc_t = typename MF::acc_t;

@@ -2,15 +2,23 @@
 // transforms in registers (gfx950).
 //
 // Four-step 16 x 16 decomposition, z = n1 + 16 n2, k = k2 + 16 k1:
-//   Y[n1][k2] = W_256^(n1 k2) sum_n2 x[n1 + 16 n2] W_16^(n2 k2)   (thread (col, n1))
-//   X[k2 + 16 k1] = sum_n1 Y[n1][k2] W_16^(n1 k1)                   (thread (col, k2))
+//   Y[n1][k2] = W_256^(n1 k2) sum_n2 x[n1 + 16 n2] W_16^(n2 k2)   (lane pair (col, n1))
+//   X[k2 + 16 k1] = sum_n1 Y[n1][k2] W_16^(n1 k1)                   (lane pair (col, k2))
 // with one LDS transpose between the two 16-point DFTs; the inverse runs the
-// same steps backwards, so a thread ends holding the z values it loaded.
-// A thread keeps the three dipole components of its 16 values, so the scale
-// G = c(k) M(k^) F needs no exchange, and one TX x 256 transpose buffer is
-// reused by the components: with TX = 16 (FP32) / 8 (FP64) kx columns per
-// workgroup every load and store is a 128-byte segment, at about a third of
-// the LDS of the LDS-resident z pass.
+// same steps backwards.  A lane (k_fft_z_scale_reg256) or a lane pair
+// (k_fft_z_scale_reg256_pair) keeps the three dipole components of its 16
+// points, so the scale G = c(k) M(k^) F needs no exchange, and one TX x 256
+// transpose buffer is reused by the components: with TX = 16 (FP32) / 8
+// (FP64) kx columns per workgroup every load and store is a 128-byte
+// segment.  In the pair kernel each 16-point DFT is split over neighbouring
+// lanes: lane h holds inputs 8h .. 8h + 7, the pair exchanges halves with a
+// DPP swap, the lanes finish the radix-2 step of their half and an 8-point
+// DFT, and lane h ends with outputs 2k + h (the arithmetic of dft16, so both
+// kernels give the same results).  Eight points per lane halve the
+// registers: FP64 went from one wave per SIMD (344 registers) to two, 0.30 ->
+// 0.25 ms at 256^3; FP32, at two waves with sixteen points per lane, stays
+// with one lane per DFT (the pair kernel's exchanges and selects cost more
+// issue than its occupancy returns: 0.139 against 0.128 ms).
 #pragma once
 
 #include "mpol_kernels.h"
@@ -131,6 +139,155 @@ __global__ void __launch_bounds__(TX * 16) k_fft_z_scale_reg256(ColArgs a, Scale
       cplx<T>* dst = spec + (size_t)(rhs * NC + c) * a.nz * plane + (size_t)y * a.kxs + kx;
 #pragma unroll
       for (int n2 = 0; n2 < 16; ++n2) dst[(size_t)(j + 16 * n2) * plane] = v[c][n2];
+    }
+  }
+}
+
+// The value of the neighbouring lane (lane ^ 1).
+template <typename T>
+__device__ __forceinline__ cplx<T> lane_swap1(cplx<T> v) {
+  return cmk<T>(dpp_mov<kDppSwap1>(v.x), dpp_mov<kDppSwap1>(v.y));
+}
+
+// 16-point DFT over a lane pair (h = lane & 1) with the arithmetic of dft16:
+// lane h holds inputs 8h .. 8h + 7 in v[0..7] and returns outputs 2k + h in
+// v[k].  Lane 0 forms the sums of dft16's radix-2 step (its even outputs),
+// lane 1 the twiddled differences (its odd outputs).
+template <typename T>
+__device__ __forceinline__ void dft16_pair(cplx<T>* v, int h) {
+  const T c1 = (T)FftConst<T>::c161, s1 = (T)FftConst<T>::s161, r2 = (T)FftConst<T>::r2;
+  cplx<T> o[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) o[k] = lane_swap1(v[k]);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) v[k] = h == 0 ? cadd(v[k], o[k]) : csub(o[k], v[k]);
+  // w16^k = cos(pi k/8) - i sin(pi k/8) on the differences
+  const cplx<T> d1 = cmul(v[1], cmk(c1, -s1));
+  const cplx<T> d2 = cmul(v[2], cmk(r2, -r2));
+  const cplx<T> d3 = cmul(v[3], cmk(s1, -c1));
+  const cplx<T> d4 = cmul_mi(v[4]);
+  const cplx<T> d5 = cmul(v[5], cmk(-s1, -c1));
+  const cplx<T> d6 = cmul(v[6], cmk(-r2, -r2));
+  const cplx<T> d7 = cmul(v[7], cmk(-c1, -s1));
+  if (h) {
+    v[1] = d1;
+    v[2] = d2;
+    v[3] = d3;
+    v[4] = d4;
+    v[5] = d5;
+    v[6] = d6;
+    v[7] = d7;
+  }
+  dft8(v);
+}
+
+template <typename T>
+__device__ __forceinline__ void idft16_pair(cplx<T>* v, int h) {
+#pragma unroll
+  for (int i = 0; i < 8; ++i) v[i] = cconj(v[i]);
+  dft16_pair<T>(v, h);
+#pragma unroll
+  for (int i = 0; i < 8; ++i) v[i] = cconj(v[i]);
+}
+
+// Lane pair reorder: lane h holds x[2k + h] in v[k]; on return x[8h + k].
+// Lane 0 keeps x[0, 2, 4, 6] and takes x[1, 3, 5, 7] from lane 1; lane 1
+// keeps x[9, 11, 13, 15] and takes x[8, 10, 12, 14] from lane 0.  Both
+// halves are swapped and the lane selects among values, never between two
+// entries of v (a select of array entries would become a computed index and
+// put the array in scratch memory).
+template <typename T>
+__device__ __forceinline__ void pair_to_halves(cplx<T>* v, int h) {
+  cplx<T> lo[4], hi[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    lo[i] = lane_swap1(v[i]);      // partner's x[2i + 1] (read by lane 0)
+    hi[i] = lane_swap1(v[4 + i]);  // partner's x[8 + 2i] (read by lane 1)
+  }
+  cplx<T> w[8];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const cplx<T> own_lo = v[i], own_hi = v[4 + i];
+    w[2 * i] = h == 0 ? own_lo : hi[i];
+    w[2 * i + 1] = h == 0 ? lo[i] : own_hi;
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) v[k] = w[k];
+}
+
+// Lane (col, j, h): col = (threadIdx.x / 2) % TX, j = (threadIdx.x / 2) / TX,
+// h = threadIdx.x % 2; the pair (col, j) does the work of one 16-point lane
+// of k_fft_z_scale_reg256, with the same results.
+template <typename T, int TX>
+__global__ void __launch_bounds__(TX * 32) k_fft_z_scale_reg256_pair(ColArgs a, ScaleParams sp, cplx<T>* __restrict__ spec) {
+  constexpr int N = 256, NC = 3, LS = 273;  // LS: padded line of the transpose buffer
+  __shared__ cplx<T> tw[N];
+  __shared__ cplx<T> xb[TX * LS];
+  const int h = threadIdx.x & 1, col = (threadIdx.x >> 1) % TX, j = (threadIdx.x >> 1) / TX;
+  const int kx = blockIdx.x * TX + col, y = blockIdx.y, rhs = blockIdx.z;
+  if (rhs_idle(a.gate, a.gdiv, rhs)) return;
+  const bool valid = kx < a.kx;
+  const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
+  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];
+  const size_t plane = (size_t)a.ny * a.kxs;
+  cplx<T> v[NC][8];
+  // load z = j + 16 n2, n2 = 8h + t, of the three components (all loads in flight together)
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const cplx<T>* src = spec + (size_t)(rhs * NC + c) * a.nz * plane + (size_t)y * a.kxs + kx;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) v[c][t] = valid ? src[(size_t)(j + 16 * (8 * h + t)) * plane] : cmk<T>(0, 0);
+  }
+  cplx<T>* line = xb + col * LS;
+  __syncthreads();  // twiddles
+  // forward: DFT over n2, twiddle, transpose, DFT over n1
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    dft16_pair<T>(v[c], h);  // v[k] = Y[n1 = j][k2 = 2k + h] before the twiddle
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int k2 = 2 * k + h;
+      const cplx<T> t = cmul(v[c][k], tw[(j * k2) & (N - 1)]);
+      if (k2 != 0) v[c][k] = t;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) line[pidx((2 * k + h) * 16 + j)] = v[c][k];
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 8; ++t) v[c][t] = line[pidx(j * 16 + 8 * h + t)];
+    __syncthreads();
+    dft16_pair<T>(v[c], h);  // lane pair (col, k2 = j) now holds X[j + 16 k1], k1 = 2k + h
+  }
+  if (valid) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      dipole_scale_point<T>(a, sp, kx, y, j + 16 * (2 * k + h), v[0][k], v[1][k], v[2][k]);
+  }
+  // inverse: DFT over k1, conjugate twiddle, transpose, DFT over k2
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    pair_to_halves<T>(v[c], h);  // k1 = 8h + k
+    idft16_pair<T>(v[c], h);     // index n1 = 2k + h
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int n1 = 2 * k + h;
+      const cplx<T> t = cmul(v[c][k], cconj(tw[(j * n1) & (N - 1)]));
+      if (n1 != 0) v[c][k] = t;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) line[pidx((2 * k + h) * 16 + j)] = v[c][k];
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 8; ++t) v[c][t] = line[pidx(j * 16 + 8 * h + t)];
+    __syncthreads();
+    idft16_pair<T>(v[c], h);  // lane pair (col, n1 = j) now holds x[j + 16 n2], n2 = 2k + h
+  }
+  if (valid) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      cplx<T>* dst = spec + (size_t)(rhs * NC + c) * a.nz * plane + (size_t)y * a.kxs + kx;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) dst[(size_t)(j + 16 * (2 * k + h)) * plane] = v[c][k];
     }
   }
 }
