@@ -643,19 +643,29 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
     mgrid = dim3(cdiv(c->ng[1], tm.ty), cdiv(c->ng[2], tm.tz), cdiv(nrhs, tma.rg));
   }
 
+  // the spread's staged chunk shares LDS with the tile lines when a
+  // workgroup serves one right-hand side and keeps its accumulators in
+  // registers (with more, aliasing would restage the image per right-hand
+  // side, which the groups exist to avoid)
+  size_t spread_lds = tm.lds;
+  tma.alias = 0;
+  if (use_mfma && tma.rg == 1 && mfma_reg_acc(c->ng[0], NC, tm.rows, sizeof(T))) {
+    tma.alias = 1;
+    spread_lds = mfma_tile_lds_alias<NC>(c->ng[0], cs, c->P, tm.ncl, tm.lstride, tm.ch, tm.rows);
+  }
   // FP32: atomic-free gather spread; FP64: LDS FP64 atomics (faster there)
   const bool gather = sizeof(T) == 4 && is_ct_size(c->ng[0]) && c->P + 3 <= c->ng[0];
   if (use_mfma) {
 #define LAUNCH_SPREAD_MFMA(N, ...)                                                              \
   do {                                                                                          \
     if (tm.rows == 16) {                                                                        \
-      launch_mfma16<T, N, NC>(true, mgrid, tm.lds, st, tma, ima, spec);                         \
+      launch_mfma16<T, N, NC>(true, mgrid, spread_lds, st, tma, ima, spec);                     \
     } else if (tm.rows == 8) {                                                                  \
-      set_lds(k_spread_mfma_xfft<T, N, NC, 8>, tm.lds);                                         \
-      hipLaunchKernelGGL((k_spread_mfma_xfft<T, N, NC, 8>), mgrid, dim3(256), tm.lds, st, tma, ima, spec); \
+      set_lds(k_spread_mfma_xfft<T, N, NC, 8>, spread_lds);                                    \
+      hipLaunchKernelGGL((k_spread_mfma_xfft<T, N, NC, 8>), mgrid, dim3(256), spread_lds, st, tma, ima, spec); \
     } else {                                                                                    \
-      set_lds(k_spread_mfma_xfft<T, N, NC, 4>, tm.lds);                                         \
-      hipLaunchKernelGGL((k_spread_mfma_xfft<T, N, NC, 4>), mgrid, dim3(256), tm.lds, st, tma, ima, spec); \
+      set_lds(k_spread_mfma_xfft<T, N, NC, 4>, spread_lds);                                    \
+      hipLaunchKernelGGL((k_spread_mfma_xfft<T, N, NC, 4>), mgrid, dim3(256), spread_lds, st, tma, ima, spec); \
     }                                                                                           \
   } while (0)
     DISPATCH_CT(c->ng[0], LAUNCH_SPREAD_MFMA, 0);

@@ -546,6 +546,7 @@ struct TileArgs {
   int n;             // particles
   int nrhs;
   int rg;            // right-hand sides per workgroup (matrix-core spread/contract)
+  int alias;         // matrix-core spread: the staged chunk shares LDS with the tile lines
   const int4* rec;   // sorted: first stencil node x, y, z; caller's index
   const void* base;  // sorted: node0*h - x per axis (double4 or float4)
   const double* ms;  // sorted dipoles [nrhs][n][3]

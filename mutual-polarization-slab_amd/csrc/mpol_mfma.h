@@ -207,17 +207,6 @@ __global__ void __launch_bounds__(256) k_img_fill(TileArgs a, const int* __restr
   }
 }
 
-// First index in the sorted s_gx[0, n) whose value is >= v.
-__device__ inline int gx_lower_bound(const int* s_gx, int n, int v) {
-  int lo = 0, hi = n;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (s_gx[mid] < v) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
-
 // Spectrum row of real line l = c*rows + row of tile (ty0, tz0) for
 // right-hand side rhs, or -1 when the row lies outside the grid (tiles at the
 // upper y/z edges).  rows, ty and tz are powers of two.
@@ -230,9 +219,10 @@ __device__ __forceinline__ long long tile_line_row(const TileArgs& a, int l, int
 }
 
 // Candidate entries past a chunk's end that the stage fills with zeros: the
-// spread's MFMA steps take four candidates at a time and read up to three
-// entries past the last one without a range check.
-constexpr int kStagePad = 4;
+// spread's MFMA steps take four candidates at a time, read up to three
+// entries past the last one without a range check, and load the next step's
+// four candidates ahead (up to seven entries past the last one).
+constexpr int kStagePad = 8;
 
 // Moments per staged candidate: NC, zero-padded to the components of the
 // NB = ceil(NC*R/16) column blocks, so that a column past the last line
@@ -261,6 +251,55 @@ template <int NC>
 inline size_t mfma_tile_lds(int nx, size_t csize, int P, int ncl, int lstride, int ch, int rows) {
   return ((size_t)nx + (size_t)ncl * lstride) * csize +
          (size_t)(ch + kStagePad) * ((P + rows + mfma_ncp(NC, rows)) * (csize / 2) + sizeof(int));
+}
+
+// LDS bytes of the matrix-core spread when the staged chunk lies over the
+// tile lines (TileArgs::alias): twiddles, then the larger of the two.
+template <int NC>
+inline size_t mfma_tile_lds_alias(int nx, size_t csize, int P, int ncl, int lstride, int ch, int rows) {
+  const size_t lines = (size_t)ncl * lstride * csize;
+  const size_t stage = (size_t)(ch + kStagePad) * ((P + rows + mfma_ncp(NC, rows)) * (csize / 2) + sizeof(int));
+  return (size_t)nx * csize + (lines > stage ? lines : stage);
+}
+
+// Whether k_spread_mfma_xfft keeps a wave's accumulators in registers across
+// chunks (its kRegAcc): the x blocks of a wave (four waves) times the column
+// blocks, four values each, in at most 64 registers.
+inline bool mfma_reg_acc(int nx, int nc, int rows, size_t tbytes) {
+  const int xpw = (nx / 16 + 3) / 4, nb = (nc * rows + 15) / 16;
+  return (size_t)xpw * nb * 4 * tbytes <= 256;
+}
+
+// Candidate ranges of every 16-node x block of a staged chunk (candidates
+// sorted by first node gx): block xb needs the candidates with gx in
+// [16 xb - P + 1, 16 xb + 15] (mod N), one range r[0..1], or two when the
+// window wraps below node 0: [r[0], nc) for gx >= 16 xb - P + 1 + N, then
+// [r[2] = 0, r[3]).  Every bound is lower_bound(v) = #{u : gx[u] < v}, which
+// is the u with gx[u - 1] < v <= gx[u] (gx[-1] = -1, gx[nc] = N): thread u
+// writes the bounds whose v falls in its interval, so each bound is written
+// once, without a search.  gs: the chunk's image records.
+template <int N>
+__device__ inline void mfma_chunk_ranges(const int2* __restrict__ gs, int nc, int P, int* s_xr) {
+  constexpr int NXB = N / 16;
+  const int nwrap = (P - 1 + 15) >> 4;  // blocks with 16 xb - P + 1 < 0
+  for (int xb = threadIdx.x; xb < NXB; xb += blockDim.x) {
+    int* r = s_xr + 4 * xb;
+    r[2] = 0;
+    if (xb < nwrap) r[1] = nc;
+    else r[3] = 0;
+  }
+  for (int u = threadIdx.x; u <= nc; u += blockDim.x) {
+    const int lo = u > 0 ? gs[u - 1].x : -1;  // this thread's bounds: v in (lo, hi]
+    const int hi = u < nc ? gs[u].x : N;
+    // upper bounds v = 16 (xb + 1)
+    for (int xb = max(lo >> 4, 0); xb <= min((hi >> 4) - 1, NXB - 1); ++xb) s_xr[4 * xb + (xb < nwrap ? 3 : 1)] = u;
+    // lower bounds v = 16 xb - P + 1 (blocks xb >= nwrap)
+    for (int xb = max(((lo + P - 1) >> 4) + 1, nwrap); xb <= min((hi + P - 1) >> 4, NXB - 1); ++xb)
+      s_xr[4 * xb] = u;
+    // wrapped lower bounds v = 16 xb - P + 1 + N (blocks xb < nwrap)
+    for (int xb = max(((lo + P - 1 - N) >> 4) + 1, 0); xb <= min((hi + P - 1 - N) >> 4, nwrap - 1); ++xb)
+      s_xr[4 * xb] = u;
+  }
 }
 
 // Copy chunk [q, q + nc) of the tile image into LDS (IMAGE); the spread also
@@ -334,6 +373,14 @@ __device__ __forceinline__ int next_live_rhs(const TileArgs& a, int r0, int r1) 
   return r;
 }
 
+// Workgroups per CU that k_spread_mfma_xfft's registers are budgeted for:
+// four for FP32 lines up to 256 nodes and at most 24 tile lines (the LDS of an
+// aliased 256-candidate chunk allows four), three for longer FP32 lines or more
+// lines (whose accumulators and line FFTs would spill at four), two for FP64.
+__host__ __device__ constexpr int spread_mfma_min_blocks(size_t tbytes, int n, int ncol) {
+  return tbytes == 4 ? ((n <= 256 && ncol <= 24) ? 4 : 3) : 2;
+}
+
 // Spread + forward x-FFT.  Wave w handles x blocks w, w + 4, ...; lane l of a
 // block holds node x0 + l % 16 (A rows) and columns (c, row) = nb*16 + l % 16
 // of the tile lines (column index = line index c*R + row).  R (tile rows) is
@@ -348,11 +395,13 @@ __device__ __forceinline__ int next_live_rhs(const TileArgs& a, int r0, int r1) 
 // block (their x factor selects to 0) or the zero-filled pad past the chunk.
 // When a wave's accumulators for all its x blocks fit in 64 registers they
 // stay in registers across chunks and are stored into the tile lines once;
-// otherwise each chunk adds its accumulators into the (cleared) lines.  The
-// register budget is three FP32 / two FP64 workgroups per CU, what the LDS of
-// a 256 (FP32) / 128 (FP64) candidate chunk allows.
+// otherwise each chunk adds its accumulators into the (cleared) lines.  With
+// a.alias (register accumulators, set by the host) the staged chunk lies over
+// the tile lines, which are written only after the last chunk's steps, and
+// every right-hand side restages the image.  Registers are budgeted by
+// spread_mfma_min_blocks.
 template <typename T, int N, int NC, int R>
-__global__ void __launch_bounds__(256, sizeof(T) == 4 ? 3 : 2)
+__global__ void __launch_bounds__(256, spread_mfma_min_blocks(sizeof(T), N, NC * R))
     k_spread_mfma_xfft(TileArgs a, ImgArgs im, cplx<T>* __restrict__ spec) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   using MF = Mfma<T>;
@@ -365,7 +414,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 3 : 2)
   cplx<T>* buf = tw + N;
   const int P = a.P, PW = P + R;
   const MfmaStage<T> st =
-      mfma_stage_layout<T>(reinterpret_cast<unsigned char*>(buf + a.ncl * LS), a.ch, PW, NCP);
+      mfma_stage_layout<T>(reinterpret_cast<unsigned char*>(a.alias ? buf : buf + a.ncl * LS), a.ch, PW, NCP);
   __shared__ int s_xr[4 * NXB];
   const int tile = blockIdx.x + blockIdx.y * gridDim.x;
   const int ty0 = blockIdx.x * a.ty, tz0 = blockIdx.y * a.tz;
@@ -379,7 +428,10 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 3 : 2)
   const int total = im.off[tile + 1] - (int)qt;
   const bool one_chunk = total <= a.ch;
   // the first chunk of the tile image goes to LDS with the twiddles
-  if (total > 0) mfma_stage_chunk<T, true, NC, NCP>(a, im, st, qt, min(a.ch, total), rfirst);
+  if (total > 0) {
+    mfma_stage_chunk<T, true, NC, NCP>(a, im, st, qt, min(a.ch, total), rfirst);
+    mfma_chunk_ranges<N>(im.gs + qt, min(a.ch, total), P, s_xr);
+  }
   const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
   for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];
   const int lane = threadIdx.x & 63;
@@ -418,23 +470,11 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 3 : 2)
     for (int base = 0; base < total; base += a.ch) {
       const int nc = min(a.ch, total - base);
       if (!(first && base == 0)) {
-        mfma_stage_chunk<T, true, NC, NCP>(a, im, st, qt + base, nc, rhs, !one_chunk);
-        __syncthreads();
-      }
-      if (first || !one_chunk) {
-        // candidate ranges of every x block: gx in [x0 - P + 1, x0 + 15]
-        // (mod nx), one range, or two when the window wraps below 0 (the
-        // wrapped one first)
-        for (int xb = threadIdx.x; xb < NXB; xb += blockDim.x) {
-          const int x0 = xb * 16, lo = x0 - P + 1;
-          const int hi_idx = gx_lower_bound(st.gx, nc, x0 + 16);
-          int* r = s_xr + 4 * xb;
-          if (lo >= 0) {
-            r[0] = gx_lower_bound(st.gx, nc, lo); r[1] = hi_idx; r[2] = 0; r[3] = 0;
-          } else {
-            r[0] = gx_lower_bound(st.gx, nc, lo + N); r[1] = nc; r[2] = 0; r[3] = hi_idx;
-          }
-        }
+        // the candidate ranges depend only on the image, so a restage of the
+        // moments alone keeps them
+        const bool image = !one_chunk || a.alias;
+        mfma_stage_chunk<T, true, NC, NCP>(a, im, st, qt + base, nc, rhs, image);
+        if (image) mfma_chunk_ranges<N>(im.gs + qt + base, nc, P, s_xr);
         __syncthreads();
       }
 #pragma unroll
@@ -451,18 +491,35 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 3 : 2)
             const int qb = __builtin_amdgcn_readfirstlane(s_xr[4 * xb + 2 * sg + 1]);
             // node offset o = x - gx, plus N for the candidates that wrap below node 0
             const int xo = x0 + i16 + ((sg == 0 && x0 - P + 1 < 0) ? N : 0);
-            for (int q = qa; q < qb; q += 4) {
-              const int cand = q + k4;
-              const int o = xo - st.gx[cand];
-              const T* f = st.fw + __umul24(cand, PW);
-              // o outside [0, P) (negative ones wrap to large unsigned values)
-              // reads the stencil's last factor and selects 0
-              const T fv = f[min((unsigned)o, (unsigned)(P - 1))];
-              const T av = (unsigned)o < (unsigned)P ? fv : (T)0;
-              const T wv = f[P + colrow];
-              const T* mq = st.m + cand * NCP + colm;
+            if (qa < qb) {
+              // the first node, row weight and moments of each step's
+              // candidate are loaded one step ahead, so a step waits only for
+              // its x factor (the stage is zero-padded for the look-ahead)
+              int cn = qa + k4;
+              int gn = st.gx[cn];
+              T wn = st.fw[__umul24(cn, PW) + P + colrow];
+              T mn[NB];
 #pragma unroll
-              for (int nb = 0; nb < NB; ++nb) acc[i][nb] = MF::mac(av, wv * mq[nb * 16 / R], acc[i][nb]);
+              for (int nb = 0; nb < NB; ++nb) mn[nb] = st.m[cn * NCP + colm + nb * 16 / R];
+              for (int q = qa; q < qb; q += 4) {
+                const int cand = cn;
+                const int o = xo - gn;
+                const T wv = wn;
+                T mq[NB];
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) mq[nb] = mn[nb];
+                cn = cand + 4;
+                gn = st.gx[cn];
+                wn = st.fw[__umul24(cn, PW) + P + colrow];
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) mn[nb] = st.m[cn * NCP + colm + nb * 16 / R];
+                // o outside [0, P) (negative ones wrap to large unsigned
+                // values) reads the stencil's last factor and selects 0
+                const T fv = st.fw[__umul24(cand, PW) + min((unsigned)o, (unsigned)(P - 1))];
+                const T av = (unsigned)o < (unsigned)P ? fv : (T)0;
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) acc[i][nb] = MF::mac(av, wv * mq[nb], acc[i][nb]);
+              }
             }
           }
           if (!kRegAcc) store_block(x0, acc[i], true);
