@@ -614,8 +614,21 @@ int mpol_set_geometry(mpol_ctx* c, const double* x, int64_t n, const double box[
         kt.push_back(j1 * j2 / kk);
         kt.push_back(j2 * j2 / kk);
       }
+      // the dipole table (after the three separable ones) over |k|^2, for
+      // the rank-one register z pass
+      c->jk_off = (int64_t)kt.size();
+      const size_t jt_off = (size_t)c->ng[0] + c->ng[1] + c->ng[2];
+      for (int64_t q = 0; q <= nmax; ++q) kt.push_back(q == 0 ? 0.0 : kt[jt_off + (size_t)q] / (dk * dk * (double)q));
+      // FP32 copies of e_z and that table
+      std::vector<float> kt32;
+      const size_t ez_off = (size_t)c->ng[0] + c->ng[1];
+      for (int i = 0; i < c->ng[2]; ++i) kt32.push_back((float)kt[ez_off + i]);
+      for (int64_t q = 0; q <= nmax; ++q) kt32.push_back((float)kt[(size_t)c->jk_off + (size_t)q]);
+      HIPALLOC(c->ktab32, kt32.size());
+      HIPCHK(hipMemcpy(c->ktab32.p, kt32.data(), kt32.size() * sizeof(float), hipMemcpyHostToDevice));
     } else {
       c->jq_off = -1;
+      c->jk_off = -1;
     }
     HIPALLOC(c->ktab, kt.size());
     HIPCHK(hipMemcpyAsync(c->ktab.p, kt.data(), kt.size() * sizeof(double), hipMemcpyHostToDevice, st));
@@ -1086,7 +1099,7 @@ int mpol_set_tuning_for(mpol_ctx* c, int32_t precision, const int32_t* knobs, in
   if (!c || (nknobs > 0 && !knobs) || nknobs < 0 || nknobs > kTuneKnobs) return fail(MPOL_EINVAL, "bad tuning arguments");
   if (precision != MPOL_FP32 && precision != MPOL_FP64 && precision != -1)
     return fail(MPOL_EINVAL, "precision must be MPOL_FP32, MPOL_FP64 or -1 (both)");
-  int t[kTuneKnobs] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int t[kTuneKnobs] = {};
   for (int i = 0; i < nknobs; ++i) t[i] = knobs[i];
   const bool pow2_rows = (t[0] == 0 && t[1] == 0) ||
                          (t[0] >= 1 && t[0] <= 4 && t[1] >= 1 && t[1] <= 4 && ((t[0] * t[1]) & (t[0] * t[1] - 1)) == 0);
@@ -1097,6 +1110,7 @@ int mpol_set_tuning_for(mpol_ctx* c, int32_t precision, const int32_t* knobs, in
     return fail(MPOL_EINVAL, "z-pass threads must be a multiple of 64 in [64, 1024]");
   if (t[6] < 0 || t[6] > 2) return fail(MPOL_EINVAL, "engine must be 0 (automatic), 1 (matrix cores) or 2 (vector loops)");
   if (t[7] < 0 || t[7] > 64) return fail(MPOL_EINVAL, "right-hand sides per workgroup must be in [1, 64]");
+  if (t[8] < 0 || t[8] > 2) return fail(MPOL_EINVAL, "z register pass must be 0 (automatic), 1 (rank-one) or 2 (full)");
   for (int p = 0; p < 2; ++p) {
     if (precision != -1 && p != (precision == MPOL_FP32 ? 1 : 0)) continue;
     for (int i = 0; i < kTuneKnobs; ++i) c->tune[p][i] = t[i];

@@ -173,6 +173,11 @@ struct ColCfg {
   size_t lds = 0;
 };
 
+// Column workgroups of the y / register z passes before the tail set
+// (ColArgs::tail): kx / tx when one column is left over (kx = nx/2 + 1 with
+// nx/2 a multiple of tx), else 0 (every workgroup takes tx columns).
+inline int tail_blocks(int kx, int tx) { return (tx > 1 && kx % tx == 1) ? kx / tx : 0; }
+
 inline ColCfg col_cfg(int n, bool generic, int kx, int lines_per_col, size_t csize, size_t budget, int tx_pref) {
   ColCfg c;
   int tx = std::min(tx_pref, kx);
@@ -187,7 +192,7 @@ inline ColCfg col_cfg(int n, bool generic, int kx, int lines_per_col, size_t csi
 
 constexpr size_t kLdsTile = 48 * 1024;
 constexpr size_t kLdsMfma = 80 * 1024;  // two matrix-core tile workgroups per CU
-constexpr int kTuneKnobs = 8;
+constexpr int kTuneKnobs = 9;
 constexpr int kRhsGroup = 4;  // automatic right-hand sides per matrix-core tile workgroup
 
 // Candidates of every tile, sorted by stencil start along x within chunks of
@@ -238,6 +243,7 @@ struct mpol_ctx {
   DevBuf<float> rs_A32, rs_BA32, rs_rh32;
   FftPlan plan[3];
   DevBuf<double> ktab;      // e_x, e_y, e_z, then (cubic boxes) the Bessel factor table
+  DevBuf<float> ktab32;     // cubic boxes: e_z, then the dipole table over |k|^2, in FP32
   bool cubic = false;
   DevBuf<double> tw64[3];
   DevBuf<float> tw32[3];
@@ -256,6 +262,7 @@ struct mpol_ctx {
   DevBuf<double> rs_Q;
   DevBuf<float> rs_Q32;
   int64_t jq_off = -1;      // offset of the quadrupole Bessel table in ktab (cubic boxes)
+  int64_t jk_off = -1;      // offset of the dipole table divided by |k|^2 in ktab (cubic boxes)
   // polydisperse operator: radii (caller's order), radius types and pair tables
   // (ntypes > 0) or closed-form couplings (ntypes == 0)
   bool have_sizes = false;
@@ -748,6 +755,9 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   sp.ez = sp.ey + c->ng[1];
   sp.jt = c->cubic ? sp.ez + c->ng[2] : nullptr;
   sp.jq = (c->cubic && c->jq_off >= 0) ? c->ktab.p + c->jq_off : nullptr;
+  sp.jk = (c->cubic && c->jk_off >= 0) ? c->ktab.p + c->jk_off : nullptr;
+  sp.ez32 = sp.jk ? c->ktab32.p : nullptr;
+  sp.jk32 = sp.jk ? c->ktab32.p + c->ng[2] : nullptr;
   const dim3 zgrid(cdiv(c->kx, cz.tx), c->ng[1], nrhs);
   if (cz.lds > kLdsMax) return fail(MPOL_EINVAL, "z-pass columns do not fit in LDS");
   // one thread group per line of the z transform, all NC*tx lines at once
@@ -768,7 +778,18 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
     constexpr int TXR = sizeof(T) == 4 ? 16 : 8;
     ColArgs ar = az;
     ar.tx = TXR;
-    const dim3 rgrid(cdiv(c->kx, TXR), c->ng[1], nrhs);
+    ar.tail = tail_blocks(c->kx, TXR);
+    dim3 rgrid(cdiv(c->kx, TXR), c->ng[1], nrhs);
+    if (ar.tail > 0 && knobs<T>(c)[8] != 2 && sp.jk) {
+      // rank-one form on the lines with non-Nyquist x and y wave numbers,
+      // then the full form on the last column and the y = ny/2 row
+      hipLaunchKernelGGL((k_fft_z_scale_reg256_rank1<T, TXR>), dim3(ar.tail, c->ng[1], nrhs), dim3(TXR * 16), 0, st,
+                         ar, sp, spec);
+      ar.zsel = 1;
+      rgrid = dim3(ar.tail + 1, cdiv(c->ng[1], TXR), nrhs);
+    } else if (ar.tail > 0) {
+      rgrid = dim3(ar.tail + 1, c->ng[1], nrhs);
+    }
     if constexpr (sizeof(T) == 4)
       hipLaunchKernelGGL((k_fft_z_scale_reg256<T, TXR>), rgrid, dim3(TXR * 16), 0, st, ar, sp, spec);
     else

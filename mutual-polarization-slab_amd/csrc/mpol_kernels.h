@@ -1055,6 +1055,10 @@ struct ColArgs {
   const void* tw;   // twiddles for the transformed axis
   const int* gate;  // optional per-system state (rhs_idle) of blockIdx.z / gdiv
   int gdiv;
+  // register z passes (mpol_zreg.h, zreg_column): tail > 0 when kx =
+  // tail*tx + 1; zsel = 1 when the rank-one pass takes the regular lines
+  int tail;
+  int zsel;
 };
 
 template <typename T, int E, int N>
@@ -1111,6 +1115,11 @@ struct ScaleParams {
   // (j1^2, j1 j2, j2^2)/|k|^2 with the spherical Bessel functions j1, j2 of
   // |k| (0 at k=0); null otherwise.
   const double* jq;
+  // Cubic boxes: jk[m^2] = jt[m^2] / |k|^2 (0 at k = 0); null otherwise.
+  const double* jk;
+  // FP32 copies of jk and ez (cubic boxes; the FP32 rank-one z pass)
+  const float* jk32;
+  const float* ez32;
 };
 
 // Spherical Bessel functions j1(k), j2(k): power series below k = 0.6 (the

@@ -78,6 +78,28 @@ __device__ __forceinline__ void dipole_scale_point(const ColArgs& a, const Scale
   F2 = G2;
 }
 
+// The z line (kx, y) of thread column col in the full register passes.
+// Without a tail (a.tail == 0) workgroup (bx, by) holds columns bx*TX + col of
+// row by.  With one (kx = tail*TX + 1) the last column kx - 1 is held by the
+// workgroups bx == tail, TX rows each (y = by*TX + col); of the other
+// workgroups, all rows take part (a.zsel == 0) or only the row y = ny/2 of an
+// even ny, at by == 0 (a.zsel == 1: k_fft_z_scale_reg256_rank1 takes the
+// rest).  Returns false when the workgroup holds no line.
+__device__ __forceinline__ bool zreg_column(const ColArgs& a, int col, int TX, int& kx, int& y) {
+  if (a.tail > 0 && (int)blockIdx.x == a.tail) {
+    kx = a.kx - 1;
+    y = blockIdx.y * TX + col;
+    return (int)blockIdx.y * TX < a.ny;
+  }
+  kx = blockIdx.x * TX + col;
+  if (a.zsel == 0) {
+    y = blockIdx.y;
+    return true;
+  }
+  y = a.ny / 2;
+  return blockIdx.y == 0 && a.ny % 2 == 0;
+}
+
 // Thread (col, j), col = threadIdx.x % TX, j = threadIdx.x / TX (16 values).
 template <typename T, int TX>
 __global__ void __launch_bounds__(TX * 16) k_fft_z_scale_reg256(ColArgs a, ScaleParams sp, cplx<T>* __restrict__ spec) {
@@ -85,9 +107,12 @@ __global__ void __launch_bounds__(TX * 16) k_fft_z_scale_reg256(ColArgs a, Scale
   __shared__ cplx<T> tw[N];
   __shared__ cplx<T> xb[TX * LS];
   const int col = threadIdx.x % TX, j = threadIdx.x / TX;
-  const int kx = blockIdx.x * TX + col, y = blockIdx.y, rhs = blockIdx.z;
+  const int rhs = blockIdx.z;
   if (rhs_idle(a.gate, a.gdiv, rhs)) return;
-  const bool valid = kx < a.kx;
+  int kx, yr;
+  if (!zreg_column(a, col, TX, kx, yr)) return;  // uniform over the workgroup
+  const bool valid = kx < a.kx && yr < a.ny;
+  const int y = valid ? yr : 0;
   const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
   for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];
   const size_t plane = (size_t)a.ny * a.kxs;
@@ -224,9 +249,12 @@ __global__ void __launch_bounds__(TX * 32) k_fft_z_scale_reg256_pair(ColArgs a, 
   __shared__ cplx<T> tw[N];
   __shared__ cplx<T> xb[TX * LS];
   const int h = threadIdx.x & 1, col = (threadIdx.x >> 1) % TX, j = (threadIdx.x >> 1) / TX;
-  const int kx = blockIdx.x * TX + col, y = blockIdx.y, rhs = blockIdx.z;
+  const int rhs = blockIdx.z;
   if (rhs_idle(a.gate, a.gdiv, rhs)) return;
-  const bool valid = kx < a.kx;
+  int kx, yr;
+  if (!zreg_column(a, col, TX, kx, yr)) return;  // uniform over the workgroup
+  const bool valid = kx < a.kx && yr < a.ny;
+  const int y = valid ? yr : 0;
   const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
   for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];
   const size_t plane = (size_t)a.ny * a.kxs;
@@ -288,6 +316,119 @@ __global__ void __launch_bounds__(TX * 32) k_fft_z_scale_reg256_pair(ColArgs a, 
       cplx<T>* dst = spec + (size_t)(rhs * NC + c) * a.nz * plane + (size_t)y * a.kxs + kx;
 #pragma unroll
       for (int k = 0; k < 8; ++k) dst[(size_t)(j + 16 * (2 * k + h)) * plane] = v[c][k];
+    }
+  }
+}
+
+// Dipole z pass over the lines whose x and y wave numbers are not Nyquist
+// frequencies (every column but the last, every row but y = ny/2 of an even
+// ny; k_fft_z_scale_reg256 with a.zsel = 1 takes those).  There the scale is
+// rank one, G = c k (k.F)/|k|^2, except at kz = nz/2, where the Hermitian part
+// (cross terms with exactly one Nyquist axis vanish) keeps the (x, y) block
+// and the z entry apart: G_xy = c k_xy (k_xy.F_xy)/|k|^2, G_z = c k_z^2
+// F_z/|k|^2.  k_x and k_y are constant along a z line, so the forward pass
+// transforms two lines, h = k_x f_x + k_y f_y and f_z (k.F = DFT(h) +
+// k_z DFT(f_z)), and the inverse pass two, psi and k_z psi_z with psi =
+// c (k.F)/|k|^2 (psi = c DFT(h)/|k|^2 and psi_z = c k_z DFT(f_z)/|k|^2 at
+// kz = nz/2): g_x = k_x IDFT(psi), g_y = k_y IDFT(psi), g_z = IDFT(k_z psi_z).
+// Four 256-point transforms per line instead of six, two lines in registers
+// instead of three.  c(k)/|k|^2 = jk[m^2] e_x e_y e_z / G from the cubic-box
+// tables (sp.jk; FP32: float copies sp.jk32, sp.ez32, whose eight loads per
+// thread in flight stay within three waves' registers).  Thread (col, j) as in
+// k_fft_z_scale_reg256; grid (tail, ny, nrhs).
+template <typename T, int TX>
+__global__ void __launch_bounds__(TX * 16) k_fft_z_scale_reg256_rank1(ColArgs a, ScaleParams sp,
+                                                                      cplx<T>* __restrict__ spec) {
+  constexpr int N = 256, LS = 273;  // LS: padded line of the transpose buffer
+  __shared__ cplx<T> tw[N];
+  __shared__ cplx<T> xb[TX * LS];
+  const int col = threadIdx.x % TX, j = threadIdx.x / TX;
+  const int kx = blockIdx.x * TX + col, y = blockIdx.y, rhs = blockIdx.z;
+  if (rhs_idle(a.gate, a.gdiv, rhs)) return;
+  if (a.ny % 2 == 0 && 2 * y == a.ny) return;  // the Nyquist row (uniform)
+  const bool valid = kx < a.kx;
+  const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
+  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];
+  const size_t plane = (size_t)a.ny * a.kxs;
+  const int mx = fft_freq(kx, a.nx), my = fft_freq(y, a.ny);
+  const T k0t = (T)(mx * sp.twopi_over_l[0]), k1t = (T)(my * sp.twopi_over_l[1]);
+  cplx<T> v[2][16];  // h, then f_z
+  {
+    const cplx<T>* s0 = spec + (size_t)(rhs * 3) * a.nz * plane + (size_t)y * a.kxs + kx;
+    const cplx<T>* s1 = s0 + (size_t)a.nz * plane;
+    const cplx<T>* s2 = s1 + (size_t)a.nz * plane;
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) {
+      const size_t o = (size_t)(j + 16 * n2) * plane;
+      const cplx<T> a0 = valid ? s0[o] : cmk<T>(0, 0);
+      const cplx<T> a1 = valid ? s1[o] : cmk<T>(0, 0);
+      v[1][n2] = valid ? s2[o] : cmk<T>(0, 0);
+      v[0][n2] = cmk<T>(k0t * a0.x + k1t * a1.x, k0t * a0.y + k1t * a1.y);
+    }
+  }
+  cplx<T>* line = xb + col * LS;
+  __syncthreads();  // twiddles
+  // forward: DFT over n2, twiddle, transpose, DFT over n1
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    dft16<T>(v[c]);
+#pragma unroll
+    for (int k2 = 1; k2 < 16; ++k2) v[c][k2] = cmul(v[c][k2], tw[(j * k2) & (N - 1)]);
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) line[pidx(k2 * 16 + j)] = v[c][k2];
+    __syncthreads();
+#pragma unroll
+    for (int n1 = 0; n1 < 16; ++n1) v[c][n1] = line[pidx(j * 16 + n1)];
+    __syncthreads();
+    dft16<T>(v[c]);  // thread (col, k2 = j) now holds X[j + 16 k1]
+  }
+  if (valid) {
+    const int mxy2 = mx * mx + my * my;
+    const T tz = (T)sp.twopi_over_l[2];
+    T exy;
+    if constexpr (sizeof(T) == 4) exy = (T)(sp.ex[kx] * sp.ey[y] * sp.inv_g);
+    else exy = sp.ex[kx] * sp.ey[y] * sp.inv_g;
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) {
+      const int kz = j + 16 * k1;
+      const int mz = fft_freq(kz, a.nz);
+      T q;  // c(k)/|k|^2 (0 at k = 0)
+      if constexpr (sizeof(T) == 4) q = sp.jk32[mxy2 + mz * mz] * sp.ez32[kz] * exy;
+      else q = sp.jk[mxy2 + mz * mz] * sp.ez[kz] * exy;
+      const T k2 = (T)mz * tz;
+      const cplx<T> txy = v[0][k1];
+      const cplx<T> tzv = cmk<T>(k2 * v[1][k1].x, k2 * v[1][k1].y);
+      const bool split = a.nz % 2 == 0 && 2 * mz == -a.nz;  // kz = nz/2
+      const cplx<T> s = split ? txy : cmk<T>(txy.x + tzv.x, txy.y + tzv.y);
+      const cplx<T> sz = split ? tzv : s;
+      v[0][k1] = cmk<T>(q * s.x, q * s.y);
+      v[1][k1] = cmk<T>(q * k2 * sz.x, q * k2 * sz.y);
+    }
+  }
+  // inverse: DFT over k1, conjugate twiddle, transpose, DFT over k2
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    idft16<T>(v[c]);  // index n1
+#pragma unroll
+    for (int n1 = 1; n1 < 16; ++n1) v[c][n1] = cmul(v[c][n1], cconj(tw[(j * n1) & (N - 1)]));
+#pragma unroll
+    for (int n1 = 0; n1 < 16; ++n1) line[pidx(n1 * 16 + j)] = v[c][n1];
+    __syncthreads();
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) v[c][k2] = line[pidx(j * 16 + k2)];
+    __syncthreads();
+    idft16<T>(v[c]);  // thread (col, n1 = j) now holds x[j + 16 n2]
+  }
+  if (valid) {
+    cplx<T>* d0 = spec + (size_t)(rhs * 3) * a.nz * plane + (size_t)y * a.kxs + kx;
+    cplx<T>* d1 = d0 + (size_t)a.nz * plane;
+    cplx<T>* d2 = d1 + (size_t)a.nz * plane;
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) {
+      const size_t o = (size_t)(j + 16 * n2) * plane;
+      d0[o] = cmk<T>(k0t * v[0][n2].x, k0t * v[0][n2].y);
+      d1[o] = cmk<T>(k1t * v[0][n2].x, k1t * v[0][n2].y);
+      d2[o] = v[1][n2];
     }
   }
 }
