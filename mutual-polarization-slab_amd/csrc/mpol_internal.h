@@ -716,33 +716,6 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   if (ev) HIPCHK(hipEventRecord(ev[3], st));
   if (stop_after <= 2) return MPOL_OK;
 
-  // y axis forward
-  const ColCfg cy = col_cfg(c->ng[1], generic_fft(c->plan[1]), c->kx, 1, cs, kLdsCol,
-                            knobs<T>(c)[3] > 0 ? knobs<T>(c)[3] : (sizeof(T) == 8 ? 8 : 16));  // 128-byte rows
-  ColArgs ay{};
-  ay.nx = c->ng[0]; ay.ny = c->ng[1]; ay.nz = c->ng[2]; ay.kx = c->kx; ay.kxs = c->kxs; ay.tx = cy.tx; ay.lstride = cy.lstride;
-  ay.plan = c->plan[1]; ay.tw = tw_ptr<T>(c, 1);
-  ay.gate = gate;
-  ay.gdiv = NC * rps;  // batch index = rhs * NC + component
-  const dim3 ygrid(cdiv(c->kx, cy.tx), c->ng[2], NC * nrhs);
-  if (cy.lds > kLdsMax) return fail(MPOL_EINVAL, "y-pass columns do not fit in LDS");
-#define LAUNCH_FFTY(E, N, INV)                                                                  \
-  do {                                                                                          \
-    set_lds(k_fft_y<T, E, N>, cy.lds);                                                          \
-    hipLaunchKernelGGL((k_fft_y<T, E, N>), ygrid, dim3(256), cy.lds, st, ay, spec, INV);       \
-  } while (0)
-  DISPATCH_FFT(c->ng[1], ay.plan.e, LAUNCH_FFTY, 0);
-  if (ev) HIPCHK(hipEventRecord(ev[4], st));
-  if (stop_after <= 3) return MPOL_OK;
-
-  // z axis forward + scale + inverse
-  const ColCfg cz = col_cfg(c->ng[2], generic_fft(c->plan[2]), c->kx, NC, cs, kLdsCol,
-                            knobs<T>(c)[4] > 0 ? knobs<T>(c)[4] : (sizeof(T) == 8 ? 4 : 8));
-  ColArgs az{};
-  az.nx = c->ng[0]; az.ny = c->ng[1]; az.nz = c->ng[2]; az.kx = c->kx; az.kxs = c->kxs; az.tx = cz.tx; az.lstride = cz.lstride;
-  az.plan = c->plan[2]; az.tw = tw_ptr<T>(c, 2);
-  az.gate = gate;
-  az.gdiv = rps;
   ScaleParams sp{};
   for (int d = 0; d < 3; ++d) {
     sp.twopi_over_l[d] = 2.0 * kPi / c->box[d];
@@ -758,6 +731,64 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   sp.jk = (c->cubic && c->jk_off >= 0) ? c->ktab.p + c->jk_off : nullptr;
   sp.ez32 = sp.jk ? c->ktab32.p : nullptr;
   sp.jk32 = sp.jk ? c->ktab32.p + c->ng[2] : nullptr;
+
+  // the rank-one scheme (mpol_zreg.h) for nz = 256 dipole grids of cubic
+  // boxes: the register z pass in columns of TXR with one left over, not
+  // forced off (z_rank1 = 2) or to the LDS z pass (z_cols), a compile-time
+  // y plan; the full operator only (the stage stops read the standard
+  // three-component grids)
+  constexpr int TXR = sizeof(T) == 4 ? 16 : 8;
+  const bool zreg = NC == 3 && c->ng[2] == 256 && knobs<T>(c)[4] == 0;
+  const bool rank1 = zreg && tail_blocks(c->kx, TXR) > 0 && knobs<T>(c)[8] != 2 && sp.jk != nullptr &&
+                     is_ct_size(c->ng[1]) && stop_after >= kStages;
+
+  // y axis forward
+  const ColCfg cy = col_cfg(c->ng[1], generic_fft(c->plan[1]), c->kx, 1, cs, kLdsCol,
+                            knobs<T>(c)[3] > 0 ? knobs<T>(c)[3] : (sizeof(T) == 8 ? 8 : 16));  // 128-byte rows
+  ColArgs ay{};
+  ay.nx = c->ng[0]; ay.ny = c->ng[1]; ay.nz = c->ng[2]; ay.kx = c->kx; ay.kxs = c->kxs; ay.tx = cy.tx; ay.lstride = cy.lstride;
+  ay.plan = c->plan[1]; ay.tw = tw_ptr<T>(c, 1);
+  ay.gate = gate;
+  ay.gdiv = NC * rps;  // batch index = rhs * NC + component
+  const dim3 ygrid(cdiv(c->kx, cy.tx), c->ng[2], NC * nrhs);
+  if (cy.lds > kLdsMax) return fail(MPOL_EINVAL, "y-pass columns do not fit in LDS");
+  // rank-one scheme: x and y components together (k_fft_y_xy), the z
+  // component alone (k_fft_y on component 2 of every right-hand side)
+  ColArgs axy = ay;
+  axy.gdiv = rps;  // batch index = rhs
+  ColArgs ayz = ay;
+  ayz.bmul = NC;
+  ayz.badd = 2;
+  const dim3 xygrid(cdiv(c->kx, cy.tx), c->ng[2], nrhs);
+  const size_t xylds = ((size_t)c->ng[1] + 2 * (size_t)cy.tx * cy.lstride * (generic_fft(c->plan[1]) ? 2 : 1)) * cs;
+  // the 2 tx lines in one pass of the line FFTs (ny / e threads per line), 256 to 512 threads
+  const int xythreads = std::min(512, std::max(256, 2 * cy.tx * (c->ng[1] / ct_plan(c->ng[1]).e)));
+  if (rank1 && xylds > kLdsMax) return fail(MPOL_EINVAL, "y-pass columns do not fit in LDS");
+#define LAUNCH_FFTY(E, N, INV)                                                                  \
+  do {                                                                                          \
+    set_lds(k_fft_y<T, E, N>, cy.lds);                                                          \
+    if constexpr (NC == 3 && N > 0) {                                                           \
+      if (rank1) {                                                                              \
+        set_lds(k_fft_y_xy<T, E, N>, xylds);                                                    \
+        hipLaunchKernelGGL((k_fft_y_xy<T, E, N>), xygrid, dim3(xythreads), xylds, st, axy, sp, spec, INV); \
+        hipLaunchKernelGGL((k_fft_y<T, E, N>), xygrid, dim3(256), cy.lds, st, ayz, spec, INV); \
+        break;                                                                                  \
+      }                                                                                         \
+    }                                                                                           \
+    hipLaunchKernelGGL((k_fft_y<T, E, N>), ygrid, dim3(256), cy.lds, st, ay, spec, INV);       \
+  } while (0)
+  DISPATCH_FFT(c->ng[1], ay.plan.e, LAUNCH_FFTY, 0);
+  if (ev) HIPCHK(hipEventRecord(ev[4], st));
+  if (stop_after <= 3) return MPOL_OK;
+
+  // z axis forward + scale + inverse
+  const ColCfg cz = col_cfg(c->ng[2], generic_fft(c->plan[2]), c->kx, NC, cs, kLdsCol,
+                            knobs<T>(c)[4] > 0 ? knobs<T>(c)[4] : (sizeof(T) == 8 ? 4 : 8));
+  ColArgs az{};
+  az.nx = c->ng[0]; az.ny = c->ng[1]; az.nz = c->ng[2]; az.kx = c->kx; az.kxs = c->kxs; az.tx = cz.tx; az.lstride = cz.lstride;
+  az.plan = c->plan[2]; az.tw = tw_ptr<T>(c, 2);
+  az.gate = gate;
+  az.gdiv = rps;
   const dim3 zgrid(cdiv(c->kx, cz.tx), c->ng[1], nrhs);
   if (cz.lds > kLdsMax) return fail(MPOL_EINVAL, "z-pass columns do not fit in LDS");
   // one thread group per line of the z transform, all NC*tx lines at once
@@ -774,26 +805,27 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
     hipLaunchKernelGGL((k_fft_z_scale<T, E, N, NC>), zgrid, dim3(zthreads), cz.lds, st, az, sp, spec); \
   } while (0)
   // nz = 256 dipole grids: transforms in registers (unless a z column width is forced)
-  if (NC == 3 && c->ng[2] == 256 && knobs<T>(c)[4] == 0) {
-    constexpr int TXR = sizeof(T) == 4 ? 16 : 8;
+  if (zreg) {
     ColArgs ar = az;
     ar.tx = TXR;
     ar.tail = tail_blocks(c->kx, TXR);
-    dim3 rgrid(cdiv(c->kx, TXR), c->ng[1], nrhs);
-    if (ar.tail > 0 && knobs<T>(c)[8] != 2 && sp.jk) {
-      // rank-one form on the lines with non-Nyquist x and y wave numbers,
-      // then the full form on the last column and the y = ny/2 row
-      hipLaunchKernelGGL((k_fft_z_scale_reg256_rank1<T, TXR>), dim3(ar.tail, c->ng[1], nrhs), dim3(TXR * 16), 0, st,
-                         ar, sp, spec);
-      ar.zsel = 1;
-      rgrid = dim3(ar.tail + 1, cdiv(c->ng[1], TXR), nrhs);
-    } else if (ar.tail > 0) {
-      rgrid = dim3(ar.tail + 1, c->ng[1], nrhs);
+    if (rank1) {
+      // rank-one form on the lines with non-Nyquist x and y wave numbers, the
+      // full form on the last column and the row ky = ny/2 (eight lines per
+      // workgroup) in the same launch
+      const int nfull = (c->ng[1] % 2 == 0 ? c->kx - 1 : 0) + c->ng[1];
+      const dim3 r1grid(ar.tail + 1, std::max(c->ng[1], cdiv(nfull, 8)), nrhs);
+      if constexpr (sizeof(T) == 4)
+        hipLaunchKernelGGL((k_fft_z_scale_reg256_rank1<T, TXR, false>), r1grid, dim3(256), 0, st, ar, sp, spec);
+      else
+        hipLaunchKernelGGL((k_fft_z_scale_reg256_rank1<T, TXR, true>), r1grid, dim3(256), 0, st, ar, sp, spec);
+    } else {
+      const dim3 rgrid(ar.tail > 0 ? ar.tail + 1 : cdiv(c->kx, TXR), c->ng[1], nrhs);
+      if constexpr (sizeof(T) == 4)
+        hipLaunchKernelGGL((k_fft_z_scale_reg256<T, TXR>), rgrid, dim3(TXR * 16), 0, st, ar, sp, spec);
+      else
+        hipLaunchKernelGGL((k_fft_z_scale_reg256_pair<T, TXR>), rgrid, dim3(TXR * 32), 0, st, ar, sp, spec);
     }
-    if constexpr (sizeof(T) == 4)
-      hipLaunchKernelGGL((k_fft_z_scale_reg256<T, TXR>), rgrid, dim3(TXR * 16), 0, st, ar, sp, spec);
-    else
-      hipLaunchKernelGGL((k_fft_z_scale_reg256_pair<T, TXR>), rgrid, dim3(TXR * 32), 0, st, ar, sp, spec);
   } else {
     DISPATCH_FFT(c->ng[2], az.plan.e, LAUNCH_FFTZ, 0);
   }

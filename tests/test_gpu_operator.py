@@ -259,3 +259,57 @@ def test_sparse_tiles_repeatable(gpu):
     for o in outs:
         assert np.array_equal(o, outs[0])
     assert relerr(outs[0], ref) < 1e-13
+
+
+def test_rank_one_z_scheme_matches_oracle(gpu):
+    """Cubic box, nz = 256: the rank-one scheme (x and y components as one
+    grid between the y passes, four z transforms per regular line, the last
+    column and the row ky = ny/2 in the full form) against the FP64 oracle and
+    against the full three-component passes (z_rank1 = 2), in both precisions,
+    with two right-hand sides, then with complex permittivities."""
+    from mpol_hip import FP32, Context
+    rng = np.random.default_rng(47)
+    ng = np.array([64, 96, 256])
+    L = 40.0
+    box = np.array([L, L, L])
+    n = 400
+    x = rng.random((n, 3)) * box
+    xi, P, rc = 1.2, 8, 3.5
+    perp, para, rv = tables(xi)
+    eps = 2.0 + 10 * rng.random(n)
+    c = Context()
+    c.set_tables(perp, para, rv)
+    c.set_geometry(x, box, ng, P, xi, P * (box / ng * xi) ** 2 / np.pi, rc)
+    c.set_permittivity(eps)
+    m = rng.standard_normal((2, n, 3))
+    out = {}
+    for mode in (0, 2):
+        c.set_tuning(z_rank1=mode)
+        out[mode, 64] = c.matvec(m, slab_factor=1.0).cpu().numpy()
+        out[mode, 32] = c.matvec(m, precision=FP32, slab_factor=1.0).cpu().numpy()
+    geo = eo.OperatorGeometry(x, box, ng, P, xi, rc, perp, para, rv)
+    for k in range(2):
+        ref = np.real(eo.field(geo, m[k], eps, 1.0))
+        print("rhs", k, "rank-one vs oracle", relerr(out[0, 64][k], ref), "full vs oracle",
+              relerr(out[2, 64][k], ref), flush=True)
+        assert relerr(out[0, 64][k], ref) < 1e-12
+        assert relerr(out[0, 32][k], ref) < 2e-6
+    print("rank-one vs full", relerr(out[0, 64], out[2, 64]), relerr(out[0, 32], out[2, 32]), flush=True)
+    assert relerr(out[0, 64], out[2, 64]) < 1e-13
+    assert relerr(out[0, 32], out[2, 32]) < 2e-6
+    # the two schemes round differently: equal outputs would mean one path ran twice
+    assert not np.array_equal(out[0, 64], out[2, 64])
+    # complex permittivities (each system a (re, im) pair of right-hand sides):
+    # against the full passes (on an even grid the oracle's complex FFT keeps
+    # the Nyquist planes' anti-Hermitian part, which the Hermitian operator
+    # here drops, so the oracle is compared on odd grids elsewhere)
+    epsc = eps + 1j * (0.5 + rng.random(n))
+    mc = rng.standard_normal((2, n, 3)) + 1j * rng.standard_normal((2, n, 3))
+    c.set_permittivity(epsc)
+    outc = {}
+    for mode in (0, 2):
+        c.set_tuning(z_rank1=mode)
+        outc[mode] = c.matvec(mc, slab_factor=0.0).cpu().numpy()
+    print("complex rank-one vs full", relerr(outc[0], outc[2]), flush=True)
+    assert relerr(outc[0], outc[2]) < 1e-13
+    assert not np.array_equal(outc[0], outc[2])
