@@ -752,14 +752,11 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   ay.gdiv = NC * rps;  // batch index = rhs * NC + component
   const dim3 ygrid(cdiv(c->kx, cy.tx), c->ng[2], NC * nrhs);
   if (cy.lds > kLdsMax) return fail(MPOL_EINVAL, "y-pass columns do not fit in LDS");
-  // rank-one scheme: x and y components together (k_fft_y_xy), the z
-  // component alone (k_fft_y on component 2 of every right-hand side)
+  // rank-one scheme: x and y components together, the z component in
+  // workgroups of the other role (k_fft_y_xy)
   ColArgs axy = ay;
-  axy.gdiv = rps;  // batch index = rhs
-  ColArgs ayz = ay;
-  ayz.bmul = NC;
-  ayz.badd = 2;
-  const dim3 xygrid(cdiv(c->kx, cy.tx), c->ng[2], nrhs);
+  axy.gdiv = rps;  // right-hand side = blockIdx.z / 2
+  const dim3 xygrid(cdiv(c->kx, cy.tx), c->ng[2], 2 * nrhs);
   const size_t xylds = ((size_t)c->ng[1] + 2 * (size_t)cy.tx * cy.lstride * (generic_fft(c->plan[1]) ? 2 : 1)) * cs;
   // the 2 tx lines in one pass of the line FFTs (ny / e threads per line), 256 to 512 threads
   const int xythreads = std::min(512, std::max(256, 2 * cy.tx * (c->ng[1] / ct_plan(c->ng[1]).e)));
@@ -771,7 +768,6 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
       if (rank1) {                                                                              \
         set_lds(k_fft_y_xy<T, E, N>, xylds);                                                    \
         hipLaunchKernelGGL((k_fft_y_xy<T, E, N>), xygrid, dim3(xythreads), xylds, st, axy, sp, spec, INV); \
-        hipLaunchKernelGGL((k_fft_y<T, E, N>), xygrid, dim3(256), cy.lds, st, ayz, spec, INV); \
         break;                                                                                  \
       }                                                                                         \
     }                                                                                           \

@@ -1057,9 +1057,6 @@ struct ColArgs {
   int gdiv;
   // register z passes (mpol_zreg.h): tail > 0 when kx = tail*tx + 1
   int tail;
-  // k_fft_y: the batch index is blockIdx.z * bmul + badd (bmul 0 = 1), so a
-  // launch can take one component of every right-hand side
-  int bmul, badd;
 };
 
 template <typename T, int E, int N>
@@ -1068,7 +1065,7 @@ __global__ void __launch_bounds__(256) k_fft_y(ColArgs a, cplx<T>* __restrict__ 
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* buf = tw + a.ny;
   cplx<T>* scratch = buf + a.tx * a.lstride;
-  const int kx0 = blockIdx.x * a.tx, z = blockIdx.y, b = blockIdx.z * max(a.bmul, 1) + a.badd;
+  const int kx0 = blockIdx.x * a.tx, z = blockIdx.y, b = blockIdx.z;
   if (rhs_idle(a.gate, a.gdiv, b)) return;
   const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
   for (int k = threadIdx.x; k < a.ny; k += blockDim.x) tw[k] = twg[k];

@@ -556,27 +556,60 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 3 : 2)
   }
 }
 
-// The x and y components through the y transforms of the rank-one scheme.
-// Forward (inverse = 0): F_x, F_y (slots 0, 1) of tx columns of plane z are
-// y-transformed together and combined into h = k_x F_x + k_y F_y (slot 0),
-// except on the full-form lines (the last column; ky = ny/2 of an even ny),
-// where F_x and F_y stay in slots 0 and 1.  Inverse: slot 0 holds psi' (or
-// G_x, with G_y in slot 1, on the full-form lines); k_x psi' and k_y psi'
-// (or G_x, G_y) are inverse-transformed into slots 0 and 1.  The z component
-// goes through k_fft_y with bmul = 3, badd = 2.  Workgroup (column group, z,
-// right-hand side) of up to 512 threads (the 2 tx lines in one pass of the
-// line FFTs); 2 tx lines in LDS (x components, then y).
+// The y transforms of the rank-one scheme, workgroups of two roles (grid
+// (column groups, nz, 2 nrhs), blockIdx.z = 2 rhs + role).  Role 0, the x and
+// y components: forward (inverse = 0), F_x, F_y (slots 0, 1) of tx columns
+// of plane z are y-transformed together and combined into h = k_x F_x +
+// k_y F_y (slot 0), except on the full-form lines (the last column; ky = ny/2
+// of an even ny), where F_x and F_y stay in slots 0 and 1; inverse, slot 0
+// holds psi' (or G_x, with G_y in slot 1, on the full-form lines) and
+// k_x psi', k_y psi' (or G_x, G_y) are inverse-transformed into slots 0 and
+// 1.  Role 1, the z component (slot 2): 2 tx columns per workgroup, the
+// column groups past kx return at once.  Up to 512 threads (2 tx lines in
+// one pass of the line FFTs); 2 tx lines in LDS.
 template <typename T, int E, int N>
 __global__ void __launch_bounds__(512) k_fft_y_xy(ColArgs a, ScaleParams sp, cplx<T>* __restrict__ spec, int inverse) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* buf = tw + a.ny;
   cplx<T>* scratch = buf + 2 * a.tx * a.lstride;
-  const int kx0 = blockIdx.x * a.tx, z = blockIdx.y, rhs = blockIdx.z;
+  const int z = blockIdx.y, rhs = blockIdx.z >> 1, role = blockIdx.z & 1;
   if (rhs_idle(a.gate, a.gdiv, rhs)) return;
+  const size_t cgrid = (size_t)a.nz * a.ny * a.kxs;  // one component
+  if (role == 1) {
+    const int tz = 2 * a.tx, kz0 = blockIdx.x * tz;
+    if (kz0 >= a.kx) return;  // uniform
+    const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
+    for (int k = threadIdx.x; k < a.ny; k += blockDim.x) tw[k] = twg[k];
+    cplx<T>* b2 = spec + (size_t)(rhs * 3 + 2) * cgrid + (size_t)z * a.ny * a.kxs;
+    const int c = threadIdx.x % tz, ystep = blockDim.x / tz;
+    const int kx = kz0 + c;
+    const bool act = (int)threadIdx.x < ystep * tz && kx < a.kx;
+    cplx<T>* l = buf + c * a.lstride;
+    if (act) {
+      for (int y0 = threadIdx.x / tz; y0 < a.ny; y0 += 8 * ystep) {
+        cplx<T> v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int y = y0 + u * ystep;
+          v[u] = y < a.ny ? b2[(size_t)y * a.kxs + kx] : cmk<T>(0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int y = y0 + u * ystep;
+          if (y < a.ny) l[pidx(y)] = v[u];
+        }
+      }
+    }
+    __syncthreads();
+    fft_any<T, E, N>(buf, a.lstride, min(tz, a.kx - kz0), a.plan, tw, inverse != 0, scratch);
+    if (act)
+      for (int y = threadIdx.x / tz; y < a.ny; y += ystep) b2[(size_t)y * a.kxs + kx] = l[pidx(y)];
+    return;
+  }
+  const int kx0 = blockIdx.x * a.tx;
   const cplx<T>* twg = reinterpret_cast<const cplx<T>*>(a.tw);
   for (int k = threadIdx.x; k < a.ny; k += blockDim.x) tw[k] = twg[k];
-  const size_t cgrid = (size_t)a.nz * a.ny * a.kxs;  // one component
   cplx<T>* b0 = spec + (size_t)(rhs * 3) * cgrid + (size_t)z * a.ny * a.kxs;
   cplx<T>* b1 = b0 + cgrid;
   const int c = threadIdx.x % a.tx, ystep = blockDim.x / a.tx;
