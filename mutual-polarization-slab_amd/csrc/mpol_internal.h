@@ -766,8 +766,13 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
     set_lds(k_fft_y<T, E, N>, cy.lds);                                                          \
     if constexpr (NC == 3 && N > 0) {                                                           \
       if (rank1) {                                                                              \
-        set_lds(k_fft_y_xy<T, E, N>, xylds);                                                    \
-        hipLaunchKernelGGL((k_fft_y_xy<T, E, N>), xygrid, dim3(xythreads), xylds, st, axy, sp, spec, INV); \
+        if (INV) {                                                                              \
+          set_lds(k_fft_y_xy<T, E, N, true>, xylds);                                            \
+          hipLaunchKernelGGL((k_fft_y_xy<T, E, N, true>), xygrid, dim3(xythreads), xylds, st, axy, sp, spec); \
+        } else {                                                                                \
+          set_lds(k_fft_y_xy<T, E, N, false>, xylds);                                           \
+          hipLaunchKernelGGL((k_fft_y_xy<T, E, N, false>), xygrid, dim3(xythreads), xylds, st, axy, sp, spec); \
+        }                                                                                       \
         break;                                                                                  \
       }                                                                                         \
     }                                                                                           \

@@ -567,8 +567,9 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 3 : 2)
 // 1.  Role 1, the z component (slot 2): 2 tx columns per workgroup, the
 // column groups past kx return at once.  Up to 512 threads (2 tx lines in
 // one pass of the line FFTs); 2 tx lines in LDS.
-template <typename T, int E, int N>
-__global__ void __launch_bounds__(512) k_fft_y_xy(ColArgs a, ScaleParams sp, cplx<T>* __restrict__ spec, int inverse) {
+template <typename T, int E, int N, bool INV>
+__global__ void __launch_bounds__(512) k_fft_y_xy(ColArgs a, ScaleParams sp, cplx<T>* __restrict__ spec) {
+  constexpr bool inverse = INV;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* buf = tw + a.ny;
@@ -602,7 +603,7 @@ __global__ void __launch_bounds__(512) k_fft_y_xy(ColArgs a, ScaleParams sp, cpl
       }
     }
     __syncthreads();
-    fft_any<T, E, N>(buf, a.lstride, min(tz, a.kx - kz0), a.plan, tw, inverse != 0, scratch);
+    fft_any<T, E, N>(buf, a.lstride, min(tz, a.kx - kz0), a.plan, tw, inverse, scratch);
     if (act)
       for (int y = threadIdx.x / tz; y < a.ny; y += ystep) b2[(size_t)y * a.kxs + kx] = l[pidx(y)];
     return;
@@ -623,7 +624,10 @@ __global__ void __launch_bounds__(512) k_fft_y_xy(ColArgs a, ScaleParams sp, cpl
   cplx<T>* ly = buf + (a.tx + c) * a.lstride;
   const bool act = (int)threadIdx.x < ystep * a.tx;
   if (act) {
-    // batches of 4 rows (8 loads) in flight before the LDS writes
+    // batches of 4 rows (8 loads) in flight before the LDS writes; the
+    // inverse reads slot 1 only in the last column (slot 1 of the row
+    // ky = ny/2 is patched after the loop by the lane that holds that row)
+    const bool two = !inverse || fullcol;
     for (int y0 = threadIdx.x / a.tx; y0 < a.ny; y0 += 4 * ystep) {
       cplx<T> vx[4], vy[4];
 #pragma unroll
@@ -631,30 +635,32 @@ __global__ void __launch_bounds__(512) k_fft_y_xy(ColArgs a, ScaleParams sp, cpl
         const int y = y0 + u * ystep;
         const bool ok = colok && y < a.ny;
         const size_t o = (size_t)y * a.kxs + kx;
-        if (!inverse) {
-          vx[u] = ok ? b0[o] : cmk<T>(0, 0);
-          vy[u] = ok ? b1[o] : cmk<T>(0, 0);
-        } else {
-          const bool full = fullcol || y == nyq;
-          const cplx<T> s0 = ok ? b0[o] : cmk<T>(0, 0);
-          const cplx<T> s1 = (ok && full) ? b1[o] : cmk<T>(0, 0);
-          const T k1 = (T)fft_freq(y, a.ny) * ky1;
-          vx[u] = full ? s0 : cmk<T>(k0 * s0.x, k0 * s0.y);
-          vy[u] = full ? s1 : cmk<T>(k1 * s0.x, k1 * s0.y);
-        }
+        vx[u] = ok ? b0[o] : cmk<T>(0, 0);
+        vy[u] = (ok && two) ? b1[o] : cmk<T>(0, 0);
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int y = y0 + u * ystep;
         if (y < a.ny) {
-          lx[pidx(y)] = vx[u];
-          ly[pidx(y)] = vy[u];
+          if (inverse && !fullcol) {
+            const T k1 = (T)fft_freq(y, a.ny) * ky1;
+            lx[pidx(y)] = cmk<T>(k0 * vx[u].x, k0 * vx[u].y);
+            ly[pidx(y)] = cmk<T>(k1 * vx[u].x, k1 * vx[u].y);
+          } else {
+            lx[pidx(y)] = vx[u];
+            ly[pidx(y)] = vy[u];
+          }
         }
       }
     }
+    if (inverse && !fullcol && colok && nyq >= 0 && (int)threadIdx.x / a.tx == nyq % ystep) {
+      const size_t o = (size_t)nyq * a.kxs + kx;
+      lx[pidx(nyq)] = b0[o];
+      ly[pidx(nyq)] = b1[o];
+    }
   }
   __syncthreads();
-  fft_any<T, E, N>(buf, a.lstride, 2 * a.tx, a.plan, tw, inverse != 0, scratch);
+  fft_any<T, E, N>(buf, a.lstride, 2 * a.tx, a.plan, tw, inverse, scratch);
   if (act && colok) {
     for (int y = threadIdx.x / a.tx; y < a.ny; y += ystep) {
       const size_t o = (size_t)y * a.kxs + kx;
