@@ -256,7 +256,7 @@ int mpol_ewald_forces(mpol_ctx* ctx, const double* p_dev, int32_t nrhs, const do
                       const double* f2_host, double* forces_dev);
 
 /* Launch-shape tuning (no counterpart in the reference; used by the
- * autotuner): knobs[0..8] = spread/contract tile rows along y and z (ty*tz a
+ * autotuner): knobs[0..9] = spread/contract tile rows along y and z (ty*tz a
  * power of two, each <= 4), particles staged per chunk (32..256), y-pass and
  * z-pass columns per workgroup (<= 64), z-pass threads (multiple of 64),
  * spread/contract engine (1 matrix cores over per-tile candidate images,
@@ -266,8 +266,12 @@ int mpol_ewald_forces(mpol_ctx* ctx, const double* p_dev, int32_t nrhs, const do
  * y/z passes), right-hand sides per matrix-core spread/contract workgroup
  * (1..64: one staged tile image serves them in turn), the dipole register
  * z pass for nz = 256 (1 rank-one form on the lines with non-Nyquist x and y
- * wave numbers, 2 the full three-component form everywhere).  0 keeps the
- * automatic choice; nknobs may be shorter than 9. */
+ * wave numbers, 2 the full three-component form everywhere), the wave-space
+ * operator for complex permittivities (1 the reference's, whose Nyquist
+ * planes carry k = -N/2 on both halves of the spectrum: the Hermitian part
+ * plus an extra wave-space pass for the anti-Hermitian part; 2 the
+ * Hermitian part only, one pass).  0 keeps the automatic choice (1 for the
+ * last two); nknobs may be shorter than 10. */
 int mpol_set_tuning(mpol_ctx* ctx, const int32_t* knobs, int32_t nknobs);
 
 /* The same knobs for the operator in one precision only (MPOL_FP32 or

@@ -1111,6 +1111,8 @@ int mpol_set_tuning_for(mpol_ctx* c, int32_t precision, const int32_t* knobs, in
   if (t[6] < 0 || t[6] > 2) return fail(MPOL_EINVAL, "engine must be 0 (automatic), 1 (matrix cores) or 2 (vector loops)");
   if (t[7] < 0 || t[7] > 64) return fail(MPOL_EINVAL, "right-hand sides per workgroup must be in [1, 64]");
   if (t[8] < 0 || t[8] > 2) return fail(MPOL_EINVAL, "z register pass must be 0 (automatic), 1 (rank-one) or 2 (full)");
+  if (t[9] < 0 || t[9] > 2)
+    return fail(MPOL_EINVAL, "complex Nyquist part must be 0 (automatic), 1 (reference operator) or 2 (Hermitian part)");
   for (int p = 0; p < 2; ++p) {
     if (precision != -1 && p != (precision == MPOL_FP32 ? 1 : 0)) continue;
     for (int i = 0; i < kTuneKnobs; ++i) c->tune[p][i] = t[i];

@@ -192,7 +192,7 @@ inline ColCfg col_cfg(int n, bool generic, int kx, int lines_per_col, size_t csi
 
 constexpr size_t kLdsTile = 48 * 1024;
 constexpr size_t kLdsMfma = 80 * 1024;  // two matrix-core tile workgroups per CU
-constexpr int kTuneKnobs = 9;
+constexpr int kTuneKnobs = 10;
 constexpr int kRhsGroup = 4;  // automatic right-hand sides per matrix-core tile workgroup
 
 // Candidates of every tile, sorted by stencil start along x within chunks of
@@ -251,6 +251,10 @@ struct mpol_ctx {
   // set, of the quadrupole rows); imaginary parts when complex
   int nsets = 0;
   bool cplx_eps = false;
+  // the pass that applies the anti-Hermitian (Nyquist) part of the wave-space
+  // operator for complex permittivities (run_operator), and its output
+  bool odd_pass = false;
+  DevBuf<double> odd_tmp;
   DevBuf<double> eps_re, eps_im;
   DevBuf<double> diag, diag_im, diag_q, diag_qim;
   bool diag_q_valid = false;
@@ -567,6 +571,7 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   hipStream_t st = c->stream;
   const int* gate = c->gate;
   const int rps = c->cplx_eps ? 2 : 1;  // operator right-hand sides per system
+  const bool odd = c->odd_pass;         // wave space only, anti-Hermitian part
 
   if (ev) HIPCHK(hipEventRecord(ev[0], st));
   const T* pA;
@@ -577,7 +582,12 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   } else {
     pA = c->rs_A32.p; pBA = c->rs_BA32.p; prh = c->rs_rh32.p;
   }
-  if constexpr (NC == kOpDipole) {
+  if (odd) {
+    // the contraction adds onto zeros
+    if (ev) HIPCHK(hipEventRecord(ev[1], st));
+    hipLaunchKernelGGL(k_fill_double, dim3(cdiv((size_t)nrhs * NC * n, 256)), dim3(256), 0, st, out,
+                       (int64_t)nrhs * NC * n, 0.0);
+  } else if constexpr (NC == kOpDipole) {
     if (slab != 0.0) {
       double* zpart = c->zsum.p + nrhs;
       hipLaunchKernelGGL(k_sum_z, dim3(kZsumBlocks, nrhs), dim3(256), 0, st, m, n, zpart);
@@ -738,7 +748,8 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   // y plan; the full operator only (the stage stops read the standard
   // three-component grids)
   constexpr int TXR = sizeof(T) == 4 ? 16 : 8;
-  const bool zreg = NC == 3 && c->ng[2] == 256 && knobs<T>(c)[4] == 0;
+  const bool zreg = NC == 3 && c->ng[2] == 256 && knobs<T>(c)[4] == 0 && !odd;
+  sp.odd = odd ? 1 : 0;
   const bool rank1 = zreg && tail_blocks(c->kx, TXR) > 0 && knobs<T>(c)[8] != 2 && sp.jk != nullptr &&
                      is_ct_size(c->ng[1]) && stop_after >= kStages;
 
@@ -871,6 +882,21 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
 #undef LAUNCH_FFTZ
 #undef LAUNCH_CONTRACT
   HIPCHK(hipGetLastError());
+  // complex permittivities: the reference's wave-space operator, applied to
+  // the complex moments p = a + i b, adds i B p to the Hermitian part's
+  // result (B real, nonzero on the Nyquist planes only), so the real output
+  // gains -B b and the imaginary output +B a (knob 9 = 2 keeps the Hermitian
+  // part only)
+  if (c->cplx_eps && !odd && stop_after >= kStages && knobs<T>(c)[9] != 2) {
+    HIPALLOC(c->odd_tmp, (size_t)nrhs * NC * n);
+    c->odd_pass = true;
+    const int s = run_operator<T, NC>(c, m, c->odd_tmp.p, nullptr, nullptr, nrhs, 0.0, nullptr, kStages);
+    c->odd_pass = false;
+    CHECK_STATUS(s);
+    hipLaunchKernelGGL(k_add_odd_part, dim3(cdiv((size_t)NC * n, 256), nrhs / 2), dim3(256), 0, st,
+                       (int64_t)NC * n, c->odd_tmp.p, out, gate);
+    HIPCHK(hipGetLastError());
+  }
   return MPOL_OK;
 }
 

@@ -1118,6 +1118,13 @@ struct ScaleParams {
   // FP32 copies of jk and ez (cubic boxes; the FP32 rank-one z pass)
   const float* jk32;
   const float* ez32;
+  // k_fft_z_scale, dipole and quadrupole blocks: 0 the Hermitian part of the
+  // reference's operator (real grids stay real); 1 its anti-Hermitian part
+  // times -i, (M(k) - M(k'))/2 F * (-i) with k' the wave vector with its
+  // Nyquist components' signs flipped (zero off the Nyquist planes): the
+  // reference applied to a real grid gives the Hermitian part's result plus
+  // i times this one's
+  int odd;
 };
 
 // Spherical Bessel functions j1(k), j2(k): power series below k = 0.6 (the
@@ -1166,6 +1173,29 @@ __device__ inline double kspace_coef(const ScaleParams& s, double k0, double k1,
   const double j = sqrt(2.0 / (kPi * kn)) * (sin(kn) / kn - cos(kn));
   const double ek = s.one_minus_eta[0] * k0 * k0 + s.one_minus_eta[1] * k1 * k1 + s.one_minus_eta[2] * k2 * k2;
   return 9.0 * kPi / (2.0 * kn) * (j * j) * exp(-ek * s.inv_4xi2) / kk;
+}
+
+// a[i] = v, i < n.
+static __global__ void k_fill_double(double* __restrict__ a, int64_t n, double v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// Complex systems as (real, imaginary) right-hand side pairs: system s =
+// blockIdx.y adds the anti-Hermitian wave-space part B (odd[rhs], from the
+// pass with ScaleParams::odd) as out[2s] -= odd[2s + 1], out[2s + 1] +=
+// odd[2s]; len values per right-hand side.
+static __global__ void k_add_odd_part(int64_t len, const double* __restrict__ odd, double* __restrict__ out,
+                                      const int* __restrict__ gate) {
+  const int s = blockIdx.y;
+  if (rhs_idle(gate, 1, s)) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  const double* oa = odd + (size_t)(2 * s) * len;
+  double* re = out + (size_t)(2 * s) * len;
+  const double ob = oa[len + i], oa_i = oa[i];
+  re[i] -= ob;
+  re[len + i] += oa_i;
 }
 
 template <typename T, int E, int N, int NC>
@@ -1245,6 +1275,19 @@ __global__ void __launch_bounds__(1024) k_fft_z_scale(ColArgs a, ScaleParams sp,
       const T x02 = (nyq_x != nyq_z) ? (T)0 : u0 * u2;
       const T x12 = (nyq_y != nyq_z) ? (T)0 : u1 * u2;
       const cplx<T> F0 = *f[0], F1 = *f[1], F2 = *f[2];
+      if (sp.odd) {
+        // the cross terms the Hermitian part drops, times -i
+        const T o01 = (nyq_x != nyq_y) ? u0 * u1 : (T)0;
+        const T o02 = (nyq_x != nyq_z) ? u0 * u2 : (T)0;
+        const T o12 = (nyq_y != nyq_z) ? u1 * u2 : (T)0;
+        const cplx<T> G0 = cmk<T>(c * (o01 * F1.x + o02 * F2.x), c * (o01 * F1.y + o02 * F2.y));
+        const cplx<T> G1 = cmk<T>(c * (o01 * F0.x + o12 * F2.x), c * (o01 * F0.y + o12 * F2.y));
+        const cplx<T> G2 = cmk<T>(c * (o02 * F0.x + o12 * F1.x), c * (o02 * F0.y + o12 * F1.y));
+        *f[0] = cmk<T>(G0.y, -G0.x);
+        *f[1] = cmk<T>(G1.y, -G1.x);
+        *f[2] = cmk<T>(G2.y, -G2.x);
+        continue;
+      }
       const T d00 = u0 * u0, d11 = u1 * u1, d22 = u2 * u2;
       *f[0] = cmk<T>(c * (d00 * F0.x + x01 * F1.x + x02 * F2.x), c * (d00 * F0.y + x01 * F1.y + x02 * F2.y));
       *f[1] = cmk<T>(c * (x01 * F0.x + d11 * F1.x + x12 * F2.x), c * (x01 * F0.y + d11 * F1.y + x12 * F2.y));
@@ -1273,8 +1316,17 @@ __global__ void __launch_bounds__(1024) k_fft_z_scale(ColArgs a, ScaleParams sp,
       if (nyq_x || nyq_y || nyq_z) {
         cplx<T> G2[NC];
         quad_scale_point<T>(nyq_x ? -u0 : u0, nyq_y ? -u1 : u1, nyq_z ? -u2 : u2, ces, ceq, cgs, cgq, F, G2);
+        if (sp.odd) {
+          // (G - G2)/2 times -i
 #pragma unroll
-        for (int c = 0; c < NC; ++c) G[c] = cmk<T>((G[c].x + G2[c].x) * (T)0.5, (G[c].y + G2[c].y) * (T)0.5);
+          for (int c = 0; c < NC; ++c) G[c] = cmk<T>((G[c].y - G2[c].y) * (T)0.5, (G2[c].x - G[c].x) * (T)0.5);
+        } else {
+#pragma unroll
+          for (int c = 0; c < NC; ++c) G[c] = cmk<T>((G[c].x + G2[c].x) * (T)0.5, (G[c].y + G2[c].y) * (T)0.5);
+        }
+      } else if (sp.odd) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) G[c] = cmk<T>(0, 0);
       }
 #pragma unroll
       for (int c = 0; c < NC; ++c) *f[c] = G[c];

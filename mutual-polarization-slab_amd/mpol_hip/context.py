@@ -245,7 +245,8 @@ class Context:
             raise ValueError("quadrupole tables must have shape (7, ntab)")
         _lib.check(self._lib.mpol_set_quad_tables(self._ctx, _lib.dptr(q), q.shape[1]))
 
-    TUNING_KNOBS = ("tile_ty", "tile_tz", "chunk", "y_cols", "z_cols", "z_threads", "engine", "rhs_group", "z_rank1")
+    TUNING_KNOBS = ("tile_ty", "tile_tz", "chunk", "y_cols", "z_cols", "z_threads", "engine", "rhs_group", "z_rank1",
+                    "cplx_nyquist")
 
     def set_tuning(self, precision=None, **knobs):
         """Override launch shapes (0 = automatic); see TUNING_KNOBS.  precision

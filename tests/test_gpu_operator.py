@@ -300,9 +300,10 @@ def test_rank_one_z_scheme_matches_oracle(gpu):
     # the two schemes round differently: equal outputs would mean one path ran twice
     assert not np.array_equal(out[0, 64], out[2, 64])
     # complex permittivities (each system a (re, im) pair of right-hand sides):
-    # against the full passes (on an even grid the oracle's complex FFT keeps
-    # the Nyquist planes' anti-Hermitian part, which the Hermitian operator
-    # here drops, so the oracle is compared on odd grids elsewhere)
+    # against the full passes and the oracle, which follows the reference's
+    # complex FFTs (their Nyquist planes' anti-Hermitian part is applied by
+    # an extra wave-space pass; without it, cplx_nyquist = 2, the result is
+    # off by that part)
     epsc = eps + 1j * (0.5 + rng.random(n))
     mc = rng.standard_normal((2, n, 3)) + 1j * rng.standard_normal((2, n, 3))
     c.set_permittivity(epsc)
@@ -310,6 +311,17 @@ def test_rank_one_z_scheme_matches_oracle(gpu):
     for mode in (0, 2):
         c.set_tuning(z_rank1=mode)
         outc[mode] = c.matvec(mc, slab_factor=0.0).cpu().numpy()
+    c.set_tuning(cplx_nyquist=2)
+    herm = c.matvec(mc, slab_factor=0.0).cpu().numpy()
+    c.set_tuning(precision=FP32)
+    out32 = c.matvec(mc, precision=FP32, slab_factor=0.0).cpu().numpy()
+    for k in range(2):
+        refc = eo.field(geo, mc[k], epsc, 0.0, complex_eps=True)
+        print("complex rhs", k, "vs oracle", relerr(outc[0][k], refc), "FP32", relerr(out32[k], refc),
+              "Hermitian part only", relerr(herm[k], refc), flush=True)
+        assert relerr(outc[0][k], refc) < 1e-12
+        assert relerr(out32[k], refc) < 2e-6
+        assert relerr(herm[k], refc) > 1e-11
     print("complex rank-one vs full", relerr(outc[0], outc[2]), flush=True)
     assert relerr(outc[0], outc[2]) < 1e-13
     assert not np.array_equal(outc[0], outc[2])

@@ -124,10 +124,13 @@ def test_quad_operator_batched_rhs_and_sets(gpu):
         assert _rel(out[k], _oracle(c, m[k], lam3[k]).real) < 1e-12
 
 
-def test_quad_operator_complex_permittivity(gpu):
+@pytest.mark.parametrize("ngrid", [[21, 21, 21], [24, 24, 24]], ids=["odd", "even"])
+def test_quad_operator_complex_permittivity(gpu, ngrid):
     """Complex eps (a dielectric function): the diagonal couples real and
-    imaginary parts; odd grid so the oracle's complex result is exact."""
-    c = Case(gpu, n=18, ngrid=[21, 21, 21], P=8)
+    imaginary parts.  On the even grid the reference's Nyquist planes give its
+    wave-space operator an anti-Hermitian part that couples them too; the
+    extra wave-space pass applies it (the oracle follows the reference)."""
+    c = Case(gpu, n=18, ngrid=ngrid, P=8)
     rng = np.random.default_rng(7)
     lam = rng.uniform(2.0, 20.0, c.n) + 1j * rng.uniform(0.5, 10.0, c.n)
     c.ctx.set_permittivity(lam)
@@ -259,7 +262,8 @@ def test_dipole_tensor_spectrum_complex_eps(gpu):
             A[:, j] = eo.field(geo, e.reshape(10, 3), eps[:, k], 0.0, complex_eps=True).reshape(-1)
         for d in range(3):
             sol = np.linalg.solve(A, np.tile(np.eye(3)[d], 10)).reshape(10, 3)
-            assert _rel(p[:, :, d, k], sol) < 1e-8
+            print("eps", eps[0, k], "field", d, "vs dense solve", _rel(p[:, :, d, k], sol), flush=True)
+            assert _rel(p[:, :, d, k], sol) < 1e-10
 
 
 @pytest.mark.parametrize("tile,chunk", [((2, 2), 0), ((4, 2), 32)])
