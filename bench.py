@@ -40,11 +40,14 @@ HBM_PEAK_GBS = 8000.0
 WORKLOAD = "n1e5_phi0.4"
 
 
-def stage_bytes(n, ng, P, npairs, nrhs, fbytes):
+def stage_bytes(n, ng, P, npairs, nrhs, fbytes, rank1=True):
     """Algorithmic HBM bytes of each operator stage for one launch (nrhs
     right-hand sides): the half spectrum is 3 components x nz*ny*(nx/2+1)
     complex values; particle data is the stencil factors (3P values), the
-    stencil start (3 int32), the permutation (int32) and the FP64 dipoles."""
+    stencil start (3 int32), the permutation (int32) and the FP64 dipoles.
+    rank1: the rank-one scheme of the y and z passes (x and y components as
+    one grid between the y passes: 5/3, 4/3 and 5/3 of a read + write of the
+    spectrum instead of 2 each)."""
     nx, ny, nz = ng
     kx = nx // 2 + 1
     spec = 3 * nrhs * nz * ny * kx * 2 * fbytes
@@ -53,9 +56,9 @@ def stage_bytes(n, ng, P, npairs, nrhs, fbytes):
         "slab_sum": nrhs * n * 24,
         "real_space": npairs * (4 + 5 * fbytes) + n * (4 + 8) + nrhs * n * (24 + 24),
         "spread_xfft": spec + part + nrhs * n * 24,
-        "fft_y_fwd": 2 * spec,
-        "fft_z_scale": 2 * spec,
-        "fft_y_inv": 2 * spec,
+        "fft_y_fwd": 5 * spec // 3 if rank1 else 2 * spec,
+        "fft_z_scale": 4 * spec // 3 if rank1 else 2 * spec,
+        "fft_y_inv": 5 * spec // 3 if rank1 else 2 * spec,
         "xifft_contract": spec + part,
         "contract_reduce": nrhs * n * (4 + 24),
     }
@@ -337,7 +340,7 @@ def main():
     dom = max(stages32, key=stages32.get)
     achieved = by32[dom] / (stages32[dom] * 1e-3) / 1e9
     kernel_names = {"spread_xfft": "k_spread_mfma_xfft", "xifft_contract": "k_xifft_contract_mfma",
-                    "fft_y_fwd": "k_fft_y", "fft_y_inv": "k_fft_y", "fft_z_scale": "k_fft_z_scale_reg256",
+                    "fft_y_fwd": "k_fft_y_xy", "fft_y_inv": "k_fft_y_xy", "fft_z_scale": "k_fft_z_scale_reg256_rank1",
                     "real_space": "k_real_space", "slab_sum": "k_sum_z", "contract_reduce": "k_contract_reduce"}
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

@@ -440,15 +440,19 @@ __global__ void __launch_bounds__(256, spread_mfma_min_blocks(sizeof(T), N, NC *
   const int colrow = i16 % R;  // row of column nb*16 + i16 for every nb (R | 16)
   const int colm = i16 / R;    // component of column i16; block nb adds nb*16/R
   // one x block's accumulators go to its tile lines (ADD: onto what is there)
+  // pidx(x0 + row) = x0 + x0/16 + row for the block's rows (row < 16,
+  // x0 % 16 == 0): one lane base per column block, the rest uniform or
+  // immediate, so the addresses need no registers across the kernel
   auto store_block = [&](int x0, const acc_t* acc, bool add) {
+    const int xo = x0 + (x0 >> 4);
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
       const int l = nb * 16 + i16;
       if (l < NCOL) {
-        T* pl = reinterpret_cast<T*>(buf + (l >> 1) * LS) + (l & 1);
+        T* pl = reinterpret_cast<T*>(buf + (l >> 1) * LS) + (l & 1) + 2 * xo;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          T* d = pl + 2 * pidx(x0 + MF::row(lane, r));
+          T* d = pl + 2 * MF::row(lane, r);
           *d = add ? *d + acc[nb][r] : acc[nb][r];
         }
       }
