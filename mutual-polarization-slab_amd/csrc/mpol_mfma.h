@@ -569,6 +569,32 @@ __global__ void __launch_bounds__(256, spread_mfma_min_blocks(sizeof(T), N, NC *
   }
 }
 
+// Sum over groups of R lanes (R = 4 or 8, aligned) of V values per lane (V a
+// power of two, V >= R), transposed: on return lane g (= its index in the
+// group) holds in v[0 .. V/R) the group sums of its values g V/R .. (g + 1)
+// V/R - 1.  Stages pair lanes g and R-1-g (half-mirror, R = 8), then g^2, g^1
+// (quad swaps); the lane in the upper half of a pair keeps the upper half of
+// its current values.  Fixed order, so the sums are deterministic.
+template <typename T, int CNT, int CTRL>
+__device__ __forceinline__ void transpose_sum_stage(T* v, bool upper) {
+  constexpr int H = CNT / 2;
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    const T keep = upper ? v[H + k] : v[k];
+    const T send = upper ? v[k] : v[H + k];
+    v[k] = keep + dpp_mov<CTRL>(send);
+  }
+}
+
+template <typename T, int V, int R>
+__device__ __forceinline__ void transpose_sum(T* v, int g) {
+  static_assert((R == 4 || R == 8) && V % R == 0, "lane groups of 4 or 8");
+  constexpr int V2 = R == 8 ? V / 2 : V;
+  if constexpr (R == 8) transpose_sum_stage<T, V, kDppRowHalfMirror>(v, (g & 4) != 0);
+  transpose_sum_stage<T, V2, kDppSwap2>(v, (g & 2) != 0);
+  transpose_sum_stage<T, V2 / 2, kDppSwap1>(v, (g & 1) != 0);
+}
+
 // Inverse x-FFT + contraction.  Wave w handles blocks of 16 consecutive
 // sorted candidates w, w + 4, ...: A[q][x] over the x span of the block (in
 // unwrapped coordinates, the grid read at x mod nx), then the row weights and
@@ -584,6 +610,8 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 1 : 2)
   static_assert(R == 4 || R == 8 || R == 16, "tile rows");
   constexpr int NCOL = NC * R, NB = (NCOL + 15) / 16, NW = 4, LS = padded_len(N);
   constexpr int NCP = mfma_ncp(NC, R), NPAIR = NCOL / 2, HALF = N / 2, KPL = (HALF + 63) / 64;
+  // the transposing lane sum needs 4 NB (values per lane) a power of two >= R
+  constexpr bool kTransposeSum = (R == 4 || R == 8) && (4 * NB) % R == 0 && ((4 * NB) & (4 * NB - 1)) == 0;
   // line pairs whose spectrum loads a wave keeps in flight together (32 VGPRs)
   constexpr int GP = (8 / (KPL * (int)sizeof(T) / 4)) > 0 ? 8 / (KPL * (int)sizeof(T) / 4) : 1;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
@@ -707,23 +735,51 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 1 : 2)
             const int o = xk - gxc;
             const T fv = f[min((unsigned)o, (unsigned)(P - 1))];
             const T av = (unsigned)o < (unsigned)P ? fv : (T)0;
-            const int px = 2 * pidx(xk - wrap);
+            // pidx(xw + k4) = xw + xw/16 + k4 (xw % 4 == 0, k4 < 4): the
+            // lane adds 2 k4 to a uniform offset
+            const int xw = xx - wrap;
+            const int px = 2 * (xw + (xw >> 4)) + 2 * k4;
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb) acc[nb] = MF::mac(av, colp[nb][px], acc[nb]);
           }
         }
         // out[q][c] = sum_row w_q[row] D[q][(c, row)]: weight, then sum the R
         // lanes of each component
+        if constexpr (kTransposeSum) {
+          // the 4 NB weighted values of a lane, v[nb*4 + r], summed over the
+          // R lanes of each component by a transposing butterfly: every stage
+          // pairs lanes (mirror, then swaps), each keeps half of its values
+          // and adds the partner's copy of them, so lane g of the R ends with
+          // the sums of values g*VL .. g*VL + VL - 1 and writes them itself
+          T v[4 * NB];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int cr = c0 + MF::row(lane, r);
-          const bool okr = cr < nc;
-          const int crq = okr ? cr : c0;
-          const T wq = st.fw[__umul24(crq, PW) + P + colrow];
+          for (int r = 0; r < 4; ++r) {
+            const int cr = c0 + MF::row(lane, r);
+            const T wq = st.fw[__umul24(cr < nc ? cr : c0, PW) + P + colrow];
 #pragma unroll
-          for (int nb = 0; nb < NB; ++nb) {
-            const T v = group_sum<T>(acc[nb][r] * wq, R);
-            if (colc[nb] < NC && colrow == 0 && okr) pb[cr * NC + colc[nb]] = v;
+            for (int nb = 0; nb < NB; ++nb) v[nb * 4 + r] = acc[nb][r] * wq;
+          }
+          transpose_sum<T, 4 * NB, R>(v, colrow);
+          constexpr int VL = 4 * NB / R;
+#pragma unroll
+          for (int k = 0; k < VL; ++k) {
+            const int idx = colrow * VL + k, nb = idx >> 2, r = idx & 3;
+            const int cr = c0 + MF::row(lane, r);
+            const int cc = nb * 16 / R + i16 / R;
+            if (cc < NC && cr < nc) pb[cr * NC + cc] = v[k];
+          }
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int cr = c0 + MF::row(lane, r);
+            const bool okr = cr < nc;
+            const int crq = okr ? cr : c0;
+            const T wq = st.fw[__umul24(crq, PW) + P + colrow];
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+              const T v = group_sum<T>(acc[nb][r] * wq, R);
+              if (colc[nb] < NC && colrow == 0 && okr) pb[cr * NC + colc[nb]] = v;
+            }
           }
         }
       }
