@@ -624,14 +624,14 @@ __global__ void __launch_bounds__(512) k_fft_y_xy(ColArgs a, ScaleParams sp, cpl
   cplx<T>* ly = buf + (a.tx + c) * a.lstride;
   const bool act = (int)threadIdx.x < ystep * a.tx;
   if (act) {
-    // batches of 4 rows (8 loads) in flight before the LDS writes; the
+    // batches of 8 rows (16 loads) in flight before the LDS writes; the
     // inverse reads slot 1 only in the last column (slot 1 of the row
     // ky = ny/2 is patched after the loop by the lane that holds that row)
     const bool two = !inverse || fullcol;
-    for (int y0 = threadIdx.x / a.tx; y0 < a.ny; y0 += 4 * ystep) {
-      cplx<T> vx[4], vy[4];
+    for (int y0 = threadIdx.x / a.tx; y0 < a.ny; y0 += 8 * ystep) {
+      cplx<T> vx[8], vy[8];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < 8; ++u) {
         const int y = y0 + u * ystep;
         const bool ok = colok && y < a.ny;
         const size_t o = (size_t)y * a.kxs + kx;
@@ -639,7 +639,7 @@ __global__ void __launch_bounds__(512) k_fft_y_xy(ColArgs a, ScaleParams sp, cpl
         vy[u] = (ok && two) ? b1[o] : cmk<T>(0, 0);
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < 8; ++u) {
         const int y = y0 + u * ystep;
         if (y < a.ny) {
           if (inverse && !fullcol) {
