@@ -24,10 +24,19 @@ def _counter(db, name):
     return per
 
 
-def _short(name):
+def _short(name, templated=False):
     base = name.split("(")[0]
-    base = base.split("<")[0]
+    if not templated:
+        base = base.split("<")[0]
     return base.replace("void ", "").replace("mpol::", "").strip()
+
+
+def _keys(names):
+    """Kernel name -> key: the bare name where one instance of the template
+    ran, the name with its template arguments where several did (the forward
+    and inverse instances of one y pass, say), so no instance hides another."""
+    count = collections.Counter(_short(k) for k in names)
+    return {k: _short(k, templated=count[_short(k)] > 1) for k in names}
 
 
 def main(argv=None):
@@ -42,17 +51,17 @@ def main(argv=None):
     sf = copy_bytes / (1024 * max(copy_f))
     sw = copy_bytes / (1024 * max(copy_w))
     res = {"copy_mb": float(copy_mb), "fetch_scale": sf, "write_scale": sw, "kernels": {}}
-    for k in sorted(set(fetch) & set(write)):
-        if "copy" in k.lower():
-            continue
+    common = sorted(k for k in set(fetch) & set(write) if "copy" not in k.lower())
+    keys = _keys(common)
+    for k in common:
         f = 1024 * sf * sum(fetch[k]) / len(fetch[k])
         w = 1024 * sw * sum(write[k]) / len(write[k])
-        res["kernels"][_short(k)] = {"fetch_bytes": round(f), "write_bytes": round(w), "bytes_per_launch": round(f + w),
+        res["kernels"][keys[k]] = {"fetch_bytes": round(f), "write_bytes": round(w), "bytes_per_launch": round(f + w),
                                      "launches": len(fetch[k]), "full_name": k[:160]}
     with open(out, "w") as fh:
         json.dump(res, fh, indent=1)
     for k, v in res["kernels"].items():
-        print(f"{k:32s} fetch {v['fetch_bytes'] / 1e6:9.2f} MB  write {v['write_bytes'] / 1e6:9.2f} MB")
+        print(f"{k:48s} fetch {v['fetch_bytes'] / 1e6:9.2f} MB  write {v['write_bytes'] / 1e6:9.2f} MB")
 
 
 if __name__ == "__main__":
