@@ -313,13 +313,14 @@ __device__ inline void mfma_stage_chunk(const TileArgs& a, const ImgArgs& im, co
   const T* src = reinterpret_cast<const T*>(im.fw) + q * PW;
   const int nt = blockDim.x, t = threadIdx.x;
   const int tot = image ? nc * PW : 0;
-  // every load of the chunk issued before the first LDS write: the image
-  // rows (up to kBatch per thread in one batch), this thread's slot record,
-  // then the moments it points to
-  constexpr int kBatch = 16;
+  // every load of the chunk issued before the first LDS write: this thread's
+  // slot record, the image rows (up to kBatch per thread in one batch: 256
+  // FP32 candidates of P + rows = 18 entries), then the moments the record
+  // points to (issued once the record is in, while the rows are in flight)
+  constexpr int kBatch = 18;
   int2 g = make_int2(0, 0);
   if (t < nc) g = im.gs[q + t];
-  int k0 = 0;
+  int k0 = 0, u0 = t;
   if (tot <= kBatch * nt) {
     T v[kBatch];
 #pragma unroll
@@ -327,12 +328,26 @@ __device__ inline void mfma_stage_chunk(const TileArgs& a, const ImgArgs& im, co
       const int k = u * nt + t;
       v[u] = k < tot ? src[k] : (T)0;
     }
+    double mv[NC];
+    if (WITH_M) {
+      const double* mp = a.ms + NC * ((size_t)rhs * a.n + g.y);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) mv[c] = t < nc ? mp[c] : 0.0;
+    }
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
       const int k = u * nt + t;
       if (k < tot) st.fw[k] = v[u];
     }
+    if (t < nc) {
+      if (image) st.gx[t] = g.x;
+      if (WITH_M) {
+#pragma unroll
+        for (int c = 0; c < NCP; ++c) st.m[NCP * t + c] = c < NC ? (T)mv[c] : (T)0;
+      }
+    }
     k0 = tot;
+    u0 = t + nt;
   }
   for (; k0 < tot; k0 += 8 * nt) {
     T v[8];
@@ -347,7 +362,7 @@ __device__ inline void mfma_stage_chunk(const TileArgs& a, const ImgArgs& im, co
       if (k < tot) st.fw[k] = v[u];
     }
   }
-  for (int u = t; u < nc; u += nt) {
+  for (int u = u0; u < nc; u += nt) {
     if (u != t) g = im.gs[q + u];
     if (image) st.gx[u] = g.x;
     if (WITH_M) {
