@@ -18,28 +18,48 @@ constexpr int kDotBlocks = 256;  // partial-sum blocks per right-hand side (one 
 constexpr int kDotGroup = 64;
 constexpr int kDotSweep = 8;
 
+// Elements of a thread whose loads a sweep issues together (KU per vector).
+constexpr int kDotUnroll = 4;
+
 // Partial dot products of one block's chunk [lo, hi) of right-hand side rhs:
 // partial[(rhs*cnt + i)*gridDim.x + blockIdx.x] = sum over the chunk of
 // a_i . w, a_i = A + i*stride_i + rhs*n (kDotSweep vectors per sweep, w read
-// once per sweep).  Thread t handles elements lo + t, lo + t + blockDim.x, ...;
-// each wave's sums go to LDS (red[vector][wave]) for up to kDotGroup vectors
-// between two barriers, so the sweeps of a group follow each other without
-// waiting; the four wave sums are added in a fixed order.
+// once per sweep).  Thread t handles elements lo + t, lo + t + blockDim.x, ...
+// in increasing order, kDotUnroll of them per batch: every load of a batch is
+// unconditional (indices clamped into the chunk and to the last vector, the
+// products past the chunk multiplied by a zero w), so a batch waits for memory
+// once.  Each wave's sums go to LDS (red[vector][wave]) for up to kDotGroup
+// vectors between two barriers, so the sweeps of a group follow each other
+// without waiting; the four wave sums are added in a fixed order.
 __device__ __forceinline__ void block_dots(const double* __restrict__ A, size_t stride_i, int cnt,
                                            const double* __restrict__ wr, size_t roff, int64_t lo, int64_t hi,
                                            int rhs, double* __restrict__ partial, double (*red)[4]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t step = blockDim.x;
   for (int g0 = 0; g0 < cnt; g0 += kDotGroup) {
     const int gc = min(kDotGroup, cnt - g0);
     for (int i0 = 0; i0 < gc; i0 += kDotSweep) {
       double s[kDotSweep];
 #pragma unroll
       for (int u = 0; u < kDotSweep; ++u) s[u] = 0.0;
-      for (int64_t k = lo + threadIdx.x; k < hi; k += blockDim.x) {
-        const double wv = wr[k];
+      const double* av[kDotSweep];
 #pragma unroll
-        for (int u = 0; u < kDotSweep; ++u)
-          if (i0 + u < gc) s[u] += A[(size_t)(g0 + i0 + u) * stride_i + roff + k] * wv;
+      for (int u = 0; u < kDotSweep; ++u) av[u] = A + (size_t)(g0 + min(i0 + u, gc - 1)) * stride_i + roff;
+      for (int64_t kb = lo + threadIdx.x; kb < hi; kb += kDotUnroll * step) {
+        double wv[kDotUnroll], a[kDotUnroll][kDotSweep];
+#pragma unroll
+        for (int e = 0; e < kDotUnroll; ++e) {
+          const int64_t k = kb + e * step;
+          const int64_t kc = k < hi ? k : kb;
+          const double w0 = wr[kc];
+          wv[e] = k < hi ? w0 : 0.0;
+#pragma unroll
+          for (int u = 0; u < kDotSweep; ++u) a[e][u] = av[u][kc];
+        }
+#pragma unroll
+        for (int e = 0; e < kDotUnroll; ++e)
+#pragma unroll
+          for (int u = 0; u < kDotSweep; ++u) s[u] += a[e][u] * wv[e];
       }
 #pragma unroll
       for (int u = 0; u < kDotSweep; ++u)
@@ -76,13 +96,22 @@ static __global__ void __launch_bounds__(256) k_dots(const double* __restrict__ 
 }
 
 // out[t] = sum of the nblk partials of output t (fixed order): one wave per
-// output, lanes load the partials in parallel, then a fixed butterfly.
+// output, lanes load the partials in parallel (four per lane in flight), then
+// a fixed butterfly.
 static __global__ void k_dots_finish(const double* __restrict__ partial, int nblk, int total, double* __restrict__ out) {
   const int t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (t >= total) return;
+  const double* pt = partial + (size_t)t * nblk;
   double s = 0.0;
-  for (int b = lane; b < nblk; b += 64) s += partial[(size_t)t * nblk + b];
+  for (int b0 = lane; b0 < nblk; b0 += 4 * 64) {
+    double p[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) p[e] = pt[min(b0 + 64 * e, nblk - 1)];
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      if (b0 + 64 * e < nblk) s += p[e];
+  }
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
   if (lane == 0) out[t] = s;
 }
@@ -105,10 +134,37 @@ static __global__ void __launch_bounds__(256) k_sub_dots(const double* __restric
   const size_t roff = (size_t)rhs * n;
   double* wr = w + roff;
   const double* hr = h + (size_t)rhs * cnt;
-  for (int64_t k = lo + threadIdx.x; k < hi; k += blockDim.x) {
-    double acc = wr[k];
-    for (int i = 0; i < cnt; ++i) acc -= hr[i] * V[(size_t)i * stride_i + roff + k];
-    wr[k] = acc;
+  // w_k -= h_i V_i,k in increasing i, kDotSweep vectors' loads of kDotUnroll
+  // elements in flight together (clamped indices; the padded terms subtract
+  // a zero coefficient)
+  const int64_t step = blockDim.x;
+  for (int64_t kb = lo + threadIdx.x; kb < hi; kb += kDotUnroll * step) {
+    double acc[kDotUnroll];
+    int64_t kc[kDotUnroll];
+#pragma unroll
+    for (int e = 0; e < kDotUnroll; ++e) {
+      const int64_t k = kb + e * step;
+      kc[e] = k < hi ? k : kb;
+      acc[e] = wr[kc[e]];
+    }
+    for (int i0 = 0; i0 < cnt; i0 += kDotSweep) {
+      double v[kDotUnroll][kDotSweep], hc[kDotSweep];
+#pragma unroll
+      for (int u = 0; u < kDotSweep; ++u) {
+        const int i = min(i0 + u, cnt - 1);
+        hc[u] = i0 + u < cnt ? hr[i] : 0.0;
+        const double* vi = V + (size_t)i * stride_i + roff;
+#pragma unroll
+        for (int e = 0; e < kDotUnroll; ++e) v[e][u] = vi[kc[e]];
+      }
+#pragma unroll
+      for (int e = 0; e < kDotUnroll; ++e)
+#pragma unroll
+        for (int u = 0; u < kDotSweep; ++u) acc[e] -= hc[u] * v[e][u];
+    }
+#pragma unroll
+    for (int e = 0; e < kDotUnroll; ++e)
+      if (kb + e * step < hi) wr[kb + e * step] = acc[e];
   }
   if (next_norm) block_dots(wr - roff, 0, 1, wr, roff, lo, hi, rhs, partial, red);
   else block_dots(V, stride_i, cnt, wr, roff, lo, hi, rhs, partial, red);
@@ -277,9 +333,22 @@ static __global__ void k_update_x(const double* __restrict__ V, size_t stride_i,
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const int kd = kdim[rhs];
-  double acc = x[(size_t)rhs * n + k];
-  for (int i = 0; i < kd; ++i) acc += y[(size_t)rhs * restart + i] * V[i * stride_i + (size_t)rhs * n + k];
-  x[(size_t)rhs * n + k] = acc;
+  const size_t o = (size_t)rhs * n + k;
+  const double* yr = y + (size_t)rhs * restart;
+  double acc = x[o];
+  // increasing i, kDotSweep loads in flight (clamped; padded terms add 0)
+  for (int i0 = 0; i0 < kd; i0 += kDotSweep) {
+    double v[kDotSweep], yc[kDotSweep];
+#pragma unroll
+    for (int u = 0; u < kDotSweep; ++u) {
+      const int i = min(i0 + u, kd - 1);
+      yc[u] = i0 + u < kd ? yr[i] : 0.0;
+      v[u] = V[i * stride_i + o];
+    }
+#pragma unroll
+    for (int u = 0; u < kDotSweep; ++u) acc += yc[u] * v[u];
+  }
+  x[o] = acc;
 }
 
 static __global__ void k_solve_to_wait(int nrhs, int* __restrict__ state) {
