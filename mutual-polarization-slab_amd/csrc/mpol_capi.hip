@@ -174,6 +174,7 @@ int gmres_core(mpol_ctx* c, int kind, const double* b, double* x, int nrhs, doub
   HIPALLOC(c->gh1, (size_t)nrhs * (restart + 1));
   HIPALLOC(c->gh2, (size_t)nrhs * (restart + 1));
   HIPALLOC(c->gpart, (size_t)nrhs * (restart + 1) * kDotBlocks);
+  HIPALLOC(c->gpart2, (size_t)nrhs * (restart + 1) * kDotBlocks);
   HIPALLOC(c->gbn, (size_t)nrhs);
   HIPALLOC(c->gbeta, (size_t)nrhs);
   HIPALLOC(c->grel, (size_t)nrhs);
@@ -254,24 +255,25 @@ int gmres_core(mpol_ctx* c, int kind, const double* b, double* x, int nrhs, doub
         CHECK_STATUS(apply(c, kind, vj, c->gW.p, nrhs, precision, slab));
       }
       ++*nmv;
-      // classical Gram-Schmidt, twice; each subtraction pass also forms the
-      // partial sums of the next reduction (second-pass coefficients, ||w||^2)
+      // classical Gram-Schmidt, twice; each subtraction pass finishes its
+      // coefficients from the previous launch's partial sums and forms the
+      // partial sums of the next reduction (second-pass coefficients into
+      // gpart2, ||w||^2 into gpart); the last reduction's finish runs with
+      // the Arnoldi step
       const int cj = j + 1;
       const int nb = dot_blocks(n3, nrhs);
       const dim3 dgrid(nb, nrhs);
-      dots(c, c->gV.p, sv, cj, c->gW.p, n3, nrhs, c->gh1.p);
-      hipLaunchKernelGGL(k_sub_dots, dgrid, dim3(256), 0, st, c->gV.p, sv, cj, c->gh1.p, c->gW.p, n3, c->gstate.p, 0,
-                         c->gpart.p);
-      hipLaunchKernelGGL(k_dots_finish, dim3(cdiv((int64_t)nrhs * cj, 4)), dim3(256), 0, st, c->gpart.p, nb,
-                         nrhs * cj, c->gh2.p);
-      hipLaunchKernelGGL(k_sub_dots, dgrid, dim3(256), 0, st, c->gV.p, sv, cj, c->gh2.p, c->gW.p, n3, c->gstate.p, 1,
-                         c->gpart.p);
-      hipLaunchKernelGGL(k_dots_finish, dim3(cdiv(nrhs, 4)), dim3(256), 0, st, c->gpart.p, nb, nrhs, c->grn2.p);
+      hipLaunchKernelGGL(k_dots, dgrid, dim3(256), 0, st, c->gV.p, sv, cj, c->gW.p, n3, nrhs, c->gpart.p);
+      hipLaunchKernelGGL(k_sub_dots, dgrid, dim3(256), 0, st, c->gV.p, sv, cj, c->gpart.p, c->gh1.p, c->gW.p, n3,
+                         c->gstate.p, 0, c->gpart2.p);
+      hipLaunchKernelGGL(k_sub_dots, dgrid, dim3(256), 0, st, c->gV.p, sv, cj, c->gpart2.p, c->gh2.p, c->gW.p, n3,
+                         c->gstate.p, 1, c->gpart.p);
+      hipLaunchKernelGGL(k_finish_arnoldi, dim3(nrhs), dim3(64), 0, st, c->gpart.p, nb, restart, j, c->gh1.p,
+                         c->gh2.p, c->grn2.p, c->gH.p, c->gcs.p, c->gsn.p, c->gg.p, c->gbn.p, tol, c->gstate.p,
+                         c->gkdim.p, c->grel.p, c->gy.p, c->dstep + (size_t)(j & 1) * nrhs,
+                         c->dsteprel + (size_t)(j & 1) * nrhs);
       hipLaunchKernelGGL(k_normalize_next, vgrid, dim3(256), 0, st, c->gW.p, c->gV.p + (size_t)(j + 1) * sv,
                          c->grn2.p, n3, c->gstate.p);
-      hipLaunchKernelGGL(k_arnoldi_step, dim3(sgrid), dim3(256), 0, st, nrhs, restart, j, c->gh1.p, c->gh2.p,
-                         c->grn2.p, c->gH.p, c->gcs.p, c->gsn.p, c->gg.p, c->gbn.p, tol, c->gstate.p, c->gkdim.p,
-                         c->grel.p, c->gy.p, c->dstep + (size_t)(j & 1) * nrhs, c->dsteprel + (size_t)(j & 1) * nrhs);
       HIPCHK(hipEventRecord(c->step_ev[j & 1], st));
       if (!cycle_read) {
         // the cycle start, read while step 0 runs

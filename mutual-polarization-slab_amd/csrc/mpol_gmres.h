@@ -95,14 +95,10 @@ static __global__ void __launch_bounds__(256) k_dots(const double* __restrict__ 
   block_dots(A, stride_i, cnt, w + (size_t)rhs * n, (size_t)rhs * n, lo, hi, rhs, partial, red);
 }
 
-// out[t] = sum of the nblk partials of output t (fixed order): one wave per
-// output, lanes load the partials in parallel (four per lane in flight), then
-// a fixed butterfly.
-static __global__ void k_dots_finish(const double* __restrict__ partial, int nblk, int total, double* __restrict__ out) {
-  const int t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (t >= total) return;
-  const double* pt = partial + (size_t)t * nblk;
+// Sum of the nblk partials at pt in a fixed order, by one wave: lane l adds
+// partials l, l + 64, ... (four loads in flight), then a fixed butterfly;
+// every lane returns the sum.
+__device__ __forceinline__ double finish_sum(const double* __restrict__ pt, int nblk, int lane) {
   double s = 0.0;
   for (int b0 = lane; b0 < nblk; b0 += 4 * 64) {
     double p[4];
@@ -113,27 +109,56 @@ static __global__ void k_dots_finish(const double* __restrict__ partial, int nbl
       if (b0 + 64 * e < nblk) s += p[e];
   }
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  return s;
+}
+
+// out[t] = sum of the nblk partials of output t (finish_sum, one wave per output).
+static __global__ void k_dots_finish(const double* __restrict__ partial, int nblk, int total, double* __restrict__ out) {
+  const int t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (t >= total) return;
+  const double s = finish_sum(partial + (size_t)t * nblk, nblk, lane);
   if (lane == 0) out[t] = s;
 }
 
-// One Gram-Schmidt pass fused with the next reduction, per rhs in kIter
-// (grid (blocks, nrhs), chunks as in k_dots):
+// Largest Krylov dimension per cycle (the C ABI's restart bound).
+constexpr int kMaxKrylov = 200;
+
+// One Gram-Schmidt pass fused with the reductions on either side, per rhs in
+// kIter (grid (blocks, nrhs), chunks as in k_dots; hpart holds the partial
+// sums of the pass's coefficients from a launch of the same grid):
+//   h[rhs*cnt + i] = sum of the partials of coefficient i (finish_sum, the
+//                    order of k_dots_finish; block 0 also stores them to h)
 //   w[rhs] -= sum_{i<cnt} h[rhs*cnt + i] V_i[rhs]
 // then the partial sums of V_i . w (i < cnt; next_norm = 0, the second pass's
-// coefficients) or of w . w (next_norm = 1, ||w||^2) over the block's chunk.
-// A thread re-reads only the elements of w that it wrote itself.
+// coefficients) or of w . w (next_norm = 1, ||w||^2) over the block's chunk
+// into partial (not hpart: other blocks still read it).  A thread re-reads
+// only the elements of w that it wrote itself.
 static __global__ void __launch_bounds__(256) k_sub_dots(const double* __restrict__ V, size_t stride_i, int cnt,
-                                                  const double* __restrict__ h, double* __restrict__ w, int64_t n,
+                                                  const double* __restrict__ hpart, double* __restrict__ h,
+                                                  double* __restrict__ w, int64_t n,
                                                   const int* __restrict__ state, int next_norm,
                                                   double* __restrict__ partial) {
   __shared__ double red[kDotGroup][4];
+  __shared__ double hs[kMaxKrylov + 1];
   const int rhs = blockIdx.y;
   if (state[rhs] != kIter) return;  // uniform over the block
+  {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int t = wave; t < cnt; t += blockDim.x >> 6) {
+      const double v = finish_sum(hpart + ((size_t)rhs * cnt + t) * gridDim.x, gridDim.x, lane);
+      if (lane == 0) {
+        hs[t] = v;
+        if (blockIdx.x == 0) h[(size_t)rhs * cnt + t] = v;
+      }
+    }
+    __syncthreads();
+  }
   int64_t lo, hi;
   block_chunk(n, lo, hi);
   const size_t roff = (size_t)rhs * n;
   double* wr = w + roff;
-  const double* hr = h + (size_t)rhs * cnt;
+  const double* hr = hs;
   // w_k -= h_i V_i,k in increasing i, kDotSweep vectors' loads of kDotUnroll
   // elements in flight together (clamped indices; the padded terms subtract
   // a zero coefficient)
@@ -264,15 +289,13 @@ static __global__ void k_normalize_next(const double* __restrict__ w, double* __
 // that ends, the back substitution of the rotated H y = g.  Every system's
 // state and residual estimate after the step also go to post_state /
 // post_rel (page-locked host memory the host reads after an event).
-static __global__ void k_arnoldi_step(int nrhs, int restart, int j, const double* __restrict__ h1,
-                                      const double* __restrict__ h2, const double* __restrict__ hnorm2,
-                                      double* __restrict__ H, double* __restrict__ cs, double* __restrict__ sn,
-                                      double* __restrict__ g, const double* __restrict__ bnorm, double tol,
-                                      int* __restrict__ state, int* __restrict__ kdim, double* __restrict__ relest,
-                                      double* __restrict__ y, int* __restrict__ post_state,
-                                      double* __restrict__ post_rel) {
-  const int rhs = blockIdx.x * blockDim.x + threadIdx.x;
-  if (rhs >= nrhs) return;
+// hn2 = ||w||^2 of this system.
+__device__ inline void arnoldi_step_rhs(int rhs, int restart, int j, const double* __restrict__ h1,
+                                        const double* __restrict__ h2, double hn2, double* __restrict__ H,
+                                        double* __restrict__ cs, double* __restrict__ sn, double* __restrict__ g,
+                                        const double* __restrict__ bnorm, double tol, int* __restrict__ state,
+                                        int* __restrict__ kdim, double* __restrict__ relest, double* __restrict__ y,
+                                        int* __restrict__ post_state, double* __restrict__ post_rel) {
   if (state[rhs] != kIter) {
     post_state[rhs] = state[rhs];
     post_rel[rhs] = relest[rhs];
@@ -293,7 +316,7 @@ static __global__ void k_arnoldi_step(int nrhs, int restart, int j, const double
     Hr[i * restart + j] = c[i] * a + s[i] * b;
     a = -s[i] * a + c[i] * b;
   }
-  const double b = sqrt(hnorm2[rhs]);
+  const double b = sqrt(hn2);
   const bool breakdown = (b == 0.0);
   const double den = sqrt(a * a + b * b);
   const double cj = den == 0.0 ? 1.0 : a / den, sj = den == 0.0 ? 0.0 : b / den;
@@ -322,6 +345,25 @@ static __global__ void k_arnoldi_step(int nrhs, int restart, int j, const double
     const double d = Hr[i * restart + i];
     yr[i] = d != 0.0 ? sum / d : 0.0;
   }
+}
+
+// ||w||^2 of every system from the partials of the second Gram-Schmidt pass
+// (finish_sum; one 64-thread block per system) into hnorm2, then the Arnoldi
+// step (arnoldi_step_rhs) of the system.
+static __global__ void k_finish_arnoldi(const double* __restrict__ partial, int nblk, int restart, int j,
+                                        const double* __restrict__ h1, const double* __restrict__ h2,
+                                        double* __restrict__ hnorm2, double* __restrict__ H, double* __restrict__ cs,
+                                        double* __restrict__ sn, double* __restrict__ g,
+                                        const double* __restrict__ bnorm, double tol, int* __restrict__ state,
+                                        int* __restrict__ kdim, double* __restrict__ relest, double* __restrict__ y,
+                                        int* __restrict__ post_state, double* __restrict__ post_rel) {
+  const int rhs = blockIdx.x;
+  const int lane = threadIdx.x & 63;
+  const double hn2 = finish_sum(partial + (size_t)rhs * nblk, nblk, lane);
+  if (threadIdx.x != 0) return;
+  hnorm2[rhs] = hn2;
+  arnoldi_step_rhs(rhs, restart, j, h1, h2, hn2, H, cs, sn, g, bnorm, tol, state, kdim, relest, y, post_state,
+                   post_rel);
 }
 
 // x[rhs] += sum_i y_i V_i[rhs] for rhs in kSolve.

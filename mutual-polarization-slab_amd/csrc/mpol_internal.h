@@ -296,7 +296,8 @@ struct mpol_ctx {
   // per-tile candidate images of the matrix-core spread/contract ([0] FP32, [1] FP64)
   TileImage img[2];
   // gmres
-  DevBuf<double> gV, gW, gR, gAx, gH, gcs, gsn, gg, gy, gh1, gh2, gpart, gbn, gbeta, grel, grn2, gx, gd, gnorm;
+  DevBuf<double> gV, gW, gR, gAx, gH, gcs, gsn, gg, gy, gh1, gh2, gpart, gpart2, gbn, gbeta, grel, grn2, gx, gd,
+      gnorm;
   DevBuf<int> gstate, gkdim;
   // while set (inside an Arnoldi step), the per-system GMRES state: operator
   // work of systems not in kIter is skipped
