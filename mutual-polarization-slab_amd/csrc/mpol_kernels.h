@@ -351,6 +351,10 @@ __global__ void __launch_bounds__(256) k_real_space(int n, int nrhs, int nsets, 
   const double* mr = m + (size_t)rhs * n * 3;
   T a0 = 0, a1 = 0, a2 = 0;
   const int q0 = off[ii], q1 = valid ? off[ii + 1] : q0;
+  // the particle's own terms load with its pair range (lane 0 uses them)
+  const int set = (nsets == 1) ? 0 : (diag_im ? rhs >> 1 : rhs);
+  const double dg = diag[(size_t)set * n + ii];
+  const double mi0 = mr[3 * ii], mi1 = mr[3 * ii + 1], mi2 = mr[3 * ii + 2];
   for (int q = q0 + g; q < q1; q += 16) {
     const int j = pj[q];
     const T m0 = (T)mr[3 * j], m1 = (T)mr[3 * j + 1], m2 = (T)mr[3 * j + 2];
@@ -365,11 +369,9 @@ __global__ void __launch_bounds__(256) k_real_space(int n, int nrhs, int nsets, 
   a1 = group_sum<T>(a1, 16);
   a2 = group_sum<T>(a2, 16);
   if (!valid || g != 0) return;
-  const int set = (nsets == 1) ? 0 : (diag_im ? rhs >> 1 : rhs);
-  const double dg = diag[(size_t)set * n + i];
-  double h0 = dg * mr[3 * i] + (double)a0;
-  double h1 = dg * mr[3 * i + 1] + (double)a1;
-  double h2 = dg * mr[3 * i + 2] + (double)a2;
+  double h0 = dg * mi0 + (double)a0;
+  double h1 = dg * mi1 + (double)a1;
+  double h2 = dg * mi2 + (double)a2;
   if (diag_im) {
     const double di = (rhs & 1) ? diag_im[(size_t)set * n + i] : -diag_im[(size_t)set * n + i];
     const double* mp = m + (size_t)(rhs ^ 1) * n * 3;

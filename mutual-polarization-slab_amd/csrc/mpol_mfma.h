@@ -119,6 +119,11 @@ __global__ void __launch_bounds__(256) k_contract_reduce(int n, long long total,
   for (int c = 0; c < NC; ++c) acc[c] = 0.0;
   const T* pr = part + (size_t)rhs * total * NC;
   const int k0 = valid ? poff[s] : 0, k1 = valid ? poff[s + 1] : 0;
+  // the output the sums add to loads with the slot range (lane 0 uses it)
+  double* o = out + ((size_t)rhs * n + perm[valid ? s : 0]) * NC;
+  double ov[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) ov[c] = o[c];
   for (int k = k0 + g; k < k1; k += 16) {
     const T* p = pr + (size_t)pslot[k] * NC;
 #pragma unroll
@@ -127,9 +132,8 @@ __global__ void __launch_bounds__(256) k_contract_reduce(int n, long long total,
 #pragma unroll
   for (int c = 0; c < NC; ++c) acc[c] = group_sum<double>(acc[c], 16);
   if (!valid || g != 0) return;
-  double* o = out + ((size_t)rhs * n + perm[s]) * NC;
 #pragma unroll
-  for (int c = 0; c < NC; ++c) o[c] += acc[c] * hvol;
+  for (int c = 0; c < NC; ++c) o[c] = ov[c] + acc[c] * hvol;
 }
 
 // Candidates of tile (ty0, tz0): the total of the column-bin ranges that
