@@ -65,6 +65,27 @@ def test_solve_mixed_within_1e6(gpu, path):
     assert relerr(p, np.real(d["p_scipy"])) < 1e-6
 
 
+def test_solve_long_krylov_cycle(gpu):
+    """One GMRES cycle of 150 Arnoldi steps (tolerance out of reach, so every
+    step runs): the Gram-Schmidt passes then orthogonalise against more than
+    the 64 vectors whose wave sums one LDS group of the dot-product kernels
+    holds, and the subtraction kernels finish up to 150 coefficients each from
+    the partial sums.  The dipoles still meet the reference's solve."""
+    path = SOLVE_FILES[0]
+    d = np.load(path)
+    op = golden("operator_n64_g21.npz")
+    print("case", os.path.basename(path), flush=True)
+    c = solve_ctx(d, op)
+    slab = 1.0 if "slab1" in path else 0.0
+    n = d["x"].shape[0]
+    b = np.tile(d["H"], n).reshape(1, n, 3)
+    res = c.solve(b, d["guess"][None], 1e-300, restart=150, maxit=1, slab_factor=slab)
+    p = res.x.cpu().numpy()[0]
+    assert res.matvecs_fp64 >= 150
+    assert res.max_relres <= 1e-12
+    assert relerr(p, np.real(d["p_scipy"])) < 1e-10
+
+
 def test_dropin_magnetic_dipole_and_run_capacitance(gpu):
     import capacitance
     for slab in (0, 1):
