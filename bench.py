@@ -11,6 +11,11 @@ with the mixed-precision path (FP32 operator inside FP64 iterative
 refinement, FP64 Arnoldi/Givens), at the loosest residual tolerance of the
 ladder 1e-6, 5e-7, 2e-7, 1e-7, 5e-8 whose dipoles are within 1e-6 (max-norm,
 relative) of a tight (tol 1e-10) FP64 solve -- chosen untimed, per path.
+The FP32 correction solves run either on the same operator (one level) or
+on a coarser one (two levels: a smaller grid at a looser Ewald tolerance,
+chosen untimed by mpol_hip.autotune.tune_inner); the FP64 residuals, which
+decide convergence, are always on the 256^3, P = 10 operator.  The faster of
+the two at the error bar is the step; both are reported.
 Inputs are resident in HBM before the timed region.  Also reported: the plain
 FP64 solve at its own such tolerance, a batched solve of the three field
 directions (capacitance tensor), matvec throughput in both precisions,
@@ -277,19 +282,49 @@ def main():
                 return t, e
         return TOL_LADDER[-1], e
 
-    tol, _ = calibrate(True)
-    log(f"timed mixed solves at tol {tol:g}: {args.warmup} warmup + {args.steps}")
-    for _ in range(args.warmup):
-        solve(True, tol)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    last = None
-    for _ in range(args.steps):
-        last = solve(True, tol)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    ms_step = 1e3 * dt / args.steps
-    mixed_err = rel_err(last.x.cpu().numpy(), p_ref)
+    def timed_mixed(t, label):
+        log(f"timed mixed solves ({label}) at tol {t:g}: {args.warmup} warmup + {args.steps}")
+        for _ in range(args.warmup):
+            solve(True, t)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = None
+        for _ in range(args.steps):
+            res = solve(True, t)
+        torch.cuda.synchronize()
+        return 1e3 * (time.perf_counter() - t0) / args.steps, res
+
+    # one-level mixed scheme: correction solves on this operator in FP32
+    tol1, _ = calibrate(True)
+    ms1, last1 = timed_mixed(tol1, "FP32 correction solves on the same operator")
+    err1 = rel_err(last1.x.cpu().numpy(), p_ref)
+    one_level = {"solves_per_s": round(1e3 / ms1, 3), "ms_per_solve": round(ms1, 3), "tol": tol1,
+                 "rel_dipole_err_vs_fp64": err1,
+                 "matvecs_per_solve": {"fp32": last1.matvecs_fp32, "fp64": last1.matvecs_fp64}}
+    # two-level: correction solves on a coarser operator (autotuned, untimed),
+    # FP64 residuals on this one
+    log("correction operator: timing the coarse candidates at the error bar (untimed)")
+    from mpol_hip.autotune import tune_inner
+    inner, inner_cfg, inner_report = tune_inner(
+        ctx, cfg["x"], cfg["box"], cfg["eps"], b, x0, TOL_LADDER,
+        accept=lambda r: rel_err(r.x.cpu().numpy(), p_ref) <= ERR_BAR, slab_factor=slab)
+    two_level = None
+    tol, ms_step, last, mixed_err = tol1, ms1, last1, err1
+    if inner is not None:
+        tol2 = inner_cfg["tol"]
+        ms2, last2 = timed_mixed(tol2, f"FP32 correction solves on the coarse operator {inner_cfg}")
+        err2 = rel_err(last2.x.cpu().numpy(), p_ref)
+        two_level = {"solves_per_s": round(1e3 / ms2, 3), "ms_per_solve": round(ms2, 3), "tol": tol2,
+                     "rel_dipole_err_vs_fp64": err2, "correction_operator": inner_cfg,
+                     "matvecs_per_solve": {"coarse_fp32": last2.matvecs_inner,
+                                           "fp32": last2.matvecs_fp32 - last2.matvecs_inner,
+                                           "fp64": last2.matvecs_fp64},
+                     "refinements": last2.refinements}
+        if ms2 < ms1 and err2 <= ERR_BAR:
+            tol, ms_step, last, mixed_err = tol2, ms2, last2, err2
+        if last is last1:
+            ctx.set_inner_operator(None)
+    inner_used = last is not last1
 
     # plain FP64 path at its own tolerance
     tol64, _ = calibrate(False)
@@ -414,17 +449,28 @@ def main():
         "config": {"workload": "N=1e5 unit spheres, phi=0.4, beta=0.9, E0=z, 256^3, P=10, xi=1.0612, "
                                "rc=3.503, slab_factor=0, 1 rhs, dipole error <= 1e-6; positions: FCC lattice "
                                f"with random vacancies + {cfg['sweeps']} hard-sphere MC sweeps (the workload "
-                               "names RSA + 1e4 relaxation sweeps; RSA saturates below phi = 0.4)",
+                               "names RSA + 1e4 relaxation sweeps; RSA saturates below phi = 0.4)"
+                               + ("; mixed scheme: FP64 residuals on this operator, FP32 correction solves on "
+                                  f"a coarser one ({inner_cfg['grid']}^3, P={inner_cfg['P']}, rule tol "
+                                  f"{inner_cfg['ewald_tol']:g}, floor {inner_cfg['floor']:g})" if inner_used else
+                                  "; mixed scheme: FP64 residuals, FP32 correction solves on this operator"),
                    "n": n, "ngrid": [int(v) for v in cfg["ngrid"]], "P": cfg["P"], "xi": round(cfg["xi"], 6),
                    "rc": round(cfg["rc"], 6), "tol": tol, "seed": cfg["seed"], "mc_sweeps": cfg["sweeps"],
                    "npairs": ginfo["npairs"]},
-        "matvecs_per_solve": {"fp32": last.matvecs_fp32, "fp64": last.matvecs_fp64},
+        "matvecs_per_solve": {"coarse_fp32": last.matvecs_inner, "fp32": last.matvecs_fp32 - last.matvecs_inner,
+                              "fp64": last.matvecs_fp64},
         "mixed_rel_dipole_err_vs_fp64": mixed_err,
+        "mixed_one_level": one_level,
+        "mixed_two_level": two_level,
+        "correction_operator_report": [{"setting": lab, "tol": t, "ms": round(ms, 3), "coarse_fp32": mi,
+                                        "fp32": m32 - mi, "fp64": m64}
+                                       for lab, t, ms, m32, m64, mi in inner_report],
         "fp64_solves_per_s": round(1e3 / ms64, 3),
         "fp64_tol": tol64,
         "fp64_rel_dipole_err_vs_tight": fp64_err,
         "fp64_matvecs_per_solve": r64.matvecs_fp64,
         "speedup_mixed_vs_fp64": round(ms64 / ms_step, 3),
+        "speedup_one_level_mixed_vs_fp64": round(ms64 / ms1, 3),
         "batched_three_fields": batched,
         "speedup_batched_mixed_vs_fp64": round((3e3 / ms3) / (1e3 / ms64), 3),
         "matvecs_per_s": {k: round(v, 1) for k, v in out.items()},

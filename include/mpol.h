@@ -66,6 +66,8 @@ typedef struct {
   int32_t converged;      /* number of right-hand sides that met tol        */
   int32_t refinements;    /* refinement sweeps performed (mixed mode)       */
   double max_relres;      /* max over rhs of the final true ||b-Ax||/||b||  */
+  int64_t matvecs_inner;  /* of matvecs_fp32, applications of the inner
+                             operator (mpol_set_inner_operator)             */
 } mpol_solve_info;
 
 typedef struct {
@@ -167,6 +169,23 @@ int mpol_matvec_components(mpol_ctx* ctx, const double* m_dev, double* hk_dev, d
 int mpol_solve(mpol_ctx* ctx, const double* b_dev, const double* x0_dev, double* x_dev, int32_t nrhs,
                const mpol_solve_opts* opts, mpol_solve_info* info);
 
+/* Operator of the mixed scheme's correction solves (no counterpart in the
+ * reference, whose solves have one operator): `inner` is another context on
+ * the same device and particles, with the same permittivity sets, typically
+ * the same system on a coarser grid at a looser Ewald tolerance.  With it,
+ * mpol_solve in mixed mode takes the first residual and runs every
+ * correction solve on the inner context's FP32 operator (on this context's
+ * stream), while every later residual stays on this context's operator: in
+ * FP32 while it only seeds the next correction, in FP64 once it may meet tol
+ * (and the solve ends on FP64 residuals only), so the result meets tol for
+ * this context's operator.  A correction solve stops at `floor` (relative)
+ * at the lowest, about the relative distance between the two operators
+ * (refining past it gains nothing).  matvecs_inner counts the inner
+ * operator's applications (also in matvecs_fp32).  The dipole
+ * operator only; the inner context must outlive its use and not be used
+ * concurrently.  inner = NULL removes it. */
+int mpol_set_inner_operator(mpol_ctx* ctx, mpol_ctx* inner, double floor);
+
 /* Dipole + quadrupole operator (8 unknowns per particle).  The seven
  * quadrupole tables (host FP64, 7 x ntab, on the rvals of mpol_set_tables, in
  * the order field_quad_1, field_quad_2, field_quad_3, grad_quad_1,
@@ -256,7 +275,7 @@ int mpol_ewald_forces(mpol_ctx* ctx, const double* p_dev, int32_t nrhs, const do
                       const double* f2_host, double* forces_dev);
 
 /* Launch-shape tuning (no counterpart in the reference; used by the
- * autotuner): knobs[0..9] = spread/contract tile rows along y and z (ty*tz a
+ * autotuner): knobs[0..10] = spread/contract tile rows along y and z (ty*tz a
  * power of two, each <= 4), particles staged per chunk (32..256), y-pass and
  * z-pass columns per workgroup (<= 64), z-pass threads (multiple of 64),
  * spread/contract engine (1 matrix cores over per-tile candidate images,
@@ -264,14 +283,22 @@ int mpol_ewald_forces(mpol_ctx* ctx, const double* p_dev, int32_t nrhs, const do
  * kernels (atomic scatter, LDS-staged contraction) and the separate y/z
  * passes instead of the stored-entry gather/contraction and the one-kernel
  * y/z passes), right-hand sides per matrix-core spread/contract workgroup
- * (1..64: one staged tile image serves them in turn), the dipole register
+ * (1..64: one staged tile image serves them in turn; automatic: up to four,
+ * and one for the dipole spread where its staged chunk can share LDS with
+ * the tile lines), the dipole register
  * z pass for nz = 256 (1 rank-one form on the lines with non-Nyquist x and y
  * wave numbers, 2 the full three-component form everywhere), the wave-space
  * operator for complex permittivities (1 the reference's, whose Nyquist
  * planes carry k = -N/2 on both halves of the spectrum: the Hermitian part
  * plus an extra wave-space pass for the anti-Hermitian part; 2 the
- * Hermitian part only, one pass).  0 keeps the automatic choice (1 for the
- * last two); nknobs may be shorter than 10. */
+ * Hermitian part only, one pass), the order of the y, z and inverse y
+ * passes over several systems (1 system by system, so that each pass reads
+ * what the one before wrote while it is still in the last-level cache; 2 each
+ * pass over all systems).  0 keeps the automatic choice (1 for the z pass
+ * and the complex operator; for the pass order, system by system when the
+ * systems' spectra overflow the 256 MB last-level cache and one system's
+ * fill a quarter of it, except in timed stage runs, mpol_matvec_timed);
+ * nknobs may be shorter than 11. */
 int mpol_set_tuning(mpol_ctx* ctx, const int32_t* knobs, int32_t nknobs);
 
 /* The same knobs for the operator in one precision only (MPOL_FP32 or

@@ -411,6 +411,21 @@ int mpol_destroy(mpol_ctx* c) {
   return MPOL_OK;
 }
 
+int mpol_set_inner_operator(mpol_ctx* c, mpol_ctx* inner, double floor) {
+  if (!c) return fail(MPOL_EINVAL, "null context");
+  if (inner == c) return fail(MPOL_EINVAL, "a context cannot be its own inner operator");
+  if (inner != nullptr) {
+    if (inner->inner != nullptr) return fail(MPOL_EINVAL, "the inner operator has an inner operator of its own");
+    if (!(floor >= 0.0 && floor < 1.0)) return fail(MPOL_EINVAL, "inner floor must be in [0, 1)");
+    if (inner->device != c->device) return fail(MPOL_EINVAL, "inner operator on another device");
+    if (!inner->have_geo || !c->have_geo) return fail(MPOL_ESTATE, "set the geometry of both contexts first");
+    if (inner->n != c->n) return fail(MPOL_EINVAL, "inner operator on another number of particles");
+  }
+  c->inner = inner;
+  c->inner_floor = inner != nullptr ? floor : 0.0;
+  return MPOL_OK;
+}
+
 int mpol_set_stream(mpol_ctx* c, void* stream) {
   if (!c) return fail(MPOL_EINVAL, "null context");
   c->stream = reinterpret_cast<hipStream_t>(stream);
@@ -778,6 +793,21 @@ int solve_impl(mpol_ctx* c, int kind, const double* b, const double* x0, double*
                             &conv));
     inf.matvecs_fp64 = nmv;
   } else {
+    // the correction operator: the inner context when one is set (dipole
+    // operator), launching on this context's stream for the solve
+    mpol_ctx* ci = (kind == kOpDipole && c->inner != nullptr) ? c->inner : c;
+    if (ci != c) {
+      CHECK_STATUS(check_ready(ci));
+      if (ci->n != c->n || ci->nsets != c->nsets || ci->cplx_eps != c->cplx_eps || ci->device != c->device)
+        return fail(MPOL_EINVAL, "inner operator: particles, permittivity sets and device must match the context's");
+    }
+    struct StreamSwap {
+      mpol_ctx* ctx;
+      hipStream_t saved;
+      ~StreamSwap() { ctx->stream = saved; }
+    } swap{ci, ci->stream};
+    ci->stream = st;
+    const double floor_in = ci != c ? std::max(c->inner_floor, kInnerFloor) : kInnerFloor;
     HIPALLOC(c->gd, (size_t)tot);
     HIPALLOC(c->gx, (size_t)tot);
     HIPALLOC(c->gR, (size_t)tot);
@@ -791,14 +821,25 @@ int solve_impl(mpol_ctx* c, int kind, const double* b, const double* x0, double*
     int64_t nmv64 = 0, nmv32 = 0;
     int ref = 0;
     // The first residual (from the caller's guess) only seeds the first
-    // correction solve, so it is taken with the FP32 operator; every later
-    // residual, and any that would end the loop, is FP64.
-    bool fp32_residual = o->max_refine > 0;
+    // correction solve, so it is taken with the FP32 operator of the
+    // correction solves.  With one operator every later residual is FP64.
+    // With an inner operator the later ones are FP32 on this operator while
+    // they only seed the next correction (its rounding, ~4e-8 relative, is
+    // far below what a sweep removes) and FP64 once the residual may meet
+    // tol: predicted, per sweep, as the largest residual times the larger of
+    // the correction's tolerance and the reduction the last sweep achieved.
+    // Any FP32 residual that meets tol is certified by an FP64 one, so the
+    // loop ends on FP64 residuals only.
+    enum { kResFp64, kResInner, kResFp32 };
+    int rkind = o->max_refine > 0 ? kResInner : kResFp64;
+    int64_t nmv_inner = 0;
+    double last_max = 0.0, reduction = floor_in;
     for (;; ++ref) {
-      const int rprec = fp32_residual ? MPOL_FP32 : MPOL_FP64;
-      CHECK_STATUS(apply(c, kind, x, c->gAx.p, nrhs, rprec, slab));
-      if (fp32_residual) ++nmv32;
-      else ++nmv64;
+      mpol_ctx* const rc = rkind == kResInner ? ci : c;
+      CHECK_STATUS(apply(rc, kind, x, c->gAx.p, nrhs, rkind == kResFp64 ? MPOL_FP64 : MPOL_FP32, slab));
+      if (rkind == kResFp64) ++nmv64;
+      else ++nmv32;
+      if (rkind == kResInner && ci != c) ++nmv_inner;
       hipLaunchKernelGGL(k_residual, dim3(cdiv(tot, 256)), dim3(256), 0, st, b, c->gAx.p, c->gd.p, tot);
       dots(c, c->gd.p, 0, 1, c->gd.p, n3, nrhs, c->gnorm.p + nrhs);
       if (!have_bn2) {
@@ -809,35 +850,42 @@ int solve_impl(mpol_ctx* c, int kind, const double* b, const double* x0, double*
       } else {
         CHECK_STATUS(read_doubles(c, c->gnorm.p + nrhs, nrhs, rn2.data()));
       }
-      double ratio = 1e300;
+      double ratio = 1e300, max_rel = 0.0;
       conv = 0;
       for (int r = 0; r < nrhs; ++r) {
         rel[r] = bn2[r] > 0.0 ? std::sqrt(rn2[r] / bn2[r]) : 0.0;
+        max_rel = std::max(max_rel, rel[r]);
         if (rel[r] <= o->tol) ++conv;
         else if (rn2[r] > 0.0) ratio = std::min(ratio, std::sqrt(bn2[r] / rn2[r]));
       }
-      if (fp32_residual) {
-        fp32_residual = false;
-        if (conv == nrhs) {  // certify with an FP64 residual
+      if (rkind != kResFp64) {
+        if (conv == nrhs || ref >= o->max_refine) {  // certify with an FP64 residual
+          rkind = kResFp64;
           --ref;
           continue;
         }
       } else if (conv == nrhs || ref >= o->max_refine) {
         break;
       }
+      if (last_max > 0.0 && max_rel > 0.0) reduction = std::min(1.0, max_rel / last_max);
+      last_max = max_rel;
       // inner tolerance relative to the current residual, not below
       // kInnerFloor: the FP32 operator's rounding (~1e-7 relative) limits how
-      // far its own residual estimate can usefully fall
+      // far its own residual estimate can usefully fall (an inner operator's
+      // distance from this one, by the floor set with it)
       double tin = 0.3 * o->tol * ratio;
-      tin = std::min(0.1, std::max(tin, kInnerFloor));
+      tin = std::min(0.1, std::max(tin, floor_in));
       // the residual becomes the correction solve's rhs (gx); the correction starts at zero in gd
       hipLaunchKernelGGL(k_take, dim3(cdiv(tot, 256)), dim3(256), 0, st, c->gd.p, c->gx.p, tot);
       int64_t nmv = 0;
-      CHECK_STATUS(gmres_core(c, kind, c->gx.p, c->gd.p, nrhs, tin, std::max(o->restart, kInnerRestart),
+      CHECK_STATUS(gmres_core(ci, kind, c->gx.p, c->gd.p, nrhs, tin, std::max(o->restart, kInnerRestart),
                               std::min(o->maxit, kInnerCycles), MPOL_FP32, slab, &nmv, nullptr, nullptr, true));
       nmv32 += nmv;
+      if (ci != c) nmv_inner += nmv;
       hipLaunchKernelGGL(k_axpy_all, dim3(cdiv(tot, 256)), dim3(256), 0, st, 1.0, c->gd.p, x, tot);
+      rkind = (ci == c || max_rel * std::max(tin, reduction) <= o->tol) ? kResFp64 : kResFp32;
     }
+    inf.matvecs_inner = nmv_inner;
     inf.matvecs_fp64 = nmv64;
     inf.matvecs_fp32 = nmv32;
     inf.refinements = ref;
@@ -1115,6 +1163,8 @@ int mpol_set_tuning_for(mpol_ctx* c, int32_t precision, const int32_t* knobs, in
   if (t[8] < 0 || t[8] > 2) return fail(MPOL_EINVAL, "z register pass must be 0 (automatic), 1 (rank-one) or 2 (full)");
   if (t[9] < 0 || t[9] > 2)
     return fail(MPOL_EINVAL, "complex Nyquist part must be 0 (automatic), 1 (reference operator) or 2 (Hermitian part)");
+  if (t[10] < 0 || t[10] > 2)
+    return fail(MPOL_EINVAL, "y/z pass order must be 0 (automatic), 1 (system by system) or 2 (pass by pass)");
   for (int p = 0; p < 2; ++p) {
     if (precision != -1 && p != (precision == MPOL_FP32 ? 1 : 0)) continue;
     for (int i = 0; i < kTuneKnobs; ++i) c->tune[p][i] = t[i];

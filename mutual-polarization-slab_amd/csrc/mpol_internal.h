@@ -192,7 +192,8 @@ inline ColCfg col_cfg(int n, bool generic, int kx, int lines_per_col, size_t csi
 
 constexpr size_t kLdsTile = 48 * 1024;
 constexpr size_t kLdsMfma = 80 * 1024;  // two matrix-core tile workgroups per CU
-constexpr int kTuneKnobs = 10;
+constexpr int kTuneKnobs = 11;
+constexpr size_t kLastLevelCache = (size_t)256 << 20;  // MI355X Infinity Cache bytes
 constexpr int kRhsGroup = 4;  // automatic right-hand sides per matrix-core tile workgroup
 
 // Candidates of every tile, sorted by stencil start along x within chunks of
@@ -220,6 +221,12 @@ struct mpol_ctx {
   }
   int device = 0;
   hipStream_t stream = nullptr;  // launch stream; nullptr is the default stream
+  // optional operator of the mixed scheme's correction solves and first
+  // residual (mpol_set_inner_operator): another context on the same
+  // particles, typically a coarser grid at a looser Ewald tolerance; its
+  // correction solves stop at inner_floor at the lowest
+  mpol_ctx* inner = nullptr;
+  double inner_floor = 0.0;
   // tables
   int ntab = 0;
   double self_perp = 0.0;
@@ -664,12 +671,24 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   // the spread's staged chunk shares LDS with the tile lines when a
   // workgroup serves one right-hand side and keeps its accumulators in
   // registers (with more, aliasing would restage the image per right-hand
-  // side, which the groups exist to avoid)
+  // side, which the groups exist to avoid).  With the group knob automatic,
+  // the FP32 dipole spread takes one right-hand side per workgroup where it
+  // can alias (more workgroups per CU: three-right-hand-side FP32 operator
+  // 1.55 -> 1.51 ms at N = 1e5, 256^3; the FP64 one 3.21 -> 3.30 ms, so FP64
+  // keeps the group), the contraction keeps the group.
+  TileArgs tms = tma;
+  dim3 msgrid = mgrid;
   size_t spread_lds = tm.lds;
-  tma.alias = 0;
-  if (use_mfma && tma.rg == 1 && mfma_reg_acc(c->ng[0], NC, tm.rows, sizeof(T))) {
-    tma.alias = 1;
-    spread_lds = mfma_tile_lds_alias<NC>(c->ng[0], cs, c->P, tm.ncl, tm.lstride, tm.ch, tm.rows);
+  tms.alias = 0;
+  if (use_mfma && mfma_reg_acc(c->ng[0], NC, tm.rows, sizeof(T))) {
+    if (NC == kOpDipole && sizeof(T) == 4 && knobs<T>(c)[7] == 0) {
+      tms.rg = 1;
+      msgrid.z = nrhs;
+    }
+    if (tms.rg == 1) {
+      tms.alias = 1;
+      spread_lds = mfma_tile_lds_alias<NC>(c->ng[0], cs, c->P, tm.ncl, tm.lstride, tm.ch, tm.rows);
+    }
   }
   // FP32: atomic-free gather spread; FP64: LDS FP64 atomics (faster there)
   const bool gather = sizeof(T) == 4 && is_ct_size(c->ng[0]) && c->P + 3 <= c->ng[0];
@@ -677,13 +696,13 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
 #define LAUNCH_SPREAD_MFMA(N, ...)                                                              \
   do {                                                                                          \
     if (tm.rows == 16) {                                                                        \
-      launch_mfma16<T, N, NC>(true, mgrid, spread_lds, st, tma, ima, spec);                     \
+      launch_mfma16<T, N, NC>(true, msgrid, spread_lds, st, tms, ima, spec);                    \
     } else if (tm.rows == 8) {                                                                  \
       set_lds(k_spread_mfma_xfft<T, N, NC, 8>, spread_lds);                                    \
-      hipLaunchKernelGGL((k_spread_mfma_xfft<T, N, NC, 8>), mgrid, dim3(256), spread_lds, st, tma, ima, spec); \
+      hipLaunchKernelGGL((k_spread_mfma_xfft<T, N, NC, 8>), msgrid, dim3(256), spread_lds, st, tms, ima, spec); \
     } else {                                                                                    \
       set_lds(k_spread_mfma_xfft<T, N, NC, 4>, spread_lds);                                    \
-      hipLaunchKernelGGL((k_spread_mfma_xfft<T, N, NC, 4>), mgrid, dim3(256), spread_lds, st, tma, ima, spec); \
+      hipLaunchKernelGGL((k_spread_mfma_xfft<T, N, NC, 4>), msgrid, dim3(256), spread_lds, st, tms, ima, spec); \
     }                                                                                           \
   } while (0)
     DISPATCH_CT(c->ng[0], LAUNCH_SPREAD_MFMA, 0);
@@ -754,6 +773,26 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   const bool rank1 = zreg && tail_blocks(c->kx, TXR) > 0 && knobs<T>(c)[8] != 2 && sp.jk != nullptr &&
                      is_ct_size(c->ng[1]) && stop_after >= kStages;
 
+  // Several systems: the y, z and inverse y passes run system by system, so
+  // that each pass finds much of the spectrum the one before it wrote still
+  // in the last-level cache (256 MB; a system's half spectra are 200 MB in
+  // FP32 at 256^3, three systems' 600 MB).  Knob 10: 1 system by system, 2
+  // each pass over all systems; automatic: system by system when all the
+  // systems' spectra overflow that cache and one system's fill a quarter of
+  // it (smaller launches leave the chip part idle: the three-system FP32
+  // operator at 128^3 is slower system by system), except in timed runs,
+  // whose stage events need the pass order.  The full operator only (the
+  // stage stops return between passes).
+  const int nsys = nrhs / rps;
+  const int fft_order = knobs<T>(c)[10];
+  const bool big = spec_bytes > kLastLevelCache && spec_bytes / nsys >= kLastLevelCache / 4;
+  const bool by_sys = nsys > 1 && stop_after >= kStages &&
+                      (fft_order == 1 || (fft_order == 0 && ev == nullptr && big));
+  const int nbatch = by_sys ? nsys : 1;  // launch batches of the three passes
+  const int rb = by_sys ? rps : nrhs;    // right-hand sides per batch
+  const size_t batch_stride = (size_t)rb * NC * c->ng[2] * c->ng[1] * c->kxs;
+  cplx<T>* const spec_all = spec;
+
   // y axis forward
   const ColCfg cy = col_cfg(c->ng[1], generic_fft(c->plan[1]), c->kx, 1, cs, kLdsCol,
                             knobs<T>(c)[3] > 0 ? knobs<T>(c)[3] : (sizeof(T) == 8 ? 8 : 16));  // 128-byte rows
@@ -762,13 +801,13 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   ay.plan = c->plan[1]; ay.tw = tw_ptr<T>(c, 1);
   ay.gate = gate;
   ay.gdiv = NC * rps;  // batch index = rhs * NC + component
-  const dim3 ygrid(cdiv(c->kx, cy.tx), c->ng[2], NC * nrhs);
+  const dim3 ygrid(cdiv(c->kx, cy.tx), c->ng[2], NC * rb);
   if (cy.lds > kLdsMax) return fail(MPOL_EINVAL, "y-pass columns do not fit in LDS");
   // rank-one scheme: x and y components together, the z component in
   // workgroups of the other role (k_fft_y_xy)
   ColArgs axy = ay;
   axy.gdiv = rps;  // right-hand side = blockIdx.z / 2
-  const dim3 xygrid(cdiv(c->kx, cy.tx), c->ng[2], 2 * nrhs);
+  const dim3 xygrid(cdiv(c->kx, cy.tx), c->ng[2], 2 * rb);
   const size_t xylds = ((size_t)c->ng[1] + 2 * (size_t)cy.tx * cy.lstride * (generic_fft(c->plan[1]) ? 2 : 1)) * cs;
   // the 2 tx lines in one pass of the line FFTs (ny / e threads per line), 256 to 512 threads
   const int xythreads = std::min(512, std::max(256, 2 * cy.tx * (c->ng[1] / ct_plan(c->ng[1]).e)));
@@ -790,9 +829,6 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
     }                                                                                           \
     hipLaunchKernelGGL((k_fft_y<T, E, N>), ygrid, dim3(256), cy.lds, st, ay, spec, INV);       \
   } while (0)
-  DISPATCH_FFT(c->ng[1], ay.plan.e, LAUNCH_FFTY, 0);
-  if (ev) HIPCHK(hipEventRecord(ev[4], st));
-  if (stop_after <= 3) return MPOL_OK;
 
   // z axis forward + scale + inverse
   const ColCfg cz = col_cfg(c->ng[2], generic_fft(c->plan[2]), c->kx, NC, cs, kLdsCol,
@@ -802,7 +838,7 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
   az.plan = c->plan[2]; az.tw = tw_ptr<T>(c, 2);
   az.gate = gate;
   az.gdiv = rps;
-  const dim3 zgrid(cdiv(c->kx, cz.tx), c->ng[1], nrhs);
+  const dim3 zgrid(cdiv(c->kx, cz.tx), c->ng[1], rb);
   if (cz.lds > kLdsMax) return fail(MPOL_EINVAL, "z-pass columns do not fit in LDS");
   // one thread group per line of the z transform, all NC*tx lines at once
   int zthreads = 256;
@@ -817,35 +853,53 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
     set_lds(k_fft_z_scale<T, E, N, NC>, cz.lds);                                                \
     hipLaunchKernelGGL((k_fft_z_scale<T, E, N, NC>), zgrid, dim3(zthreads), cz.lds, st, az, sp, spec); \
   } while (0)
-  // nz = 256 dipole grids: transforms in registers (unless a z column width is forced)
-  if (zreg) {
-    ColArgs ar = az;
-    ar.tx = TXR;
-    ar.tail = tail_blocks(c->kx, TXR);
-    if (rank1) {
-      // rank-one form on the lines with non-Nyquist x and y wave numbers, the
-      // full form on the last column and the row ky = ny/2 (eight lines per
-      // workgroup) in the same launch
-      const int nfull = (c->ng[1] % 2 == 0 ? c->kx - 1 : 0) + c->ng[1];
-      const dim3 r1grid(ar.tail + 1, std::max(c->ng[1], cdiv(nfull, 8)), nrhs);
-      if constexpr (sizeof(T) == 4)
-        hipLaunchKernelGGL((k_fft_z_scale_reg256_rank1<T, TXR, false>), r1grid, dim3(256), 0, st, ar, sp, spec);
-      else
-        hipLaunchKernelGGL((k_fft_z_scale_reg256_rank1<T, TXR, true>), r1grid, dim3(256), 0, st, ar, sp, spec);
-    } else {
-      const dim3 rgrid(ar.tail > 0 ? ar.tail + 1 : cdiv(c->kx, TXR), c->ng[1], nrhs);
-      if constexpr (sizeof(T) == 4)
-        hipLaunchKernelGGL((k_fft_z_scale_reg256<T, TXR>), rgrid, dim3(TXR * 16), 0, st, ar, sp, spec);
-      else
-        hipLaunchKernelGGL((k_fft_z_scale_reg256_pair<T, TXR>), rgrid, dim3(TXR * 32), 0, st, ar, sp, spec);
-    }
-  } else {
-    DISPATCH_FFT(c->ng[2], az.plan.e, LAUNCH_FFTZ, 0);
-  }
-  if (ev) HIPCHK(hipEventRecord(ev[5], st));
-  if (stop_after <= 4) return MPOL_OK;
+  for (int sb = 0; sb < nbatch; ++sb) {
+    // this batch's spectra and system states (the kernels count right-hand
+    // sides from the batch's first; the launch macros read spec)
+    cplx<T>* const spec = spec_all + (size_t)sb * batch_stride;
+    const int* const gate_b = gate != nullptr ? gate + (size_t)sb * (rb / rps) : nullptr;
+    ay.gate = gate_b;
+    axy.gate = gate_b;
+    az.gate = gate_b;
+    DISPATCH_FFT(c->ng[1], ay.plan.e, LAUNCH_FFTY, 0);
+    if (ev && !by_sys) HIPCHK(hipEventRecord(ev[4], st));
+    if (stop_after <= 3) return MPOL_OK;
 
-  DISPATCH_FFT(c->ng[1], ay.plan.e, LAUNCH_FFTY, 1);
+    // nz = 256 dipole grids: transforms in registers (unless a z column width is forced)
+    if (zreg) {
+      ColArgs ar = az;
+      ar.tx = TXR;
+      ar.tail = tail_blocks(c->kx, TXR);
+      if (rank1) {
+        // rank-one form on the lines with non-Nyquist x and y wave numbers, the
+        // full form on the last column and the row ky = ny/2 (eight lines per
+        // workgroup) in the same launch
+        const int nfull = (c->ng[1] % 2 == 0 ? c->kx - 1 : 0) + c->ng[1];
+        const dim3 r1grid(ar.tail + 1, std::max(c->ng[1], cdiv(nfull, 8)), rb);
+        if constexpr (sizeof(T) == 4)
+          hipLaunchKernelGGL((k_fft_z_scale_reg256_rank1<T, TXR, false>), r1grid, dim3(256), 0, st, ar, sp, spec);
+        else
+          hipLaunchKernelGGL((k_fft_z_scale_reg256_rank1<T, TXR, true>), r1grid, dim3(256), 0, st, ar, sp, spec);
+      } else {
+        const dim3 rgrid(ar.tail > 0 ? ar.tail + 1 : cdiv(c->kx, TXR), c->ng[1], rb);
+        if constexpr (sizeof(T) == 4)
+          hipLaunchKernelGGL((k_fft_z_scale_reg256<T, TXR>), rgrid, dim3(TXR * 16), 0, st, ar, sp, spec);
+        else
+          hipLaunchKernelGGL((k_fft_z_scale_reg256_pair<T, TXR>), rgrid, dim3(TXR * 32), 0, st, ar, sp, spec);
+      }
+    } else {
+      DISPATCH_FFT(c->ng[2], az.plan.e, LAUNCH_FFTZ, 0);
+    }
+    if (ev && !by_sys) HIPCHK(hipEventRecord(ev[5], st));
+    if (stop_after <= 4) return MPOL_OK;
+
+    DISPATCH_FFT(c->ng[1], ay.plan.e, LAUNCH_FFTY, 1);
+  }
+  if (ev && by_sys) {
+    // system by system: the forward y stage carries the three passes
+    HIPCHK(hipEventRecord(ev[4], st));
+    HIPCHK(hipEventRecord(ev[5], st));
+  }
   if (ev) HIPCHK(hipEventRecord(ev[6], st));
   if (stop_after <= 5) return MPOL_OK;
 

@@ -22,7 +22,7 @@ class SolveOpts(ctypes.Structure):
 class SolveInfo(ctypes.Structure):
     _fields_ = [("matvecs_fp64", ctypes.c_int64), ("matvecs_fp32", ctypes.c_int64),
                 ("converged", ctypes.c_int32), ("refinements", ctypes.c_int32),
-                ("max_relres", ctypes.c_double)]
+                ("max_relres", ctypes.c_double), ("matvecs_inner", ctypes.c_int64)]
 
 
 class GeometrySummary(ctypes.Structure):
@@ -42,6 +42,7 @@ SIGNATURES = [
     ("mpol_create", ctypes.c_int, [ctypes.POINTER(_VP), ctypes.c_int, _VP]),
     ("mpol_destroy", ctypes.c_int, [_VP]),
     ("mpol_set_stream", ctypes.c_int, [_VP, _VP]),
+    ("mpol_set_inner_operator", ctypes.c_int, [_VP, _VP, ctypes.c_double]),
     ("mpol_synchronize", ctypes.c_int, [_VP]),
     ("mpol_set_tables", ctypes.c_int, [_VP, _DP, _DP, _DP, ctypes.c_int64]),
     ("mpol_set_geometry", ctypes.c_int, [_VP, _VP, ctypes.c_int64, _DP, _I32P, ctypes.c_int32, ctypes.c_double,

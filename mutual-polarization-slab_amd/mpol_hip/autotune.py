@@ -25,6 +25,16 @@ tune_ewald(x, box, eps, target)
     one's operator error is measured on these particles against a tight
     (tol 1e-11) FP64 operator, and the fastest candidate whose error is at
     most target is returned.
+
+coarse_operator(x, box, eps, ewald_tol, grid) / tune_inner(ctx, ...)
+    The correction operator of the mixed solves: the same particles on a
+    coarser grid at a looser rule tolerance (e.g. 128^3, P = 6 at 1e-4 for a
+    256^3, P = 10 operator at 1e-6), set with Context.set_inner_operator.
+    Its FP32 correction solves cost less per step; the FP64 residuals on the
+    full operator still decide convergence, so the solve meets its tolerance
+    for the full operator, in more refinement sweeps (each removes about the
+    two operators' relative distance, the floor).  tune_inner times the
+    solve with none and with each candidate and keeps the fastest.
 """
 from __future__ import annotations
 
@@ -193,3 +203,93 @@ def tune_ewald(x, box, eps, target, precision=FP32, tols=None, sizes=CT_SIZES, r
     ms, err, xi, ng, P, rc = ok[0]
     eta = P * (np.asarray(box, dtype=np.float64) / ng * xi) ** 2 / math.pi
     return xi, ng, P, eta, rc, ms, report
+
+
+# Correction-operator candidates of the mixed solves, (Ewald tolerance of the
+# parameter rule, grid nodes along the longest axis), and the floors their
+# correction solves stop at.
+INNER_CANDIDATES = ((1e-4, 128), (1e-4, 96), (1e-3, 96), (1e-5, 128), (1e-5, 160))
+INNER_FLOORS = (1e-3, 3e-4, 1e-4)
+
+
+def coarse_operator(x, box, eps, ewald_tol=1e-4, grid=128, device=None, tables=None):
+    """(Context, (xi, ngrid, P, eta, rc)): the particles x with permittivities
+    eps (one set) on the Ewald parameters of `grid` nodes along the longest
+    box axis at rule tolerance ewald_tol (ewald_candidates), for
+    Context.set_inner_operator.  tables(xi) -> (perp, para, rvals), default
+    the reference's table grid."""
+    from .context import Context, default_tables
+    cands = ewald_candidates(box, ewald_tol, sizes=(int(grid),))
+    if not cands:
+        raise ValueError(f"no Ewald parameters with {grid} nodes at tol {ewald_tol} for box {box}")
+    xi, ng, P, eta, rc = cands[0]
+    perp, para, rv = (tables or default_tables)(xi)
+    c = Context(device)
+    c.set_tables(perp, para, rv)
+    c.set_geometry(x, box, ng, P, xi, eta, rc)
+    c.set_permittivity(eps)
+    return c, (xi, ng, P, eta, rc)
+
+
+def tune_inner(ctx, x, box, eps, b, x0, tols, accept=None, candidates=INNER_CANDIDATES, floors=INNER_FLOORS,
+               repeats=3, slab_factor=1.0, restart=10, maxit=100):
+    """Choose the correction operator of ctx's mixed solves by timing them:
+    none (ctx's own FP32 operator) and each coarse candidate x floor.  For
+    each setting the loosest tolerance of `tols` (loosest first) whose solve
+    of b from x0 converges (its FP64 residual on ctx's operator meets it) and
+    passes accept(result) (default: always) is timed over `repeats` solves;
+    the fastest setting is left on ctx.  Returns (inner Context or None,
+    {"ewald_tol", "grid", "P", "floor", "tol"} or None for none, report
+    [(label, tol, ms, matvecs_fp32, matvecs_fp64, matvecs_inner)])."""
+    import torch
+
+    nrhs = b.shape[0] if b.ndim == 3 else 1
+
+    def run(t):
+        return ctx.solve(b, x0, t, restart=restart, maxit=maxit, mixed=True, slab_factor=slab_factor)
+
+    def timed():
+        for t in tols:
+            r = run(t)
+            if r.converged == nrhs and (accept is None or accept(r)):
+                break
+        else:
+            return None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(repeats):
+            r = run(t)
+        torch.cuda.synchronize()
+        return t, 1e3 * (time.perf_counter() - t0) / repeats, r
+
+    ctx.set_inner_operator(None)
+    report = []
+    best = (math.inf, None, None)
+    got = timed()
+    if got is not None:
+        t, ms, r = got
+        report.append(("none", t, ms, r.matvecs_fp32, r.matvecs_fp64, r.matvecs_inner))
+        best = (ms, None, None)
+    for etol, grid in candidates:
+        try:
+            inner, (xi, ng, P, eta, rc) = coarse_operator(x, box, eps, etol, grid, device=ctx.device)
+        except ValueError:
+            continue
+        for fl in floors:
+            ctx.set_inner_operator(inner, fl)
+            got = timed()
+            if got is None:
+                continue
+            t, ms, r = got
+            report.append((f"tol {etol:g} grid {grid} P {P} floor {fl:g}", t, ms, r.matvecs_fp32, r.matvecs_fp64,
+                           r.matvecs_inner))
+            if ms < best[0]:
+                if best[1] is not None and best[1] is not inner:
+                    best[1].close()
+                best = (ms, inner, {"ewald_tol": etol, "grid": int(grid), "P": int(P), "floor": fl, "tol": t})
+        ctx.set_inner_operator(None)
+        if best[1] is not inner:
+            inner.close()
+    if best[1] is not None:
+        ctx.set_inner_operator(best[1], best[2]["floor"])
+    return best[1], best[2], report

@@ -131,6 +131,7 @@ class SolveResult:
     converged: int
     refinements: int
     max_relres: float
+    matvecs_inner: int = 0  # of matvecs_fp32, on the inner operator (set_inner_operator)
 
     @property
     def flag(self) -> int:
@@ -153,6 +154,7 @@ class Context:
         self._ntab = 0
         self.complex = False
         self._keep = []
+        self._inner = None
 
     # -- lifetime -------------------------------------------------------
     def close(self):
@@ -246,7 +248,7 @@ class Context:
         _lib.check(self._lib.mpol_set_quad_tables(self._ctx, _lib.dptr(q), q.shape[1]))
 
     TUNING_KNOBS = ("tile_ty", "tile_tz", "chunk", "y_cols", "z_cols", "z_threads", "engine", "rhs_group", "z_rank1",
-                    "cplx_nyquist")
+                    "cplx_nyquist", "fft_order")
 
     def set_tuning(self, precision=None, **knobs):
         """Override launch shapes (0 = automatic); see TUNING_KNOBS.  precision
@@ -421,10 +423,22 @@ class Context:
         _lib.check(fn(self._ctx, ctypes.c_void_p(bt.data_ptr()), ctypes.c_void_p(x0t.data_ptr()),
                       ctypes.c_void_p(x.data_ptr()), bt.shape[0], ctypes.byref(opts), ctypes.byref(info)))
         return SolveResult(self._from_rhs(x), info.matvecs_fp64, info.matvecs_fp32, info.converged,
-                           info.refinements, info.max_relres)
+                           info.refinements, info.max_relres, info.matvecs_inner)
 
     def solve(self, b, x0, tol, restart=10, maxit=100, mixed=False, max_refine=8, slab_factor=1.0):
         return self._solve(self._lib.mpol_solve, 3, b, x0, tol, restart, maxit, mixed, max_refine, slab_factor)
+
+    def set_inner_operator(self, inner, floor=1e-3):
+        """Operator of the mixed solves' correction steps: `inner`, a Context
+        on the same particles and permittivity sets (coarse_operator builds
+        one on a coarser grid), takes the first residual and the FP32
+        correction solves, which stop at `floor` (relative) at the lowest;
+        every residual that decides convergence stays on this context's FP64
+        operator.  Dipole solves only.  None removes it."""
+        self._sync_stream()
+        _lib.check(self._lib.mpol_set_inner_operator(self._ctx, None if inner is None else inner._ctx,
+                                                     float(floor)))
+        self._inner = inner  # the C context keeps a pointer to it
 
     # -- forces of the Ewald-summed polarisation energy --------------------
     def ewald_forces(self, p, f1, f2):

@@ -325,3 +325,50 @@ def test_rank_one_z_scheme_matches_oracle(gpu):
     print("complex rank-one vs full", relerr(outc[0], outc[2]), flush=True)
     assert relerr(outc[0], outc[2]) < 1e-13
     assert not np.array_equal(outc[0], outc[2])
+
+
+@pytest.mark.parametrize("ng", [(64, 96, 256), (64, 64, 64)])
+def test_pass_order_and_spread_group_keep_the_operator(gpu, ng):
+    """Several systems: the y, z and inverse y passes system by system
+    (fft_order 1, the automatic order) and pass by pass over all systems
+    (fft_order 2) give bitwise the same outputs, with the FP32 spread at one
+    right-hand side per workgroup (automatic) or in groups of three, for
+    real permittivities (three systems) and complex ones (two systems of two
+    right-hand sides), each output also against the oracle.  Both grids take
+    the matrix-core tile kernels (fixed summation orders; nz = 256 the
+    register z pass, nz = 64 the LDS one); off that path the spread sums
+    with atomics and repeats agree to rounding only."""
+    from mpol_hip import FP32, FP64, Context
+    rng = np.random.default_rng(53)
+    ng = np.array(ng)
+    box = np.array([40.0, 40.0, 40.0])
+    n = 500
+    x = rng.random((n, 3)) * box
+    xi, P, rc = 1.2, 8, 3.5
+    perp, para, rv = tables(xi)
+    eps = 2.0 + 10 * rng.random(n)
+    c = Context()
+    c.set_tables(perp, para, rv)
+    c.set_geometry(x, box, ng, P, xi, P * (box / ng * xi) ** 2 / np.pi, rc)
+    geo = eo.OperatorGeometry(x, box, ng, P, xi, rc, perp, para, rv)
+    cases = (("real", eps, rng.standard_normal((3, n, 3))),
+             ("complex", eps + 1j * (0.5 + rng.random(n)),
+              rng.standard_normal((2, n, 3)) + 1j * rng.standard_normal((2, n, 3))))
+    for name, e, m in cases:
+        c.set_permittivity(e)
+        for prec, tol in ((FP64, 1e-12), (FP32, 2e-6)):
+            outs = {}
+            for fo, rg in ((2, 3), (1, 3), (0, 0), (2, 0)):
+                c.set_tuning(precision=prec, fft_order=fo, rhs_group=rg)
+                outs[fo, rg] = c.matvec(m, precision=prec, slab_factor=0.0).cpu().numpy()
+            c.set_tuning()
+            base = outs[2, 3]
+            for key, o in outs.items():
+                assert np.array_equal(o, base), (name, prec, key, relerr(o, base))
+            for k in range(m.shape[0]):
+                ref = eo.field(geo, m[k], e, 0.0, complex_eps=(name == "complex"))
+                if name == "real":
+                    ref = np.real(ref)
+                err = relerr(base[k], ref)
+                print(name, "precision", int(prec), "rhs", k, "vs oracle", err, flush=True)
+                assert err < tol

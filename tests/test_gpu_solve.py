@@ -147,3 +147,78 @@ def test_run_capacitance_trajectory_frames_concurrent(gpu, precision):
     assert relerr(p4[:, :, 0], d["p"][:, :, 0]) < 1e-4
     assert abs(C4[0] - d["C"][0]) / abs(d["C"][0]) < 1e-5
     assert len(set(np.round(np.real(C4), 12))) == 6  # the frames differ
+
+
+def test_mixed_solve_with_a_coarse_correction_operator(gpu):
+    """Two-level mixed scheme: correction solves on a coarser operator (32^3,
+    P = 6 at rule tolerance 1e-4) while the FP64 residuals stay on the full
+    one (64^3, P = 8).  The result meets tol for the full operator (its true
+    FP64 residual), agrees with the full operator's tight FP64 solve to the
+    tolerance's reach, in more refinement sweeps than one level, for one and
+    for three right-hand sides; misuse is refused."""
+    import torch
+
+    import workloads
+    from mpol_hip import FP64, Context
+    from mpol_hip.autotune import coarse_operator
+    n = 3000
+    x, box = workloads.hard_spheres(n, 0.3, 11, sweeps=20)
+    xi, P, ng = 1.0, 8, np.array([64, 64, 64])
+    rc = float(np.sqrt(-np.log(1e-6)) / xi)
+    eps = np.full(n, 28.0)
+    perp, para, rv = tables(xi)
+    c = Context(gpu)
+    c.set_tables(perp, para, rv)
+    c.set_geometry(x, box, ng, P, xi, P * (box / ng * xi) ** 2 / np.pi, rc)
+    c.set_permittivity(eps)
+    inner, (xi2, ng2, P2, _, rc2) = coarse_operator(x, box, eps, 1e-4, 32, device=gpu)
+    print("coarse", xi2, ng2, P2, rc2, flush=True)
+    assert int(ng2[0]) == 32 and P2 == 6
+    phi = 4 * np.pi * n / (3 * np.prod(box))
+    mg = 4 * np.pi * (9.0 / 10.0) / (1 - 0.9 * phi)
+    tol = 1e-8
+    for fields in (np.eye(3)[2:], np.eye(3)):
+        b = torch.as_tensor(np.stack([np.tile(e, n).reshape(n, 3) for e in fields]), device=gpu)
+        ref = c.solve(b, mg * b, 1e-12, restart=10, maxit=200).x
+        c.set_inner_operator(None)
+        one = c.solve(b, mg * b, tol, restart=10, maxit=100, mixed=True)
+        c.set_inner_operator(inner, 1e-3)
+        two = c.solve(b, mg * b, tol, restart=10, maxit=100, mixed=True)
+        c.set_inner_operator(None)
+        res = b - c.matvec(two.x, precision=FP64)
+        rel = (torch.linalg.norm(res.reshape(b.shape[0], -1), dim=1)
+               / torch.linalg.norm(b.reshape(b.shape[0], -1), dim=1)).cpu().numpy()
+        err = relerr(two.x.cpu().numpy(), ref.cpu().numpy())
+        print("rhs", b.shape[0], "one level", one.matvecs_fp32, one.matvecs_fp64, one.refinements,
+              "two levels", two.matvecs_fp32, two.matvecs_fp64, two.refinements, "true relres", rel,
+              "err vs tight FP64", err, flush=True)
+        assert two.converged == b.shape[0]
+        assert np.all(rel <= tol * 1.001)
+        assert err < 1e-6
+        assert two.refinements > one.refinements
+        assert one.matvecs_inner == 0 and 0 < two.matvecs_inner <= two.matvecs_fp32
+    with pytest.raises(ValueError):
+        c.set_inner_operator(c)
+    other = Context(gpu)
+    other.set_tables(perp, para, rv)
+    other.set_geometry(x[:100], box, ng, P, xi, P * (box / ng * xi) ** 2 / np.pi, rc)
+    with pytest.raises(ValueError):
+        c.set_inner_operator(other)
+    inner.set_permittivity(np.stack([eps, eps]))
+    c.set_inner_operator(inner)
+    b = torch.as_tensor(np.tile([0.0, 0.0, 1.0], n).reshape(1, n, 3), device=gpu)
+    with pytest.raises(ValueError):
+        c.solve(b, mg * b, tol, mixed=True)  # permittivity sets differ
+    c.set_inner_operator(None)
+    # the tuner: the fastest of none and the candidate's floors, left on c
+    from mpol_hip.autotune import tune_inner
+    inner.set_permittivity(eps)
+    picked, cfg, report = tune_inner(c, x, box, eps, b, mg * b, (1e-8,), candidates=((1e-4, 32),),
+                                     floors=(1e-3, 3e-4), repeats=2, slab_factor=1.0)
+    print("tuner", cfg, report, flush=True)
+    assert [r[0] for r in report][0] == "none" and len(report) == 3
+    assert min(r[2] for r in report) == (report[0][2] if cfg is None else
+                                         [r[2] for r in report if r[0] != "none" and f"floor {cfg['floor']:g}" in r[0]][0])
+    r = c.solve(b, mg * b, 1e-8, mixed=True)
+    assert r.converged == 1
+    c.set_inner_operator(None)
