@@ -208,16 +208,16 @@ def tune_ewald(x, box, eps, target, precision=FP32, tols=None, sizes=CT_SIZES, r
 # Correction-operator candidates of the mixed solves, (Ewald tolerance of the
 # parameter rule, grid nodes along the longest axis), and the floors their
 # correction solves stop at.
-INNER_CANDIDATES = ((1e-4, 128), (1e-4, 96), (1e-3, 96), (1e-5, 128), (1e-5, 160))
+INNER_CANDIDATES = ((1e-4, 128), (1e-3, 128), (1e-4, 96), (1e-3, 96), (1e-5, 128), (1e-5, 160))
 INNER_FLOORS = (1e-3, 3e-4, 1e-4)
 
 
-def coarse_operator(x, box, eps, ewald_tol=1e-4, grid=128, device=None, tables=None):
+def coarse_operator(x, box, eps, ewald_tol=1e-4, grid=128, device=None, tables=None, tune=True):
     """(Context, (xi, ngrid, P, eta, rc)): the particles x with permittivities
     eps (one set) on the Ewald parameters of `grid` nodes along the longest
     box axis at rule tolerance ewald_tol (ewald_candidates), for
-    Context.set_inner_operator.  tables(xi) -> (perp, para, rvals), default
-    the reference's table grid."""
+    Context.set_inner_operator; tune: its FP32 launch shapes by tune_launch.
+    tables(xi) -> (perp, para, rvals), default the reference's table grid."""
     from .context import Context, default_tables
     cands = ewald_candidates(box, ewald_tol, sizes=(int(grid),))
     if not cands:
@@ -228,6 +228,8 @@ def coarse_operator(x, box, eps, ewald_tol=1e-4, grid=128, device=None, tables=N
     c.set_tables(perp, para, rv)
     c.set_geometry(x, box, ng, P, xi, eta, rc)
     c.set_permittivity(eps)
+    if tune:
+        tune_launch(c, FP32)
     return c, (xi, ng, P, eta, rc)
 
 
