@@ -181,9 +181,10 @@ int mpol_solve(mpol_ctx* ctx, const double* b_dev, const double* x0_dev, double*
  * this context's operator.  A correction solve stops at `floor` (relative)
  * at the lowest, about the relative distance between the two operators
  * (refining past it gains nothing).  matvecs_inner counts the inner
- * operator's applications (also in matvecs_fp32).  The dipole
- * operator only; the inner context must outlive its use and not be used
- * concurrently.  inner = NULL removes it. */
+ * operator's applications (also in matvecs_fp32).  The dipole and the
+ * dipole + quadrupole solves (mpol_solve, mpol_quad_solve; for the latter the
+ * inner context needs its quadrupole tables); the inner context must outlive
+ * its use and not be used concurrently.  inner = NULL removes it. */
 int mpol_set_inner_operator(mpol_ctx* ctx, mpol_ctx* inner, double floor);
 
 /* Dipole + quadrupole operator (8 unknowns per particle).  The seven

@@ -793,9 +793,10 @@ int solve_impl(mpol_ctx* c, int kind, const double* b, const double* x0, double*
                             &conv));
     inf.matvecs_fp64 = nmv;
   } else {
-    // the correction operator: the inner context when one is set (dipole
-    // operator), launching on this context's stream for the solve
-    mpol_ctx* ci = (kind == kOpDipole && c->inner != nullptr) ? c->inner : c;
+    // the correction operator: the inner context when one is set (dipole and
+    // dipole + quadrupole operators), launching on this context's stream for
+    // the solve
+    mpol_ctx* ci = ((kind == kOpDipole || kind == kOpQuad) && c->inner != nullptr) ? c->inner : c;
     if (ci != c) {
       CHECK_STATUS(check_ready(ci));
       if (ci->n != c->n || ci->nsets != c->nsets || ci->cplx_eps != c->cplx_eps || ci->device != c->device)
@@ -807,6 +808,7 @@ int solve_impl(mpol_ctx* c, int kind, const double* b, const double* x0, double*
       ~StreamSwap() { ctx->stream = saved; }
     } swap{ci, ci->stream};
     ci->stream = st;
+    if (ci != c && kind == kOpQuad) CHECK_STATUS(prepare_quad(ci));
     const double floor_in = ci != c ? std::max(c->inner_floor, kInnerFloor) : kInnerFloor;
     HIPALLOC(c->gd, (size_t)tot);
     HIPALLOC(c->gx, (size_t)tot);

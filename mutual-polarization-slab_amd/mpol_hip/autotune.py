@@ -212,13 +212,16 @@ INNER_CANDIDATES = ((1e-4, 128), (1e-3, 128), (1e-4, 96), (1e-3, 96), (1e-5, 128
 INNER_FLOORS = (1e-3, 3e-4, 1e-4)
 
 
-def coarse_operator(x, box, eps, ewald_tol=1e-4, grid=128, device=None, tables=None, tune=True):
+def coarse_operator(x, box, eps, ewald_tol=1e-4, grid=128, device=None, tables=None, tune=True,
+                    quadrupole=False):
     """(Context, (xi, ngrid, P, eta, rc)): the particles x with permittivities
     eps (one set) on the Ewald parameters of `grid` nodes along the longest
     box axis at rule tolerance ewald_tol (ewald_candidates), for
-    Context.set_inner_operator; tune: its FP32 launch shapes by tune_launch.
-    tables(xi) -> (perp, para, rvals), default the reference's table grid."""
-    from .context import Context, default_tables
+    Context.set_inner_operator; tune: its FP32 launch shapes by tune_launch
+    (dipole operator); quadrupole: with the seven quadrupole tables, for
+    dipole + quadrupole solves.  tables(xi) -> (perp, para, rvals), default
+    the reference's table grid."""
+    from .context import Context, default_tables, real_space_table_quad
     cands = ewald_candidates(box, ewald_tol, sizes=(int(grid),))
     if not cands:
         raise ValueError(f"no Ewald parameters with {grid} nodes at tol {ewald_tol} for box {box}")
@@ -227,20 +230,25 @@ def coarse_operator(x, box, eps, ewald_tol=1e-4, grid=128, device=None, tables=N
     c = Context(device)
     c.set_tables(perp, para, rv)
     c.set_geometry(x, box, ng, P, xi, eta, rc)
+    if quadrupole:
+        # field_quad_1..3, grad_quad_1..4 on the same rvals (entry 0 the self term)
+        c.set_quad_tables(np.stack(real_space_table_quad(rv[1:], xi)[1:]))
     c.set_permittivity(eps)
-    if tune:
+    if tune and not quadrupole:
         tune_launch(c, FP32)
     return c, (xi, ng, P, eta, rc)
 
 
 def tune_inner(ctx, x, box, eps, b, x0, tols, accept=None, candidates=INNER_CANDIDATES, floors=INNER_FLOORS,
-               repeats=3, slab_factor=1.0, restart=10, maxit=100):
+               repeats=3, slab_factor=1.0, restart=10, maxit=100, quadrupole=False):
     """Choose the correction operator of ctx's mixed solves by timing them:
     none (ctx's own FP32 operator) and each coarse candidate x floor.  For
     each setting the loosest tolerance of `tols` (loosest first) whose solve
     of b from x0 converges (its FP64 residual on ctx's operator meets it) and
     passes accept(result) (default: always) is timed over `repeats` solves;
-    the fastest setting is left on ctx.  Returns (inner Context or None,
+    the fastest setting is left on ctx (quadrupole: the dipole + quadrupole
+    solve, quad_solve, and coarse operators with quadrupole tables).  Returns
+    (inner Context or None,
     {"ewald_tol", "grid", "P", "floor", "tol"} or None for none, report
     [(label, tol, ms, matvecs_fp32, matvecs_fp64, matvecs_inner)])."""
     import torch
@@ -248,6 +256,8 @@ def tune_inner(ctx, x, box, eps, b, x0, tols, accept=None, candidates=INNER_CAND
     nrhs = b.shape[0] if b.ndim == 3 else 1
 
     def run(t):
+        if quadrupole:
+            return ctx.quad_solve(b, x0, t, restart=restart, maxit=maxit, mixed=True)
         return ctx.solve(b, x0, t, restart=restart, maxit=maxit, mixed=True, slab_factor=slab_factor)
 
     def timed():
@@ -274,7 +284,8 @@ def tune_inner(ctx, x, box, eps, b, x0, tols, accept=None, candidates=INNER_CAND
         best = (ms, None, None)
     for etol, grid in candidates:
         try:
-            inner, (xi, ng, P, eta, rc) = coarse_operator(x, box, eps, etol, grid, device=ctx.device)
+            inner, (xi, ng, P, eta, rc) = coarse_operator(x, box, eps, etol, grid, device=ctx.device,
+                                                          quadrupole=quadrupole)
         except ValueError:
             continue
         for fl in floors:

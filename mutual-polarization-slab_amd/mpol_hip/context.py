@@ -434,7 +434,7 @@ class Context:
         one on a coarser grid), takes the first residual and the FP32
         correction solves, which stop at `floor` (relative) at the lowest;
         every residual that decides convergence stays on this context's FP64
-        operator.  Dipole solves only.  None removes it."""
+        operator.  Dipole and dipole + quadrupole solves.  None removes it."""
         self._sync_stream()
         _lib.check(self._lib.mpol_set_inner_operator(self._ctx, None if inner is None else inner._ctx,
                                                      float(floor)))

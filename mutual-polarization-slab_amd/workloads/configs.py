@@ -12,7 +12,8 @@ quadrupole: N = 2e4, phi = 0.35, body-centred tetragonal lattice c/a = 1.2
 Each reports FP64 and mixed (FP32 operator inside FP64 refinement) solves/s
 at the loosest tolerance of {1e-6, 5e-7, ..., 1e-8} whose solution is
 within 1e-6 of a tight (1e-11) FP64 solve, the operator applications per
-solve and the error.
+solve and the error; and, when autotune.tune_inner finds one faster, the
+two-level mixed solve (correction solves on a coarser operator).
 
     python -m workloads.configs [--only tensor|quadrupole]
 """
@@ -103,6 +104,21 @@ def tensor(reps=3, sweeps=200):
         out[name] = {"tol": t, "rhs_per_s": round(3.0 / s, 2), "ms_per_batch": round(1e3 * s, 3), "rel_err": e,
                      "matvecs": {"fp32": r.matvecs_fp32, "fp64": r.matvecs_fp64}}
     out["speedup_mixed_vs_fp64"] = round(out["mixed"]["rhs_per_s"] / out["fp64"]["rhs_per_s"], 3)
+    # two levels: correction solves on a coarser operator
+    from mpol_hip.autotune import tune_inner
+    inner, icfg, _ = tune_inner(
+        ctx, cfg["x"], cfg["box"], cfg["eps"], b, x0, TOL_LADDER, repeats=reps, slab_factor=0.0,
+        candidates=((1e-4, 64), (1e-3, 64), (1e-3, 32)),
+        accept=lambda r: float(np.max(np.abs(r.x.cpu().numpy() - ref)) / np.max(np.abs(ref))) <= ERR_BAR)
+    if inner is not None:
+        t, s, r, e = _ladder(lambda tt: solve(True, tt), ref, reps)
+        out["mixed_two_level"] = {"tol": t, "rhs_per_s": round(3.0 / s, 2), "ms_per_batch": round(1e3 * s, 3),
+                                  "rel_err": e, "correction_operator": icfg,
+                                  "matvecs": {"coarse_fp32": r.matvecs_inner, "fp32": r.matvecs_fp32 - r.matvecs_inner,
+                                              "fp64": r.matvecs_fp64}}
+        out["speedup_two_level_vs_fp64"] = round(out["mixed_two_level"]["rhs_per_s"] / out["fp64"]["rhs_per_s"], 3)
+        ctx.set_inner_operator(None)
+        inner.close()
     C = ref.mean(axis=1)  # capacitance tensor rows (field i -> mean dipole)
     out["capacitance_tensor_diag"] = [float(C[i, i]) for i in range(3)]
     ctx.close()
@@ -149,6 +165,23 @@ def quadrupole(reps=3):
         out[name] = {"tol": t, "solves_per_s": round(1.0 / s_, 2), "ms_per_solve": round(1e3 * s_, 3), "rel_err": e,
                      "matvecs": {"fp32": r_.matvecs_fp32, "fp64": r_.matvecs_fp64}}
     out["speedup_mixed_vs_fp64"] = round(out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"], 3)
+    # two levels: correction solves on a coarser dipole + quadrupole operator
+    from mpol_hip.autotune import tune_inner
+    inner, icfg, _ = tune_inner(
+        ctx, x, box, eps, bt, x0t, TOL_LADDER, quadrupole=True, repeats=reps,
+        candidates=((1e-4, 96), (1e-4, 64), (1e-3, 64), (1e-5, 96)),
+        accept=lambda r: float(np.max(np.abs(r.x.cpu().numpy() - ref)) / np.max(np.abs(ref))) <= ERR_BAR)
+    if inner is not None:
+        t, s_, r_, e = _ladder(lambda tt: solve(True, tt), ref, reps)
+        out["mixed_two_level"] = {"tol": t, "solves_per_s": round(1.0 / s_, 2), "ms_per_solve": round(1e3 * s_, 3),
+                                  "rel_err": e, "correction_operator": icfg,
+                                  "matvecs": {"coarse_fp32": r_.matvecs_inner,
+                                              "fp32": r_.matvecs_fp32 - r_.matvecs_inner,
+                                              "fp64": r_.matvecs_fp64}}
+        out["speedup_two_level_vs_fp64"] = round(out["mixed_two_level"]["solves_per_s"]
+                                                 / out["fp64"]["solves_per_s"], 3)
+        ctx.set_inner_operator(None)
+        inner.close()
     out["mean_S_z"] = float(ref[0, :, 2].mean())
     return out
 

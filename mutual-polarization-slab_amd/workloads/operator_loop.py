@@ -4,6 +4,12 @@ context and applies the dipole operator a few times.
     python -m workloads.operator_loop [--workload n1e5_phi0.4] [--precision fp32|fp64]
                                       [--reps 3] [--nrhs 1] [--engine 0|1|2] [--sweeps 20]
                                       [--chunk 0] [--rhs-group 0] [--copy-mb 256]
+                                      [--coarse 128 --coarse-tol 1e-4]
+
+--coarse G: the workload's particles on the correction operator of the
+two-level mixed solves instead (G nodes along the longest axis at rule
+tolerance --coarse-tol, launch shapes tuned, as autotune.coarse_operator
+builds it).
 
 (run from mutual-polarization-slab_amd/, or with it on sys.path).  Prints
 one line per step so a profiler run shows progress.
@@ -29,6 +35,8 @@ def main(argv=None):
     ap.add_argument("--sweeps", type=int, default=20)
     ap.add_argument("--chunk", type=int, default=0, help="particles staged per chunk (0 = automatic)")
     ap.add_argument("--rhs-group", type=int, default=0, help="right-hand sides per tile workgroup (0 = automatic)")
+    ap.add_argument("--coarse", type=int, default=0, help="grid of the coarse correction operator (0 = off)")
+    ap.add_argument("--coarse-tol", type=float, default=1e-4)
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="first copy a float32 device buffer of this many MiB (counter calibration)")
     args = ap.parse_args(argv)
@@ -40,12 +48,17 @@ def main(argv=None):
 
     cfg = workloads.configuration(args.workload, sweeps=args.sweeps)
     print(f"workload {args.workload}: n={cfg['x'].shape[0]} grid={list(cfg['ngrid'])} P={cfg['P']}", flush=True)
-    perp, para, rv = default_tables(cfg["xi"])
-    ctx = Context()
-    ctx.set_tables(perp, para, rv)
-    ctx.set_geometry(cfg["x"], cfg["box"], cfg["ngrid"], cfg["P"], cfg["xi"], cfg["eta"], cfg["rc"])
-    ctx.set_permittivity(cfg["eps"])
-    ctx.set_tuning(engine=args.engine, chunk=args.chunk, rhs_group=args.rhs_group)
+    if args.coarse:
+        from mpol_hip.autotune import coarse_operator
+        ctx, (xi, ng, P, _, rc) = coarse_operator(cfg["x"], cfg["box"], cfg["eps"], args.coarse_tol, args.coarse)
+        print(f"coarse operator: grid={list(ng)} P={P} xi={xi:.4f} rc={rc:.3f}", flush=True)
+    else:
+        perp, para, rv = default_tables(cfg["xi"])
+        ctx = Context()
+        ctx.set_tables(perp, para, rv)
+        ctx.set_geometry(cfg["x"], cfg["box"], cfg["ngrid"], cfg["P"], cfg["xi"], cfg["eta"], cfg["rc"])
+        ctx.set_permittivity(cfg["eps"])
+        ctx.set_tuning(engine=args.engine, chunk=args.chunk, rhs_group=args.rhs_group)
     prec = FP32 if args.precision == "fp32" else FP64
     if args.copy_mb > 0:
         src = torch.ones(args.copy_mb << 18, dtype=torch.float32, device="cuda")

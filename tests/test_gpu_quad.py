@@ -325,3 +325,43 @@ def test_lone_sphere_quadrupole_clausius_mossotti_gpu(gpu):
         dense = np.linalg.solve(qo.dense_operator(geo, np.array([eps])), np.concatenate([np.zeros(3), g])).real
         assert _rel(out[k], dense) < 1e-10, (eps, _rel(out[k], dense))
     ctx.close()
+
+
+def test_quad_mixed_solve_with_a_coarse_correction_operator(gpu):
+    """Two-level dipole + quadrupole solve: correction solves on a coarser
+    operator (32^3, P = 6 at rule tolerance 1e-4, the library's quadrupole
+    tables at its xi) while FP64 residuals stay on the full one (64^3, P = 9,
+    the oracle's tables).  The two operators differ by the coarse one's
+    Ewald error only (a wrong quadrupole table would put them O(1) apart);
+    the two-level result meets tol for the full operator and agrees with its
+    tight FP64 solve."""
+    import torch
+
+    from mpol_hip import FP32, FP64
+    from mpol_hip.autotune import coarse_operator
+    c = Case(gpu, n=800, phi=0.2, seed=5, xi=1.04, ngrid=[64, 64, 64], P=9)
+    inner, (xi2, ng2, P2, _, rc2) = coarse_operator(c.x, c.box, c.lam, 1e-4, 32, device=gpu, quadrupole=True)
+    print("coarse", xi2, ng2, P2, rc2, flush=True)
+    m = torch.as_tensor(_moments(c.n, 9, k=1), device=gpu)
+    full = c.ctx.quad_matvec(m, precision=FP64).cpu().numpy()
+    coarse = inner.quad_matvec(m, precision=FP64).cpu().numpy()
+    dist = _rel(coarse, full)
+    print("coarse vs full operator", dist, flush=True)
+    assert 1e-8 < dist < 1e-2
+    b = np.zeros((1, c.n, 8))
+    b[0, :, 2] = 1.0
+    bt = torch.as_tensor(b, device=gpu)
+    x0 = torch.zeros_like(bt)
+    tol = 1e-8
+    ref = c.ctx.quad_solve(bt, x0, 1e-12, restart=10, maxit=200).x
+    c.ctx.set_inner_operator(inner, 1e-3)
+    two = c.ctx.quad_solve(bt, x0, tol, restart=10, maxit=100, mixed=True)
+    c.ctx.set_inner_operator(None)
+    res = bt - c.ctx.quad_matvec(two.x, precision=FP64)
+    rel = float(torch.linalg.norm(res) / torch.linalg.norm(bt))
+    err = _rel(two.x.cpu().numpy(), ref.cpu().numpy())
+    print("two levels", two.matvecs_inner, two.matvecs_fp32, two.matvecs_fp64, two.refinements, "true relres", rel,
+          "err vs tight FP64", err, flush=True)
+    assert two.converged == 1 and rel <= tol * 1.001 and err < 1e-6
+    assert two.matvecs_inner > 0
+    inner.close()
