@@ -345,17 +345,24 @@ def main():
     b3 = torch.as_tensor(np.stack([np.tile(e, n).reshape(n, 3) for e in np.eye(3)]), device=dev)
     x03 = mg * b3
     ref3 = solve(False, 1e-10, b3, x03).x.cpu().numpy()
-    solve(True, tol, b3, x03)
+    # its own loosest ladder tolerance at the error bar (the worst field decides)
+    tol3 = TOL_LADDER[-1]
+    for t in TOL_LADDER:
+        if rel_err(solve(True, t, b3, x03).x.cpu().numpy(), ref3) <= ERR_BAR:
+            tol3 = t
+            break
+    solve(True, tol3, b3, x03)
     torch.cuda.synchronize()
     nb = max(1, min(3, args.steps))
     t0 = time.perf_counter()
     for _ in range(nb):
-        r3 = solve(True, tol, b3, x03)
+        r3 = solve(True, tol3, b3, x03)
     torch.cuda.synchronize()
     ms3 = 1e3 * (time.perf_counter() - t0) / nb
-    batched = {"rhs": 3, "solves_per_s": round(3e3 / ms3, 3), "ms_per_batch": round(ms3, 3),
+    batched = {"rhs": 3, "solves_per_s": round(3e3 / ms3, 3), "ms_per_batch": round(ms3, 3), "tol": tol3,
                "rel_dipole_err_vs_fp64": rel_err(r3.x.cpu().numpy(), ref3),
-               "matvecs_per_batch": {"fp32": r3.matvecs_fp32, "fp64": r3.matvecs_fp64}}
+               "matvecs_per_batch": {"coarse_fp32": r3.matvecs_inner, "fp32": r3.matvecs_fp32 - r3.matvecs_inner,
+                                     "fp64": r3.matvecs_fp64}}
 
     # matvec throughput and per-stage times
     m = torch.randn((1, n, 3), device=dev, dtype=torch.float64)

@@ -209,7 +209,7 @@ def tune_ewald(x, box, eps, target, precision=FP32, tols=None, sizes=CT_SIZES, r
 # parameter rule, grid nodes along the longest axis), and the floors their
 # correction solves stop at.
 INNER_CANDIDATES = ((1e-4, 128), (1e-3, 128), (1e-4, 96), (1e-3, 96), (1e-5, 128), (1e-5, 160))
-INNER_FLOORS = (1e-3, 3e-4, 1e-4)
+INNER_FLOORS = (1e-2, 3e-3, 1e-3)
 
 
 def coarse_operator(x, box, eps, ewald_tol=1e-4, grid=128, device=None, tables=None, tune=True,
