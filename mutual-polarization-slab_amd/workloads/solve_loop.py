@@ -1,7 +1,7 @@
 """Solve driver for kernel traces: the benchmark workload (N = 1e5 unit
 spheres, phi = 0.4, beta = 0.9, 256^3, P = 10, field along z, slab factor 0),
 a few mixed solves at tol 5e-7, two-level mixed solves (correction solves on
-the 128^3, P = 6 operator at rule tolerance 1e-4, floor 1e-3) at tol 1e-6
+the 128^3, P = 6 operator at rule tolerance 1e-4, floor 1e-2) at tol 1e-6
 and plain FP64 solves at tol 2e-7 (the tolerances that meet the 1e-6 dipole
 bar), with default launch shapes.  Each solve is bracketed by device
 synchronisations, a printed line and a 10 ms host pause, so a trace of this
@@ -55,7 +55,7 @@ def main(argv=None):
         inner, _ = coarse_operator(cfg["x"], cfg["box"], cfg["eps"], 1e-4, 128)
     for name, count, mixed, tol in (("mixed", args.mixed, True, 5e-7), ("two-level", args.two_level, True, 1e-6),
                                     ("fp64", args.fp64, False, 2e-7)):
-        ctx.set_inner_operator(inner if name == "two-level" else None, 1e-3)
+        ctx.set_inner_operator(inner if name == "two-level" else None, 1e-2)
         for k in range(count):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
