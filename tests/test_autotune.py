@@ -81,3 +81,68 @@ def test_tuned_ewald_grid_meets_the_error_target(gpu):
         c.set_permittivity(eps)
         outs.append(c.matvec(m, slab_factor=0.0).cpu().numpy())
     assert relerr(outs[0], outs[1]) < 3 * target
+
+
+def test_tune_inner_keeps_the_fastest_accepted_setting(monkeypatch):
+    """tune_inner's selection on the host, with stand-in contexts: per setting
+    the loosest tolerance whose solve converges and passes accept is timed;
+    the fastest setting stays set on the context, every other candidate
+    context is closed, and no candidate at all leaves none set."""
+    import types
+
+    from mpol_hip import autotune
+
+    cost = {None: 9.0, (1e-4, 128, 1e-2): 6.0, (1e-4, 128, 3e-3): 7.0, (1e-3, 96, 1e-2): 5.0,
+            (1e-3, 96, 3e-3): 8.0}
+    rejected = {(1e-3, 96, 1e-2)}  # fastest, but its dipoles miss the bar at every tolerance
+
+    class Inner:
+        def __init__(self, etol, grid):
+            self.key, self.closed = (etol, grid), False
+
+        def close(self):
+            self.closed = True
+
+    made = []
+
+    def fake_coarse(x, box, eps, etol, grid, device=None, quadrupole=False):
+        made.append(Inner(etol, grid))
+        return made[-1], (0.5, np.array([grid] * 3), 6, np.zeros(3), 4.0)
+
+    class Ctx:
+        device = None
+
+        def __init__(self):
+            self.inner, self.floor, self.clock = None, None, 0.0
+
+        def set_inner_operator(self, inner, floor=1e-3):
+            self.inner, self.floor = inner, floor
+
+        def solve(self, b, x0, t, **kw):
+            key = None if self.inner is None else (*self.inner.key, self.floor)
+            self.clock += cost[key] * 1e-3
+            return types.SimpleNamespace(converged=1, matvecs_fp32=1, matvecs_fp64=1, matvecs_inner=0,
+                                         bad=key in rejected)
+
+    ctx = Ctx()
+    monkeypatch.setattr(autotune, "coarse_operator", fake_coarse)
+    monkeypatch.setattr(autotune.time, "perf_counter", lambda: ctx.clock)
+    fake_torch = types.SimpleNamespace(cuda=types.SimpleNamespace(synchronize=lambda: None))
+    monkeypatch.setitem(__import__("sys").modules, "torch", fake_torch)
+    b = np.zeros((1, 4, 3))
+    inner, cfg, report = autotune.tune_inner(ctx, None, None, None, b, b, (1e-6, 5e-7),
+                                             accept=lambda r: not r.bad,
+                                             candidates=((1e-4, 128), (1e-3, 96)), floors=(1e-2, 3e-3), repeats=2)
+    assert cfg == {"ewald_tol": 1e-4, "grid": 128, "P": 6, "floor": 1e-2, "tol": 1e-6}
+    assert inner is made[0] and not made[0].closed and made[1].closed
+    assert ctx.inner is made[0] and ctx.floor == 1e-2
+    labels = [r[0] for r in report]
+    assert labels[0] == "none" and len(report) == 4  # the rejected setting is not timed
+    assert [round(r[2], 6) for r in report] == [9.0, 6.0, 7.0, 8.0]
+    # nothing faster than none: none stays
+    cost.update({(1e-4, 128, 1e-2): 10.0, (1e-4, 128, 3e-3): 11.0, (1e-3, 96, 3e-3): 12.0})
+    made.clear()
+    ctx = Ctx()
+    inner, cfg, _ = autotune.tune_inner(ctx, None, None, None, b, b, (1e-6,), accept=lambda r: not r.bad,
+                                        candidates=((1e-4, 128), (1e-3, 96)), floors=(1e-2, 3e-3))
+    assert inner is None and cfg is None and ctx.inner is None and all(m.closed for m in made)
