@@ -140,7 +140,8 @@ int mpol_matvec(mpol_ctx* ctx, const double* m_dev, double* out_dev, int32_t nrh
 /* Diagnostic timing of mpol_matvec (no counterpart in the reference): runs it
  * `repeats` times and returns in stage_ms (host, 8 entries) the mean
  * milliseconds of each stage measured with HIP events on the context stream:
- * [0] slab sum, [1] real-space sum, [2] spread + forward x-FFT,
+ * [0] slab sum, [1] real-space sum, [2] spread + forward x-FFT (with the
+ * permutation of the moments into the sorted order),
  * [3] forward y-FFT, [4] z-FFT + k-space scale + inverse z-FFT,
  * [5] inverse y-FFT, [6] inverse x-FFT + contraction, [7] per-particle sum of
  * the contraction's tile partials (matrix-core engine; 0 otherwise). */

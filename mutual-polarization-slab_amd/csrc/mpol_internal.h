@@ -616,10 +616,12 @@ int run_operator(mpol_ctx* c, const double* m, double* out, double* hr, double* 
                        c->cplx_eps ? c->diag_qim.p : nullptr, c->rs_off.p, c->rs_j.p, pA, pBA, prh, pq,
                        hr != nullptr ? 1 : 0, hr, gate, rps);
   }
+  // the stage events bracket the real-space sum alone; the permutation of
+  // the moments into sorted order counts with the spread
+  if (ev) HIPCHK(hipEventRecord(ev[2], st));
   HIPALLOC(c->ms, (size_t)nrhs * NC * n);
   hipLaunchKernelGGL(k_permute_m<NC>, dim3(cdiv(n, 256), nrhs), dim3(256), 0, st, m, c->perm.p, n, c->ms.p, gate,
                      rps);
-  if (ev) HIPCHK(hipEventRecord(ev[2], st));
 
   // tile (x axis) kernels
   const bool gen_x = generic_fft(c->plan[0]);
