@@ -222,3 +222,45 @@ def test_mixed_solve_with_a_coarse_correction_operator(gpu):
     r = c.solve(b, mg * b, 1e-8, mixed=True)
     assert r.converged == 1
     c.set_inner_operator(None)
+
+
+def test_two_level_solve_with_complex_permittivities(gpu):
+    """Two-level mixed solve with complex permittivities (each system a
+    (re, im) pair of right-hand sides): the inner operator carries the same
+    complex permittivities; the result meets tol on the full operator's true
+    FP64 residual and agrees with its tight FP64 solve.  An inner operator
+    left at real permittivities is refused."""
+    import torch
+
+    import workloads
+    from mpol_hip import FP64, Context
+    from mpol_hip.autotune import coarse_operator
+    n = 2000
+    x, box = workloads.hard_spheres(n, 0.3, 13, sweeps=20)
+    xi, P, ng = 1.0, 8, np.array([64, 64, 64])
+    rc = float(np.sqrt(-np.log(1e-6)) / xi)
+    rng = np.random.default_rng(5)
+    eps = (4.0 + 20.0 * rng.random(n)) + 1j * (0.5 + 2.0 * rng.random(n))
+    perp, para, rv = tables(xi)
+    c = Context(gpu)
+    c.set_tables(perp, para, rv)
+    c.set_geometry(x, box, ng, P, xi, P * (box / ng * xi) ** 2 / np.pi, rc)
+    c.set_permittivity(eps)
+    inner, _ = coarse_operator(x, box, eps.real, 1e-4, 32, device=gpu)
+    b = np.zeros((1, n, 3), dtype=np.complex128)
+    b[0, :, 2] = 1.0
+    x0 = np.zeros_like(b)
+    ref = c.solve(b, x0, 1e-12, restart=10, maxit=300).x
+    c.set_inner_operator(inner, 1e-3)
+    with pytest.raises(ValueError):
+        c.solve(b, x0, 1e-8, mixed=True)  # the inner operator's permittivities are real
+    inner.set_permittivity(eps)
+    two = c.solve(b, x0, 1e-8, restart=10, maxit=100, mixed=True)
+    c.set_inner_operator(None)
+    res = torch.as_tensor(b, device=gpu) - c.matvec(two.x, precision=FP64)
+    rel = float(torch.linalg.norm(res) / np.linalg.norm(b))
+    err = relerr(two.x.cpu().numpy(), ref.cpu().numpy())
+    print("complex two levels", two.matvecs_inner, two.matvecs_fp32, two.matvecs_fp64, two.refinements,
+          "true relres", rel, "err", err, flush=True)
+    assert two.converged == 1 and rel <= 1e-8 * 1.001 and err < 1e-6 and two.matvecs_inner > 0
+    inner.close()
